@@ -1,0 +1,247 @@
+"""ctypes front end of the C oracle (oracle/pmoracle.c) and of the compiled
+reference harness (oracle/_ref/ref_driver).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+
+Arrays follow the reference layout: meshes are numpy arrays of shape
+(3, Ex, Ey, Ez) indexed [c, i, j, k] (component first, ghosts included) and
+stored in Fortran order, particles are (3, np) Fortran-ordered.
+"""
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libpmoracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+
+
+class Geom(C.Structure):
+    _fields_ = [("ncell", C.c_int32 * 3), ("nghost", C.c_int32),
+                ("bound_cond", C.c_int32 * 3), ("interp_order", C.c_int32),
+                ("xmin", C.c_double * 3), ("xmax", C.c_double * 3), ("dx", C.c_double * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [("regularisation_radius", C.c_double), ("poisson_order", C.c_int32),
+                ("poisson_kernel", C.c_int32)]
+
+
+def make_geom(ncell, nghost, xmin, dx, bound_cond=(0, 0, 0), interp_order=3):
+    g = Geom()
+    for d in range(3):
+        g.ncell[d] = int(ncell[d])
+        g.bound_cond[d] = int(bound_cond[d])
+        g.xmin[d] = float(xmin[d])
+        g.dx[d] = float(dx if np.isscalar(dx) else dx[d])
+        g.xmax[d] = g.xmin[d] + g.ncell[d] * g.dx[d]
+    g.nghost = int(nghost)
+    g.interp_order = int(interp_order)
+    return g
+
+
+def make_params(order=4, kernel=1, eps_radius=2.0):
+    p = Params()
+    p.regularisation_radius = float(eps_radius)
+    p.poisson_order = int(order)
+    p.poisson_kernel = int(kernel)
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library not built: run make -f oracle/Makefile")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        G = C.POINTER(Geom)
+        Pp = C.POINTER(Params)
+        L.or_glassman.argtypes = [P, P, C.c_int, C.c_int]
+        L.or_green_real.argtypes = [G, Pp, P]
+        L.or_green_real.restype = C.c_int
+        L.or_kernel_size.argtypes = [G, Pp]
+        L.or_kernel_size.restype = C.c_int64
+        L.or_poisson_setup.argtypes = [G, Pp, P]
+        L.or_poisson_setup.restype = C.c_int
+        L.or_poisson_solve.argtypes = [G, Pp, P, P, P, P, P, C.c_int, C.c_int, C.c_int]
+        L.or_poisson_solve.restype = C.c_int
+        L.or_cell_index.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.or_cell_index.restype = C.c_int32
+        L.or_cell_indices.argtypes = [G, P, C.c_int64, P]
+        L.or_p2m.argtypes = [G, P, P, C.c_int64, P, C.c_int]
+        L.or_p2m.restype = C.c_int
+        L.or_m2p.argtypes = [G, P, P, C.c_int64, P]
+        L.or_m2p.restype = C.c_int
+        L.or_remesh.argtypes = [G, P, P, P, C.c_double, C.c_double, C.c_int, P, P, P, P, C.c_int64]
+        L.or_remesh.restype = C.c_int64
+        L.or_advance.argtypes = [G, C.c_int, C.c_int, C.c_double, C.c_int, P, P, P, P, P, P, C.c_int64]
+        L.or_advance.restype = C.c_int64
+        L.or_penalise.argtypes = [G, P, P, P, C.c_double, P, P]
+        L.or_penalise.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def mesh_extent(g):
+    ext = []
+    for d in range(3):
+        if g.bound_cond[d] == 0:
+            ng = g.nghost
+        else:
+            ng = 3 if g.interp_order == 4 else 2
+        ext.append(g.ncell[d] + 2 * ng)
+    return tuple(ext)
+
+
+def new_mesh(g):
+    return np.zeros((3,) + mesh_extent(g), dtype=np.float64, order="F")
+
+
+def glassman(x, sign=-1):
+    u = np.array(x, dtype=np.complex128)
+    w = np.empty_like(u)
+    lib().or_glassman(_p(u), _p(w), len(u), sign)
+    return u
+
+
+def green_real(g, p):
+    n = lib().or_kernel_size(g, p)
+    G = np.zeros(n, dtype=np.float64)
+    if lib().or_green_real(g, p, _p(G)):
+        raise ValueError("bad kernel order")
+    return G
+
+
+def poisson_setup(g, p):
+    n = lib().or_kernel_size(g, p)
+    Gh = np.zeros(n, dtype=np.complex128)
+    if lib().or_poisson_setup(g, p, _p(Gh)):
+        raise ValueError("bad kernel order")
+    return Gh
+
+
+def poisson_solve(g, p, Ghat, vort, reg_vort=False, reproj=False, calc_vecpot=False):
+    vort = np.asfortranarray(vort, dtype=np.float64)
+    vel = new_mesh(g)
+    vo = new_mesh(g) if (reg_vort or reproj) else None
+    ps = new_mesh(g) if calc_vecpot else None
+    e = lib().or_poisson_solve(g, p, _p(Ghat), _p(vort), _p(vel), _p(vo), _p(ps),
+                               int(reg_vort), int(reproj), int(calc_vecpot))
+    if e:
+        raise ValueError("or_poisson_solve failed %d" % e)
+    return vel, vo, ps
+
+
+def cell_indices(g, pos):
+    pos = np.asfortranarray(pos, dtype=np.float64)
+    out = np.zeros(pos.shape, dtype=np.int32, order="F")
+    lib().or_cell_indices(g, _p(pos), pos.shape[1], _p(out))
+    return out
+
+
+def p2m(g, pos, val, mesh=None, clear=True):
+    pos = np.asfortranarray(pos, dtype=np.float64)
+    val = np.asfortranarray(val, dtype=np.float64)
+    if mesh is None:
+        mesh = new_mesh(g)
+    lib().or_p2m(g, _p(pos), _p(val), pos.shape[1], _p(mesh), int(clear))
+    return mesh
+
+
+def m2p(g, mesh, pos):
+    pos = np.asfortranarray(pos, dtype=np.float64)
+    mesh = np.asfortranarray(mesh, dtype=np.float64)
+    out = np.zeros(pos.shape, dtype=np.float64, order="F")
+    lib().or_m2p(g, _p(mesh), _p(pos), pos.shape[1], _p(out))
+    return out
+
+
+def remesh(g, vort_m, dvort_m=None, vel_m=None, min_vort=0.0, min_dvort=0.0, incl_ghost=False):
+    vort_m = np.asfortranarray(vort_m, dtype=np.float64)
+    dv = None if dvort_m is None else np.asfortranarray(dvort_m, dtype=np.float64)
+    vl = None if vel_m is None else np.asfortranarray(vel_m, dtype=np.float64)
+    n = lib().or_remesh(g, _p(vort_m), _p(dv), _p(vl), min_vort, min_dvort, int(incl_ghost),
+                        None, None, None, None, 0)
+    out = [np.zeros((3, n), order="F") for _ in range(4)]
+    lib().or_remesh(g, _p(vort_m), _p(dv), _p(vl), min_vort, min_dvort, int(incl_ghost),
+                    *[_p(a) for a in out], n)
+    return out  # pos, vort, vel, dvort
+
+
+def advance(g, rk_order, stage, dt, pos, vel, vort, dvort, vel_rk, dvort_rk, euler_vort=False):
+    arrs = [np.array(a, dtype=np.float64, order="F", copy=True)
+            for a in (pos, vel, vort, dvort, vel_rk, dvort_rk)]
+    n = lib().or_advance(g, rk_order, stage, dt, int(euler_vort), *[_p(a) for a in arrs],
+                         arrs[0].shape[1])
+    return [a[:, :n] for a in arrs]
+
+
+def penalise(g, vel, chi, vort, solid_vel=None, relax=1.0):
+    vel = np.asfortranarray(vel, dtype=np.float64)
+    chi = np.asfortranarray(chi, dtype=np.float64)
+    vort = np.array(vort, dtype=np.float64, order="F", copy=True)
+    sv = None if solid_vel is None else np.asfortranarray(solid_vel, dtype=np.float64)
+    diag = np.zeros(5)
+    if lib().or_penalise(g, _p(vel), _p(chi), _p(sv), relax, _p(vort), _p(diag)):
+        raise ValueError("penalise needs >= 2 ghost layers")
+    return vort, diag
+
+
+# --------------------------------------------------------------------------
+# compiled reference (oracle/_ref), run in this container only
+
+
+def ref_available():
+    return os.path.exists(REF_DRIVER)
+
+
+def _fmt(v):
+    if isinstance(v, bool):
+        return ".true." if v else ".false."
+    if isinstance(v, str):
+        return "'%s'" % v
+    if isinstance(v, (list, tuple)):
+        return ",".join(_fmt(x) for x in v)
+    if isinstance(v, float):
+        return repr(v)
+    return str(v)
+
+
+def ref_run(job, inputs, nproc=1, timeout=600):
+    """Run the reference harness with namelist `job` (dict); `inputs` is a list
+    of arrays written back to back (Fortran order) as the input file.
+    Returns (raw float64 output, stdout)."""
+    with tempfile.TemporaryDirectory() as td:
+        inf = os.path.join(td, "in.bin")
+        outf = os.path.join(td, "out.bin")
+        with open(inf, "wb") as f:
+            for a in inputs:
+                np.asarray(a).ravel(order="F").tofile(f)
+        job = dict(job, infile=inf, outfile=outf)
+        nml = "&job " + ", ".join("%s=%s" % (k, _fmt(v)) for k, v in job.items()) + " /\n"
+        cmd = [REF_DRIVER] if nproc == 1 else ["mpiexec", "-n", str(nproc), REF_DRIVER]
+        r = subprocess.run(cmd, input=nml, text=True, capture_output=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError("ref_driver failed: %s %s" % (r.stdout[-2000:], r.stderr[-2000:]))
+        raw = open(outf, "rb").read() if os.path.exists(outf) else b""
+        return raw, r.stdout
+
+
+def geom_job(g, p=None):
+    j = dict(ncell=list(g.ncell), nghost=g.nghost, bound_cond=list(g.bound_cond),
+             xmin=list(g.xmin), xmax=list(g.xmax), dx=list(g.dx), interp_order=g.interp_order)
+    if p is not None:
+        j.update(order=p.poisson_order, kernel=p.poisson_kernel, eps_radius=p.regularisation_radius)
+    return j

@@ -1,0 +1,343 @@
+!-------------------------------------------------------------------------------
+! ref_driver.f90 -- harness around the compiled reference library (oracle/_ref).
+!
+! Test infrastructure only: it runs the reference pmlib3d routines on inputs
+! written by the test suite and dumps their FP64 results, so that the C
+! restatement in oracle/ and the GPU path can be pinned against the real thing.
+!
+! The geometry is set directly on class_patch (pmlib_patch_adjust is bypassed,
+! it would always grow the grid: patch/pmlib_patch_adjust.f:53).  The job is
+! read as a namelist from stdin; raw arrays use stream I/O in Fortran order.
+!
+! Modes
+!   solve    : pmlib_poisson_setup + pmlib_poisson_solve   (poisson/*.f)
+!   p2m      : pmlib_interp_particle_mesh                   (interpolation/)
+!   m2p      : pmlib_interp_mesh_particle (vel)             (interpolation/)
+!   remesh   : pmlib_remesh                                 (remesh/)
+!   advance  : pmlib_particles_advance                      (particles/)
+!   step     : 2 x (P2M, solve, M2P, advance) + (P2M, remesh), timed
+!-------------------------------------------------------------------------------
+PROGRAM ref_driver
+  USE pmlib_mod_parameters
+  USE pmlib_mod_write
+  USE pmlib_mod_patch
+  USE pmlib_mod_topology
+  USE pmlib_mod_mesh
+  USE pmlib_mod_particles
+  USE pmlib_mod_poisson
+  USE pmlib_mod_interpolation
+  USE pmlib_mod_remesh
+  IMPLICIT NONE
+
+  CHARACTER(LEN=16)  :: mode = 'solve'
+  INTEGER            :: ncell(3) = 32, nghost = 0, bound_cond(3) = 0, fft_seq(3) = (/1,2,3/)
+  REAL(MK)           :: xmin(3) = -1.0_MK, xmax(3) = 1.0_MK, dx(3) = 1.0_MK/16.0_MK
+  INTEGER            :: order = 4, kernel = 1, interp_order = 3, rk_order = 2
+  REAL(MK)           :: eps_radius = 2.0_MK
+  LOGICAL            :: reg_vort = .FALSE., reproj = .FALSE., calc_vecpot = .FALSE.
+  LOGICAL            :: clear = .TRUE., incl_ghost = .FALSE., euler_vort = .FALSE.
+  INTEGER            :: npart = 0, stage = 1, nrep = 1, nsteps = 1
+  REAL(MK)           :: dtime = 0.0_MK, min_vort = 0.0_MK, min_dvort = 0.0_MK
+  CHARACTER(LEN=512) :: infile = 'in.bin', outfile = 'out.bin'
+  NAMELIST /job/ mode, ncell, nghost, bound_cond, fft_seq, xmin, xmax, dx, order, kernel, &
+       & interp_order, rk_order, eps_radius, reg_vort, reproj, calc_vecpot, clear, &
+       & incl_ghost, euler_vort, npart, stage, nrep, nsteps, dtime, min_vort, min_dvort, &
+       & infile, outfile
+
+  TYPE(class_patch)        :: patch
+  TYPE(class_topology_all) :: topo_all
+  TYPE(class_mesh), SAVE   :: mesh
+  TYPE(class_particles)    :: part
+  INTEGER                  :: ierr, rank, nproc, irep, istep, s, nstage
+  INTEGER                  :: lo(3), hi(3), ng6(6)
+  REAL(MK), ALLOCATABLE    :: vort_in(:,:,:,:)
+  REAL(MK)                 :: t0, t1, tsetup, tmin, tsum
+  REAL(MK)                 :: tp2m, tsolve, tm2p, tadv, tremesh
+
+  CALL MPI_INIT(ierr)
+  CALL MPI_COMM_RANK(MPI_COMM_WORLD, rank, ierr)
+  CALL MPI_COMM_SIZE(MPI_COMM_WORLD, nproc, ierr)
+  mpi_comm = MPI_COMM_WORLD
+  mpi_rank = rank
+  mpi_nproc = nproc
+
+  IF (rank .EQ. 0) READ(*, NML=job)
+  CALL bcast_job()
+
+  pmlib_regularisation_radius  = eps_radius
+  pmlib_poisson_order          = order
+  pmlib_poisson_kernel         = kernel
+  pmlib_interpolation_order    = interp_order
+  pmlib_time_integration_order = rk_order
+  pmlib_fd_order               = 4
+
+  patch%ptype = 1
+  patch%level = 1
+  patch%nsubs = nproc
+  patch%parent = (/ 0, 0 /)
+  patch%xmin = xmin
+  patch%xmax = xmax
+  patch%dx = dx
+  patch%ncell = ncell
+  patch%bound_cond = bound_cond
+  patch%nghost = nghost
+  patch%fft_seq = fft_seq
+
+  CALL pmlib_topology_cuboid(patch, topo_all%cuboid, ierr)
+  IF (ierr .NE. 0) CALL die('topology')
+  ng6 = topo_all%cuboid(rank)%nghost
+  lo = 1 - ng6(1:5:2)
+  hi = topo_all%cuboid(rank)%ncell + ng6(2:6:2)
+  IF (nproc .GT. 1 .AND. mode .NE. 'solve') CALL die('only solve runs on several ranks')
+
+  CALL pmlib_mesh_allocate(topo_all%cuboid, mesh, ierr, vort=.TRUE., dvort=.TRUE., vel=.TRUE., &
+       & mask=.FALSE.)
+  IF (ierr .NE. 0) CALL die('mesh allocate')
+  mesh%vort = 0.0_MK
+  mesh%dvort = 0.0_MK
+  mesh%vel = 0.0_MK
+  ! pmlib_ifft sizes the psi transform from SIZE(mesh%vec_pot,1)
+  ! (fourier/pmlib_fourier_ifft.f:43), so the caller allocates it.
+  IF (calc_vecpot) THEN
+    ALLOCATE(mesh%vec_pot(3, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)))
+    mesh%vec_pot = 0.0_MK
+  END IF
+
+  SELECT CASE (TRIM(mode))
+
+  CASE ('solve')
+    ! Timing over nrep repetitions (setup timed separately); only rank 0
+    ! does file I/O, the field is scattered through a global read per rank.
+    t0 = MPI_WTIME()
+    CALL pmlib_poisson_setup(patch, topo_all, mesh, ierr)
+    IF (ierr .NE. 0) CALL die('poisson setup')
+    tsetup = MPI_WTIME() - t0
+    CALL read_mesh_slice(infile, mesh%vort)
+    ALLOCATE(vort_in(3, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)))
+    vort_in = mesh%vort
+    tmin = HUGE(1.0_MK)
+    tsum = 0.0_MK
+    DO irep = 1, nrep
+      IF (.NOT. ASSOCIATED(mesh%vort)) ALLOCATE(mesh%vort(3, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)))
+      mesh%vort = vort_in
+      CALL MPI_BARRIER(MPI_COMM_WORLD, ierr)
+      t0 = MPI_WTIME()
+      CALL pmlib_poisson_solve(patch, topo_all, mesh, ierr, reg_vort=reg_vort, &
+           & reproj=reproj, calc_vecpot=calc_vecpot)
+      CALL MPI_BARRIER(MPI_COMM_WORLD, ierr)
+      t1 = MPI_WTIME() - t0
+      IF (ierr .NE. 0) CALL die('poisson solve')
+      tmin = MIN(tmin, t1)
+      tsum = tsum + t1
+    END DO
+    IF (rank .EQ. 0) WRITE(*,'(A,I4,A,E16.8,A,E16.8,A,E16.8)') 'TIMING ranks=', nproc, &
+         & ' setup_s=', tsetup, ' solve_min_s=', tmin, ' solve_mean_s=', tsum/nrep
+    IF (nproc .EQ. 1) THEN
+      OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+      WRITE(20) mesh%vel
+      IF (reg_vort .OR. reproj) WRITE(20) mesh%vort
+      IF (calc_vecpot .AND. kernel .EQ. 1) WRITE(20) mesh%vec_pot
+      CLOSE(20)
+    END IF
+
+  CASE ('p2m')
+    CALL read_particles(.FALSE.)
+    CALL pmlib_interp_particle_mesh(topo_all%cuboid, part%pos, part%vort, mesh%vort, ierr, &
+         & clear=clear, clear_ghost=.FALSE.)
+    IF (ierr .NE. 0) CALL die('p2m')
+    OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(20) mesh%vort
+    CLOSE(20)
+
+  CASE ('m2p')
+    ! input: mesh vel (3,lo:hi) then npart positions
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) mesh%vel
+    ALLOCATE(part%pos(3,npart), part%vel(3,npart), part%vort(3,npart), part%dvort(3,npart))
+    READ(10) part%pos
+    CLOSE(10)
+    topo_all%cuboid(rank)%npart = npart
+    part%vel = -1.0_MK
+    CALL pmlib_interp_mesh_particle(topo_all%cuboid, mesh, part, ierr, vort=.FALSE., &
+         & dvort=.FALSE., vel=.TRUE., clear=.FALSE.)
+    IF (ierr .NE. 0) CALL die('m2p')
+    OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(20) part%vel
+    CLOSE(20)
+
+  CASE ('remesh')
+    ! input: mesh vort, dvort, vel
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) mesh%vort
+    READ(10) mesh%dvort
+    READ(10) mesh%vel
+    CLOSE(10)
+    CALL pmlib_remesh(patch, topo_all%cuboid, mesh, part, min_vort, min_dvort, ierr, &
+         & dvort=.TRUE., vel=.TRUE., mask=.FALSE., incl_ghost=incl_ghost)
+    IF (ierr .NE. 0) CALL die('remesh')
+    CALL write_particles()
+
+  CASE ('advance')
+    ! input: npart x (pos, vel, vort, dvort, vel_rk, dvort_rk)
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    ALLOCATE(part%pos(3,npart), part%vel(3,npart), part%vort(3,npart), part%dvort(3,npart))
+    ALLOCATE(part%vel_rk(3,npart), part%dvort_rk(3,npart))
+    READ(10) part%pos, part%vel, part%vort, part%dvort, part%vel_rk, part%dvort_rk
+    CLOSE(10)
+    topo_all%cuboid(rank)%npart = npart
+    CALL pmlib_particles_advance(patch, topo_all%cuboid, part, stage, dtime, ierr, &
+         & euler_vort=euler_vort)
+    IF (ierr .NE. 0) CALL die('advance')
+    npart = topo_all%cuboid(rank)%npart
+    OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(20) npart
+    WRITE(20) part%pos, part%vort, part%vel_rk, part%dvort_rk
+    CLOSE(20)
+
+  CASE ('step')
+    ! One vortex time step of the particle-mesh method, repeated nsteps times:
+    ! nstage x (P2M -> solve -> M2P(vel) -> advance) then P2M -> remesh.
+    CALL read_particles(.TRUE.)
+    t0 = MPI_WTIME()
+    CALL pmlib_poisson_setup(patch, topo_all, mesh, ierr)
+    IF (ierr .NE. 0) CALL die('poisson setup')
+    tsetup = MPI_WTIME() - t0
+    nstage = rk_order
+    tp2m = 0; tsolve = 0; tm2p = 0; tadv = 0; tremesh = 0
+    t1 = MPI_WTIME()
+    DO istep = 1, nsteps
+      DO s = 1, nstage
+        t0 = MPI_WTIME()
+        CALL pmlib_interp_particle_mesh(topo_all%cuboid, part%pos, part%vort, mesh%vort, &
+             & ierr, clear=.TRUE., clear_ghost=.FALSE.)
+        IF (ierr .NE. 0) CALL die('step p2m')
+        tp2m = tp2m + MPI_WTIME() - t0
+        t0 = MPI_WTIME()
+        CALL pmlib_poisson_solve(patch, topo_all, mesh, ierr, reg_vort=.FALSE., &
+             & reproj=.FALSE., calc_vecpot=.FALSE.)
+        IF (ierr .NE. 0) CALL die('step solve')
+        tsolve = tsolve + MPI_WTIME() - t0
+        t0 = MPI_WTIME()
+        CALL pmlib_interp_mesh_particle(topo_all%cuboid, mesh, part, ierr, vort=.FALSE., &
+         & dvort=.FALSE., vel=.TRUE., clear=.FALSE.)
+        IF (ierr .NE. 0) CALL die('step m2p')
+        tm2p = tm2p + MPI_WTIME() - t0
+        t0 = MPI_WTIME()
+        part%dvort = 0.0_MK
+        CALL pmlib_particles_advance(patch, topo_all%cuboid, part, s, dtime, ierr, &
+             & euler_vort=.FALSE.)
+        IF (ierr .NE. 0) CALL die('step advance')
+        part%dvort = 0.0_MK
+        tadv = tadv + MPI_WTIME() - t0
+      END DO
+      t0 = MPI_WTIME()
+      CALL pmlib_interp_particle_mesh(topo_all%cuboid, part%pos, part%vort, mesh%vort, &
+           & ierr, clear=.TRUE., clear_ghost=.FALSE.)
+      IF (ierr .NE. 0) CALL die('step remesh p2m')
+      mesh%dvort = 0.0_MK
+      CALL pmlib_remesh(patch, topo_all%cuboid, mesh, part, min_vort, min_dvort, ierr, &
+           & dvort=.FALSE., vel=.TRUE., mask=.FALSE., incl_ghost=.FALSE.)
+      IF (ierr .NE. 0) CALL die('step remesh')
+      tremesh = tremesh + MPI_WTIME() - t0
+    END DO
+    t1 = MPI_WTIME() - t1
+    WRITE(*,'(A,E16.8,A,E16.8,A,5E16.8)') 'TIMING setup_s=', tsetup, ' step_s=', t1/nsteps, &
+         & ' parts(p2m,solve,m2p,adv,remesh)=', tp2m/nsteps, tsolve/nsteps, tm2p/nsteps, &
+         & tadv/nsteps, tremesh/nsteps
+    CALL write_particles()
+
+  CASE DEFAULT
+    CALL die('unknown mode')
+  END SELECT
+
+  CALL MPI_FINALIZE(ierr)
+
+CONTAINS
+
+  SUBROUTINE die(msg)
+    CHARACTER(LEN=*), INTENT(IN) :: msg
+    INTEGER :: e
+    WRITE(*,*) 'REF_DRIVER_ERROR: ', msg
+    CALL MPI_ABORT(MPI_COMM_WORLD, 1, e)
+    STOP 1
+  END SUBROUTINE die
+
+  SUBROUTINE bcast_job()
+    INTEGER :: e
+    CALL MPI_BCAST(mode, 16, MPI_CHARACTER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(ncell, 3, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(nghost, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(bound_cond, 3, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(fft_seq, 3, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(xmin, 3, MPI_DOUBLE_PRECISION, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(xmax, 3, MPI_DOUBLE_PRECISION, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(dx, 3, MPI_DOUBLE_PRECISION, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(order, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(kernel, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(interp_order, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(rk_order, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(eps_radius, 1, MPI_DOUBLE_PRECISION, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(reg_vort, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(reproj, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(calc_vecpot, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(nrep, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(interp_order, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(infile, 512, MPI_CHARACTER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(outfile, 512, MPI_CHARACTER, 0, MPI_COMM_WORLD, e)
+  END SUBROUTINE bcast_job
+
+  ! Reads this rank's block of a global (3, 1-ng:N+ng, ...) field.  The file
+  ! holds the global field including the exterior ghost layers.
+  SUBROUTINE read_mesh_slice(fname, fld)
+    CHARACTER(LEN=*), INTENT(IN)         :: fname
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: fld
+    REAL(MK), ALLOCATABLE :: glob(:,:,:,:)
+    INTEGER :: gng(3), i0(3), d
+    ! exterior ghost layers of the global mesh: patch%nghost on free sides,
+    ! the interior count on periodic ones (topology/pmlib_topology_cuboid.f:71-80)
+    DO d = 1, 3
+      IF (bound_cond(d) .EQ. 0) THEN
+        gng(d) = nghost
+      ELSE IF (interp_order .EQ. 4) THEN
+        gng(d) = 3
+      ELSE
+        gng(d) = 2
+      END IF
+    END DO
+    ALLOCATE(glob(3, 1-gng(1):ncell(1)+gng(1), 1-gng(2):ncell(2)+gng(2), &
+         & 1-gng(3):ncell(3)+gng(3)))
+    OPEN(10, FILE=TRIM(fname), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) glob
+    CLOSE(10)
+    ! local index l maps to global index l + icell - 1
+    i0 = topo_all%cuboid(rank)%icell - 1
+    fld(:, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)) = &
+         & glob(:, lo(1)+i0(1):hi(1)+i0(1), lo(2)+i0(2):hi(2)+i0(2), lo(3)+i0(3):hi(3)+i0(3))
+    DEALLOCATE(glob)
+  END SUBROUTINE read_mesh_slice
+
+  SUBROUTINE read_particles(with_rk)
+    LOGICAL, INTENT(IN) :: with_rk
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    ALLOCATE(part%pos(3,npart), part%vel(3,npart), part%vort(3,npart), part%dvort(3,npart))
+    ALLOCATE(part%vel_rk(3,npart), part%dvort_rk(3,npart))
+    READ(10) part%pos
+    READ(10) part%vort
+    CLOSE(10)
+    part%vel = 0.0_MK
+    part%dvort = 0.0_MK
+    part%vel_rk = 0.0_MK
+    part%dvort_rk = 0.0_MK
+    topo_all%cuboid(rank)%npart = npart
+    IF (.NOT. with_rk) RETURN
+  END SUBROUTINE read_particles
+
+  SUBROUTINE write_particles()
+    npart = topo_all%cuboid(rank)%npart
+    OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(20) npart
+    WRITE(20) part%pos, part%vort, part%vel
+    CLOSE(20)
+  END SUBROUTINE write_particles
+
+END PROGRAM ref_driver
