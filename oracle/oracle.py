@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libpmoracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+MPIEXEC = "/opt/conda/bin/mpiexec"
 
 
 class Geom(C.Structure):
@@ -231,7 +232,7 @@ def ref_run(job, inputs, nproc=1, timeout=600):
                 np.asarray(a).ravel(order="F").tofile(f)
         job = dict(job, infile=inf, outfile=outf)
         nml = "&job " + ", ".join("%s=%s" % (k, _fmt(v)) for k, v in job.items()) + " /\n"
-        cmd = [REF_DRIVER] if nproc == 1 else ["mpiexec", "-n", str(nproc), REF_DRIVER]
+        cmd = [REF_DRIVER] if nproc == 1 else [MPIEXEC, "-n", str(nproc), REF_DRIVER]
         r = subprocess.run(cmd, input=nml, text=True, capture_output=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError("ref_driver failed: %s %s" % (r.stdout[-2000:], r.stderr[-2000:]))

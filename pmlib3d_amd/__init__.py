@@ -1,0 +1,17 @@
+"""pmlib3d_amd: MI355X-native particle-mesh hot path with the pmlib3d interface.
+
+Python mirror of the reference's Fortran interface for the hot path
+(the types class_patch / class_mesh / class_particles and the subroutines
+pmlib_poisson_setup, pmlib_poisson_solve, pmlib_interp_particle_mesh,
+pmlib_interp_mesh_particle, pmlib_remesh, pmlib_particles_advance and the
+Brinkman penalisation step).  Every routine runs the HIP kernels of
+libpmgpu.so through its C ABI; subroutines return ierr (0 or -1) and write
+"[rank](caller) : message" on failure, as pmlib_write does.
+"""
+from .core import (Patch, Mesh, Particles, Parameters, pmlib_write, mesh_extent,
+                   pmlib_poisson_setup, pmlib_poisson_solve, PoissonSolver)
+from ._lib import PmgpuError, FP32, FP64
+
+__all__ = ["Patch", "Mesh", "Particles", "Parameters", "pmlib_write", "mesh_extent",
+           "pmlib_poisson_setup", "pmlib_poisson_solve", "PoissonSolver", "PmgpuError",
+           "FP32", "FP64"]

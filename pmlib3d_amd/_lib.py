@@ -1,0 +1,87 @@
+"""ctypes binding of libpmgpu.so (the C ABI declared in include/pmgpu.h).
+
+The library is built in place by ``make -f pmlib3d_amd/Makefile`` (or
+``__graft_entry__.build()``).  There is no fallback: if the shared library is
+missing or a call fails, an error is raised.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpmgpu.so")
+
+FP64 = 0
+FP32 = 1
+
+
+class PmgpuError(RuntimeError):
+    pass
+
+
+class Geom(C.Structure):
+    _fields_ = [("ncell", C.c_int32 * 3), ("nghost", C.c_int32), ("bound_cond", C.c_int32 * 3),
+                ("interp_order", C.c_int32), ("xmin", C.c_double * 3), ("xmax", C.c_double * 3),
+                ("dx", C.c_double * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [("regularisation_radius", C.c_double), ("poisson_order", C.c_int32),
+                ("poisson_kernel", C.c_int32)]
+
+
+# name -> (restype, argtypes); every exported symbol of include/pmgpu.h
+_P = C.c_void_p
+_I = C.c_int32
+SIGNATURES = {
+    "pmgpu_init": (C.c_int, [_I]),
+    "pmgpu_finalize": (C.c_int, []),
+    "pmgpu_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "pmgpu_mesh_extent": (C.c_int, [C.POINTER(Geom), C.POINTER(C.c_int32)]),
+    "pmgpu_poisson_create": (C.c_int, [C.POINTER(Geom), C.POINTER(Params), _I, _P, C.POINTER(_P)]),
+    "pmgpu_poisson_destroy": (C.c_int, [_P]),
+    "pmgpu_poisson_setup": (C.c_int, [_P]),
+    "pmgpu_poisson_solve": (C.c_int, [_P, _P, _P, _P, _P, _I, _I, _I]),
+    "pmgpu_poisson_solve_device": (C.c_int, [_P, _P, _P, _P, _P, _I, _I, _I]),
+    "pmgpu_poisson_kernel_octant": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int32)]),
+    "pmgpu_poisson_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "pmgpu_poisson_set_timing": (C.c_int, [_P, _I]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libpmgpu.so and bind every entry point (raises if missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PmgpuError("libpmgpu.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    load().pmgpu_last_error(buf, 1024)
+    return buf.value.decode()
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise PmgpuError("%s: %s" % (what, last_error()))
+    return rc
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    if not _initialised:
+        check(load().pmgpu_init(device), "pmgpu_init")
+        _initialised = True

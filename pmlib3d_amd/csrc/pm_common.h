@@ -1,0 +1,55 @@
+// Shared device/host helpers for the pmgpu HIP library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace pmgpu {
+
+template <typename R> struct cplx_of;
+template <> struct cplx_of<float> { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+template <typename R> using cplx_t = typename cplx_of<R>::type;
+
+template <typename V> __host__ __device__ __forceinline__ V cmake(decltype(V::x) a, decltype(V::x) b) {
+  V r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cmake<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cmake<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+  return cmake<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename V> __device__ __forceinline__ V cconj(V a) { return cmake<V>(a.x, -a.y); }
+template <typename V, typename R> __device__ __forceinline__ V cscale(V a, R s) { return cmake<V>(a.x * s, a.y * s); }
+// multiply by i*s with s = +1 or -1 (compile time)
+template <int S, typename V> __device__ __forceinline__ V cmul_i(V a) {
+  return S > 0 ? cmake<V>(-a.y, a.x) : cmake<V>(a.y, -a.x);
+}
+
+// Error state of the last failing call on this host thread.
+void set_error(const std::string &msg);
+const std::string &get_error();
+
+#define PM_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      ::pmgpu::set_error(std::string(#call " failed: ") + hipGetErrorString(e_));      \
+      return -1;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+#define PM_CHECK(cond, msg)          \
+  do {                               \
+    if (!(cond)) {                   \
+      ::pmgpu::set_error(msg);       \
+      return -1;                     \
+    }                                \
+  } while (0)
+
+}  // namespace pmgpu

@@ -1,0 +1,125 @@
+// Compile-time specialised FFT engine (hot lengths).
+//
+// Same Stockham autosort algorithm as pm_fft.h, but the length, the radix
+// sequence, the threads per line P (a power of two) and the butterflies per
+// thread are template constants, so every index is folded, every register
+// array is statically indexed (no scratch) and a line that fits in one
+// wavefront (P <= 64) synchronises with wave-level fences instead of block
+// barriers.  LDS indices are padded, i -> i + i/8 + i/64, which spreads the
+// stride-8 and stride-64 element writes of the first stages over the banks.
+// Twiddles: one table load per butterfly, higher powers by multiplication.
+#pragma once
+#include "pm_fft.h"
+
+namespace pmgpu {
+namespace fx {
+
+constexpr int radix_at(int L, int s) {
+  int r = L, st = 0;
+  const int rads[5] = {8, 4, 2, 3, 5};
+  for (int q = 0; q < 5; ++q) {
+    while (r % rads[q] == 0 && r > 1) {
+      if (st == s) return rads[q];
+      r /= rads[q];
+      ++st;
+    }
+  }
+  return 0;
+}
+constexpr int nstages(int L) {
+  int s = 0;
+  while (radix_at(L, s) != 0) ++s;
+  return s;
+}
+constexpr bool supported(int L) {
+  int p = 1;
+  for (int s = 0; s < nstages(L); ++s) p *= radix_at(L, s);
+  return p == L && L >= 8;
+}
+constexpr int ns_before(int L, int s) {
+  int p = 1;
+  for (int q = 0; q < s; ++q) p *= radix_at(L, q);
+  return p;
+}
+constexpr int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
+  return p;
+}
+template <int L> struct Geo {
+  static constexpr int P = next_pow2((L + 7) / 8);   // threads per line
+  static constexpr int PADDED = L + L / 8 + L / 64 + 1;
+  static constexpr int ST = PADDED | 1;               // odd line stride
+  static constexpr bool WAVE = P <= 64;
+};
+__device__ __forceinline__ int pad(int i) { return i + (i >> 3) + (i >> 6); }
+
+template <bool WAVE> __device__ __forceinline__ void line_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <int L, int S, int R, int NS, bool WAVE, typename V>
+__device__ __forceinline__ void stage(V *line, int p, const V *tw) {
+  constexpr int P = Geo<L>::P;
+  constexpr int NBF = L / R;
+  constexpr int MB = (NBF + P - 1) / P;
+  constexpr int TSTRIDE = L / (NS * R);
+  V v[MB][R];
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    const int j = p + m * P;
+    if (MB * P == NBF || j < NBF) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[m][r] = line[pad(j + r * NBF)];
+    }
+  }
+  line_sync<WAVE>();
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    const int j = p + m * P;
+    if (MB * P == NBF || j < NBF) {
+      const int k = NS == 1 ? 0 : j % NS;
+      if (NS > 1) {
+        V w1 = tw[k * TSTRIDE];
+        if (S > 0) w1.y = -w1.y;
+        V w = w1;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          v[m][r] = cmul(v[m][r], w);
+          if (r + 1 < R) w = cmul(w, w1);
+        }
+      }
+      DftAny<R, S, V>::run(v[m], tw, L);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[pad(base + r * NS)] = v[m][r];
+    }
+  }
+  line_sync<WAVE>();
+}
+
+template <int L, int S, int I, bool WAVE, typename V, bool DONE = (I >= nstages(L))> struct StageLoop;
+template <int L, int S, int I, bool WAVE, typename V> struct StageLoop<L, S, I, WAVE, V, true> {
+  static __device__ __forceinline__ void run(V *, int, const V *) {}
+};
+template <int L, int S, int I, bool WAVE, typename V> struct StageLoop<L, S, I, WAVE, V, false> {
+  static __device__ __forceinline__ void run(V *line, int p, const V *tw) {
+    stage<L, S, radix_at(L, I), ns_before(L, I), WAVE>(line, p, tw);
+    StageLoop<L, S, I + 1, WAVE, V>::run(line, p, tw);
+  }
+};
+
+// In-place transform of one padded line held in LDS (tw: length-L table).
+// The data must be visible to the line's threads (caller syncs before).
+template <int L, int S, typename V> __device__ __forceinline__ void fft(V *line, int p, const V *tw) {
+  StageLoop<L, S, 0, Geo<L>::WAVE, V>::run(line, p, tw);
+}
+
+}  // namespace fx
+}  // namespace pmgpu

@@ -1,0 +1,505 @@
+// Host side of the Poisson solve: plans, buffers, launches, C ABI.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/pmgpu.h"
+#include "poisson_kernels.h"
+
+namespace pmgpu {
+
+int device_lds_limit();  // capi.hip
+
+// length-specialised launchers (poisson_fixed.hip): 1 launched, 0 no
+// specialisation for this length, -1 launch error
+template <typename R>
+int fx_strided(int L, int S, StridedArgs a, int nb_y, int nb_z, const cplx_t<R> *tw, hipStream_t st);
+template <typename R> int fx_z(int L, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st);
+template <typename R> int fx_xfwd(int NH, int NC, XFwdArgs a, const cplx_t<R> *tw, hipStream_t st);
+template <typename R> int fx_xinv(int NH, XInvArgs a, const cplx_t<R> *tw, hipStream_t st);
+
+// Factor L into the supported radices, largest first.
+static bool make_plan(int L, FftPlan &pl) {
+  pl.L = L;
+  pl.P = (L + 7) / 8;
+  pl.nstage = 0;
+  int r = L;
+  const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
+  for (int rad : order) {
+    while (r % rad == 0 && r > 1) {
+      if (pl.nstage >= kMaxStages) return false;
+      pl.radix[pl.nstage++] = rad;
+      r /= rad;
+    }
+  }
+  return r == 1;
+}
+
+// Twiddle tables exp(-2 pi i n / L) for a set of lengths, in one buffer.
+template <typename R> struct TwTable {
+  std::map<int, int> off;
+  std::vector<cplx_t<R>> host;
+  int add(int L) {
+    auto it = off.find(L);
+    if (it != off.end()) return it->second;
+    int o = (int)host.size();
+    off[L] = o;
+    for (int n = 0; n < L; ++n) {
+      long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)n / (long double)L;
+      host.push_back(cmake<cplx_t<R>>((R)cosl(a), (R)sinl(a)));
+    }
+    return o;
+  }
+};
+
+struct PoissonBase {
+  virtual int setup() = 0;
+  virtual int solve_device(const void *vort, void *vel, void *vort_out, void *vecpot, int reg, int reproj,
+                           int psi) = 0;
+  virtual int solve_host(const double *vort, double *vel, double *vort_out, double *vecpot, int reg,
+                         int reproj, int psi) = 0;
+  virtual int octant(void *host, int64_t cap, int32_t *dims) = 0;
+  pmgpu_geom g;
+  pmgpu_params prm;
+  hipStream_t stream = nullptr;
+  // HIP-event timing of the fused z pass and of whole solves, recorded on
+  // the handle's stream without host synchronisation; read back on demand
+  bool timing = false;
+  std::vector<hipEvent_t> evs;  // 4 per recorded solve
+  int nrec = 0;
+  static constexpr int kMaxRec = 512;
+  int record(int slot) {
+    if (!timing) return 0;
+    if (nrec >= kMaxRec) return 0;
+    size_t i = (size_t)nrec * 4 + slot;
+    while (evs.size() <= i) {
+      hipEvent_t e;
+      PM_HIP(hipEventCreate(&e));
+      evs.push_back(e);
+    }
+    PM_HIP(hipEventRecord(evs[i], stream));
+    if (slot == 3) ++nrec;
+    return 0;
+  }
+  int averages(double *z_ms, double *tot_ms) {
+    *z_ms = -1;
+    *tot_ms = -1;
+    if (nrec == 0) return 0;
+    PM_HIP(hipEventSynchronize(evs[(size_t)nrec * 4 - 1]));
+    double za = 0, ta = 0;
+    for (int r = 0; r < nrec; ++r) {
+      float a = 0, b = 0;
+      PM_HIP(hipEventElapsedTime(&a, evs[r * 4 + 1], evs[r * 4 + 2]));
+      PM_HIP(hipEventElapsedTime(&b, evs[r * 4 + 0], evs[r * 4 + 3]));
+      za += a;
+      ta += b;
+    }
+    *z_ms = za / nrec;
+    *tot_ms = ta / nrec;
+    return 0;
+  }
+  virtual ~PoissonBase() {
+    for (auto e : evs) (void)hipEventDestroy(e);
+  }
+};
+
+template <typename R> struct PoissonGPU : PoissonBase {
+  using V = cplx_t<R>;
+  int M[3], L[3], E[3], lo[3], ngs[3];
+  int Nh, Hx, Hy, Hz;
+  bool allper = false;
+  FftPlan pNh, pLy, pLz;
+  int twLx_off = 0;
+  TwTable<R> tw;
+  V *d_tw = nullptr;
+  V *A = nullptr, *B = nullptr, *Bpsi = nullptr, *Bvort = nullptr;
+  R *Ghat = nullptr;
+  R *mesh_in = nullptr, *mesh_out = nullptr;  // staging for host calls
+  size_t nA = 0, nB = 0, nmesh = 0;
+  int Tx = 1, Ts = 1, Tz = 1;
+  int lds_limit = 65536;
+
+  ~PoissonGPU() override {
+    void *bufs[] = {d_tw, A, B, Bpsi, Bvort, Ghat, mesh_in, mesh_out};
+    for (void *b : bufs)
+      if (b) (void)hipFree(b);
+  }
+
+  int init(const pmgpu_geom &gg, const pmgpu_params &pp, hipStream_t s) {
+    g = gg;
+    prm = pp;
+    stream = s;
+    PM_CHECK(prm.poisson_kernel == 1, "pmgpu: only poisson_kernel = 1 is implemented on the GPU path");
+    int o = prm.poisson_order;
+    PM_CHECK(o == 0 || o == 2 || o == 4 || o == 6 || o == 8 || o == 10,
+             "pmgpu: poisson_order must be 0, 2, 4, 6, 8 or 10");
+    PM_CHECK(prm.poisson_order == 0 || prm.regularisation_radius >= 1.0,
+             "pmgpu: regularisation_radius must be >= 1 (parameters/pmlib_parameters_check.f:67)");
+    allper = g.bound_cond[0] == 1 && g.bound_cond[1] == 1 && g.bound_cond[2] == 1;
+    for (int d = 0; d < 3; ++d) {
+      PM_CHECK(g.ncell[d] > 0 && g.dx[d] > 0, "pmgpu: bad geometry");
+      int ng = g.bound_cond[d] == 0 ? g.nghost : (g.interp_order == 4 ? 3 : 2);
+      ngs[d] = ng;
+      E[d] = g.ncell[d] + 2 * ng;
+      if (g.bound_cond[d] == 0) {
+        M[d] = g.ncell[d] + 2 * g.nghost;
+        L[d] = 2 * M[d];
+        lo[d] = 0;
+      } else {
+        M[d] = g.ncell[d];
+        L[d] = M[d];
+        lo[d] = ng;
+        PM_CHECK(g.ncell[d] >= ng, "pmgpu: periodic axis shorter than its ghost layer");
+      }
+      PM_CHECK(L[d] % 2 == 0, "pmgpu: nfft is an odd number (fourier/pmlib_fourier_fft.f:126)");
+    }
+    Nh = L[0] / 2;
+    Hx = Nh + 1;
+    Hy = L[1] / 2 + 1;
+    Hz = L[2] / 2 + 1;
+    PM_CHECK(make_plan(Nh, pNh) && make_plan(L[1], pLy) && make_plan(L[2], pLz),
+             "pmgpu: FFT length with a prime factor > 13 is not supported");
+    pNh.tw_off = tw.add(Nh);
+    pLy.tw_off = tw.add(L[1]);
+    pLz.tw_off = tw.add(L[2]);
+    twLx_off = tw.add(L[0]);
+    PM_HIP(hipMalloc(&d_tw, tw.host.size() * sizeof(V)));
+    PM_HIP(hipMemcpy(d_tw, tw.host.data(), tw.host.size() * sizeof(V), hipMemcpyHostToDevice));
+    nA = (size_t)3 * Hx * M[1] * M[2];
+    nB = (size_t)3 * Hx * L[1] * M[2];
+    nmesh = (size_t)3 * E[0] * E[1] * E[2];
+    PM_HIP(hipMalloc(&A, nA * sizeof(V)));
+    PM_HIP(hipMalloc(&B, nB * sizeof(V)));
+    lds_limit = device_lds_limit();
+    // tile widths: 128-byte rows for the strided passes where LDS allows
+    const int tmax = sizeof(R) == 4 ? 16 : 8;
+    auto pick = [&](int len, int lines_per_unit, int tcap) {
+      int t = tcap;
+      while (t > 1 && ((size_t)t * lines_per_unit * lds_stride(len) * sizeof(V) > (size_t)lds_limit ||
+                       t * lines_per_unit * ((len + 7) / 8) > 1024))
+        t /= 2;
+      return t;
+    };
+    Ts = pick(L[1] > L[2] ? L[1] : L[2], 1, tmax);
+    Tz = pick(L[2], 1, tmax);
+    Tx = pick(Nh, 3, 8);
+    return 0;
+  }
+
+  size_t lds_bytes(int len, int lines) const { return (size_t)lines * lds_stride(len) * sizeof(V); }
+
+  // ---- generic launches -------------------------------------------------
+  template <typename RR>
+  int launch_xfwd(const void *in, void *out, int64_t cs, int64_t xs, int64_t ys, int64_t zs, int64_t o_cs,
+                  int64_t o_ks, int64_t o_ys, int64_t o_zs, int ncomp, int Min, int My, int Mz, int T,
+                  const FftPlan &pl, int twL, const cplx_t<RR> *twp) {
+    XFwdArgs a{in, out, cs, xs, ys, zs, o_cs, o_ks, o_ys, o_zs, ncomp, Min, My, Mz, pl.L + 1, T, pl, twL};
+    int f = fx_xfwd<RR>(pl.L, ncomp, a, twp, stream);
+    PM_CHECK(f >= 0, "pmgpu: x-pass launch failed");
+    if (f == 1) return 0;
+    dim3 grid((My + T - 1) / T, Mz);
+    size_t sh = (size_t)T * ncomp * lds_stride(pl.L) * sizeof(cplx_t<RR>);
+    hipLaunchKernelGGL(k_x_fwd<RR>, grid, dim3(pl.P * T * ncomp), sh, stream, a, twp);
+    PM_HIP(hipGetLastError());
+    return 0;
+  }
+  template <typename RR, int S>
+  int launch_strided(const void *in, void *out, int64_t rs_in, int64_t rs_out, int64_t bs_in, int64_t bs_out,
+                     int ncols, int nin, int nout, int nb_y, int nb_z, double scale, int T, const FftPlan &pl,
+                     const cplx_t<RR> *twp) {
+    StridedArgs a{in, out, rs_in, rs_out, bs_in, bs_out, ncols, nin, nout, T, scale, pl};
+    int f = fx_strided<RR>(pl.L, S, a, nb_y, nb_z, twp, stream);
+    PM_CHECK(f >= 0, "pmgpu: strided-pass launch failed");
+    if (f == 1) return 0;
+    dim3 grid((ncols + T - 1) / T, nb_y, nb_z);
+    size_t sh = (size_t)T * lds_stride(pl.L) * sizeof(cplx_t<RR>);
+    hipLaunchKernelGGL((k_strided<RR, S>), grid, dim3(pl.P * T), sh, stream, a, twp);
+    PM_HIP(hipGetLastError());
+    return 0;
+  }
+
+  // ---- pmlib_poisson_setup ---------------------------------------------
+  int setup() override {
+    if (allper) return 0;  // G = zeta / (4 pi^2 k^2) is formed in the z pass
+    // FP64 kernel grid and its transform (x R2C, y and z C2C), then the octant
+    double *Gr = nullptr;
+    double2 *Gc = nullptr;
+    size_t nG = (size_t)L[0] * L[1] * L[2];
+    size_t nGc = (size_t)Hx * L[1] * L[2];
+    PM_HIP(hipMalloc(&Gr, nG * sizeof(double)));
+    PM_HIP(hipMalloc(&Gc, nGc * sizeof(double2)));
+    GreenArgs ga;
+    ga.out = Gr;
+    double dmax = g.dx[0] > g.dx[1] ? g.dx[0] : g.dx[1];
+    dmax = dmax > g.dx[2] ? dmax : g.dx[2];
+    ga.eps = prm.regularisation_radius * dmax;
+    ga.order = prm.poisson_order;
+    for (int d = 0; d < 3; ++d) {
+      ga.L[d] = L[d];
+      ga.bc[d] = g.bound_cond[d];
+      ga.dx[d] = g.dx[d];
+      ga.xk[d] = -(0.5 * (double)L[d] + 0.5) * g.dx[d];
+      ga.len[d] = g.dx[d] * (double)L[d];
+    }
+    hipLaunchKernelGGL(k_green, dim3(4096), dim3(256), 0, stream, ga);
+    PM_HIP(hipGetLastError());
+    TwTable<double> t64;
+    FftPlan q0, q1, q2;
+    make_plan(Nh, q0);
+    make_plan(L[1], q1);
+    make_plan(L[2], q2);
+    q0.tw_off = t64.add(Nh);
+    q1.tw_off = t64.add(L[1]);
+    q2.tw_off = t64.add(L[2]);
+    int tl = t64.add(L[0]);
+    double2 *dt = nullptr;
+    PM_HIP(hipMalloc(&dt, t64.host.size() * sizeof(double2)));
+    PM_HIP(hipMemcpyAsync(dt, t64.host.data(), t64.host.size() * sizeof(double2), hipMemcpyHostToDevice, stream));
+    auto pick64 = [&](int len, int lines_per_unit, int tcap) {
+      int t = tcap;
+      while (t > 1 && ((size_t)t * lines_per_unit * lds_stride(len) * sizeof(double2) > (size_t)lds_limit ||
+                       t * lines_per_unit * ((len + 7) / 8) > 1024))
+        t /= 2;
+      return t;
+    };
+    int tx = pick64(Nh, 1, 8), ty = pick64(L[1], 1, 8), tz = pick64(L[2], 1, 8);
+    int e = 0;
+    e |= launch_xfwd<double>(Gr, Gc, 0, 1, L[0], (int64_t)L[0] * L[1], 0, 1, Hx, (int64_t)Hx * L[1], 1, L[0],
+                             L[1], L[2], tx, q0, tl, dt);
+    e |= launch_strided<double, -1>(Gc, Gc, Hx, Hx, (int64_t)L[1] * Hx, (int64_t)L[1] * Hx, Hx, L[1], L[1], L[2],
+                                    1, 1.0, ty, q1, dt);
+    e |= launch_strided<double, -1>(Gc, Gc, (int64_t)L[1] * Hx, (int64_t)L[1] * Hx, Hx, Hx, Hx, L[2], L[2], L[1],
+                                    1, 1.0, tz, q2, dt);
+    if (e) return -1;
+    if (!Ghat) PM_HIP(hipMalloc(&Ghat, (size_t)Hx * Hy * Hz * sizeof(R)));
+    hipLaunchKernelGGL(k_g_octant<R>, dim3(2048), dim3(256), 0, stream, Gc, Ghat, Hx, L[1], Hy, Hz);
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(Gr);
+    (void)hipFree(Gc);
+    (void)hipFree(dt);
+    return 0;
+  }
+
+  int octant(void *host, int64_t cap, int32_t *dims) override {
+    dims[0] = Hx;
+    dims[1] = Hy;
+    dims[2] = Hz;
+    size_t n = (size_t)Hx * Hy * Hz * sizeof(R);
+    PM_CHECK(Ghat != nullptr, "pmgpu: kernel not set up");
+    PM_CHECK((int64_t)n <= cap, "pmgpu: buffer too small for the kernel octant");
+    PM_HIP(hipMemcpy(host, Ghat, n, hipMemcpyDeviceToHost));
+    return 0;
+  }
+
+  // ---- pmlib_poisson_solve ---------------------------------------------
+  int solve_device(const void *vort, void *vel, void *vort_out, void *vecpot, int reg, int reproj,
+                   int psi) override {
+    const bool want_vort = reg || reproj;
+    const bool want_psi = psi && prm.poisson_kernel == 1;
+    PM_CHECK(allper || Ghat != nullptr, "pmgpu: pmgpu_poisson_setup was not called");
+    PM_CHECK(vort && vel, "pmgpu: null mesh pointer");
+    PM_CHECK(!want_vort || vort_out, "pmgpu: vort_out required with reg_vort/reproj");
+    PM_CHECK(!want_psi || vecpot, "pmgpu: vecpot required with calc_vecpot");
+    if (want_psi && !Bpsi) PM_HIP(hipMalloc(&Bpsi, nB * sizeof(V)));
+    if (want_vort && !Bvort) PM_HIP(hipMalloc(&Bvort, nB * sizeof(V)));
+    if (record(0)) return -1;
+    const int64_t off = 3 * ((int64_t)lo[0] + (int64_t)E[0] * (lo[1] + (int64_t)E[1] * lo[2]));
+    const int64_t cs = 1, xs = 3, ys = 3 * (int64_t)E[0], zs = 3 * (int64_t)E[0] * E[1];
+    const R *vin = reinterpret_cast<const R *>(vort) + off;
+    int e = 0;
+    // component-interleaved spectra: A[((z*My + y)*Hx + kx)*3 + c],
+    // B[((z*Ly + ky)*Hx + kx)*3 + c]
+    const int64_t H3 = 3 * (int64_t)Hx;
+    e |= launch_xfwd<R>(vin, A, cs, xs, ys, zs, 1, 3, H3, H3 * M[1], 3, M[0], M[1], M[2], Tx, pNh, twLx_off, d_tw);
+    e |= launch_strided<R, -1>(A, B, H3, H3, H3 * M[1], H3 * L[1], (int)H3, M[1], L[1], M[2], 1, 1.0, Ts, pLy,
+                               d_tw);
+    if (e) return -1;
+    ZArgs za;
+    za.B = B;
+    za.Bpsi = Bpsi;
+    za.Bvort = Bvort;
+    za.Ghat = Ghat;
+    za.rs = H3 * L[1];
+    za.cs = 1;
+    za.kxs = 3;
+    za.kys = H3;
+    za.Hx = Hx;
+    za.Ly = L[1];
+    za.Lz = L[2];
+    za.Lx = L[0];
+    za.Mz = M[2];
+    za.T = Tz;
+    za.Hy = Hy;
+    za.Hz = Hz;
+    za.order = prm.poisson_order;
+    za.allper = allper;
+    za.want_psi = want_psi;
+    za.want_vort = want_vort;
+    za.reproj = reproj;
+    double dmax = g.dx[0] > g.dx[1] ? g.dx[0] : g.dx[1];
+    dmax = dmax > g.dx[2] ? dmax : g.dx[2];
+    for (int d = 0; d < 3; ++d) {
+      za.dk[d] = 1.0 / ((double)L[d] * g.dx[d]);
+      za.ksmp[d] = 1.0 / (2.0 * g.dx[d]);
+    }
+    za.h3 = g.dx[0] * g.dx[1] * g.dx[2];
+    za.eps2pi = prm.regularisation_radius * dmax * 2.0 * 3.14159265358979323846;
+    za.plan = pLz;
+    if (record(1)) return -1;
+    int fz = fx_z<R>(L[2], za, L[1], d_tw, stream);
+    PM_CHECK(fz >= 0, "pmgpu: z-pass launch failed");
+    if (fz == 0) {
+      dim3 zg((Hx + Tz - 1) / Tz, L[1]);
+      size_t zsh = lds_bytes(L[2], Tz);
+      int Q = (L[2] + pLz.P - 1) / pLz.P;
+      PM_CHECK(Q <= 8, "pmgpu: internal z-pass layout error");
+      hipLaunchKernelGGL((k_z_convolve<R, 8>), zg, dim3(pLz.P * Tz), zsh, stream, za, d_tw);
+      PM_HIP(hipGetLastError());
+    }
+    if (record(2)) return -1;
+    V *fields[3] = {B, want_psi ? Bpsi : nullptr, want_vort ? Bvort : nullptr};
+    void *outs[3] = {vel, vecpot, vort_out};
+    for (int f = 0; f < 3; ++f) {
+      if (!fields[f]) continue;
+      e |= launch_strided<R, 1>(fields[f], A, H3, H3, H3 * L[1], H3 * M[1], (int)H3, L[1], M[1], M[2], 1,
+                                1.0 / (double)L[1], Ts, pLy, d_tw);
+      XInvArgs xa{A,  reinterpret_cast<R *>(outs[f]) + off, cs, xs, ys, zs, 1, 3, H3, H3 * M[1], 3, M[0], M[1], M[2],
+                  Hx, Tx, 1.0 / (double)L[0], pNh, twLx_off};
+      int fxi = fx_xinv<R>(Nh, xa, d_tw, stream);
+      PM_CHECK(fxi >= 0, "pmgpu: x inverse launch failed");
+      if (fxi == 0) {
+        dim3 grid((M[1] + Tx - 1) / Tx, M[2]);
+        hipLaunchKernelGGL(k_x_inv<R>, grid, dim3(pNh.P * Tx * 3), lds_bytes(Nh, 3 * Tx), stream, xa, d_tw);
+        PM_HIP(hipGetLastError());
+      }
+      for (int d = 0; d < 3; ++d)
+        if (g.bound_cond[d] == 1) {
+          hipLaunchKernelGGL(k_ghost_fill<R>, dim3(1024), dim3(256), 0, stream, reinterpret_cast<R *>(outs[f]),
+                             E[0], E[1], E[2], d, g.ncell[d], ngs[d]);
+          PM_HIP(hipGetLastError());
+        }
+    }
+    if (e) return -1;
+    if (record(3)) return -1;
+    return 0;
+  }
+
+  int solve_host(const double *vort, double *vel, double *vort_out, double *vecpot, int reg, int reproj,
+                 int psi) override {
+    const bool want_vort = reg || reproj;
+    const bool want_psi = psi != 0;
+    if (!mesh_in) PM_HIP(hipMalloc(&mesh_in, nmesh * sizeof(R)));
+    size_t nout = 1 + (want_vort ? 1 : 0) + (want_psi ? 1 : 0);
+    if (!mesh_out) PM_HIP(hipMalloc(&mesh_out, 3 * nmesh * sizeof(R)));
+    std::vector<R> stage(nmesh);
+    const R *src;
+    if (sizeof(R) == sizeof(double)) {
+      src = reinterpret_cast<const R *>(vort);
+    } else {
+      for (size_t i = 0; i < nmesh; ++i) stage[i] = (R)vort[i];
+      src = stage.data();
+    }
+    PM_HIP(hipMemcpyAsync(mesh_in, src, nmesh * sizeof(R), hipMemcpyHostToDevice, stream));
+    // cells outside the solve domain (periodic ghosts) are refilled by the solve;
+    // zero the outputs first so untouched cells are defined
+    PM_HIP(hipMemsetAsync(mesh_out, 0, 3 * nmesh * sizeof(R), stream));
+    R *o_vel = mesh_out, *o_psi = mesh_out + nmesh, *o_vort = mesh_out + 2 * nmesh;
+    if (solve_device(mesh_in, o_vel, want_vort ? o_vort : nullptr, want_psi ? o_psi : nullptr, reg, reproj, psi))
+      return -1;
+    PM_HIP(hipStreamSynchronize(stream));
+    (void)nout;
+    double *dst[3] = {vel, want_psi ? vecpot : nullptr, want_vort ? vort_out : nullptr};
+    R *from[3] = {o_vel, o_psi, o_vort};
+    for (int f = 0; f < 3; ++f) {
+      if (!dst[f]) continue;
+      if (sizeof(R) == sizeof(double)) {
+        PM_HIP(hipMemcpy(dst[f], from[f], nmesh * sizeof(R), hipMemcpyDeviceToHost));
+      } else {
+        PM_HIP(hipMemcpy(stage.data(), from[f], nmesh * sizeof(R), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nmesh; ++i) dst[f][i] = (double)stage[i];
+      }
+    }
+    return 0;
+  }
+};
+
+}  // namespace pmgpu
+
+using namespace pmgpu;
+
+struct pmgpu_poisson_s {
+  std::unique_ptr<PoissonBase> impl;
+};
+
+extern "C" {
+
+int pmgpu_poisson_create(const pmgpu_geom *g, const pmgpu_params *p, int32_t precision, void *stream,
+                         pmgpu_poisson_t *out) {
+  PM_CHECK(g && p && out, "pmgpu_poisson_create: null argument");
+  PM_CHECK(precision == PMGPU_FP64 || precision == PMGPU_FP32, "pmgpu_poisson_create: bad precision");
+  auto h = new pmgpu_poisson_s;
+  int e;
+  if (precision == PMGPU_FP64) {
+    auto s = new PoissonGPU<double>;
+    h->impl.reset(s);
+    e = s->init(*g, *p, (hipStream_t)stream);
+  } else {
+    auto s = new PoissonGPU<float>;
+    h->impl.reset(s);
+    e = s->init(*g, *p, (hipStream_t)stream);
+  }
+  if (e) {
+    delete h;
+    return -1;
+  }
+  *out = h;
+  return 0;
+}
+
+int pmgpu_poisson_destroy(pmgpu_poisson_t h) {
+  delete h;
+  return 0;
+}
+
+int pmgpu_poisson_setup(pmgpu_poisson_t h) {
+  PM_CHECK(h, "pmgpu_poisson_setup: null handle");
+  return h->impl->setup();
+}
+
+int pmgpu_poisson_solve(pmgpu_poisson_t h, const double *vort, double *vel, double *vort_out, double *vecpot,
+                        int32_t reg_vort, int32_t reproj, int32_t calc_vecpot) {
+  PM_CHECK(h && vort && vel, "pmgpu_poisson_solve: null argument");
+  return h->impl->solve_host(vort, vel, vort_out, vecpot, reg_vort, reproj, calc_vecpot);
+}
+
+int pmgpu_poisson_solve_device(pmgpu_poisson_t h, const void *d_vort, void *d_vel, void *d_vort_out,
+                               void *d_vecpot, int32_t reg_vort, int32_t reproj, int32_t calc_vecpot) {
+  PM_CHECK(h, "pmgpu_poisson_solve_device: null handle");
+  return h->impl->solve_device(d_vort, d_vel, d_vort_out, d_vecpot, reg_vort, reproj, calc_vecpot);
+}
+
+int pmgpu_poisson_kernel_octant(pmgpu_poisson_t h, void *host, int64_t cap_bytes, int32_t dims[3]) {
+  PM_CHECK(h && host && dims, "pmgpu_poisson_kernel_octant: null argument");
+  return h->impl->octant(host, cap_bytes, dims);
+}
+
+int pmgpu_poisson_last_timing(pmgpu_poisson_t h, double *zpass_ms, double *total_ms) {
+  PM_CHECK(h, "pmgpu_poisson_last_timing: null handle");
+  double a, b;
+  if (h->impl->averages(&a, &b)) return -1;
+  if (zpass_ms) *zpass_ms = a;
+  if (total_ms) *total_ms = b;
+  return 0;
+}
+
+int pmgpu_poisson_set_timing(pmgpu_poisson_t h, int32_t enable) {
+  PM_CHECK(h, "pmgpu_poisson_set_timing: null handle");
+  h->impl->timing = enable != 0;
+  h->impl->nrec = 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,266 @@
+// Poisson solve kernels specialised on the transform length (see
+// poisson_kernels.h for the data path; argument structs are shared).
+#pragma once
+#include "pm_fft_fixed.h"
+#include "poisson_kernels.h"
+
+namespace pmgpu {
+namespace fx {
+
+// ---------------------------------------------------------------- strided
+template <typename R, int L, int T, int S>
+__global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const cplx_t<R> *tw) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<L>::P, ST = Geo<L>::ST, NT = P * T;
+  __shared__ V lds[T * ST];
+  const int col0 = blockIdx.x * T;
+  const int64_t b = blockIdx.y + (int64_t)blockIdx.z * gridDim.y;
+  const V *in = reinterpret_cast<const V *>(a.in) + b * a.bs_in + col0;
+  V *out = reinterpret_cast<V *>(a.out) + b * a.bs_out + col0;
+  const int tid = threadIdx.x;
+  const int c_ld = tid % T;
+  const bool col_ok = col0 + c_ld < a.ncols;
+  for (int row = tid / T; row < L; row += NT / T) {
+    V v = cmake<V>(0, 0);
+    if (row < a.nin && col_ok) v = in[(int64_t)row * a.rs_in + c_ld];
+    lds[c_ld * ST + pad(row)] = v;
+  }
+  __syncthreads();
+  const int line = tid / P, p = tid % P;
+  fft<L, S>(lds + line * ST, p, tw + a.plan.tw_off);
+  __syncthreads();
+  const R sc = (R)a.scale;
+  if (col_ok)
+    for (int row = tid / T; row < a.nout; row += NT / T)
+      out[(int64_t)row * a.rs_out + c_ld] = cscale(lds[c_ld * ST + pad(row)], sc);
+}
+
+// -------------------------------------------------------------- x forward
+// Lines of ncomp (= 3 or 1) components; NH = half transform length.
+template <typename R, int NH, int T, int NC>
+__global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const cplx_t<R> *tw) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<NH>::P, ST = Geo<NH>::ST, NL = T * NC, NT = P * NL;
+  __shared__ V lds[NL * ST];
+  const int y0 = blockIdx.x * T, z = blockIdx.y;
+  const R *in = reinterpret_cast<const R *>(a.in);
+  const int tid = threadIdx.x;
+  // packed pairs x[2m] + i x[2m+1]; m >= ceil(Min/2) are zero
+  const int half = (a.Min + 1) / 2;
+  for (int idx = tid; idx < NL * NH; idx += NT) {
+    int l = idx / NH, m = idx - l * NH;
+    int t = l / NC, c = l - t * NC;
+    int y = y0 + t;
+    V v = cmake<V>(0, 0);
+    if (m < half && y < a.My) {
+      const R *src = in + (int64_t)c * a.cs + (int64_t)y * a.ys + (int64_t)z * a.zs;
+      int x0 = 2 * m;
+      v.x = src[(int64_t)x0 * a.xs];
+      v.y = x0 + 1 < a.Min ? src[(int64_t)(x0 + 1) * a.xs] : (R)0;
+    }
+    lds[l * ST + pad(m)] = v;
+  }
+  __syncthreads();
+  const int line = tid / P, p = tid % P;
+  fft<NH, -1>(lds + line * ST, p, tw + a.plan.tw_off);
+  __syncthreads();
+  V *out = reinterpret_cast<V *>(a.out);
+  const V *twL = tw + a.twL_off;
+  constexpr int HX = NH + 1;
+  // c fastest: with the interleaved spectral layout consecutive threads
+  // write consecutive complex words
+  for (int idx = tid; idx < T * HX * NC; idx += NT) {
+    int c = idx % NC, k = (idx / NC) % HX, t = idx / (NC * HX);
+    int y = y0 + t;
+    if (y >= a.My) continue;
+    const V *zl = lds + (t * NC + c) * ST;
+    V zk = zl[pad(k == NH ? 0 : k)];
+    V zm = cconj(zl[pad(k == 0 ? 0 : NH - k)]);
+    V e = cscale(cadd(zk, zm), (R)0.5);
+    V o = cscale(csub(zk, zm), (R)0.5);
+    V wo = cmul(twL[k], o);
+    out[c * a.o_cs + k * a.o_ks + (int64_t)y * a.o_ys + (int64_t)z * a.o_zs] = cadd(e, cmake<V>(wo.y, -wo.x));
+  }
+}
+
+// -------------------------------------------------------------- x inverse
+template <typename R, int NH, int T, int NC>
+__global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const cplx_t<R> *tw) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<NH>::P, ST = Geo<NH>::ST, NL = T * NC, NT = P * NL;
+  constexpr int HX = NH + 1;
+  __shared__ V lds[NL * ST];
+  const int y0 = blockIdx.x * T, z = blockIdx.y;
+  const V *in = reinterpret_cast<const V *>(a.in);
+  const V *twL = tw + a.twL_off;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < NL * NH; idx += NT) {
+    int c = idx % NC, k = (idx / NC) % NH, t = idx / (NC * NH);
+    int l = t * NC + c;
+    int y = y0 + t;
+    V Zk = cmake<V>(0, 0);
+    if (y < a.My) {
+      const V *xl = in + c * a.i_cs + (int64_t)y * a.i_ys + (int64_t)z * a.i_zs;
+      V xk = xl[k * a.i_ks], xm = xl[(NH - k) * a.i_ks];
+      if (k == 0) {
+        xk.y = 0;
+        xm.y = 0;
+      }
+      V s = cadd(xk, cconj(xm));
+      V d = csub(xk, cconj(xm));
+      V wd = cmul(cconj(twL[k]), d);
+      Zk = cadd(s, cmake<V>(-wd.y, wd.x));
+    }
+    lds[l * ST + pad(k)] = Zk;
+  }
+  __syncthreads();
+  const int line = tid / P, p = tid % P;
+  fft<NH, 1>(lds + line * ST, p, tw + a.plan.tw_off);
+  __syncthreads();
+  R *out = reinterpret_cast<R *>(a.out);
+  const R sc = (R)a.scale;
+  const int halfo = (a.Mout + 1) / 2;
+  for (int idx = tid; idx < NL * halfo; idx += NT) {
+    int l = idx / halfo, m = idx - l * halfo;
+    int t = l / NC, c = l - t * NC;
+    int y = y0 + t;
+    if (y >= a.My) continue;
+    V v = lds[l * ST + pad(m)];
+    R *dst = out + (int64_t)c * a.cs + (int64_t)y * a.ys + (int64_t)z * a.zs;
+    int x0 = 2 * m;
+    dst[(int64_t)x0 * a.xs] = v.x * sc;
+    if (x0 + 1 < a.Mout) dst[(int64_t)(x0 + 1) * a.xs] = v.y * sc;
+  }
+}
+
+// ------------------------------------------------------------ fused z pass
+// Component-interleaved spectra B[((z*Ly + ky)*Hx + kx)*3 + c]: a block owns
+// TC consecutive kx columns of one ky row, i.e. 3*TC contiguous complex words
+// per z row, and transforms the 3*TC component lines together.  The curl at
+// (kx, ky, kz) reads the other two components from LDS, so no spectrum has to
+// stay in registers across an FFT (only the thread's own forward values when
+// psi or the filtered vorticity are also wanted).
+template <typename R, int L, int TC, bool EXTRA>
+__global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx_t<R> *tw_all) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<L>::P, ST = Geo<L>::ST;
+  constexpr int NLN = 3 * TC, NT = P * NLN;
+  constexpr int E = (L + P - 1) / P;
+  constexpr bool FULL = E * P == L;
+  __shared__ V lds[NLN * ST];
+  const V *tw = tw_all + a.plan.tw_off;
+  const int kx0 = blockIdx.x * TC, ky = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int line = tid / P, p = tid % P;
+  const int col = line / 3, comp = line - 3 * col;
+  const int kx = kx0 + col;
+  const bool col_ok = kx < a.Hx;
+  const int ncl = (a.Hx - kx0 < TC ? a.Hx - kx0 : TC) * 3;  // valid lines in this block
+  V *B = reinterpret_cast<V *>(a.B);
+  const int64_t tile = (int64_t)ky * a.kys + (int64_t)kx0 * 3;
+  V *myl = lds + line * ST;
+  // load the 3*TC interleaved lines, zero rows >= Mz
+  for (int idx = tid; idx < L * NLN; idx += NT) {
+    int row = idx / NLN, cl = idx - row * NLN;
+    V v = cmake<V>(0, 0);
+    if (row < a.Mz && cl < ncl) v = B[tile + (int64_t)row * a.rs + cl];
+    lds[cl * ST + pad(row)] = v;
+  }
+  __syncthreads();
+  fft<L, -1>(myl, p, tw);
+  __syncthreads();
+  const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
+  const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
+  const bool nx = (2 * kx == a.Lx), ny = (2 * ky == a.Ly);
+  const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
+  const R pi = (R)3.14159265358979323846;
+  const R twopi = (R)6.28318530717958647692;
+  const R *G = reinterpret_cast<const R *>(a.Ghat);
+  const R scale = (R)1 / (R)L;
+  // EXTRA = psi and/or filtered vorticity requested (else velocity only)
+  constexpr bool extra = EXTRA;
+  const V *cl3 = lds + (3 * col) * ST;  // the three component lines of my column
+  V own[EXTRA ? E : 1];                // my component's forward spectrum (extra outputs)
+  const int nkind = EXTRA ? 1 + (a.want_psi ? 1 : 0) + (a.want_vort ? 1 : 0) : 1;
+  for (int kd = 0; kd < nkind; ++kd) {
+    const int kind = kd == 0 ? 0 : ((kd == 1 && a.want_psi) ? 1 : 2);
+    if (EXTRA && kd > 0 && kind == 2) {
+      // the filtered vorticity needs all three forward spectra again
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int n = p + e * P;
+        if (FULL || n < L) myl[pad(n)] = own[EXTRA ? e : 0];
+      }
+      __syncthreads();
+    }
+    V r[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int n = p + e * P;
+      r[e] = cmake<V>(0, 0);
+      if (!(FULL || n < L)) continue;
+      const R kk = wavenumber<R>(n, L, a.dk[2], a.ksmp[2]);
+      const bool nz = (2 * n == L);
+      R g;
+      R zeta = (R)1;
+      if (kind == 2 || a.allper)
+        if (a.want_vort) zeta = zeta_filter<R>(a.order, (R)a.eps2pi * sqrt(ki * ki + kj * kj + kk * kk));
+      if (a.allper) {
+        const R kmag = ki * ki + kj * kj + kk * kk;
+        g = kmag > (R)0 ? zeta / ((R)4 * (pi * pi) * kmag) : (R)0;
+      } else {
+        const int kzf = n <= L - n ? n : L - n;
+        g = col_ok ? G[((int64_t)kyf * a.Hx + kx) * a.Hz + kzf] * (R)a.h3 : (R)0;
+      }
+      if (EXTRA && kind == 1) {
+        r[e] = cscale(own[EXTRA ? e : 0], g);
+        continue;
+      }
+      const V w0 = cl3[pad(n)], w1 = cl3[ST + pad(n)], w2 = cl3[2 * ST + pad(n)];
+      const V wc = comp == 0 ? w0 : (comp == 1 ? w1 : w2);
+      if (EXTRA && kd == 0) own[EXTRA ? e : 0] = wc;
+      if (!EXTRA || kind == 0) {
+        // u-hat = G 2 pi i (k x w-hat); Nyquist wavenumbers drop out of the
+        // real part of the inverse (Hermitian projection)
+        const R kx_ = nx ? (R)0 : ki, ky_ = ny ? (R)0 : kj, kz_ = nz ? (R)0 : kk;
+        V cr;
+        if (comp == 0) cr = csub(cscale(w2, ky_), cscale(w1, kz_));
+        else if (comp == 1) cr = csub(cscale(w0, kz_), cscale(w2, kx_));
+        else cr = csub(cscale(w1, kx_), cscale(w0, ky_));
+        const R f = g * twopi;
+        r[e] = cmake<V>(-cr.y * f, cr.x * f);
+      } else {
+        V q = cscale(wc, zeta);
+        if (a.reproj) {
+          // -4 pi^2 G k_c sum_d k_d w_d over pairs of equal Nyquist parity
+          const bool nc = comp == 0 ? nx : (comp == 1 ? ny : nz);
+          const R kc = comp == 0 ? ki : (comp == 1 ? kj : kk);
+          V acc = cmake<V>(0, 0);
+          if (nx == nc) acc = cadd(acc, cscale(w0, ki));
+          if (ny == nc) acc = cadd(acc, cscale(w1, kj));
+          if (nz == nc) acc = cadd(acc, cscale(w2, kk));
+          q = cadd(q, cscale(acc, -(R)4 * (pi * pi) * g * kc));
+        }
+        r[e] = q;
+      }
+    }
+    __syncthreads();  // every read of the forward spectra is done
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int n = p + e * P;
+      if (FULL || n < L) myl[pad(n)] = r[e];
+    }
+    line_sync<Geo<L>::WAVE>();
+    fft<L, 1>(myl, p, tw);
+    __syncthreads();
+    V *dst = (kind == 0 ? B : (kind == 1 ? reinterpret_cast<V *>(a.Bpsi) : reinterpret_cast<V *>(a.Bvort))) + tile;
+    for (int idx = tid; idx < a.Mz * NLN; idx += NT) {
+      int row = idx / NLN, cl = idx - row * NLN;
+      if (cl < ncl) dst[(int64_t)row * a.rs + cl] = cscale(lds[cl * ST + pad(row)], scale);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace fx
+}  // namespace pmgpu

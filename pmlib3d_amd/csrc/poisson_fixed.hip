@@ -1,0 +1,135 @@
+// Launchers of the length-specialised Poisson kernels.  Each returns 1 when
+// it launched, 0 when the length has no specialisation (the caller then uses
+// the runtime-planned kernels of poisson_kernels.h), -1 on a launch error.
+#include "poisson_fixed.h"
+
+namespace pmgpu {
+namespace fx {
+
+constexpr int kLdsBudget = 80 * 1024;  // two workgroups per CU (160 KB LDS)
+
+template <typename R, int L> constexpr int tile_cols() {
+  constexpr int P = Geo<L>::P;
+  constexpr int want = sizeof(R) == 4 ? 16 : 8;
+  int t = want;
+  while (t > 1 && (P * t > 1024 || t * Geo<L>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget)) t /= 2;
+  return t;
+}
+template <typename R, int NH, int NC> constexpr int tile_lines() {
+  constexpr int P = Geo<NH>::P;
+  int t = sizeof(R) == 4 ? 8 : 4;
+  while (t > 1 && (P * t * NC > 1024 || t * NC * Geo<NH>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget)) t /= 2;
+  return t;
+}
+
+template <typename R, int L>
+int strided_L(int S, StridedArgs a, int nb_y, int nb_z, const cplx_t<R> *tw, hipStream_t st) {
+  constexpr int T = tile_cols<R, L>();
+  a.T = T;
+  dim3 grid((a.ncols + T - 1) / T, nb_y, nb_z);
+  if (S < 0)
+    hipLaunchKernelGGL((k_strided_f<R, L, T, -1>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+  else
+    hipLaunchKernelGGL((k_strided_f<R, L, T, 1>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
+// columns per block of the interleaved z pass (3 component lines each)
+template <typename R, int L, bool EXTRA> constexpr int z_cols() {
+  constexpr int P = Geo<L>::P;
+  int t = sizeof(R) == 4 ? (EXTRA ? 2 : 5) : (EXTRA ? 1 : 2);
+  while (t > 1 && (P * 3 * t > 1024 || 3 * t * Geo<L>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget)) --t;
+  return t;
+}
+
+template <typename R, int L> int z_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  if (a.want_psi || a.want_vort) {
+    constexpr int TC = z_cols<R, L, true>();
+    a.T = TC;
+    dim3 grid((a.Hx + TC - 1) / TC, Ly);
+    hipLaunchKernelGGL((k_z3_f<R, L, TC, true>), grid, dim3(Geo<L>::P * 3 * TC), 0, st, a, tw);
+  } else {
+    constexpr int TC = z_cols<R, L, false>();
+    a.T = TC;
+    dim3 grid((a.Hx + TC - 1) / TC, Ly);
+    hipLaunchKernelGGL((k_z3_f<R, L, TC, false>), grid, dim3(Geo<L>::P * 3 * TC), 0, st, a, tw);
+  }
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
+template <typename R, int NH> int xfwd_L(int NC, XFwdArgs a, const cplx_t<R> *tw, hipStream_t st) {
+  if (NC == 3) {
+    constexpr int T = tile_lines<R, NH, 3>();
+    a.lines_per_block = T;
+    dim3 grid((a.My + T - 1) / T, a.Mz);
+    hipLaunchKernelGGL((k_x_fwd_f<R, NH, T, 3>), grid, dim3(Geo<NH>::P * T * 3), 0, st, a, tw);
+  } else {
+    constexpr int T = tile_lines<R, NH, 1>();
+    a.lines_per_block = T;
+    dim3 grid((a.My + T - 1) / T, a.Mz);
+    hipLaunchKernelGGL((k_x_fwd_f<R, NH, T, 1>), grid, dim3(Geo<NH>::P * T), 0, st, a, tw);
+  }
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
+template <typename R, int NH> int xinv_L(XInvArgs a, const cplx_t<R> *tw, hipStream_t st) {
+  constexpr int T = tile_lines<R, NH, 3>();
+  a.lines_per_block = T;
+  dim3 grid((a.My + T - 1) / T, a.Mz);
+  hipLaunchKernelGGL((k_x_inv_f<R, NH, T, 3>), grid, dim3(Geo<NH>::P * T * 3), 0, st, a, tw);
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
+#define PM_FX_LENGTHS(X) \
+  X(16) X(32) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024)
+
+}  // namespace fx
+
+template <typename R>
+int fx_strided(int L, int S, StridedArgs a, int nb_y, int nb_z, const cplx_t<R> *tw, hipStream_t st) {
+  switch (L) {
+#define X(n) \
+  case n: return fx::strided_L<R, n>(S, a, nb_y, nb_z, tw, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+template <typename R> int fx_z(int L, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  switch (L) {
+#define X(n) \
+  case n: return fx::z_L<R, n>(a, Ly, tw, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+template <typename R> int fx_xfwd(int NH, int NC, XFwdArgs a, const cplx_t<R> *tw, hipStream_t st) {
+  switch (NH) {
+#define X(n) \
+  case n: return fx::xfwd_L<R, n>(NC, a, tw, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+template <typename R> int fx_xinv(int NH, XInvArgs a, const cplx_t<R> *tw, hipStream_t st) {
+  switch (NH) {
+#define X(n) \
+  case n: return fx::xinv_L<R, n>(a, tw, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+
+template int fx_strided<float>(int, int, StridedArgs, int, int, const float2 *, hipStream_t);
+template int fx_strided<double>(int, int, StridedArgs, int, int, const double2 *, hipStream_t);
+template int fx_z<float>(int, ZArgs, int, const float2 *, hipStream_t);
+template int fx_z<double>(int, ZArgs, int, const double2 *, hipStream_t);
+template int fx_xfwd<float>(int, int, XFwdArgs, const float2 *, hipStream_t);
+template int fx_xfwd<double>(int, int, XFwdArgs, const double2 *, hipStream_t);
+template int fx_xinv<float>(int, XInvArgs, const float2 *, hipStream_t);
+template int fx_xinv<double>(int, XInvArgs, const double2 *, hipStream_t);
+
+}  // namespace pmgpu

@@ -1,0 +1,440 @@
+// Kernels of the free-space / periodic Poisson solve (HIP, gfx950).
+//
+// Data path of one solve (kernel 1, fft_seq (1,2,3)), all on device:
+//   X  : real mesh (3, Ex, Ey, Ez) -> A[c][z][y][kx]   R2C, implicit zero pad
+//   Y  : A -> B[c][z][ky][kx]                         C2C, implicit zero pad
+//   Z  : B -> B (+ psi / vort buffers)                 fused pad, FFT, G-hat
+//        product, spectral curl, inverse FFT, truncate (one kernel)
+//   Y' : B -> A                                        C2C inverse, truncate
+//   X' : A -> real mesh                                C2R, truncate
+// Semantics follow poisson/pmlib_poisson_solve.f:119-1013 and
+// fourier/pmlib_fourier_{fft,ifft}.f of the reference.
+#pragma once
+#include "pm_fft.h"
+
+namespace pmgpu {
+
+// Real-input line pass (R2C) along x with implicit zero padding.
+// Line (y, z) of component c: element x at in[c*cs + x*xs + y*ys + z*zs],
+// x < Min; the transform length is 2*Nh.  Output element (c, kx, y, z) at
+// out[c*o_cs + kx*o_ks + y*o_ys + z*o_zs]; the solve uses the
+// component-interleaved spectral layout A[((z*My + y)*Hx + kx)*3 + c].
+struct XFwdArgs {
+  const void *in;
+  void *out;
+  int64_t cs, xs, ys, zs;
+  int64_t o_cs, o_ks, o_ys, o_zs;
+  int ncomp, Min, My, Mz, Hx, lines_per_block;
+  FftPlan plan;     // length Nh
+  int twL_off;      // offset of the length-2Nh table (post-processing twiddles)
+};
+
+template <typename R>
+__global__ void __launch_bounds__(1024) k_x_fwd(XFwdArgs a, const cplx_t<R> *tw) {
+  using V = cplx_t<R>;
+  extern __shared__ unsigned char smem_raw[];
+  V *lds = reinterpret_cast<V *>(smem_raw);
+  const int Nh = a.plan.L, P = a.plan.P, st = lds_stride(Nh);
+  const int T = a.lines_per_block, NL = T * a.ncomp;
+  const int y0 = blockIdx.x * T, z = blockIdx.y;
+  const R *in = reinterpret_cast<const R *>(a.in);
+  // load + pack x[2m] + i x[2m+1] into line (t, c); zero the rest
+  for (int idx = threadIdx.x; idx < NL * st; idx += blockDim.x) lds[idx] = cmake<V>(0, 0);
+  __syncthreads();
+  const int per_line = a.ncomp * a.Min;
+  for (int idx = threadIdx.x; idx < T * per_line; idx += blockDim.x) {
+    int t = idx / per_line, e = idx - t * per_line;
+    int x = e / a.ncomp, c = e - x * a.ncomp;
+    int y = y0 + t;
+    if (y < a.My) {
+      R v = in[c * a.cs + (int64_t)x * a.xs + (int64_t)y * a.ys + (int64_t)z * a.zs];
+      R *slot = reinterpret_cast<R *>(&lds[(t * a.ncomp + c) * st + (x >> 1)]);
+      slot[x & 1] = v;
+    }
+  }
+  __syncthreads();
+  const int line = threadIdx.x / P, p = threadIdx.x - line * P;
+  V *myline = lds + (line < NL ? line : 0) * st;
+  fft_lds<-1>(myline, a.plan, p, tw);
+  // post-process: X[k] = (Z[k] + conj Z[Nh-k])/2 - i/2 W^k (Z[k] - conj Z[Nh-k])
+  V *out = reinterpret_cast<V *>(a.out);
+  const V *twL = tw + a.twL_off;
+  const int Hx = a.Hx;
+  for (int idx = threadIdx.x; idx < NL * Hx; idx += blockDim.x) {
+    int l = idx / Hx, k = idx - l * Hx;
+    int t = l / a.ncomp, c = l - t * a.ncomp;
+    int y = y0 + t;
+    if (y >= a.My) continue;
+    const V *zl = lds + l * st;
+    V zk = zl[k == Nh ? 0 : k];
+    V zm = cconj(zl[k == 0 ? 0 : Nh - k]);
+    V e = cscale(cadd(zk, zm), (R)0.5);
+    V o = cscale(csub(zk, zm), (R)0.5);
+    // -i * W^k * o
+    V w = twL[k];
+    V wo = cmul(w, o);
+    V X = cadd(e, cmake<V>(wo.y, -wo.x));
+    out[c * a.o_cs + k * a.o_ks + (int64_t)y * a.o_ys + (int64_t)z * a.o_zs] = X;
+  }
+}
+
+// Strided complex pass along a non-contiguous axis.  Element (col, row, b) at
+// base + b*bstride + row*rstride + col; a block transforms T consecutive
+// columns of one batch entry.  Rows >= nin read as zero, rows < nout written,
+// multiplied by scale.  in and out may alias (each block owns its columns).
+struct StridedArgs {
+  const void *in;
+  void *out;
+  int64_t rs_in, rs_out, bs_in, bs_out;
+  int ncols, nin, nout, T;
+  double scale;
+  FftPlan plan;
+};
+
+template <typename R, int S>
+__global__ void __launch_bounds__(1024) k_strided(StridedArgs a, const cplx_t<R> *tw) {
+  using V = cplx_t<R>;
+  extern __shared__ unsigned char smem_raw[];
+  V *lds = reinterpret_cast<V *>(smem_raw);
+  const int L = a.plan.L, P = a.plan.P, st = lds_stride(L), T = a.T;
+  const int col0 = blockIdx.x * T;
+  const int64_t b = blockIdx.y + (int64_t)blockIdx.z * gridDim.y;
+  const V *in = reinterpret_cast<const V *>(a.in) + b * a.bs_in + col0;
+  V *out = reinterpret_cast<V *>(a.out) + b * a.bs_out + col0;
+  for (int idx = threadIdx.x; idx < T * L; idx += blockDim.x) {
+    int row = idx / T, c = idx - row * T;
+    V v = cmake<V>(0, 0);
+    if (row < a.nin && col0 + c < a.ncols) v = in[(int64_t)row * a.rs_in + c];
+    lds[c * st + row] = v;
+  }
+  __syncthreads();
+  const int line = threadIdx.x / P, p = threadIdx.x - line * P;
+  fft_lds<S>(lds + (line < T ? line : 0) * st, a.plan, p, tw);
+  const R sc = (R)a.scale;
+  for (int idx = threadIdx.x; idx < T * a.nout; idx += blockDim.x) {
+    int row = idx / T, c = idx - row * T;
+    if (col0 + c < a.ncols) out[(int64_t)row * a.rs_out + c] = cscale(lds[c * st + row], sc);
+  }
+}
+
+// Complex-to-real line pass along x (inverse of k_x_fwd), keeping x < Mout.
+// Input element (c, kx, y, z) at in[c*i_cs + kx*i_ks + y*i_ys + z*i_zs].
+struct XInvArgs {
+  const void *in;
+  void *out;
+  int64_t cs, xs, ys, zs;
+  int64_t i_cs, i_ks, i_ys, i_zs;
+  int ncomp, Mout, My, Mz, Hx, lines_per_block;
+  double scale;
+  FftPlan plan;  // length Nh
+  int twL_off;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(1024) k_x_inv(XInvArgs a, const cplx_t<R> *tw) {
+  using V = cplx_t<R>;
+  extern __shared__ unsigned char smem_raw[];
+  V *lds = reinterpret_cast<V *>(smem_raw);
+  const int Nh = a.plan.L, P = a.plan.P, st = lds_stride(Nh);
+  const int T = a.lines_per_block, NL = T * a.ncomp;
+  const int y0 = blockIdx.x * T, z = blockIdx.y;
+  const V *in = reinterpret_cast<const V *>(a.in);
+  const V *twL = tw + a.twL_off;
+  // Z[k] = (X[k] + conj X[Nh-k]) + i W^-k (X[k] - conj X[Nh-k]), k < Nh;
+  // imaginary parts of X[0] and X[Nh] are dropped (Hermitian projection).
+  for (int idx = threadIdx.x; idx < NL * Nh; idx += blockDim.x) {
+    int l = idx / Nh, k = idx - l * Nh;
+    int t = l / a.ncomp, c = l - t * a.ncomp;
+    int y = y0 + t;
+    V Zk = cmake<V>(0, 0);
+    if (y < a.My) {
+      const V *xl = in + c * a.i_cs + (int64_t)y * a.i_ys + (int64_t)z * a.i_zs;
+      V xk = xl[k * a.i_ks], xm = xl[(Nh - k) * a.i_ks];
+      if (k == 0) { xk.y = 0; xm.y = 0; }
+      V s = cadd(xk, cconj(xm));
+      V d = csub(xk, cconj(xm));
+      V w = cconj(twL[k]);
+      V wd = cmul(w, d);
+      Zk = cadd(s, cmake<V>(-wd.y, wd.x));
+    }
+    lds[l * st + k] = Zk;
+  }
+  __syncthreads();
+  const int line = threadIdx.x / P, p = threadIdx.x - line * P;
+  fft_lds<1>(lds + (line < NL ? line : 0) * st, a.plan, p, tw);
+  R *out = reinterpret_cast<R *>(a.out);
+  const R sc = (R)a.scale;
+  const int per_line = a.ncomp * a.Mout;
+  for (int idx = threadIdx.x; idx < T * per_line; idx += blockDim.x) {
+    int t = idx / per_line, e = idx - t * per_line;
+    int x = e / a.ncomp, c = e - x * a.ncomp;
+    int y = y0 + t;
+    if (y < a.My) {
+      const R *slot = reinterpret_cast<const R *>(&lds[(t * a.ncomp + c) * st + (x >> 1)]);
+      out[c * a.cs + (int64_t)x * a.xs + (int64_t)y * a.ys + (int64_t)z * a.zs] = slot[x & 1] * sc;
+    }
+  }
+}
+
+// Fused third-direction pass: zero pad, forward FFT of the 3 vorticity
+// components, G-hat product + spectral curl (+ psi, + regularised /
+// reprojected vorticity), inverse FFTs, truncation to Mz rows, 1/Lz.
+struct ZArgs {
+  void *B;            // in: w-hat (3 comps), out: u-hat (in place)
+  void *Bpsi;         // optional psi-hat output
+  void *Bvort;        // optional regularised vorticity output
+  const void *Ghat;   // octant table [(ky' * Hx + kx) * Hz + kz'] (kernel 1)
+  int64_t rs, cs;     // row (z) stride and component stride of B
+  int64_t kxs, kys;   // kx and ky strides of B (interleaved layout: 3 and 3*Hx)
+  int Hx, Ly, Lz, Lx, Mz, T, Hy, Hz;
+  int order;          // regularisation order for zeta
+  int allper;         // all axes periodic: G = zeta / (4 pi^2 |k|^2)
+  int want_psi, want_vort, reproj;
+  double dk[3], ksmp[3], h3, eps2pi;
+  FftPlan plan;       // length Lz
+};
+
+template <typename R>
+__device__ __forceinline__ R wavenumber(int n, int L, double dk, double ksmp) {
+  // poisson/pmlib_poisson_solve.f:724-766 with kmin = -ksmp on one rank,
+  // evaluated in the working precision
+  const R kmin = -(R)ksmp;
+  const R base = kmin + (R)n * (R)dk;
+  return (n < L / 2) ? base + (R)ksmp : base - (R)ksmp;
+}
+
+template <typename R>
+__device__ __forceinline__ R zeta_filter(int order, R s) {
+  R s2 = s * s, e = exp((R)-0.5 * s2);
+  switch (order) {
+    case 0: return (R)1;
+    case 2: return e;
+    case 4: return ((R)1 + (R)0.5 * s2) * e;
+    case 6: return ((R)1 + (R)0.5 * s2 + (R)0.125 * (s2 * s2)) * e;
+    case 8: return ((R)1 + (R)0.5 * s2 + (R)0.125 * (s2 * s2) + (R)(1.0 / 48.0) * (s2 * s2 * s2)) * e;
+    default:
+      return ((R)1 + (R)0.5 * s2 + (R)0.125 * (s2 * s2) + (R)(1.0 / 48.0) * (s2 * s2 * s2) +
+              (R)(1.0 / 384.0) * (s2 * s2 * s2 * s2)) * e;
+  }
+}
+
+template <typename R, int Q>
+__global__ void __launch_bounds__(1024) k_z_convolve(ZArgs a, const cplx_t<R> *tw) {
+  using V = cplx_t<R>;
+  extern __shared__ unsigned char smem_raw[];
+  V *lds = reinterpret_cast<V *>(smem_raw);
+  const int L = a.plan.L, P = a.plan.P, st = lds_stride(L), T = a.T;
+  const int kx0 = blockIdx.x * T, ky = blockIdx.y;
+  const int line = threadIdx.x / P, p = threadIdx.x - line * P;
+  const int myl = line < T ? line : 0;
+  V *B = reinterpret_cast<V *>(a.B);
+  V w[3][Q];
+  for (int q = 0; q < 3; ++q) {
+    const V *src = B + q * a.cs + (int64_t)ky * a.kys + (int64_t)kx0 * a.kxs;
+    for (int idx = threadIdx.x; idx < T * L; idx += blockDim.x) {
+      int row = idx / T, c = idx - row * T;
+      V v = cmake<V>(0, 0);
+      if (row < a.Mz && kx0 + c < a.Hx) v = src[(int64_t)row * a.rs + c * a.kxs];
+      lds[c * st + row] = v;
+    }
+    __syncthreads();
+    fft_lds<-1>(lds + myl * st, a.plan, p, tw);
+#pragma unroll
+    for (int e = 0; e < Q; ++e) {
+      int n = p + e * P;
+      w[q][e] = n < L ? lds[myl * st + n] : cmake<V>(0, 0);
+    }
+    __syncthreads();
+  }
+  // wavenumbers of this column
+  const int kx = kx0 + myl;
+  const bool col_ok = line < T && kx < a.Hx;
+  const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
+  const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
+  const bool nx = (2 * kx == a.Lx), ny = (2 * ky == a.Ly);
+  const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
+  const R pi = (R)3.14159265358979323846;
+  const R twopi = (R)6.28318530717958647692;
+  const R *G = reinterpret_cast<const R *>(a.Ghat);
+  const int nout = 3 + (a.want_psi ? 3 : 0) + (a.want_vort ? 3 : 0);
+  const R scale = (R)1 / (R)L;
+  for (int o = 0; o < nout; ++o) {
+    int kind = o / 3, comp = o - kind * 3;  // kind 0 vel, 1 psi or vort, 2 vort
+    if (kind == 1 && !a.want_psi) kind = 2;
+#pragma unroll
+    for (int e = 0; e < Q; ++e) {
+      int n = p + e * P;
+      if (n >= L) continue;
+      const R kk = wavenumber<R>(n, a.Lz, a.dk[2], a.ksmp[2]);
+      const bool nz = (2 * n == a.Lz);
+      const int kzf = n <= a.Lz - n ? n : a.Lz - n;
+      R zeta = (R)1;
+      if (a.want_vort || a.allper) {
+        if (a.want_vort) {
+          R s = (R)a.eps2pi * sqrt(ki * ki + kj * kj + kk * kk);
+          zeta = zeta_filter<R>(a.order, s);
+        }
+      }
+      R g;
+      if (a.allper) {
+        R kmag = ki * ki + kj * kj + kk * kk;
+        g = kmag > (R)0 ? zeta / ((R)4 * (pi * pi) * kmag) : (R)0;
+      } else {
+        g = col_ok ? G[((int64_t)kyf * a.Hx + kx) * a.Hz + kzf] * (R)a.h3 : (R)0;
+      }
+      V r;
+      if (kind == 0) {
+        // u-hat = G 2 pi i (k x w-hat); Nyquist wavenumbers drop out of the
+        // real part of the inverse (Hermitian projection)
+        R kx_ = nx ? (R)0 : ki, ky_ = ny ? (R)0 : kj, kz_ = nz ? (R)0 : kk;
+        V cr;
+        if (comp == 0) cr = csub(cscale(w[2][e], ky_), cscale(w[1][e], kz_));
+        else if (comp == 1) cr = csub(cscale(w[0][e], kz_), cscale(w[2][e], kx_));
+        else cr = csub(cscale(w[1][e], kx_), cscale(w[0][e], ky_));
+        R f = g * twopi;
+        r = cmake<V>(-cr.y * f, cr.x * f);
+      } else if (kind == 1) {
+        r = cscale(w[comp][e], g);
+      } else {
+        r = cscale(w[comp][e], zeta);
+        if (a.reproj) {
+          // + (2 pi i k_c) G 2 pi i (k . w-hat) = -4 pi^2 G k_c sum_d k_d w_d,
+          // keeping only pairs (c, d) of equal Nyquist parity
+          R kv[3] = {ki, kj, kk};
+          bool nq[3] = {nx, ny, nz};
+          V acc = cmake<V>(0, 0);
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            if (nq[d] == nq[comp]) acc = cadd(acc, cscale(w[d][e], kv[d]));
+          R f = -(R)4 * (pi * pi) * g * kv[comp];
+          r = cadd(r, cscale(acc, f));
+        }
+      }
+      lds[myl * st + n] = r;
+    }
+    __syncthreads();
+    fft_lds<1>(lds + myl * st, a.plan, p, tw);
+    V *dst = kind == 0 ? B : (kind == 1 ? reinterpret_cast<V *>(a.Bpsi) : reinterpret_cast<V *>(a.Bvort));
+    dst += comp * a.cs + (int64_t)ky * a.kys + (int64_t)kx0 * a.kxs;
+    for (int idx = threadIdx.x; idx < T * a.Mz; idx += blockDim.x) {
+      int row = idx / T, c = idx - row * T;
+      if (kx0 + c < a.Hx) dst[(int64_t)row * a.rs + c * a.kxs] = cscale(lds[c * st + row], scale);
+    }
+    __syncthreads();
+  }
+}
+
+// Regularised Green's function (kernel 1) on the kernel grid, written in
+// FFT order (index n <-> grid point m = n or n - L, the fft_shift of
+// fourier/pmlib_fourier_fft.f:139-150).  Cell centres follow
+// topology/pmlib_topology_kernel.f:109-110: x = xk + ((i-1) + 1/2) dx with
+// xk = -(L/2 + 1/2) dx, i = m + L/2 + 1; periodic axes sum images +-1.
+struct GreenArgs {
+  double *out;       // real, [(z * Ly + y) * Lx + x]
+  int L[3];
+  int bc[3];
+  double dx[3], xk[3], len[3];
+  double eps;
+  int order;
+};
+
+static __global__ void k_green(GreenArgs a) {
+  const int64_t n = (int64_t)a.L[0] * a.L[1] * a.L[2];
+  const double pi = 3.14159265358979323846;
+  const double c_1_4pi = 0.25 / pi;
+  const double c_1_sqrt2 = 0.70710678118654752440;
+  const double c_1_sqrt2pi = 0.39894228040143267794;
+  const double c_sq2_4pieps = 1.41421356237309504880 / (4.0 * pow(pi, 1.5) * a.eps);
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int ix = idx % a.L[0];
+    int iy = (idx / a.L[0]) % a.L[1];
+    int iz = idx / ((int64_t)a.L[0] * a.L[1]);
+    int nn[3] = {ix, iy, iz};
+    double base[3];
+    for (int d = 0; d < 3; ++d) {
+      int m = nn[d] < a.L[d] / 2 ? nn[d] : nn[d] - a.L[d];
+      int i1 = m + a.L[d] / 2 + 1;
+      base[d] = a.xk[d] + ((double)(i1 - 1) + 0.5) * a.dx[d];
+    }
+    double acc = 0.0;
+    for (int kk = -a.bc[2]; kk <= a.bc[2]; ++kk)
+      for (int jj = -a.bc[1]; jj <= a.bc[1]; ++jj)
+        for (int ii = -a.bc[0]; ii <= a.bc[0]; ++ii) {
+          double px = base[0] + a.len[0] * (double)ii;
+          double py = base[1] + a.len[1] * (double)jj;
+          double pz = base[2] + a.len[2] * (double)kk;
+          double r = sqrt(px * px + py * py + pz * pz);
+          double rho = r / a.eps;
+          double v;
+          if (r == 0.0) {
+            switch (a.order) {
+              case 0: v = 1.0; break;
+              case 2: v = c_sq2_4pieps; break;
+              case 4: v = 1.5 * c_sq2_4pieps; break;
+              case 6: v = 1.875 * c_sq2_4pieps; break;
+              case 8: v = 2.1875 * c_sq2_4pieps; break;
+              default: v = 2.4609375 * c_sq2_4pieps; break;
+            }
+          } else if (a.order == 0) {
+            v = c_1_4pi / r;
+          } else {
+            double E = exp(-0.5 * (rho * rho)) * c_1_sqrt2pi;
+            double F = erf(c_1_sqrt2 * rho);
+            double r2 = rho * rho, r3 = r2 * rho, r5 = r3 * r2, r7 = r5 * r2;
+            double poly;
+            switch (a.order) {
+              case 2: poly = 0.0; break;
+              case 4: poly = rho; break;
+              case 6: poly = 1.75 * rho - 0.25 * r3; break;
+              case 8: poly = 2.375 * rho + (-2.0 / 3.0) * r3 + (1.0 / 24.0) * r5; break;
+              default:
+                poly = 2.921875 * rho + (-233.0 / 192.0) * r3 + (29.0 / 192.0) * r5 + (-1.0 / 192.0) * r7;
+                break;
+            }
+            v = c_1_4pi / r * (poly * E + F);
+          }
+          acc += v;
+        }
+    a.out[idx] = acc;
+  }
+}
+
+// Octant extraction of the real, even kernel spectrum:
+// Ghat[(ky' * Hx + kx) * Hz + kz'] = Re B[(kz' * Ly + ky') * Hx + kx].
+template <typename R>
+__global__ void k_g_octant(const double2 *B, R *G, int Hx, int Ly, int Hy, int Hz) {
+  int64_t n = (int64_t)Hy * Hx * Hz;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int kz = idx % Hz;
+    int64_t r = idx / Hz;
+    int kx = r % Hx;
+    int ky = r / Hx;
+    G[idx] = (R)B[((int64_t)kz * Ly + ky) * Hx + kx].x;
+  }
+}
+
+// Periodic forward ghost fill of a (3, E...) mesh along axis d
+// (mesh/pmlib_mesh_map_ghost.f with incl_edges = .FALSE., op = replace).
+template <typename R>
+__global__ void k_ghost_fill(R *m, int E0, int E1, int E2, int d, int N, int ng) {
+  int E[3] = {E0, E1, E2};
+  int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+  int64_t st[3] = {3, 3 * (int64_t)E0, 3 * (int64_t)E0 * E1};
+  int64_t n = (int64_t)E[o1] * E[o2] * 3 * ng;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int c = idx % 3;
+    int64_t r = idx / 3;
+    int t = r % ng;
+    r /= ng;
+    int a1 = r % E[o1];
+    int a2 = r / E[o1];
+    int64_t base = c + st[o1] * a1 + st[o2] * a2;
+    m[base + st[d] * t] = m[base + st[d] * (N + t)];
+    m[base + st[d] * (ng + N + t)] = m[base + st[d] * (ng + t)];
+  }
+}
+
+}  // namespace pmgpu

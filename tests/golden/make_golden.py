@@ -1,0 +1,126 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference
+(oracle/_ref/ref_driver, built from the reference sources by
+oracle/Makefile.ref).  Run in the build container only:
+
+    python tests/golden/make_golden.py
+
+Outputs (small, committed):
+  kat_ring_ref.json      rms velocity errors of the validation/poisson ring on
+                         the pmlib_patch_adjust grids (order 6, eps 2h, ng 4)
+  ref_solve_*.npz        inputs + reference outputs of small solves
+  ref_particles_*.npz    inputs + reference outputs of P2M/M2P/remesh/advance
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from oracle import oracle as O  # noqa: E402
+import kat_ring as K  # noqa: E402
+from tests.helpers import blob_field  # noqa: E402
+
+SOLVE_CASES = {
+    "free_o4_psi": ((16, 16, 16), 0, (0, 0, 0), 4, (False, False, True)),
+    "free_ng2_o6_all": ((12, 10, 8), 2, (0, 0, 0), 6, (True, True, True)),
+    "mixed_o4": ((8, 8, 12), 2, (1, 1, 0), 4, (True, False, True)),
+    "periodic_o6": ((12, 8, 8), 0, (1, 1, 1), 6, (True, True, True)),
+    "free_o2_r35": ((20, 12, 18), 0, (0, 0, 0), 2, (False, False, True)),
+}
+
+
+def solve_case(name, spec):
+    ncell, ng, bc, order, (rv, rp, cv) = spec
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+    p = O.make_params(order, 1, 2.0)
+    w = blob_field(g, seed=sum(ncell) + order)
+    job = dict(O.geom_job(g, p), mode="solve", reg_vort=rv, reproj=rp, calc_vecpot=cv)
+    raw, _ = O.ref_run(job, [w])
+    ref = np.frombuffer(raw, dtype=np.float64)
+    n = w.size
+    outs = {"vel": ref[:n]}
+    q = 1
+    if rv or rp:
+        outs["vort"] = ref[q * n:(q + 1) * n]
+        q += 1
+    if cv:
+        outs["psi"] = ref[q * n:(q + 1) * n]
+    np.savez_compressed(os.path.join(HERE, "ref_solve_%s.npz" % name), vort_in=w,
+                        spec=np.array(json.dumps(spec)),
+                        **{k: v.reshape(w.shape, order="F") for k, v in outs.items()})
+
+
+def particle_case(name, ncell, ng, bc, io, seed):
+    rng = np.random.default_rng(seed)
+    h = 1.0 / 8
+    g = O.make_geom(ncell, ng, [-0.5, -0.5, -0.5], h, bc, io)
+    xmin, xmax = np.array(g.xmin), np.array(g.xmax)
+    npart = 300
+    pos = rng.uniform(xmin - 2 * h, xmax + 2 * h, size=(npart, 3)).T.copy(order="F")
+    pos[:, :40] = xmin[:, None] + ((rng.integers(0, 8, size=(3, 40)) - 1) + 0.5) * h
+    pos[:, 40:60] = xmin[:, None] + rng.integers(0, 8, size=(3, 20)) * h
+    if io == 4:   # keep M6* M2P inside its defined range (k3 + 2 >= 1 - ng)
+        pos[2] = np.maximum(pos[2], xmin[2] - (ng + 0.5) * h)
+    val = np.asfortranarray(rng.normal(size=(3, npart)))
+    ext = O.mesh_extent(g)
+    out = {"pos": pos, "val": val}
+    raw, _ = O.ref_run(dict(O.geom_job(g), mode="p2m", npart=npart, interp_order=io), [pos, val])
+    out["p2m"] = np.frombuffer(raw).reshape((3,) + ext, order="F")
+    mesh = np.asfortranarray(rng.normal(size=(3,) + ext))
+    out["mesh"] = mesh
+    raw, _ = O.ref_run(dict(O.geom_job(g), mode="m2p", npart=npart, interp_order=io), [mesh, pos])
+    out["m2p"] = np.frombuffer(raw).reshape((3, npart), order="F")
+    vm = np.asfortranarray(rng.normal(size=(3,) + ext))
+    dvm = np.asfortranarray(0.1 * rng.normal(size=vm.shape))
+    velm = np.asfortranarray(rng.normal(size=vm.shape))
+    out.update(rm_vort=vm, rm_dvort=dvm, rm_vel=velm)
+    for ig in (0, 1):
+        raw, _ = O.ref_run(dict(O.geom_job(g), mode="remesh", min_vort=1.2, min_dvort=0.15,
+                                incl_ghost=bool(ig), interp_order=io), [vm, dvm, velm])
+        n = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+        r = np.frombuffer(raw[4:]).reshape((3, 3 * n), order="F")
+        out["remesh%d" % ig] = r
+    arrs = [pos] + [np.asfortranarray(rng.normal(size=(3, npart)) * s) for s in (1.0, 1.0, 0.5, 0.3, 0.2)]
+    for k, a in zip(("a_pos", "a_vel", "a_vort", "a_dvort", "a_vel_rk", "a_dvort_rk"), arrs):
+        out[k] = a
+    for st in (1, 2):
+        raw, _ = O.ref_run(dict(O.geom_job(g), mode="advance", npart=npart, stage=st, dtime=0.2,
+                                rk_order=2, interp_order=io), arrs)
+        n = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+        out["advance%d" % st] = np.frombuffer(raw[4:]).reshape((3, 4 * n), order="F")
+    out["geom"] = np.array(json.dumps([ncell, ng, bc, io]))
+    np.savez_compressed(os.path.join(HERE, "ref_particles_%s.npz" % name), **out)
+
+
+def kat():
+    cases = []
+    for (h, ncell, xmin, stored) in K.KAT_CASES:
+        g = O.make_geom(ncell, K.KAT_NGHOST, xmin, h, (0, 0, 0))
+        p = O.make_params(K.KAT_ORDER, 1, K.KAT_EPS_RADIUS)
+        w, u = K.ring_fields(xmin, h, ncell, K.KAT_NGHOST)
+        raw, _ = O.ref_run(dict(O.geom_job(g, p), mode="solve", reg_vort=True), [w], timeout=1200)
+        vel = np.frombuffer(raw, dtype=np.float64)[:w.size].reshape(w.shape, order="F")
+        err = K.rms_velocity_error(vel, u, K.KAT_NGHOST)
+        print("KAT h=%g reference rms error %.15e (convergence.dat stores %.12e)" % (h, err, stored))
+        cases.append({"h": h, "ncell": list(ncell), "xmin": list(xmin), "rms_error": err,
+                      "convergence_dat": stored})
+    json.dump({"source": "compiled reference (oracle/_ref), order 6, eps 2h, nghost 4, reg_vort",
+               "cases": cases}, open(os.path.join(HERE, "kat_ring_ref.json"), "w"), indent=1)
+
+
+if __name__ == "__main__":
+    if not O.ref_available():
+        sys.exit("oracle/_ref is not built: make -f oracle/Makefile.ref")
+    for name, spec in SOLVE_CASES.items():
+        solve_case(name, spec)
+    particle_case("m4_free", (10, 8, 12), 0, (0, 0, 0), 3, 11)
+    particle_case("m4_ng2", (8, 10, 8), 2, (0, 0, 0), 3, 12)
+    particle_case("m4_mixed", (8, 8, 10), 0, (1, 1, 0), 3, 13)
+    particle_case("m6_ng1", (10, 8, 8), 1, (0, 0, 0), 4, 14)
+    particle_case("m6_periodic", (8, 8, 8), 0, (1, 1, 1), 4, 15)
+    kat()

@@ -1,0 +1,224 @@
+"""Pins the oracle (oracle/pmoracle.c) before it is trusted as the GPU checker:
+  * against the golden vectors written by the compiled reference
+    (tests/golden/ref_*.npz, tests/golden/make_golden.py),
+  * against closed forms and invariants that a plausible mistake would break
+    (textbook DFT, single Fourier mode, kernel limits, partition of unity,
+    interpolation moments, RK coefficients, NINT rounding at exact halves),
+  * and, when oracle/_ref is built in this container, against a live run of
+    the reference on a fresh random case.
+"""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import blob_field, cell_axes, rel_linf
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ----------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6, 7, 8, 9, 12, 14, 15, 16, 20, 22, 26, 28, 30, 32, 48, 64, 96,
+                               100, 128, 160])
+def test_glassman_is_the_textbook_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    f = O.glassman(x, -1)
+    assert np.abs(f - np.fft.fft(x)).max() <= 1e-13 * np.abs(np.fft.fft(x)).max() * max(1, np.log2(n))
+    b = O.glassman(x, +1)
+    assert np.abs(b - np.fft.ifft(x) * n).max() <= 1e-13 * np.abs(b).max() * max(1, np.log2(n))
+
+
+def test_glassman_brute_force_and_impulse():
+    n = 12
+    x = np.exp(1j * np.arange(n) ** 1.3)
+    dft = np.array([sum(x[m] * np.exp(-2j * np.pi * k * m / n) for m in range(n)) for k in range(n)])
+    assert np.abs(O.glassman(x) - dft).max() < 1e-12
+    e = np.zeros(n, complex)
+    e[3] = 1.0
+    assert np.abs(O.glassman(e) - np.exp(-2j * np.pi * 3 * np.arange(n) / n)).max() < 1e-14
+
+
+# ------------------------------------------------------- Green's function
+@pytest.mark.parametrize("order,g0", [(2, 1.0), (4, 1.5), (6, 1.875), (8, 2.1875), (10, 2.4609375)])
+def test_green_r0_value_is_the_limit(order, g0):
+    """G(0) is the r -> 0 limit of the regularised kernel (closed form in
+    poisson/pmlib_poisson_setup.f, e.g. :562-570): check against the series
+    of the kernel at a tiny radius evaluated in extended precision."""
+    h = 1.0 / 16
+    g = O.make_geom((8, 8, 8), 0, [0, 0, 0], h)
+    G = O.green_real(g, O.make_params(order, 1, 2.0)).reshape((16, 16, 16), order="F")
+    eps = 2.0 * h
+    c = np.sqrt(2.0) / (4.0 * np.pi ** 1.5 * eps)
+    assert abs(G[8, 8, 8] - g0 * c) <= 1e-15 * g0 * c
+    # continuity: value at r = h against the kernel formula at r = h
+    from math import erf, exp, pi, sqrt
+    rho = h / eps
+    E = exp(-0.5 * rho * rho) / sqrt(2 * pi)
+    poly = {2: 0.0, 4: rho, 6: 1.75 * rho - 0.25 * rho ** 3,
+            8: 2.375 * rho - (2 / 3) * rho ** 3 + rho ** 5 / 24,
+            10: 2.921875 * rho - 233 / 192 * rho ** 3 + 29 / 192 * rho ** 5 - rho ** 7 / 192}[order]
+    ref = (poly * E + erf(rho / sqrt(2))) / (4 * pi * h)
+    assert abs(G[9, 8, 8] - ref) <= 1e-14 * ref
+    # far field: G -> 1/(4 pi r) (3.5 eps away the smoothing error is < 1%)
+    r = 7 * h
+    assert abs(G[15, 8, 8] * 4 * pi * r - 1.0) < 1e-2
+
+
+def test_green_spectrum_is_real_and_even():
+    g = O.make_geom((6, 5, 4), 1, [0, 0, 0], 0.1)
+    Gh = O.poisson_setup(g, O.make_params(6, 1, 2.0)).reshape((16, 14, 12), order="F")
+    assert np.abs(Gh.imag).max() <= 1e-14 * np.abs(Gh).max()
+    assert np.abs(Gh - Gh[::-1][np.r_[-1, 0:15]][:, ::-1][:, np.r_[-1, 0:13]]).max() <= 1e-14 * np.abs(Gh).max()
+
+
+# --------------------------------------------------------------- solve
+def test_periodic_single_mode_is_exact():
+    """All-periodic solve (poisson_solve.f:799-808): omega_z = sin(2 pi x / Lx)
+    has psi_z = omega_z / k^2 and u_y = -dpsi_z/dx exactly."""
+    g = O.make_geom((12, 8, 8), 0, [-0.5, -0.6, -0.7], 1.0 / 8, (1, 1, 1))
+    p = O.make_params(6, 1, 2.0)
+    X, Y, Z = np.meshgrid(*cell_axes(g), indexing="ij")
+    k = 2 * np.pi / 1.5
+    w = np.zeros((3,) + X.shape, order="F")
+    w[2] = np.sin(k * X)
+    vel, _, psi = O.poisson_solve(g, p, O.poisson_setup(g, p), w, calc_vecpot=True)
+    assert np.abs(psi[2] - np.sin(k * X) / k ** 2).max() < 1e-14
+    assert np.abs(vel[1] + np.cos(k * X) / k).max() < 1e-14
+    assert np.abs(vel[0]).max() < 1e-14 and np.abs(vel[2]).max() < 1e-14
+
+
+def test_free_space_solve_converges_to_closed_form_ring():
+    """The compact ring of validation/poisson (closed-form omega and u, see
+    tests/golden/kat_ring.py): the rms velocity error of the order-6, eps = 2h
+    solve falls by more than 2^3 when h halves (a dropped h^3, a wrong
+    wavenumber scale or a sign error in the curl breaks this)."""
+    import sys
+    sys.path.insert(0, GOLD)
+    import kat_ring as K
+    errs = []
+    for n in (16, 32):
+        h = 1.0 / n
+        ncell = (n, 2 * n, 2 * n)
+        xmin = (-0.5, -1.0, -1.0)
+        g = O.make_geom(ncell, 4, xmin, h)
+        p = O.make_params(6, 1, 2.0)
+        w, u = K.ring_fields(xmin, h, ncell, 4)
+        vel, _, _ = O.poisson_solve(g, p, O.poisson_setup(g, p), w)
+        errs.append(K.rms_velocity_error(vel, u, 4))
+    assert errs[1] < 0.05 and errs[0] / errs[1] > 8.0, errs
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "ref_solve_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[10:-4])
+def test_oracle_solve_matches_reference_golden(path):
+    d = np.load(path, allow_pickle=False)
+    ncell, ng, bc, order, (rv, rp, cv) = json.loads(str(d["spec"]))
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+    p = O.make_params(order, 1, 2.0)
+    vel, vo, ps = O.poisson_solve(g, p, O.poisson_setup(g, p), d["vort_in"], rv, rp, cv)
+    assert rel_linf(vel, d["vel"]) <= 2e-15
+    if rv or rp:
+        assert rel_linf(vo, d["vort"]) <= 2e-15
+    if cv:
+        assert rel_linf(ps, d["psi"]) <= 2e-15
+
+
+# -------------------------------------------------------- interpolation
+def test_cell_index_rounds_half_away_from_zero():
+    """NINT((x - xmin) * (1/h)) (interpolation/pmlib_interp_particle_mesh.f:153)."""
+    f = O.lib().or_cell_index
+    assert f(1.5, 0.0, 1.0) == 2 and f(2.5, 0.0, 1.0) == 3 and f(-0.5, 0.0, 1.0) == -1
+    assert f(-1.5, 0.0, 1.0) == -2
+    x = 0.0 + 2.5 * 0.1
+    assert f(np.nextafter(2.5, 0), 0.0, 1.0) == 2 and f(np.nextafter(2.5, 3), 0.0, 1.0) == 3
+    # the subtraction and the multiply are each correctly rounded (no FMA):
+    xmin, h = -1.0, 2.0 / 128
+    for i in range(-3, 131):
+        x = xmin + ((i - 1) + 0.5) * h
+        t = (x - xmin) * (1.0 / h)
+        assert f(x, xmin, 1.0 / h) == int(np.sign(t) * np.floor(abs(t) + 0.5))
+
+
+@pytest.mark.parametrize("io", [3, 4])
+def test_interpolation_partition_of_unity_and_moments(io):
+    """One particle deposited away from walls: weights sum to 1, first and
+    second moments vanish (M'4 and M6* reproduce quadratics)."""
+    g = O.make_geom((16, 16, 16), 0, [0, 0, 0], 1.0 / 16, (0, 0, 0), io)
+    rng = np.random.default_rng(io)
+    for _ in range(20):
+        x = rng.uniform(0.3, 0.7, 3)
+        m = O.p2m(g, x.reshape(3, 1), np.array([[1.0], [0.0], [0.0]]))[0]
+        X, Y, Z = np.meshgrid(*cell_axes(g), indexing="ij")
+        assert abs(m.sum() - 1.0) < 1e-13
+        for A, c in ((X, x[0]), (Y, x[1]), (Z, x[2])):
+            assert abs((m * (A - c)).sum()) < 1e-13
+            assert abs((m * (A - c) ** 2).sum()) < 1e-13
+
+
+def _geom_from(d):
+    ncell, ng, bc, io = json.loads(str(d["geom"]))
+    return O.make_geom(ncell, ng, [-0.5, -0.5, -0.5], 1.0 / 8, bc, io)
+
+
+PART = sorted(glob.glob(os.path.join(GOLD, "ref_particles_*.npz")))
+
+
+@pytest.mark.parametrize("path", PART, ids=lambda p: os.path.basename(p)[14:-4])
+def test_oracle_particles_match_reference_golden_bitwise(path):
+    d = np.load(path, allow_pickle=False)
+    g = _geom_from(d)
+    assert np.array_equal(O.p2m(g, d["pos"], d["val"]), d["p2m"])
+    assert np.array_equal(O.m2p(g, d["mesh"], d["pos"]), d["m2p"])
+    for ig in (0, 1):
+        P, W, V, D = O.remesh(g, d["rm_vort"], d["rm_dvort"], d["rm_vel"], 1.2, 0.15, bool(ig))
+        assert np.array_equal(np.concatenate([P, W, V], axis=1), d["remesh%d" % ig])
+    for st in (1, 2):
+        out = O.advance(g, 2, st, 0.2, d["a_pos"], d["a_vel"], d["a_vort"], d["a_dvort"], d["a_vel_rk"],
+                        d["a_dvort_rk"])
+        mine = np.concatenate([out[0], out[2], out[4], out[5]], axis=1)
+        assert np.array_equal(mine, d["advance%d" % st])
+
+
+def test_rk_coefficients_closed_form():
+    """Constant velocity: every scheme moves x by dt*u over its stages
+    (particles/pmlib_particles_advance.f:49-88)."""
+    g = O.make_geom((8, 8, 8), 0, [-10, -10, -10], 2.5, (0, 0, 0))
+    rng = np.random.default_rng(0)
+    x0 = rng.uniform(-1, 1, (3, 5))
+    u = rng.uniform(-1, 1, (3, 5))
+    for order in (1, 2, 3):
+        arrs = [x0, u, np.zeros((3, 5)), np.zeros((3, 5)), np.zeros((3, 5)), np.zeros((3, 5))]
+        for st in range(1, order + 1):
+            arrs = O.advance(g, order, st, 0.1, *arrs)
+            arrs[1] = u.copy()
+        assert np.abs(arrs[0] - (x0 + 0.1 * u)).max() < 1e-15
+
+
+def test_remesh_order_and_threshold():
+    g = O.make_geom((3, 2, 2), 0, [0, 0, 0], 1.0)
+    vm = np.zeros((3, 3, 2, 2), order="F")
+    vm[0, 2, 0, 0] = 2.0
+    vm[1, 0, 1, 1] = 3.0
+    vm[2, 1, 0, 1] = 0.5
+    P, W, V, D = O.remesh(g, vm, None, None, 1.0, 0.0)
+    assert P.shape[1] == 2
+    assert np.array_equal(P[:, 0], [2.5, 0.5, 0.5]) and np.array_equal(P[:, 1], [0.5, 1.5, 1.5])
+
+
+# ------------------------------------------------------ live reference
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+def test_oracle_matches_live_reference_random_case():
+    rng = np.random.default_rng(99)
+    ncell = tuple(int(v) for v in rng.integers(6, 13, 3))
+    g = O.make_geom(ncell, 1, [-0.4, -0.5, -0.6], 1.0 / 10, (0, 0, 0))
+    p = O.make_params(8, 1, 2.0)
+    w = blob_field(g, seed=5)
+    raw, _ = O.ref_run(dict(O.geom_job(g, p), mode="solve", reg_vort=True), [w])
+    ref = np.frombuffer(raw, dtype=np.float64)
+    vel, vo, _ = O.poisson_solve(g, p, O.poisson_setup(g, p), w, True)
+    assert rel_linf(vel, ref[:w.size].reshape(w.shape, order="F")) <= 2e-15
+    assert rel_linf(vo, ref[w.size:].reshape(w.shape, order="F")) <= 2e-15
