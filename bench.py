@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the single-GPU box runs --gpus 1 "
@@ -171,6 +173,15 @@ def main():
     t64 = time.perf_counter() - t0
     z64_ms, tot64_ms = s64.last_timing()
     s64.set_timing(False)
+
+    if args.copy_calibration:
+        # an elementwise kernel of known size: reads 1 GiB, writes 1 GiB
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x torch.neg over 1 GiB (1 GiB read + 1 GiB written)")
 
     a = v32.double()
     b = v64

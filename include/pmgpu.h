@@ -91,6 +91,71 @@ int pmgpu_poisson_kernel_octant(pmgpu_poisson_t h, void *host, int64_t cap_bytes
 int pmgpu_poisson_last_timing(pmgpu_poisson_t h, double *zpass_ms, double *total_ms);
 int pmgpu_poisson_set_timing(pmgpu_poisson_t h, int32_t enable);
 
+/* ------------------------------------------------------------------------
+ * Particles (FP64 throughout; the *_device variants take device pointers and
+ * a hipStream_t, NULL for the default stream).  Particle arrays are (3, np).
+ * ------------------------------------------------------------------------ */
+
+/* Replaces pmlib_interp_particle_mesh(topo, xpart, part, mesh, ierr, clear,
+ * clear_ghost) (interpolation/pmlib_interp_particle_mesh.f:5-6): deposits
+ * part at xpart onto the (3, E...) mesh with the M'4 or M6* kernel of
+ * g->interp_order, then adds the periodic ghost layers back onto their
+ * images.  clear zeroes the whole mesh first; otherwise clear_ghost zeroes the
+ * ghost layers (:81-135).  Deterministic: no atomics touch the mesh. */
+int pmgpu_interp_particle_mesh(const pmgpu_geom *g, const double *xpart, const double *part, int64_t np,
+                               double *mesh, int32_t clear, int32_t clear_ghost);
+int pmgpu_interp_particle_mesh_device(const pmgpu_geom *g, const void *d_xpart, const void *d_part,
+                                      int64_t np, void *d_mesh, int32_t clear, int32_t clear_ghost,
+                                      void *stream);
+
+/* Replaces one field of pmlib_interp_mesh_particle(topo, mesh, part, ierr,
+ * vort, dvort, vel, clear) (interpolation/pmlib_interp_mesh_particle.f:5-6):
+ * out(:, q) = sum of kernel weights times mesh around pos(:, q), in the
+ * reference's term order (bit-identical to it). */
+int pmgpu_interp_mesh_particle(const pmgpu_geom *g, const double *mesh, const double *pos, int64_t np,
+                               double *out);
+int pmgpu_interp_mesh_particle_device(const pmgpu_geom *g, const void *d_mesh, const void *d_pos,
+                                      int64_t np, void *d_out, void *stream);
+
+/* Replaces pmlib_remesh(patch, topo, mesh, part, min_vort, min_dvort, ierr,
+ * dvort, vel, mask, incl_ghost) (remesh/pmlib_remesh.f:5-6): one particle at
+ * each cell centre whose |vort| > min_vort or |dvort| > min_dvort (and whose
+ * mask byte is non-zero when mask is given), created in (k, j, i) order.
+ * dvort_m may be NULL (a zero field); vel_m may be NULL unless vel_flag.
+ * dvort_flag / vel_flag copy the mesh dvort / vel onto the particles (else
+ * zero).  mask is one byte per mesh cell (E...) or NULL.  The count is
+ * returned in *np_out; the particle arrays (3, cap) are written only when pos
+ * is non-NULL and the count fits in cap (call once with pos = NULL to size
+ * them). */
+int pmgpu_remesh(const pmgpu_geom *g, const double *vort_m, const double *dvort_m, const double *vel_m,
+                 const uint8_t *mask, double min_vort, double min_dvort, int32_t dvort_flag,
+                 int32_t vel_flag, int32_t incl_ghost, double *pos, double *vort, double *vel,
+                 double *dvort, int64_t cap, int64_t *np_out);
+/* Device variant; also zeroes d_vel_rk and d_dvort_rk (either may be NULL). */
+int pmgpu_remesh_device(const pmgpu_geom *g, const void *d_vort_m, const void *d_dvort_m,
+                        const void *d_vel_m, const void *d_mask, double min_vort, double min_dvort,
+                        int32_t dvort_flag, int32_t vel_flag, int32_t incl_ghost, void *d_pos,
+                        void *d_vort, void *d_vel, void *d_dvort, void *d_vel_rk, void *d_dvort_rk,
+                        int64_t cap, int64_t *np_out, void *stream);
+
+/* Replaces pmlib_particles_advance(patch, topo, part, itimestage, dtime, ierr,
+ * euler_vort) (particles/pmlib_particles_advance.f:5-6) on one rank: the
+ * low-storage RK stage update of order rk_order (1-3) followed by the
+ * periodic wrap / free-side removal of pmlib_particles_map.  The arrays are
+ * compacted in place; the new count is returned in *np_out. */
+int pmgpu_particles_advance(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
+                            int32_t euler_vort, double *pos, double *vel, double *vort, double *dvort,
+                            double *vel_rk, double *dvort_rk, int64_t np, int64_t *np_out);
+/* Device variant; d_arrays = {pos, vel, vort, dvort, vel_rk, dvort_rk}. */
+int pmgpu_particles_advance_device(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
+                                   int32_t euler_vort, void *const d_arrays[6], int64_t np,
+                                   int64_t *np_out, void *stream);
+
+/* The particle -> cell map of every interpolation and mapping routine:
+ * idx(d, q) = NINT((pos(d, q) - xmin(d)) * (1 / dx(d)))
+ * (interpolation/pmlib_interp_particle_mesh.f:153-155). */
+int pmgpu_cell_index(const pmgpu_geom *g, const double *pos, int64_t np, int32_t *idx);
+
 #ifdef __cplusplus
 }
 #endif

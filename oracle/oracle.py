@@ -81,7 +81,8 @@ def lib():
         L.or_p2m.restype = C.c_int
         L.or_m2p.argtypes = [G, P, P, C.c_int64, P]
         L.or_m2p.restype = C.c_int
-        L.or_remesh.argtypes = [G, P, P, P, C.c_double, C.c_double, C.c_int, P, P, P, P, C.c_int64]
+        L.or_remesh.argtypes = [G, P, P, P, P, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                P, P, P, P, C.c_int64]
         L.or_remesh.restype = C.c_int64
         L.or_advance.argtypes = [G, C.c_int, C.c_int, C.c_double, C.c_int, P, P, P, P, P, P, C.c_int64]
         L.or_advance.restype = C.c_int64
@@ -169,15 +170,19 @@ def m2p(g, mesh, pos):
     return out
 
 
-def remesh(g, vort_m, dvort_m=None, vel_m=None, min_vort=0.0, min_dvort=0.0, incl_ghost=False):
+def remesh(g, vort_m, dvort_m=None, vel_m=None, min_vort=0.0, min_dvort=0.0, incl_ghost=False,
+           mask=None, dvort=True, vel=True):
+    """pos, vort, vel, dvort of the remeshed particles; mask is a boolean
+    (Ex, Ey, Ez) array or None, dvort/vel the reference's copy flags."""
     vort_m = np.asfortranarray(vort_m, dtype=np.float64)
     dv = None if dvort_m is None else np.asfortranarray(dvort_m, dtype=np.float64)
     vl = None if vel_m is None else np.asfortranarray(vel_m, dtype=np.float64)
-    n = lib().or_remesh(g, _p(vort_m), _p(dv), _p(vl), min_vort, min_dvort, int(incl_ghost),
-                        None, None, None, None, 0)
+    mk = None if mask is None else np.asfortranarray(mask, dtype=np.uint8)
+    args = (g, _p(vort_m), _p(dv), _p(vl), _p(mk), min_vort, min_dvort, int(dvort), int(vel),
+            int(incl_ghost))
+    n = lib().or_remesh(*args, None, None, None, None, 0)
     out = [np.zeros((3, n), order="F") for _ in range(4)]
-    lib().or_remesh(g, _p(vort_m), _p(dv), _p(vl), min_vort, min_dvort, int(incl_ghost),
-                    *[_p(a) for a in out], n)
+    lib().or_remesh(*args, *[_p(a) for a in out], n)
     return out  # pos, vort, vel, dvort
 
 

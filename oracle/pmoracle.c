@@ -18,7 +18,7 @@
  *                      mesh/pmlib_mesh_map_ghost.f (forward ghost fill)
  *   or_p2m             interpolation/pmlib_interp_particle_mesh.f:137-1134
  *   or_m2p             interpolation/pmlib_interp_mesh_particle.f:76-1266
- *   or_remesh          remesh/pmlib_remesh.f:84-226
+ *   or_remesh          remesh/pmlib_remesh.f:84-226 (incl. mask, dvort, vel)
  *   or_advance         particles/pmlib_particles_advance.f:49-113 +
  *                      particles/pmlib_particles_map.f:57-133 (one rank)
  *   or_penalise        vortex_codes/vortex_penalisation/source/penalisation/
@@ -633,6 +633,13 @@ int or_m2p(const or_geom *g, const double *mesh, const double *pos, int64_t np, 
         }
         if (idx[d][s] > hi[d]) { w[d][s] = 0.0; idx[d][s] = hi[d]; }
       }
+    /* j5/j6 = 1-nghost(5) can lie outside the y range when the z axis has more
+     * ghost layers; their weight is zero, so the read is kept inside the mesh
+     * (the reference reads a neighbouring plane there, a finite value times 0). */
+    for (int s = 0; s < ns; ++s) {
+      if (idx[1][s] < lo[1]) idx[1][s] = lo[1];
+      if (idx[1][s] > hi[1]) idx[1][s] = hi[1];
+    }
     double acc[3] = {0.0, 0.0, 0.0};
     int first = 1;
     for (int c3 = 0; c3 < ns; ++c3)
@@ -649,11 +656,15 @@ int or_m2p(const or_geom *g, const double *mesh, const double *pos, int64_t np, 
   return 0;
 }
 
-/* pmlib_remesh: count pass then create pass over (k,j,i).  Returns the
- * particle count; pos/vort/vel/dvort (3,cap) are written when non-NULL and
- * the count fits in cap.  dvort_m may be NULL (treated as zero). */
+/* pmlib_remesh (remesh/pmlib_remesh.f:5-233): count pass then create pass
+ * over (k,j,i).  Returns the particle count; pos/vort/vel/dvort (3,cap) are
+ * written when non-NULL and the count fits in cap.  dvort_m may be NULL
+ * (treated as zero; the reference always reads mesh%dvort for the threshold,
+ * :102-105); mask (1 byte per cell) may be NULL (the mask=.FALSE. case,
+ * :107-113).  copy_dvort / copy_vel are the dvort / vel flags (:208-219). */
 int64_t or_remesh(const or_geom *g, const double *vort_m, const double *dvort_m,
-                  const double *vel_m, double min_vort, double min_dvort, int incl_ghost,
+                  const double *vel_m, const uint8_t *mask, double min_vort, double min_dvort,
+                  int copy_dvort, int copy_vel, int incl_ghost,
                   double *pos, double *vort, double *vel, double *dvort, int64_t cap) {
   int E[3], ngl[3];
   for (int d = 0; d < 3; ++d) { E[d] = mesh_ext(g, d); ngl[d] = ng_side(g, d); }
@@ -676,12 +687,13 @@ int64_t or_remesh(const or_geom *g, const double *vort_m, const double *dvort_m,
         if (dvort_m)
           ds = sqrt(dvort_m[o] * dvort_m[o] + dvort_m[o + 1] * dvort_m[o + 1] + dvort_m[o + 2] * dvort_m[o + 2]);
         if (!(place && (s > min_vort || ds > min_dvort))) continue;
+        if (mask && !mask[o / 3]) continue;
         if (pos && n < cap) {
           pos[3 * n] = px; pos[3 * n + 1] = py; pos[3 * n + 2] = pz;
           for (int c = 0; c < 3; ++c) {
             vort[3 * n + c] = vort_m[o + c];
-            if (vel) vel[3 * n + c] = vel_m ? vel_m[o + c] : 0.0;
-            if (dvort) dvort[3 * n + c] = dvort_m ? dvort_m[o + c] : 0.0;
+            if (vel) vel[3 * n + c] = (copy_vel && vel_m) ? vel_m[o + c] : 0.0;
+            if (dvort) dvort[3 * n + c] = (copy_dvort && dvort_m) ? dvort_m[o + c] : 0.0;
           }
         }
         ++n;

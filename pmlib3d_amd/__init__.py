@@ -10,8 +10,11 @@ libpmgpu.so through its C ABI; subroutines return ierr (0 or -1) and write
 """
 from .core import (Patch, Mesh, Particles, Parameters, pmlib_write, mesh_extent,
                    pmlib_poisson_setup, pmlib_poisson_solve, PoissonSolver)
+from .particles import (cell_index, pmlib_interp_particle_mesh, pmlib_interp_mesh_particle,
+                        interp_mesh_field, pmlib_remesh, pmlib_particles_advance, ParticleOps)
 from ._lib import PmgpuError, FP32, FP64
 
 __all__ = ["Patch", "Mesh", "Particles", "Parameters", "pmlib_write", "mesh_extent",
            "pmlib_poisson_setup", "pmlib_poisson_solve", "PoissonSolver", "PmgpuError",
-           "FP32", "FP64"]
+           "FP32", "FP64", "cell_index", "pmlib_interp_particle_mesh", "pmlib_interp_mesh_particle",
+           "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps"]

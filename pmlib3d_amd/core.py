@@ -82,6 +82,7 @@ class Mesh:
     vort: np.ndarray = None
     vec_pot: np.ndarray = None
     dvort: np.ndarray = None
+    patch_mask: np.ndarray = None
     pkernel_fft: object = None
 
     @staticmethod

@@ -42,6 +42,7 @@ int pmgpu_init(int32_t device) {
 
 int pmgpu_finalize(void) {
   PM_HIP(hipDeviceSynchronize());
+  pmgpu::particles_release();
   return 0;
 }
 

@@ -1,0 +1,533 @@
+// Host side of the particle kernels: workspace, scans, C ABI.
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/pmgpu.h"
+#include "particles_kernels.h"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+namespace pmgpu {
+
+// ---------------------------------------------------------------- scan
+constexpr int kScanThreads = 256, kScanItems = 16, kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int block_exclusive(int v, int *sh, int &total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = 1; o < kScanThreads; o <<= 1) {
+    int x = t >= o ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  total = sh[kScanThreads - 1];
+  int r = sh[t] - v;
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int *in, int64_t n, int *tile_sums) {
+  __shared__ int sh[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int s = 0;
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) s += in[base + i];
+  int tot;
+  block_exclusive(s, sh, tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// exclusive scan of tile sums in one block; writes the grand total at [nt]
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int *sums, int nt) {
+  __shared__ int sh[kScanThreads];
+  int carry = 0;
+  for (int b0 = 0; b0 < nt; b0 += kScanThreads) {
+    int i = b0 + threadIdx.x;
+    int v = i < nt ? sums[i] : 0;
+    int tot;
+    int ex = block_exclusive(v, sh, tot);
+    if (i < nt) sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) sums[nt] = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_final(const int *in, int64_t n, const int *tile_offs,
+                                                             int *out) {
+  __shared__ int sh[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = base + i < n ? in[base + i] : 0;
+    s += v[i];
+  }
+  int tot;
+  int ex = block_exclusive(s, sh, tot) + tile_offs[blockIdx.x];
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = ex;
+    ex += v[i];
+  }
+}
+
+struct Scratch {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    PM_HIP(hipMalloc(&p, bytes));
+    cap = bytes;
+    return 0;
+  }
+  ~Scratch() {
+    if (p) (void)hipFree(p);
+  }
+  Scratch() = default;
+  Scratch(const Scratch &) = delete;
+  Scratch &operator=(const Scratch &) = delete;
+  template <typename T> T *as() { return reinterpret_cast<T *>(p); }
+};
+
+// exclusive scan of n ints (in -> out, may alias); out[n] receives the total
+static int scan_exclusive(const int *in, int *out, int64_t n, Scratch &tmp, hipStream_t st) {
+  int nt = (int)((n + kScanTile - 1) / kScanTile);
+  if (nt < 1) nt = 1;
+  if (tmp.ensure((size_t)(nt + 1) * sizeof(int))) return -1;
+  int *sums = tmp.as<int>();
+  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, in, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, sums, nt);
+  hipLaunchKernelGGL(k_scan_final, dim3(nt), dim3(kScanThreads), 0, st, in, n, sums, out);
+  PM_HIP(hipMemcpyAsync(out + n, sums + nt, sizeof(int), hipMemcpyDeviceToDevice, st));
+  PM_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------ geometry
+static int make_pgeom(const pmgpu_geom *g, PGeom &pg) {
+  PM_CHECK(g, "pmgpu: null geometry");
+  PM_CHECK(g->interp_order == 3 || g->interp_order == 4,
+           "pmgpu: Interpolation scheme/order unknown (interp_particle_mesh.f:1118)");
+  pg.m6 = g->interp_order == 4;
+  pg.ns = pg.m6 ? 6 : 4;
+  pg.off = pg.m6 ? 2 : 1;
+  for (int d = 0; d < 3; ++d) {
+    PM_CHECK(g->ncell[d] > 0 && g->dx[d] > 0, "pmgpu: bad geometry");
+    pg.ncell[d] = g->ncell[d];
+    pg.bc[d] = g->bound_cond[d];
+    pg.ngl[d] = g->bound_cond[d] == 0 ? g->nghost : (pg.m6 ? 3 : 2);
+    pg.E[d] = g->ncell[d] + 2 * pg.ngl[d];
+    pg.lo[d] = 1 - pg.ngl[d];
+    pg.hi[d] = g->ncell[d] + pg.ngl[d];
+    pg.xmin[d] = g->xmin[d];
+    pg.xmax[d] = g->xmax[d];
+    pg.dx[d] = g->dx[d];
+    pg.c1dx[d] = 1.0 / g->dx[d];
+    pg.blo[d] = pg.lo[d] + pg.off - pg.ns + 1;
+    pg.nb[d] = (pg.hi[d] + pg.off) - pg.blo[d] + 1;
+  }
+  return 0;
+}
+
+static int64_t mesh_cells(const PGeom &g) { return (int64_t)g.E[0] * g.E[1] * g.E[2]; }
+static dim3 grid_for(int64_t n, int threads = 256) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+// Workspace shared by the particle entry points of one process.
+struct PartWS {
+  Scratch key, ids, sort, count, start, scan, flag, offs, cls, tmp6;
+};
+// Allocated on first use and released by pmgpu_finalize (never from a static
+// destructor, which may run after the HIP runtime has shut down).
+static PartWS *g_ws = nullptr;
+static PartWS &ws() {
+  if (!g_ws) g_ws = new PartWS;
+  return *g_ws;
+}
+
+void particles_release() {
+  delete g_ws;
+  g_ws = nullptr;
+}
+
+// ---------------------------------------------------------------- P2M
+int p2m_device(const pmgpu_geom *geom, const double *xpart, const double *part, int64_t np, double *mesh,
+               int clear, int clear_ghost, hipStream_t st) {
+  PGeom g;
+  if (make_pgeom(geom, g)) return -1;
+  for (int d = 0; d < 3; ++d)
+    PM_CHECK(g.bc[d] == 0 || g.ncell[d] >= 2 * g.ngl[d], "pmgpu: periodic axis shorter than two ghost layers");
+  PartWS &w = ws();
+  const int nbins = g.nb[0] * g.nb[1] * g.nb[2];
+  const int64_t nn = mesh_cells(g);
+  PM_CHECK(np >= 0 && np < (int64_t)1 << 31, "pmgpu: particle count out of range");
+  PM_CHECK((int64_t)nbins < ((int64_t)1 << 30), "pmgpu: mesh too large for deposition");
+  int end_bit = 1;
+  while ((1 << end_bit) <= nbins) ++end_bit;
+  const int n = (int)np;
+  size_t sort_bytes = 0;
+  PM_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (int *)nullptr, (int *)nullptr, (int *)nullptr,
+                                   (int *)nullptr, n, 0, end_bit, st));
+  if (w.key.ensure(sizeof(int) * 2 * ((size_t)np + 1)) || w.ids.ensure(sizeof(int) * 2 * ((size_t)np + 1)) ||
+      w.count.ensure(sizeof(int) * (nbins + 1)) || w.start.ensure(sizeof(int) * (nbins + 1)) ||
+      w.sort.ensure(sort_bytes + 16))
+    return -1;
+  int *key_in = w.key.as<int>(), *key_out = key_in + np + 1;
+  int *id_in = w.ids.as<int>(), *id_out = id_in + np + 1;
+  PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * (nbins + 1), st));
+  if (np > 0) {
+    hipLaunchKernelGGL(k_p2m_keys, grid_for(np), dim3(256), 0, st, g, xpart, np, nbins, key_in, id_in,
+                       w.count.as<int>());
+    PM_HIP(hipGetLastError());
+    PM_HIP(rocprim::radix_sort_pairs(w.sort.p, sort_bytes, key_in, key_out, id_in, id_out, n, 0, end_bit, st));
+  }
+  if (scan_exclusive(w.count.as<int>(), w.start.as<int>(), nbins, w.scan, st)) return -1;
+  int accumulate = clear ? 0 : 1;
+  hipLaunchKernelGGL(k_p2m_gather, grid_for(nn), dim3(256), 0, st, g, xpart, part, w.start.as<int>(), id_out,
+                     mesh, accumulate);
+  PM_HIP(hipGetLastError());
+  for (int d = 0; d < 3; ++d)
+    if (g.bc[d] != 0) {
+      hipLaunchKernelGGL(k_halo_add, grid_for(2LL * g.ngl[d] * 3 * g.E[(d + 1) % 3] * g.E[(d + 2) % 3]), dim3(256),
+                         0, st, mesh, g, d);
+      PM_HIP(hipGetLastError());
+    }
+  return 0;
+}
+
+__global__ void k_zero_ghosts(PGeom g, double *mesh) {
+  const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(idx % g.E[0]) + g.lo[0];
+    int j = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
+    int k = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
+    bool interior = i >= 1 && i <= g.ncell[0] && j >= 1 && j <= g.ncell[1] && k >= 1 && k <= g.ncell[2];
+    if (!interior) mesh[3 * idx] = mesh[3 * idx + 1] = mesh[3 * idx + 2] = 0.0;
+  }
+}
+
+int p2m_device_full(const pmgpu_geom *geom, const double *xpart, const double *part, int64_t np, double *mesh,
+                    int clear, int clear_ghost, hipStream_t st) {
+  if (!clear && clear_ghost) {
+    PGeom g;
+    if (make_pgeom(geom, g)) return -1;
+    hipLaunchKernelGGL(k_zero_ghosts, grid_for(mesh_cells(g)), dim3(256), 0, st, g, mesh);
+    PM_HIP(hipGetLastError());
+  }
+  return p2m_device(geom, xpart, part, np, mesh, clear, 0, st);
+}
+
+// ---------------------------------------------------------------- M2P
+int m2p_device(const pmgpu_geom *geom, const double *mesh, const double *pos, int64_t np, double *out,
+               hipStream_t st) {
+  PGeom g;
+  if (make_pgeom(geom, g)) return -1;
+  if (np > 0) hipLaunchKernelGGL(k_m2p, grid_for(np), dim3(256), 0, st, g, mesh, pos, np, out);
+  PM_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------- remesh
+int remesh_device(const pmgpu_geom *geom, const double *vort_m, const double *dvort_m, const double *vel_m,
+                  const uint8_t *mask, double min_vort, double min_dvort, int copy_dvort, int copy_vel,
+                  int incl_ghost, double *pos, double *vort, double *vel, double *dvort, double *vel_rk,
+                  double *dvort_rk, int64_t cap, int64_t *np_out, hipStream_t st) {
+  PGeom g;
+  if (make_pgeom(geom, g)) return -1;
+  PartWS &w = ws();
+  const int64_t nn = mesh_cells(g);
+  PM_CHECK(nn < (int64_t)1 << 31, "pmgpu: mesh too large for remeshing");
+  if (w.flag.ensure(sizeof(int) * (nn + 1)) || w.offs.ensure(sizeof(int) * (nn + 1))) return -1;
+  PM_CHECK(vort_m, "pmgpu_remesh: null vorticity mesh");
+  PM_CHECK(!copy_vel || vel_m, "pmgpu_remesh: vel requested without a velocity mesh");
+  hipLaunchKernelGGL(k_remesh_flags, grid_for(nn), dim3(256), 0, st, g, vort_m, dvort_m, mask, min_vort, min_dvort,
+                     incl_ghost, w.flag.as<int>());
+  if (scan_exclusive(w.flag.as<int>(), w.offs.as<int>(), nn, w.scan, st)) return -1;
+  int total = 0;
+  PM_HIP(hipMemcpyAsync(&total, w.offs.as<int>() + nn, sizeof(int), hipMemcpyDeviceToHost, st));
+  PM_HIP(hipStreamSynchronize(st));
+  *np_out = total;
+  if (pos == nullptr || total > cap) return 0;  // count only
+  hipLaunchKernelGGL(k_remesh_create, grid_for(nn), dim3(256), 0, st, g, w.flag.as<int>(), w.offs.as<int>(),
+                     vort_m, copy_dvort ? dvort_m : nullptr, copy_vel ? vel_m : nullptr, pos, vort, vel, dvort,
+                     vel_rk, dvort_rk);
+  PM_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------ advance
+__global__ void k_class_keys(const int *cls, int64_t np, int want, int *key) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x)
+    key[q] = cls[q] == want ? 1 : 0;
+}
+
+// destination of particle q: west movers, then east movers, then stayers,
+// each in their original order; dropped particles are not copied
+__global__ void k_migrate(int64_t np, const int *cls, const int *o1, const int *o2, const int *o0, int n1, int n2,
+                          int d, double len, double *const *src, double *const *dst, int narr) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    int c = cls[q];
+    if (c == 3) continue;
+    int64_t to = c == 1 ? o1[q] : (c == 2 ? n1 + o2[q] : n1 + n2 + o0[q]);
+    for (int a = 0; a < narr; ++a) {
+      double v0 = src[a][3 * q], v1 = src[a][3 * q + 1], v2 = src[a][3 * q + 2];
+      if (a == 0) {
+        double &x = d == 0 ? v0 : (d == 1 ? v1 : v2);
+        if (c == 1) x = x + len;
+        if (c == 2) x = x - len;
+      }
+      dst[a][3 * to] = v0;
+      dst[a][3 * to + 1] = v1;
+      dst[a][3 * to + 2] = v2;
+    }
+  }
+}
+
+static RkCoef rk_coef(int order, int stage, int euler_vort) {
+  double A1[3] = {0, 0, 0}, B1[3] = {1, 0, 0}, C1[3] = {1, 0, 0};
+  double A2[3] = {0, 0, 0}, B2[3] = {1, 0, 0}, C2[3] = {1, 0, 0};
+  if (order == 2) {
+    A1[1] = -0.5; B1[1] = 1.0; C1[0] = 0.5; C1[1] = 1.0;
+    if (euler_vort) { A2[1] = 1.0; B2[1] = 0.0; C2[0] = 0.5; C2[1] = 0.5; }
+    else { A2[1] = -0.5; B2[1] = 1.0; C2[0] = 0.5; C2[1] = 1.0; }
+  } else if (order == 3) {
+    A1[1] = -5.0 / 9.0; A1[2] = -153.0 / 128.0; B1[1] = B1[2] = 1.0;
+    C1[0] = 1.0 / 3.0; C1[1] = 15.0 / 16.0; C1[2] = 8.0 / 15.0;
+    if (euler_vort) { A2[1] = A2[2] = 1.0; B2[1] = B2[2] = 0.0; C2[0] = C2[1] = C2[2] = 1.0 / 3.0; }
+    else {
+      A2[1] = -5.0 / 9.0; A2[2] = -153.0 / 128.0; B2[1] = B2[2] = 1.0;
+      C2[0] = 1.0 / 3.0; C2[1] = 15.0 / 16.0; C2[2] = 8.0 / 15.0;
+    }
+  }
+  int s = stage - 1;
+  return RkCoef{A1[s], B1[s], C1[s], A2[s], B2[s], C2[s]};
+}
+
+// arrays: pos, vel, vort, dvort, vel_rk, dvort_rk (3, np) on the device; the
+// survivors are compacted in place (through a scratch copy)
+int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, int euler_vort, double **arr,
+                   int64_t np, int64_t *np_out, hipStream_t st) {
+  PM_CHECK(rk_order >= 1 && rk_order <= 3, "pmgpu: time integration order must be 1, 2 or 3");
+  PM_CHECK(stage >= 1 && stage <= rk_order, "pmgpu: RK stage out of range");
+  PM_CHECK(np >= 0 && np < (int64_t)1 << 31, "pmgpu: particle count out of range");
+  RkCoef k = rk_coef(rk_order, stage, euler_vort);
+  if (np > 0)
+    hipLaunchKernelGGL(k_rk_update, grid_for(3 * np), dim3(256), 0, st, k, dt, np, arr[0], arr[1], arr[2], arr[3],
+                       arr[4], arr[5]);
+  PM_HIP(hipGetLastError());
+  PartWS &w = ws();
+  int64_t n = np;
+  for (int d = 0; d < 3 && n > 0; ++d) {
+    const double lo = geom->xmin[d], hi = geom->xmax[d], len = geom->xmax[d] - geom->xmin[d];
+    const int per = geom->bound_cond[d] != 0;
+    for (int pass = 0; pass < 64; ++pass) {
+      if (w.cls.ensure(sizeof(int) * (n + 1) * 5) || w.count.ensure(sizeof(int) * 8)) return -1;
+      int *cls = w.cls.as<int>();
+      int *o1 = cls + (n + 1), *o2 = o1 + (n + 1), *o0 = o2 + (n + 1), *kk = o0 + (n + 1);
+      PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * 4, st));
+      hipLaunchKernelGGL(k_classify, grid_for(n), dim3(256), 0, st, n, arr[0], d, lo, hi, per, cls,
+                         w.count.as<int>());
+      int counts[4];
+      PM_HIP(hipMemcpyAsync(counts, w.count.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+      PM_HIP(hipStreamSynchronize(st));
+      if (counts[1] == 0 && counts[2] == 0 && counts[3] == 0) break;  // nobody moves
+      int *outs[3] = {o1, o2, o0};
+      int wants[3] = {1, 2, 0};
+      for (int c = 0; c < 3; ++c) {
+        hipLaunchKernelGGL(k_class_keys, grid_for(n), dim3(256), 0, st, cls, n, wants[c], kk);
+        if (scan_exclusive(kk, outs[c], n, w.scan, st)) return -1;
+      }
+      int64_t nnew = n - counts[3];
+      if (w.tmp6.ensure(sizeof(double) * 3 * (n + 1) * 6 + sizeof(double *) * 12)) return -1;
+      double *base = w.tmp6.as<double>();
+      double *dst[6];
+      for (int a = 0; a < 6; ++a) dst[a] = base + (size_t)a * 3 * (n + 1);
+      double **ptrs = reinterpret_cast<double **>(base + (size_t)6 * 3 * (n + 1));
+      double *hp[12];
+      for (int a = 0; a < 6; ++a) {
+        hp[a] = arr[a];
+        hp[6 + a] = dst[a];
+      }
+      PM_HIP(hipMemcpyAsync(ptrs, hp, sizeof(hp), hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_migrate, grid_for(n), dim3(256), 0, st, n, cls, o1, o2, o0, counts[1], counts[2], d, len,
+                         ptrs, ptrs + 6, 6);
+      PM_HIP(hipGetLastError());
+      for (int a = 0; a < 6; ++a)
+        PM_HIP(hipMemcpyAsync(arr[a], dst[a], sizeof(double) * 3 * nnew, hipMemcpyDeviceToDevice, st));
+      PM_HIP(hipStreamSynchronize(st));
+      n = nnew;
+      if (!per || counts[1] + counts[2] == 0) break;
+    }
+  }
+  *np_out = n;
+  return 0;
+}
+
+int cell_index_device(const pmgpu_geom *geom, const double *pos, int64_t np, int32_t *out, hipStream_t st) {
+  PGeom g;
+  if (make_pgeom(geom, g)) return -1;
+  if (np > 0) hipLaunchKernelGGL(k_cell_index, grid_for(3 * np), dim3(256), 0, st, g, pos, np, out);
+  PM_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace pmgpu
+
+using namespace pmgpu;
+
+// ------------------------------------------------------------------ C ABI
+namespace {
+struct Dev {
+  void *p = nullptr;
+  ~Dev() {
+    if (p) (void)hipFree(p);
+  }
+};
+int to_dev(Dev &d, const void *h, size_t bytes) {
+  PM_HIP(hipMalloc(&d.p, bytes > 0 ? bytes : 8));
+  if (h && bytes) PM_HIP(hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+int mesh_bytes(const pmgpu_geom *g, size_t &b) {
+  int32_t e[3];
+  if (pmgpu_mesh_extent(g, e)) return -1;
+  b = sizeof(double) * 3 * (size_t)e[0] * e[1] * e[2];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int pmgpu_interp_particle_mesh(const pmgpu_geom *g, const double *xpart, const double *part, int64_t np,
+                               double *mesh, int32_t clear, int32_t clear_ghost) {
+  PM_CHECK(g && mesh && (np == 0 || (xpart && part)), "pmgpu_interp_particle_mesh: null argument");
+  size_t mb;
+  if (mesh_bytes(g, mb)) return -1;
+  Dev dp, dv, dm;
+  if (to_dev(dp, xpart, sizeof(double) * 3 * np) || to_dev(dv, part, sizeof(double) * 3 * np) ||
+      to_dev(dm, clear ? nullptr : mesh, mb))
+    return -1;
+  if (p2m_device_full(g, (double *)dp.p, (double *)dv.p, np, (double *)dm.p, clear, clear_ghost, nullptr)) return -1;
+  PM_HIP(hipMemcpy(mesh, dm.p, mb, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmgpu_interp_particle_mesh_device(const pmgpu_geom *g, const void *d_xpart, const void *d_part, int64_t np,
+                                      void *d_mesh, int32_t clear, int32_t clear_ghost, void *stream) {
+  return p2m_device_full(g, (const double *)d_xpart, (const double *)d_part, np, (double *)d_mesh, clear,
+                         clear_ghost, (hipStream_t)stream);
+}
+
+int pmgpu_interp_mesh_particle(const pmgpu_geom *g, const double *mesh, const double *pos, int64_t np, double *out) {
+  PM_CHECK(g && mesh && (np == 0 || (pos && out)), "pmgpu_interp_mesh_particle: null argument");
+  size_t mb;
+  if (mesh_bytes(g, mb)) return -1;
+  Dev dm, dp, dout;
+  if (to_dev(dm, mesh, mb) || to_dev(dp, pos, sizeof(double) * 3 * np) || to_dev(dout, nullptr, sizeof(double) * 3 * np))
+    return -1;
+  if (m2p_device(g, (double *)dm.p, (double *)dp.p, np, (double *)dout.p, nullptr)) return -1;
+  if (np) PM_HIP(hipMemcpy(out, dout.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmgpu_interp_mesh_particle_device(const pmgpu_geom *g, const void *d_mesh, const void *d_pos, int64_t np,
+                                      void *d_out, void *stream) {
+  return m2p_device(g, (const double *)d_mesh, (const double *)d_pos, np, (double *)d_out, (hipStream_t)stream);
+}
+
+int pmgpu_remesh(const pmgpu_geom *g, const double *vort_m, const double *dvort_m, const double *vel_m,
+                 const uint8_t *mask, double min_vort, double min_dvort, int32_t dvort_flag, int32_t vel_flag,
+                 int32_t incl_ghost, double *pos, double *vort, double *vel, double *dvort, int64_t cap,
+                 int64_t *np_out) {
+  PM_CHECK(g && vort_m && np_out, "pmgpu_remesh: null argument");
+  size_t mb;
+  if (mesh_bytes(g, mb)) return -1;
+  Dev dw, ddw, dvl, dmk;
+  if (to_dev(dw, vort_m, mb) || (dvort_m && to_dev(ddw, dvort_m, mb)) || (vel_m && to_dev(dvl, vel_m, mb)) ||
+      (mask && to_dev(dmk, mask, mb / 24)))
+    return -1;
+  int64_t n = 0;
+  if (remesh_device(g, (double *)dw.p, (double *)ddw.p, (double *)dvl.p, (uint8_t *)dmk.p, min_vort, min_dvort,
+                    dvort_flag, vel_flag, incl_ghost, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, &n,
+                    nullptr))
+    return -1;
+  *np_out = n;
+  if (!pos || n > cap) return 0;
+  Dev op, ov, ol, od;
+  size_t pb = sizeof(double) * 3 * (size_t)n;
+  if (to_dev(op, nullptr, pb) || to_dev(ov, nullptr, pb) || to_dev(ol, nullptr, pb) || to_dev(od, nullptr, pb))
+    return -1;
+  if (remesh_device(g, (double *)dw.p, (double *)ddw.p, (double *)dvl.p, (uint8_t *)dmk.p, min_vort, min_dvort,
+                    dvort_flag, vel_flag, incl_ghost, (double *)op.p, (double *)ov.p, (double *)ol.p, (double *)od.p,
+                    nullptr, nullptr, n, &n, nullptr))
+    return -1;
+  if (n) {
+    PM_HIP(hipMemcpy(pos, op.p, pb, hipMemcpyDeviceToHost));
+    PM_HIP(hipMemcpy(vort, ov.p, pb, hipMemcpyDeviceToHost));
+    if (vel) PM_HIP(hipMemcpy(vel, ol.p, pb, hipMemcpyDeviceToHost));
+    if (dvort) PM_HIP(hipMemcpy(dvort, od.p, pb, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int pmgpu_remesh_device(const pmgpu_geom *g, const void *d_vort_m, const void *d_dvort_m, const void *d_vel_m,
+                        const void *d_mask, double min_vort, double min_dvort, int32_t dvort_flag, int32_t vel_flag,
+                        int32_t incl_ghost, void *d_pos, void *d_vort, void *d_vel, void *d_dvort, void *d_vel_rk,
+                        void *d_dvort_rk, int64_t cap, int64_t *np_out, void *stream) {
+  PM_CHECK(np_out, "pmgpu_remesh_device: null argument");
+  return remesh_device(g, (const double *)d_vort_m, (const double *)d_dvort_m, (const double *)d_vel_m,
+                       (const uint8_t *)d_mask, min_vort, min_dvort, dvort_flag, vel_flag, incl_ghost,
+                       (double *)d_pos, (double *)d_vort, (double *)d_vel, (double *)d_dvort, (double *)d_vel_rk,
+                       (double *)d_dvort_rk, cap, np_out, (hipStream_t)stream);
+}
+
+int pmgpu_particles_advance(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt, int32_t euler_vort,
+                            double *pos, double *vel, double *vort, double *dvort, double *vel_rk, double *dvort_rk,
+                            int64_t np, int64_t *np_out) {
+  PM_CHECK(g && np_out && (np == 0 || (pos && vel && vort && dvort && vel_rk && dvort_rk)),
+           "pmgpu_particles_advance: null argument");
+  double *h[6] = {pos, vel, vort, dvort, vel_rk, dvort_rk};
+  Dev d[6];
+  double *dp[6];
+  size_t pb = sizeof(double) * 3 * (size_t)np;
+  for (int a = 0; a < 6; ++a) {
+    if (to_dev(d[a], h[a], pb)) return -1;
+    dp[a] = (double *)d[a].p;
+  }
+  int64_t n = 0;
+  if (advance_device(g, rk_order, stage, dt, euler_vort, dp, np, &n, nullptr)) return -1;
+  for (int a = 0; a < 6; ++a)
+    if (n) PM_HIP(hipMemcpy(h[a], dp[a], sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost));
+  *np_out = n;
+  return 0;
+}
+
+int pmgpu_particles_advance_device(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
+                                   int32_t euler_vort, void *const d_arrays[6], int64_t np, int64_t *np_out,
+                                   void *stream) {
+  PM_CHECK(g && d_arrays && np_out, "pmgpu_particles_advance_device: null argument");
+  double *dp[6];
+  for (int a = 0; a < 6; ++a) dp[a] = (double *)d_arrays[a];
+  return advance_device(g, rk_order, stage, dt, euler_vort, dp, np, np_out, (hipStream_t)stream);
+}
+
+int pmgpu_cell_index(const pmgpu_geom *g, const double *pos, int64_t np, int32_t *idx) {
+  PM_CHECK(g && (np == 0 || (pos && idx)), "pmgpu_cell_index: null argument");
+  Dev dp, di;
+  if (to_dev(dp, pos, sizeof(double) * 3 * np) || to_dev(di, nullptr, sizeof(int32_t) * 3 * np)) return -1;
+  if (cell_index_device(g, (double *)dp.p, np, (int32_t *)di.p, nullptr)) return -1;
+  if (np) PM_HIP(hipMemcpy(idx, di.p, sizeof(int32_t) * 3 * np, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,334 @@
+// Particle <-> mesh kernels (HIP, gfx950): M'4 / M6* deposition and gather,
+// remeshing and the low-storage RK update with one-rank migration.
+//
+// Deposition is a gather, not a scatter: particles are binned by the cell of
+// their stencil base index (bit-exact NINT), a stable radix sort orders
+// them by bin and, inside a bin, by particle id, and every mesh node sums
+// the contributions of the ns^3 bins whose stencils cover it, in a fixed
+// order.  No atomics touch the mesh, so the result is deterministic run to
+// run.
+#pragma once
+#include "pm_common.h"
+
+namespace pmgpu {
+
+struct PGeom {
+  int ncell[3], ngl[3], bc[3];
+  int E[3];           // mesh extent incl. ghosts
+  int lo[3], hi[3];   // node index range 1-ng .. N+ng
+  double xmin[3], xmax[3], dx[3], c1dx[3];
+  int ns, off;        // stencil width and base offset (M'4: 4,1; M6*: 6,2)
+  int m6;
+  int nb[3];          // bins per axis: base index b in [lo + off - ns + 1, hi + off]
+  int blo[3];         // b of bin 0
+};
+
+// NINT((x - xmin) * c_1_dx): correctly rounded subtract and multiply (no
+// contraction), then round half away from zero.
+__device__ __forceinline__ int cell_index(double x, double xmin, double c1) {
+  double t = __dmul_rn(__dsub_rn(x, xmin), c1);
+  return (int)round(t);
+}
+
+// normalised offset d = (x - xmin - h (b - 1/2)) * nrm
+__device__ __forceinline__ double stencil_offset(double x, double xmin, double h, int b, double nrm) {
+  double t = __dmul_rn(h, __dsub_rn((double)b, 0.5));
+  return __dmul_rn(__dsub_rn(__dsub_rn(x, xmin), t), nrm);
+}
+
+// weight of stencil slot s for offset d (interp_particle_mesh.f:182-214, :565-602)
+__device__ __forceinline__ double weight(int m6, int s, double d) {
+  if (!m6) {
+    switch (s) {
+      case 0: { double u = __dadd_rn(d, 1.0), a = __dsub_rn(2.0, u);
+                return __dmul_rn(__dmul_rn(0.5, __dmul_rn(a, a)), __dsub_rn(1.0, u)); }
+      case 1: { double d2 = __dmul_rn(d, d), d3 = __dmul_rn(d2, d);
+                return __dadd_rn(__dsub_rn(1.0, __dmul_rn(2.5, d2)), __dmul_rn(1.5, d3)); }
+      case 2: { double u = __dsub_rn(1.0, d), u2 = __dmul_rn(u, u), u3 = __dmul_rn(u2, u);
+                return __dadd_rn(__dsub_rn(1.0, __dmul_rn(2.5, u2)), __dmul_rn(1.5, u3)); }
+      default: { double u = __dsub_rn(2.0, d), a = __dsub_rn(2.0, u);
+                 return __dmul_rn(__dmul_rn(0.5, __dmul_rn(a, a)), __dsub_rn(1.0, u)); }
+    }
+  }
+  const double c1 = -1.0 / 24.0, c2 = 1.0 / 24.0, c3 = -1.0 / 12.0;
+  double x;
+  switch (s) {
+    case 0: x = __dadd_rn(d, 2.0); break;
+    case 1: x = __dadd_rn(d, 1.0); break;
+    case 2: x = d; break;
+    case 3: x = __dsub_rn(1.0, d); break;
+    case 4: x = __dsub_rn(2.0, d); break;
+    default: x = __dsub_rn(3.0, d); break;
+  }
+  double x2 = __dmul_rn(x, x), x3 = __dmul_rn(x2, x), x4 = __dmul_rn(x3, x);
+  if (s == 0 || s == 5) {
+    double t = __dsub_rn(x, 3.0);
+    double t3 = __dmul_rn(__dmul_rn(t, t), t);
+    return __dmul_rn(__dmul_rn(__dmul_rn(c1, __dsub_rn(x, 2.0)), t3), __dsub_rn(__dmul_rn(5.0, x), 8.0));
+  }
+  if (s == 1 || s == 4) {
+    double p = __dsub_rn(__dadd_rn(__dsub_rn(__dmul_rn(25.0, x3), __dmul_rn(114.0, x2)), __dmul_rn(153.0, x)), 48.0);
+    return __dmul_rn(__dmul_rn(__dmul_rn(c2, __dsub_rn(x, 1.0)), __dsub_rn(x, 2.0)), p);
+  }
+  double p = __dadd_rn(__dadd_rn(__dsub_rn(__dsub_rn(__dmul_rn(25.0, x4), __dmul_rn(38.0, x3)), __dmul_rn(3.0, x2)),
+                                 __dmul_rn(12.0, x)), 12.0);
+  return __dmul_rn(__dmul_rn(c3, __dsub_rn(x, 1.0)), p);
+}
+
+// bin key of a particle (nbins: no stencil node touches the mesh, or the
+// particle lies outside a periodic axis, where the reference zeroes every
+// weight); ids[q] = q feeds the stable key sort
+__global__ void k_p2m_keys(PGeom g, const double *pos, int64_t np, int nbins, int *key, int *ids, int *count) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    int b[3];
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) {
+      double x = pos[3 * q + d];
+      b[d] = cell_index(x, g.xmin[d], g.c1dx[d]);
+      const int64_t bb = b[d];
+      if (bb - g.off + g.ns - 1 < g.lo[d] || bb - g.off > g.hi[d]) ok = false;
+      if (g.bc[d] != 0 && (x < g.xmin[d] || x > g.xmax[d])) ok = false;
+    }
+    int k = nbins;
+    if (ok) {
+      k = (b[0] - g.blo[0]) + g.nb[0] * ((b[1] - g.blo[1]) + g.nb[1] * (b[2] - g.blo[2]));
+      atomicAdd(&count[k], 1);
+    }
+    key[q] = k;
+    ids[q] = (int)q;
+  }
+}
+
+// one thread per mesh node: sum over the bins whose stencils cover it
+__global__ void k_p2m_gather(PGeom g, const double *pos, const double *val, const int *start, const int *ids,
+                             double *mesh, int accumulate) {
+  const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int n[3];
+    n[0] = (int)(idx % g.E[0]) + g.lo[0];
+    n[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
+    n[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    // node n is slot s = n - b + off of base b, s in [0, ns)
+    for (int sz = g.ns - 1; sz >= 0; --sz) {
+      int bz = n[2] - sz + g.off;
+      int kz = bz - g.blo[2];
+      if (kz < 0 || kz >= g.nb[2]) continue;
+      for (int sy = g.ns - 1; sy >= 0; --sy) {
+        int by = n[1] - sy + g.off;
+        int ky = by - g.blo[1];
+        if (ky < 0 || ky >= g.nb[1]) continue;
+        for (int sx = g.ns - 1; sx >= 0; --sx) {
+          int bx = n[0] - sx + g.off;
+          int kx = bx - g.blo[0];
+          if (kx < 0 || kx >= g.nb[0]) continue;
+          int k = kx + g.nb[0] * (ky + g.nb[1] * kz);
+          for (int t = start[k]; t < start[k + 1]; ++t) {
+            int q = ids[t];
+            double wx = weight(g.m6, sx, stencil_offset(pos[3 * q], g.xmin[0], g.dx[0], bx, g.c1dx[0]));
+            double wy = weight(g.m6, sy, stencil_offset(pos[3 * q + 1], g.xmin[1], g.dx[1], by, g.c1dx[1]));
+            double wz = weight(g.m6, sz, stencil_offset(pos[3 * q + 2], g.xmin[2], g.dx[2], bz, g.c1dx[2]));
+            double w = __dmul_rn(__dmul_rn(wx, wy), wz);
+            acc0 = __dadd_rn(acc0, __dmul_rn(w, val[3 * q]));
+            acc1 = __dadd_rn(acc1, __dmul_rn(w, val[3 * q + 1]));
+            acc2 = __dadd_rn(acc2, __dmul_rn(w, val[3 * q + 2]));
+          }
+        }
+      }
+    }
+    double *m = mesh + 3 * idx;
+    if (accumulate) {
+      m[0] += acc0;
+      m[1] += acc1;
+      m[2] += acc2;
+    } else {
+      m[0] = acc0;
+      m[1] = acc1;
+      m[2] = acc2;
+    }
+  }
+}
+
+// Reverse periodic halo along axis d (mesh/pmlib_mesh_map_ghost.f with
+// incl_edges = .TRUE., op = add): cells [0, 2ng) and [N, N + 2ng) exchange
+// and add, both read before either is written.
+__global__ void k_halo_add(double *m, PGeom g, int d) {
+  const int E0 = g.E[0], E1 = g.E[1];
+  int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+  int64_t st[3] = {3, 3 * (int64_t)E0, 3 * (int64_t)E0 * E1};
+  const int ng = g.ngl[d], N = g.ncell[d];
+  int64_t n = (int64_t)g.E[o1] * g.E[o2] * 3 * 2 * ng;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int c = idx % 3;
+    int64_t r = idx / 3;
+    int t = r % (2 * ng);
+    r /= 2 * ng;
+    int a1 = r % g.E[o1];
+    int a2 = r / g.E[o1];
+    int64_t base = c + st[o1] * a1 + st[o2] * a2;
+    double lo = m[base + st[d] * t], hi = m[base + st[d] * (N + t)];
+    m[base + st[d] * t] = lo + hi;
+    m[base + st[d] * (N + t)] = hi + lo;
+  }
+}
+
+// M2P: one thread per particle, the reference's term order
+// (interp_mesh_particle.f:87-409; the M'4 branch scales the y and z offsets
+// by 1/dx(1), the M6* branch reproduces the k5/k6 quirk of :587-590).
+__global__ void k_m2p(PGeom g, const double *mesh, const double *pos, int64_t np, double *out) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    int idx[3][6];
+    double w[3][6];
+    for (int d = 0; d < 3; ++d) {
+      double x = pos[3 * q + d];
+      int b = cell_index(x, g.xmin[d], g.c1dx[d]);
+      double nrm = g.m6 ? g.c1dx[d] : g.c1dx[0];
+      double dd = stencil_offset(x, g.xmin[d], g.dx[d], b, nrm);
+      for (int s = 0; s < g.ns; ++s) {
+        w[d][s] = weight(g.m6, s, dd);
+        idx[d][s] = b - g.off + s;
+      }
+    }
+    for (int d = 0; d < 3; ++d)
+      for (int s = 0; s < g.ns; ++s) {
+        if (idx[d][s] < g.lo[d]) {
+          if (g.m6 && d == 2 && s >= 4) {
+            w[1][s] = 0.0;
+            idx[1][s] = g.lo[2];
+          }
+          w[d][s] = 0.0;
+          idx[d][s] = g.lo[d];
+        }
+        if (idx[d][s] > g.hi[d]) {
+          w[d][s] = 0.0;
+          idx[d][s] = g.hi[d];
+        }
+      }
+    // The quirk can leave a y index outside the y range (when the z axis has
+    // more ghost layers); its weight is zero, so the read is clamped into the
+    // mesh instead of leaving the allocation.
+    for (int s = 0; s < g.ns; ++s) idx[1][s] = min(max(idx[1][s], g.lo[1]), g.hi[1]);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    bool first = true;
+    for (int sz = 0; sz < g.ns; ++sz)
+      for (int sy = 0; sy < g.ns; ++sy)
+        for (int sx = 0; sx < g.ns; ++sx) {
+          double wt = __dmul_rn(__dmul_rn(w[0][sx], w[1][sy]), w[2][sz]);
+          int64_t o = 3 * ((int64_t)(idx[0][sx] - g.lo[0]) +
+                           (int64_t)g.E[0] * ((idx[1][sy] - g.lo[1]) + (int64_t)g.E[1] * (idx[2][sz] - g.lo[2])));
+          double t0 = __dmul_rn(wt, mesh[o]), t1 = __dmul_rn(wt, mesh[o + 1]), t2 = __dmul_rn(wt, mesh[o + 2]);
+          if (first) {
+            a0 = t0;
+            a1 = t1;
+            a2 = t2;
+            first = false;
+          } else {
+            a0 = __dadd_rn(a0, t0);
+            a1 = __dadd_rn(a1, t1);
+            a2 = __dadd_rn(a2, t2);
+          }
+        }
+    out[3 * q] = a0;
+    out[3 * q + 1] = a1;
+    out[3 * q + 2] = a2;
+  }
+}
+
+__global__ void k_cell_index(PGeom g, const double *pos, int64_t np, int *out) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < 3 * np; q += (int64_t)gridDim.x * blockDim.x) {
+    int d = q % 3;
+    out[q] = cell_index(pos[q], g.xmin[d], g.c1dx[d]);
+  }
+}
+
+// ------------------------------------------------------------ remesh
+// flag of mesh cell idx (remesh/pmlib_remesh.f:104-125)
+__device__ __forceinline__ int remesh_flag(const PGeom &g, int64_t idx, const double *vort, const double *dvort,
+                                           const uint8_t *mask, double min_vort, double min_dvort,
+                                           int incl_ghost) {
+  int ii[3];
+  ii[0] = (int)(idx % g.E[0]) + g.lo[0];
+  ii[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
+  ii[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
+  for (int d = 0; d < 3; ++d) {
+    bool in = ii[d] >= 1 && ii[d] <= g.ncell[d];
+    bool gh = incl_ghost && g.bc[d] == 0;
+    if (!(in || gh)) return 0;
+  }
+  const double *v = vort + 3 * idx;
+  double s = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(v[0], v[0]), __dmul_rn(v[1], v[1])), __dmul_rn(v[2], v[2])));
+  double ds = 0.0;
+  if (dvort) {
+    const double *u = dvort + 3 * idx;
+    ds = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(u[0], u[0]), __dmul_rn(u[1], u[1])), __dmul_rn(u[2], u[2])));
+  }
+  if (!(s > min_vort || ds > min_dvort)) return 0;
+  return mask ? (mask[idx] ? 1 : 0) : 1;
+}
+
+__global__ void k_remesh_flags(PGeom g, const double *vort, const double *dvort, const uint8_t *mask,
+                               double min_vort, double min_dvort, int incl_ghost, int *flag) {
+  const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
+       idx += (int64_t)gridDim.x * blockDim.x)
+    flag[idx] = remesh_flag(g, idx, vort, dvort, mask, min_vort, min_dvort, incl_ghost);
+}
+
+// create particles at cell centres in (k, j, i) order from the scanned flags;
+// vel / dvort are the fields copied (NULL: zero), per the vel / dvort flags
+__global__ void k_remesh_create(PGeom g, const int *flag, const int *offs, const double *vort, const double *dvort,
+                                const double *vel, double *pos, double *pvort, double *pvel, double *pdvort,
+                                double *vel_rk, double *dvort_rk) {
+  const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    if (!flag[idx]) continue;
+    int64_t p = offs[idx];
+    int ii[3];
+    ii[0] = (int)(idx % g.E[0]) + g.lo[0];
+    ii[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
+    ii[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
+    for (int d = 0; d < 3; ++d)
+      pos[3 * p + d] = __dadd_rn(g.xmin[d], __dmul_rn(__dadd_rn((double)(ii[d] - 1), 0.5), g.dx[d]));
+    for (int c = 0; c < 3; ++c) {
+      pvort[3 * p + c] = vort[3 * idx + c];
+      if (pvel) pvel[3 * p + c] = vel ? vel[3 * idx + c] : 0.0;
+      if (pdvort) pdvort[3 * p + c] = dvort ? dvort[3 * idx + c] : 0.0;
+      if (vel_rk) vel_rk[3 * p + c] = 0.0;
+      if (dvort_rk) dvort_rk[3 * p + c] = 0.0;
+    }
+  }
+}
+
+// ------------------------------------------------------------ advance
+struct RkCoef {
+  double a1, b1, c1, a2, b2, c2;
+};
+
+__global__ void k_rk_update(RkCoef k, double dt, int64_t np, double *pos, const double *vel, double *vort,
+                            const double *dvort, double *vel_rk, double *dvort_rk) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * np; i += (int64_t)gridDim.x * blockDim.x) {
+    double vr = __dadd_rn(__dmul_rn(k.a1, vel_rk[i]), __dmul_rn(__dmul_rn(k.b1, dt), vel[i]));
+    double dr = __dadd_rn(__dmul_rn(k.a2, dvort_rk[i]), __dmul_rn(__dmul_rn(k.b2, dt), dvort[i]));
+    vel_rk[i] = vr;
+    dvort_rk[i] = dr;
+    pos[i] = __dadd_rn(pos[i], __dmul_rn(k.c1, vr));
+    vort[i] = __dadd_rn(vort[i], __dmul_rn(k.c2, dr));
+  }
+}
+
+// migration class along axis d: 0 stay, 1 west (wrap), 2 east (wrap), 3 drop
+__global__ void k_classify(int64_t np, const double *pos, int d, double lo, double hi, int per, int *cls,
+                           int *counts) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    double x = pos[3 * q + d];
+    int c = x < lo ? 1 : (x >= hi ? 2 : 0);
+    if (c != 0 && !per) c = 3;
+    cls[q] = c;
+    if (c != 0) atomicAdd(&counts[c], 1);
+  }
+}
+
+}  // namespace pmgpu

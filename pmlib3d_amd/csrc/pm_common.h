@@ -52,4 +52,7 @@ const std::string &get_error();
     }                                \
   } while (0)
 
+// frees the particle workspace (particles.hip)
+void particles_release();
+
 }  // namespace pmgpu

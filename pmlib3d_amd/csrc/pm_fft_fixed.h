@@ -54,6 +54,18 @@ template <int L> struct Geo {
 };
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3) + (i >> 6); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Map the linear block id to a logical tile id so that consecutive
+// tiles, which share cache lines at their edges, run on the same XCD.
+__device__ __forceinline__ int64_t xcd_tile(int64_t lin, int64_t n) {
+  const int64_t per = n / 8, rem = n % 8;
+  const int64_t x = lin % 8, k = lin / 8;
+  return x * per + (x < rem ? x : rem) + k;
+}
+__device__ __forceinline__ int64_t block_linear() {
+  return blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+}
+
 template <bool WAVE> __device__ __forceinline__ void line_sync() {
   if (WAVE) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

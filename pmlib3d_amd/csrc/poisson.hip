@@ -1,5 +1,6 @@
 // Host side of the Poisson solve: plans, buffers, launches, C ABI.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -119,6 +120,8 @@ template <typename R> struct PoissonGPU : PoissonBase {
   R *mesh_in = nullptr, *mesh_out = nullptr;  // staging for host calls
   size_t nA = 0, nB = 0, nmesh = 0;
   int Tx = 1, Ts = 1, Tz = 1;
+  int64_t H3p = 0;   // padded row length (complex) of the interleaved spectra
+  int tune_tc = 0;   // z-pass columns per block (0 = built-in default)
   int lds_limit = 65536;
 
   ~PoissonGPU() override {
@@ -167,8 +170,12 @@ template <typename R> struct PoissonGPU : PoissonBase {
     twLx_off = tw.add(L[0]);
     PM_HIP(hipMalloc(&d_tw, tw.host.size() * sizeof(V)));
     PM_HIP(hipMemcpy(d_tw, tw.host.data(), tw.host.size() * sizeof(V), hipMemcpyHostToDevice));
-    nA = (size_t)3 * Hx * M[1] * M[2];
-    nB = (size_t)3 * Hx * L[1] * M[2];
+    // interleaved spectral rows padded to 128 bytes so tiles align with cache lines
+    const int64_t align = 128 / (int64_t)sizeof(V);
+    H3p = ((3 * (int64_t)Hx + align - 1) / align) * align;
+    nA = (size_t)H3p * M[1] * M[2];
+    nB = (size_t)H3p * L[1] * M[2];
+    if (const char *env = getenv("PMGPU_Z_TC")) tune_tc = atoi(env);
     nmesh = (size_t)3 * E[0] * E[1] * E[2];
     PM_HIP(hipMalloc(&A, nA * sizeof(V)));
     PM_HIP(hipMalloc(&B, nB * sizeof(V)));
@@ -312,9 +319,10 @@ template <typename R> struct PoissonGPU : PoissonBase {
     int e = 0;
     // component-interleaved spectra: A[((z*My + y)*Hx + kx)*3 + c],
     // B[((z*Ly + ky)*Hx + kx)*3 + c]
-    const int64_t H3 = 3 * (int64_t)Hx;
+    const int64_t H3 = H3p;
+    const int ncols = 3 * Hx;
     e |= launch_xfwd<R>(vin, A, cs, xs, ys, zs, 1, 3, H3, H3 * M[1], 3, M[0], M[1], M[2], Tx, pNh, twLx_off, d_tw);
-    e |= launch_strided<R, -1>(A, B, H3, H3, H3 * M[1], H3 * L[1], (int)H3, M[1], L[1], M[2], 1, 1.0, Ts, pLy,
+    e |= launch_strided<R, -1>(A, B, H3, H3, H3 * M[1], H3 * L[1], ncols, M[1], L[1], M[2], 1, 1.0, Ts, pLy,
                                d_tw);
     if (e) return -1;
     ZArgs za;
@@ -339,6 +347,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     za.want_psi = want_psi;
     za.want_vort = want_vort;
     za.reproj = reproj;
+    za.tune_tc = tune_tc;
     double dmax = g.dx[0] > g.dx[1] ? g.dx[0] : g.dx[1];
     dmax = dmax > g.dx[2] ? dmax : g.dx[2];
     for (int d = 0; d < 3; ++d) {
@@ -364,7 +373,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     void *outs[3] = {vel, vecpot, vort_out};
     for (int f = 0; f < 3; ++f) {
       if (!fields[f]) continue;
-      e |= launch_strided<R, 1>(fields[f], A, H3, H3, H3 * L[1], H3 * M[1], (int)H3, L[1], M[1], M[2], 1,
+      e |= launch_strided<R, 1>(fields[f], A, H3, H3, H3 * L[1], H3 * M[1], ncols, L[1], M[1], M[2], 1,
                                 1.0 / (double)L[1], Ts, pLy, d_tw);
       XInvArgs xa{A,  reinterpret_cast<R *>(outs[f]) + off, cs, xs, ys, zs, 1, 3, H3, H3 * M[1], 3, M[0], M[1], M[2],
                   Hx, Tx, 1.0 / (double)L[0], pNh, twLx_off};

@@ -13,8 +13,10 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const
   using V = cplx_t<R>;
   constexpr int P = Geo<L>::P, ST = Geo<L>::ST, NT = P * T;
   __shared__ V lds[T * ST];
-  const int col0 = blockIdx.x * T;
-  const int64_t b = blockIdx.y + (int64_t)blockIdx.z * gridDim.y;
+  const int64_t nblk = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+  const int64_t lt = xcd_tile(block_linear(), nblk);
+  const int col0 = (int)(lt % gridDim.x) * T;
+  const int64_t b = lt / gridDim.x;
   const V *in = reinterpret_cast<const V *>(a.in) + b * a.bs_in + col0;
   V *out = reinterpret_cast<V *>(a.out) + b * a.bs_out + col0;
   const int tid = threadIdx.x;
@@ -149,7 +151,8 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
   constexpr bool FULL = E * P == L;
   __shared__ V lds[NLN * ST];
   const V *tw = tw_all + a.plan.tw_off;
-  const int kx0 = blockIdx.x * TC, ky = blockIdx.y;
+  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
+  const int kx0 = (int)(lt % gridDim.x) * TC, ky = (int)(lt / gridDim.x);
   const int tid = threadIdx.x;
   const int line = tid / P, p = tid % P;
   const int col = line / 3, comp = line - 3 * col;
@@ -157,7 +160,7 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
   const bool col_ok = kx < a.Hx;
   const int ncl = (a.Hx - kx0 < TC ? a.Hx - kx0 : TC) * 3;  // valid lines in this block
   V *B = reinterpret_cast<V *>(a.B);
-  const int64_t tile = (int64_t)ky * a.kys + (int64_t)kx0 * 3;
+  const int64_t tile = (int64_t)ky * a.kys + (int64_t)kx0 * 3;  // kys = padded row
   V *myl = lds + line * ST;
   // load the 3*TC interleaved lines, zero rows >= Mz
   for (int idx = tid; idx < L * NLN; idx += NT) {

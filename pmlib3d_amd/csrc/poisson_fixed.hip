@@ -42,7 +42,26 @@ template <typename R, int L, bool EXTRA> constexpr int z_cols() {
   return t;
 }
 
+template <typename R, int L, int TC> int z_tc(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  a.T = TC;
+  dim3 grid((a.Hx + TC - 1) / TC, Ly);
+  hipLaunchKernelGGL((k_z3_f<R, L, TC, false>), grid, dim3(Geo<L>::P * 3 * TC), 0, st, a, tw);
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
 template <typename R, int L> int z_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  // tuned tile width (columns per block) for the hot lengths, velocity only
+  if constexpr (L == 256 || L == 512) {
+    if (!(a.want_psi || a.want_vort) && a.tune_tc > 0) {
+      switch (a.tune_tc) {
+        case 1: return z_tc<R, L, 1>(a, Ly, tw, st);
+        case 2: return z_tc<R, L, 2>(a, Ly, tw, st);
+        case 3: return z_tc<R, L, 3>(a, Ly, tw, st);
+        case 4: return z_tc<R, L, 4>(a, Ly, tw, st);
+        default: break;
+      }
+    }
+  }
   if (a.want_psi || a.want_vort) {
     constexpr int TC = z_cols<R, L, true>();
     a.T = TC;

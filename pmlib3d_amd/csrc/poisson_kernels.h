@@ -190,6 +190,7 @@ struct ZArgs {
   int order;          // regularisation order for zeta
   int allper;         // all axes periodic: G = zeta / (4 pi^2 |k|^2)
   int want_psi, want_vort, reproj;
+  int tune_tc;        // z-pass columns per block (0 = default)
   double dk[3], ksmp[3], h3, eps2pi;
   FftPlan plan;       // length Lz
 };

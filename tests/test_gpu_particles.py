@@ -1,0 +1,291 @@
+"""GPU parity of the particle path (P2M, M2P, remesh, RK advance, cell index)
+through the C ABI, against the oracle (oracle/pmoracle.c, pinned bit for bit
+against the compiled reference in tests/test_oracle.py) and against the
+reference's own outputs in tests/golden/ref_particles_*.npz.
+
+Bars: cell indices, M2P, remesh and advance bit-exact; P2M within 1e-12
+relative L-inf (the deposition is a deterministic gather whose per-node sum
+runs bin by bin instead of in global particle order, so rounding differs
+from the reference's scatter in the last bits)."""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import rel_linf
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+PART = sorted(glob.glob(os.path.join(GOLD, "ref_particles_*.npz")))
+TOL_P2M = 1e-12
+
+
+def _patch(g):
+    import pmlib3d_amd as pm
+    return pm.Patch(xmin=tuple(g.xmin), xmax=tuple(g.xmax), ncell=tuple(g.ncell), dx=tuple(g.dx),
+                    bound_cond=tuple(g.bound_cond), nghost=g.nghost)
+
+
+def _geom_from(d):
+    ncell, ng, bc, io = json.loads(str(d["geom"]))
+    return O.make_geom(ncell, ng, [-0.5, -0.5, -0.5], 1.0 / 8, bc, io)
+
+
+def _p2m(g, pos, val, mesh=None, clear=True, clear_ghost=False):
+    import pmlib3d_amd as pm
+    ext = (3,) + O.mesh_extent(g)
+    m = np.zeros(ext, order="F") if mesh is None else np.array(mesh, order="F", copy=True)
+    ierr = pm.pmlib_interp_particle_mesh(_patch(g), pos, val, m, clear=clear, clear_ghost=clear_ghost,
+                                         interp_order=g.interp_order)
+    assert ierr == 0
+    return m
+
+
+def _m2p(g, mesh, pos):
+    import pmlib3d_amd as pm
+    return pm.interp_mesh_field(_patch(g), mesh, pos, interp_order=g.interp_order)
+
+
+def _remesh(g, vm, dvm, velm, lo, dlo, incl_ghost, dvort=True, vel=True, mask=None):
+    import pmlib3d_amd as pm
+    mesh = pm.Mesh(vort=vm, dvort=dvm, vel=velm, patch_mask=mask)
+    part = pm.Particles()
+    assert pm.pmlib_remesh(_patch(g), mesh, part, lo, dlo, dvort=dvort, vel=vel, mask=mask is not None,
+                           incl_ghost=incl_ghost, interp_order=g.interp_order) == 0
+    return part
+
+
+def _advance(g, order, stage, dt, arrs, euler_vort=False):
+    import pmlib3d_amd as pm
+    part = pm.Particles(*[np.array(a, order="F", copy=True) for a in arrs])
+    assert pm.pmlib_particles_advance(_patch(g), part, stage, dt, euler_vort=euler_vort, rk_order=order,
+                                      interp_order=g.interp_order) == 0
+    return [part.pos, part.vel, part.vort, part.dvort, part.vel_rk, part.dvort_rk]
+
+
+# ---------------------------------------------------------------- golden
+@pytest.mark.parametrize("path", PART, ids=lambda p: os.path.basename(p)[14:-4])
+def test_particles_match_reference_golden(path):
+    d = np.load(path, allow_pickle=False)
+    g = _geom_from(d)
+    print("golden", os.path.basename(path), flush=True)
+    got = _p2m(g, d["pos"], d["val"])
+    assert rel_linf(got, d["p2m"]) <= TOL_P2M
+    assert np.array_equal(_m2p(g, d["mesh"], d["pos"]), d["m2p"])
+    for ig in (0, 1):
+        p = _remesh(g, d["rm_vort"], d["rm_dvort"], d["rm_vel"], 1.2, 0.15, bool(ig))
+        assert np.array_equal(np.concatenate([p.pos, p.vort, p.vel], axis=1), d["remesh%d" % ig])
+    for st in (1, 2):
+        out = _advance(g, 2, st, 0.2, [d[k] for k in ("a_pos", "a_vel", "a_vort", "a_dvort", "a_vel_rk",
+                                                      "a_dvort_rk")])
+        mine = np.concatenate([out[0], out[2], out[4], out[5]], axis=1)
+        assert np.array_equal(mine, d["advance%d" % st])
+
+
+# ------------------------------------------------------------ cell index
+def test_cell_index_bit_exact_on_ties_and_boundaries():
+    """Positions on cell faces, centres and exact half-way ties, plus random
+    positions; the GPU's NINT must agree with the oracle's everywhere."""
+    import pmlib3d_amd as pm
+    rng = np.random.default_rng(7)
+    for h, xmin in ((1.0 / 128, -1.0), (0.1, -0.35), (2.0 / 3.0, 0.1), (1.0 / 3.0, -7.0)):
+        g = O.make_geom((64, 64, 64), 0, [xmin] * 3, h, (0, 0, 0))
+        i = np.arange(-70, 140)
+        cands = [xmin + i * h, xmin + (i + 0.5) * h, xmin + (i - 0.5) * h,
+                 np.nextafter(xmin + (i + 0.5) * h, np.inf), np.nextafter(xmin + (i + 0.5) * h, -np.inf),
+                 rng.uniform(xmin - 5 * h, xmin + 70 * h, 2000)]
+        x = np.concatenate(cands)
+        pos = np.asfortranarray(np.stack([x, np.roll(x, 1), np.roll(x, 2)]))
+        got = pm.cell_index(_patch(g), pos)
+        ref = np.zeros(pos.shape, dtype=np.int32, order="F")
+        O.lib().or_cell_indices(g, pos.ctypes.data_as(O.C.c_void_p), pos.shape[1],
+                                ref.ctypes.data_as(O.C.c_void_p))
+        assert np.array_equal(got, ref)
+
+
+# ------------------------------------------------------------ P2M / M2P
+GEOMS = [
+    # ncell, nghost, bound_cond, interp_order
+    ((16, 12, 20), 0, (0, 0, 0), 3),
+    ((12, 12, 12), 2, (0, 0, 0), 3),
+    ((16, 16, 16), 0, (1, 1, 1), 3),
+    ((12, 16, 10), 1, (1, 0, 1), 3),
+    ((14, 12, 12), 2, (0, 0, 0), 4),
+    ((12, 12, 12), 0, (1, 1, 1), 4),
+    ((12, 10, 14), 3, (0, 1, 0), 4),
+]
+
+
+def _particles(g, n, seed, spread=2):
+    rng = np.random.default_rng(seed)
+    lo, hi, h = np.array(g.xmin), np.array(g.xmax), np.array(g.dx)
+    pos = rng.uniform(lo - spread * h, hi + spread * h, size=(n, 3)).T.copy(order="F")
+    k = n // 5
+    pos[:, :k] = lo[:, None] + (rng.integers(-2, 12, size=(3, k)) + 0.5) * h[:, None]   # tie points
+    pos[:, k:2 * k] = lo[:, None] + rng.integers(-2, 12, size=(3, k)) * h[:, None]    # cell faces
+    if g.interp_order == 4:   # M6* M2P reads below the mesh for k3 + 2 < 1 - ng
+        ng = (O.mesh_extent(g)[2] - g.ncell[2]) // 2
+        pos[2] = np.maximum(pos[2], lo[2] - (ng + 0.5) * h[2])
+    val = np.asfortranarray(rng.normal(size=(3, n)))
+    return pos, val
+
+
+@pytest.mark.parametrize("case", GEOMS, ids=[str(i) for i in range(len(GEOMS))])
+def test_p2m_and_m2p_match_oracle(case):
+    ncell, ng, bc, io = case
+    print("case", case, flush=True)
+    g = O.make_geom(ncell, ng, [-0.4, -0.3, -0.55], 1.0 / 10, bc, io)
+    pos, val = _particles(g, 3000, sum(ncell) + io)
+    ref = O.p2m(g, pos, val)
+    got = _p2m(g, pos, val)
+    assert rel_linf(got, ref) <= TOL_P2M
+    # accumulate onto an existing mesh, and the clear_ghost variant
+    base = np.asfortranarray(np.random.default_rng(1).normal(size=ref.shape))
+    acc = _p2m(g, pos, val, base, clear=False)
+    assert rel_linf(acc, O.p2m(g, pos, val, base.copy(order="F"), clear=False)) <= TOL_P2M
+    E = O.mesh_extent(g)
+    ngl = [(E[d] - g.ncell[d]) // 2 for d in range(3)]
+    inner = np.zeros(E, dtype=bool)
+    inner[ngl[0]:E[0] - ngl[0], ngl[1]:E[1] - ngl[1], ngl[2]:E[2] - ngl[2]] = True
+    zg = base.copy(order="F")
+    zg[:, ~inner] = 0.0   # pmlib_interp_particle_mesh.f:90-133 zeroes the ghost layers
+    cg = _p2m(g, pos, val, base, clear=False, clear_ghost=True)
+    assert rel_linf(cg, O.p2m(g, pos, val, zg, clear=False)) <= TOL_P2M
+    # M2P is bit-identical (same term order as the reference)
+    mesh = np.asfortranarray(np.random.default_rng(2).normal(size=ref.shape))
+    assert np.array_equal(_m2p(g, mesh, pos), O.m2p(g, mesh, pos))
+
+
+def test_p2m_deterministic_and_empty():
+    g = O.make_geom((16, 16, 16), 0, [0, 0, 0], 1.0 / 16, (1, 1, 1), 3)
+    pos, val = _particles(g, 20000, 5)
+    a = _p2m(g, pos, val)
+    b = _p2m(g, pos, val)
+    assert np.array_equal(a, b)
+    # all particles on one point: one bin holds every particle
+    same = np.asfortranarray(np.full((3, 4000), 0.3))
+    c = _p2m(g, same, np.ones((3, 4000)))
+    assert rel_linf(c, O.p2m(g, same, np.ones((3, 4000)))) <= TOL_P2M
+    # no particles: clear gives zeros, accumulate leaves the mesh alone
+    z = _p2m(g, np.zeros((3, 0), order="F"), np.zeros((3, 0), order="F"))
+    assert not z.any()
+    m = np.asfortranarray(np.random.default_rng(0).normal(size=z.shape))
+    assert np.array_equal(_p2m(g, np.zeros((3, 0)), np.zeros((3, 0)), m, clear=False), m)
+
+
+def test_p2m_conserves_circulation_periodic_large():
+    """Size-independent property at 128^3 with 2M particles: on an
+    all-periodic patch every particle's weights sum to one over the interior
+    after the ghost fold, so the interior sum equals the total strength."""
+    g = O.make_geom((128, 128, 128), 0, [-1, -1, -1], 2.0 / 128, (1, 1, 1), 3)
+    rng = np.random.default_rng(3)
+    n = 1 << 21
+    pos = np.asfortranarray(rng.uniform(-1, 1, size=(3, n)))
+    val = np.asfortranarray(rng.normal(size=(3, n)))
+    m = _p2m(g, pos, val)
+    E = O.mesh_extent(g)
+    ng = (E[0] - 128) // 2
+    interior = m[:, ng:E[0] - ng, ng:E[1] - ng, ng:E[2] - ng]
+    tot = val.sum(axis=1)
+    assert np.abs(interior.sum(axis=(1, 2, 3)) - tot).max() <= 1e-9 * np.abs(val).sum()
+    # the ghosts hold the periodic images of the interior faces
+    assert np.array_equal(m[:, :ng], m[:, 128:128 + ng])
+
+
+# --------------------------------------------------------------- remesh
+@pytest.mark.parametrize("case", GEOMS[:5], ids=[str(i) for i in range(5)])
+def test_remesh_matches_oracle(case):
+    ncell, ng, bc, io = case
+    g = O.make_geom(ncell, ng, [-0.4, -0.3, -0.55], 1.0 / 10, bc, io)
+    rng = np.random.default_rng(sum(ncell))
+    ext = (3,) + O.mesh_extent(g)
+    vm, dvm, velm = (np.asfortranarray(rng.normal(size=ext) * s) for s in (1.0, 0.1, 1.0))
+    mask = rng.random(ext[1:]) < 0.7
+    for ig in (False, True):
+        for kw in (dict(), dict(dvort=False, vel=False), dict(mask=mask)):
+            p = _remesh(g, vm, dvm, velm, 1.2, 0.15, ig, **kw)
+            P, W, V, D = O.remesh(g, vm, dvm, velm, 1.2, 0.15, ig, **kw)
+            for a, b in zip((p.pos, p.vort, p.vel, p.dvort), (P, W, V, D)):
+                assert np.array_equal(a, b)
+            assert not p.vel_rk.any() and not p.dvort_rk.any()
+    # dvort mesh absent: threshold on |vort| alone
+    p = _remesh(g, vm, None, velm, 1.0, 0.0, False, dvort=False)
+    P, W, V, D = O.remesh(g, vm, None, velm, 1.0, 0.0, False, dvort=False)
+    assert np.array_equal(p.pos, P) and np.array_equal(p.vort, W)
+
+
+def test_remesh_large_count_and_order():
+    """256^3 mesh: count and (k, j, i) creation order at full size."""
+    g = O.make_geom((256, 256, 256), 0, [-1, -1, -1], 2.0 / 256, (0, 0, 0), 3)
+    rng = np.random.default_rng(11)
+    vm = np.asfortranarray(rng.normal(size=(3, 256, 256, 256)))
+    p = _remesh(g, vm, None, None, 1.5, 0.0, False, dvort=False, vel=False)
+    P, W, V, D = O.remesh(g, vm, None, None, 1.5, 0.0, False, dvort=False, vel=False)
+    assert p.pos.shape == P.shape and P.shape[1] > 1000000
+    assert np.array_equal(p.pos, P) and np.array_equal(p.vort, W)
+    assert not p.vel.any() and not p.dvort.any()
+
+
+# -------------------------------------------------------------- advance
+@pytest.mark.parametrize("bc", [(0, 0, 0), (1, 1, 1), (1, 0, 1)])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_advance_matches_oracle(bc, order):
+    g = O.make_geom((12, 10, 8), 1, [-0.5, -0.4, -0.3], 1.0 / 10, bc, 3)
+    rng = np.random.default_rng(order * 7 + sum(bc))
+    n = 5000
+    lo, hi = np.array(g.xmin), np.array(g.xmax)
+    arrs = [np.asfortranarray(rng.uniform(lo, hi, size=(n, 3)).T)]
+    arrs += [np.asfortranarray(rng.normal(size=(3, n)) * s) for s in (1.0, 1.0, 0.5, 0.3, 0.2)]
+    for ev in (False, True):
+        cur_g = [a.copy(order="F") for a in arrs]
+        cur_o = [a.copy(order="F") for a in arrs]
+        for st in range(1, order + 1):
+            cur_g = _advance(g, order, st, 0.4, cur_g, euler_vort=ev)
+            cur_o = O.advance(g, order, st, 0.4, *cur_o, euler_vort=ev)
+            for a, b in zip(cur_g, cur_o):
+                assert a.shape == b.shape and np.array_equal(a, b)
+
+
+def test_advance_multiple_wraps_and_empty():
+    """Displacements of several periods need the repeated map; a free patch
+    can lose every particle."""
+    g = O.make_geom((8, 8, 8), 0, [0, 0, 0], 1.0 / 8, (1, 1, 1), 3)
+    rng = np.random.default_rng(4)
+    n = 300
+    arrs = [np.asfortranarray(rng.uniform(0, 1, (3, n))), np.asfortranarray(rng.normal(size=(3, n)) * 30)]
+    arrs += [np.asfortranarray(rng.normal(size=(3, n))) for _ in range(4)]
+    got = _advance(g, 1, 1, 0.1, arrs)
+    ref = O.advance(g, 1, 1, 0.1, *arrs)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    assert ((got[0] >= 0) & (got[0] < 1)).all()
+    gf = O.make_geom((8, 8, 8), 0, [0, 0, 0], 1.0 / 8, (0, 0, 0), 3)
+    got = _advance(gf, 1, 1, 100.0, arrs)
+    assert got[0].shape == (3, 0)
+    empty = [np.zeros((3, 0), order="F") for _ in range(6)]
+    assert _advance(g, 2, 1, 0.1, empty)[0].shape == (3, 0)
+
+
+# ------------------------------------------------------------ device API
+def test_device_entry_points_match_host_entry_points():
+    import torch
+    import pmlib3d_amd as pm
+    g = O.make_geom((32, 32, 32), 0, [-1, -1, -1], 2.0 / 32, (1, 1, 0), 3)
+    pos, val = _particles(g, 40000, 9)
+    dev = torch.device("cuda:0")
+    ops = pm.ParticleOps(_patch(g), interp_order=3)
+    ext = (3,) + O.mesh_extent(g)
+    d_pos = torch.from_numpy(pos.ravel(order="F")).to(dev)
+    d_val = torch.from_numpy(val.ravel(order="F")).to(dev)
+    d_mesh = torch.empty(int(np.prod(ext)), dtype=torch.float64, device=dev)
+    ops.p2m(d_pos.data_ptr(), d_val.data_ptr(), pos.shape[1], d_mesh.data_ptr(), clear=True)
+    d_out = torch.empty_like(d_pos)
+    ops.m2p(d_mesh.data_ptr(), d_pos.data_ptr(), pos.shape[1], d_out.data_ptr())
+    torch.cuda.synchronize()
+    m = d_mesh.cpu().numpy().reshape(ext, order="F")
+    assert np.array_equal(m, _p2m(g, pos, val))
+    assert np.array_equal(d_out.cpu().numpy().reshape(pos.shape, order="F"), _m2p(g, m, pos))

@@ -209,6 +209,31 @@ def test_remesh_order_and_threshold():
     assert np.array_equal(P[:, 0], [2.5, 0.5, 0.5]) and np.array_equal(P[:, 1], [0.5, 1.5, 1.5])
 
 
+def test_remesh_mask_and_copy_flags_reduce_to_the_plain_remesh():
+    """mask=.TRUE. keeps exactly the unmasked particles of the plain remesh,
+    in the same order (remesh/pmlib_remesh.f:107-113); dvort/vel = .FALSE.
+    zero those particle fields and change nothing else (:208-219)."""
+    g = O.make_geom((9, 7, 8), 2, [-0.3, -0.2, -0.1], 0.1, (0, 1, 0))
+    rng = np.random.default_rng(21)
+    ext = (3,) + O.mesh_extent(g)
+    vm, dvm, velm = (np.asfortranarray(rng.normal(size=ext) * s) for s in (1.0, 0.1, 1.0))
+    mask = rng.random(ext[1:]) < 0.6
+    for ig in (False, True):
+        P, W, V, D = O.remesh(g, vm, dvm, velm, 1.2, 0.15, ig)
+        cells = [np.rint((P[d] - g.xmin[d]) / g.dx[d] - 0.5).astype(int) + (ext[1 + d] - g.ncell[d]) // 2
+                 for d in range(3)]
+        keep = mask[cells[0], cells[1], cells[2]]
+        Pm, Wm, Vm, Dm = O.remesh(g, vm, dvm, velm, 1.2, 0.15, ig, mask=mask)
+        for a, b in zip((Pm, Wm, Vm, Dm), (P, W, V, D)):
+            assert np.array_equal(a, b[:, keep])
+        P2, W2, V2, D2 = O.remesh(g, vm, dvm, velm, 1.2, 0.15, ig, dvort=False, vel=False)
+        assert np.array_equal(P2, P) and np.array_equal(W2, W)
+        assert not V2.any() and not D2.any()
+        # the vorticity really is copied from the cell the particle sits in
+        assert np.array_equal(W, vm[:, cells[0], cells[1], cells[2]])
+        assert np.array_equal(V, velm[:, cells[0], cells[1], cells[2]])
+
+
 # ------------------------------------------------------ live reference
 @pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
 def test_oracle_matches_live_reference_random_case():
