@@ -156,6 +156,29 @@ int pmgpu_particles_advance_device(const pmgpu_geom *g, int32_t rk_order, int32_
  * (interpolation/pmlib_interp_particle_mesh.f:153-155). */
 int pmgpu_cell_index(const pmgpu_geom *g, const double *pos, int64_t np, int32_t *idx);
 
+/* ------------------------------------------------------------------------
+ * Brinkman penalisation
+ * ------------------------------------------------------------------------ */
+
+/* One iteration of penalisation_solve(patch, topo_all, mesh, ierr)
+ * (vortex_codes/vortex_penalisation/source/penalisation/penalisation_solve.f:6,
+ * the loop body :96-111 and :146-273): f = chi (us - vel) on every cell, then
+ * vort += relax curl(f) on the interior cells with central differences of
+ * order fd_order (2 or 4; pmlib_fd_order) over 2 dx.  chi is one value per
+ * mesh cell (the mask, or lambda dt chi_s / (1 + lambda dt chi_s) for a
+ * finite penalisation factor); us is solid_vel (3, E...) if non-NULL, else
+ * the constant us_const (NULL: at rest).  diag (may be NULL) receives
+ * sum_dforce(1:3), res_enst and sum_enst_pen of the iteration (:177-190). */
+int pmgpu_penalise(const pmgpu_geom *g, const double *vel, const double *chi, const double *solid_vel,
+                   const double us_const[3], double relax, int32_t fd_order, double *vort,
+                   double diag[5]);
+/* Device variant in FP64 or FP32 (precision); the sums are FP64 either way
+ * and, when diag is non-NULL, the call synchronises the stream to return
+ * them. */
+int pmgpu_penalise_device(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_chi,
+                          const void *d_solid_vel, const double us_const[3], double relax,
+                          int32_t fd_order, void *d_vort, double *diag, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,7 +86,7 @@ def lib():
         L.or_remesh.restype = C.c_int64
         L.or_advance.argtypes = [G, C.c_int, C.c_int, C.c_double, C.c_int, P, P, P, P, P, P, C.c_int64]
         L.or_advance.restype = C.c_int64
-        L.or_penalise.argtypes = [G, P, P, P, C.c_double, P, P]
+        L.or_penalise.argtypes = [G, P, P, P, C.c_double, C.c_int, P, P]
         L.or_penalise.restype = C.c_int
         _lib = L
     return _lib
@@ -194,14 +194,16 @@ def advance(g, rk_order, stage, dt, pos, vel, vort, dvort, vel_rk, dvort_rk, eul
     return [a[:, :n] for a in arrs]
 
 
-def penalise(g, vel, chi, vort, solid_vel=None, relax=1.0):
+def penalise(g, vel, chi, vort, solid_vel=None, relax=1.0, fd_order=4):
+    """One penalisation iteration: returns (vort + relax curl(chi (us - u)),
+    diag = [sum_dforce(3), res_enst, sum_enst_pen])."""
     vel = np.asfortranarray(vel, dtype=np.float64)
     chi = np.asfortranarray(chi, dtype=np.float64)
     vort = np.array(vort, dtype=np.float64, order="F", copy=True)
     sv = None if solid_vel is None else np.asfortranarray(solid_vel, dtype=np.float64)
     diag = np.zeros(5)
-    if lib().or_penalise(g, _p(vel), _p(chi), _p(sv), relax, _p(vort), _p(diag)):
-        raise ValueError("penalise needs >= 2 ghost layers")
+    if lib().or_penalise(g, _p(vel), _p(chi), _p(sv), relax, int(fd_order), _p(vort), _p(diag)):
+        raise ValueError("penalise needs fd_order 2 or 4 and fd_order/2 ghost layers")
     return vort, diag
 
 

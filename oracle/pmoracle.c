@@ -22,8 +22,8 @@
  *   or_advance         particles/pmlib_particles_advance.f:49-113 +
  *                      particles/pmlib_particles_map.f:57-133 (one rank)
  *   or_penalise        vortex_codes/vortex_penalisation/source/penalisation/
- *                      penalisation_solve.f:96-111,133-343 (one iteration,
- *                      4th-order curl, Brinkman factor per cell)
+ *                      penalisation_solve.f:96-111,146-273 (one iteration,
+ *                      2nd/4th-order curl, Brinkman factor per cell)
  *
  * Array layout is the reference's: meshes are (3, i, j, k) with the component
  * fastest and ghost layers included, particles are (3, np).  Fixed-size fp64
@@ -788,19 +788,22 @@ int64_t or_advance(const or_geom *g, int rk_order, int stage, double dt, int eul
 }
 
 /* One Brinkman penalisation iteration on a (3, E...) mesh, restating
- * penalisation_solve.f:96-111 and the 4th-order branch :195-273:
+ * penalisation_solve.f:96-111 and the FD branches :146-193 (2nd order) and
+ * :195-273 (4th order):
  *   vel_pen = solid_vel - vel on all cells, f = chi * vel_pen,
- *   vort += relax * curl(f) on interior cells (central differences with
- *   c1 = 1/6, c2 = 4/3 over 2h), accumulating the force increment and
+ *   vort += relax * curl(f) on interior cells (central differences over 2h;
+ *   4th order with c1 = 1/6, c2 = 4/3), accumulating the force increment and
  *   enstrophy sums.  chi (1, E...) is the per-cell factor (the sharp mask,
  *   or lambda dt chi_s / (1 + lambda dt chi_s) for the finite-lambda form);
  *   solid_vel may be NULL (solid at rest).  diag[0..4] = sum_dforce(3),
  *   res_enst, sum_enst_pen. */
 int or_penalise(const or_geom *g, const double *vel, const double *chi, const double *solid_vel,
-                double relax, double *vort, double *diag) {
+                double relax, int fd_order, double *vort, double *diag) {
   int E[3], ngl[3];
   for (int d = 0; d < 3; ++d) { E[d] = mesh_ext(g, d); ngl[d] = ng_side(g, d); }
-  if (ngl[0] < 2 || ngl[1] < 2 || ngl[2] < 2) return -1;
+  const int need = fd_order == 2 ? 1 : 2;
+  if (fd_order != 2 && fd_order != 4) return -1;
+  if (ngl[0] < need || ngl[1] < need || ngl[2] < need) return -1;
   size_t n = (size_t)E[0] * E[1] * E[2];
   double *f = (double *)malloc(sizeof(double) * 3 * n);
   for (size_t q = 0; q < n; ++q)
@@ -820,10 +823,19 @@ int or_penalise(const or_geom *g, const double *vel, const double *chi, const do
         double px = g->xmin[0] + ((double)(i - 1) + 0.5) * g->dx[0];
         size_t o = 3 * ((size_t)(i - 1 + ngl[0]) + (size_t)E[0] * ((j - 1 + ngl[1]) + (size_t)E[1] * (k - 1 + ngl[2])));
 #define D4(c, s, cc) ((c1 * f[o - 2 * (s) + (c)] - c2 * f[o - (s) + (c)] + c2 * f[o + (s) + (c)] - c1 * f[o + 2 * (s) + (c)]) * (cc))
-        double dvdx = D4(1, sx, cx), dwdx = D4(2, sx, cx);
-        double dudy = D4(0, sy, cy), dwdy = D4(2, sy, cy);
-        double dudz = D4(0, sz, cz), dvdz = D4(1, sz, cz);
+#define D2(c, s, cc) ((-f[o - (s) + (c)] + f[o + (s) + (c)]) * (cc))
+        double dvdx, dwdx, dudy, dwdy, dudz, dvdz;
+        if (fd_order == 4) {
+          dvdx = D4(1, sx, cx); dwdx = D4(2, sx, cx);
+          dudy = D4(0, sy, cy); dwdy = D4(2, sy, cy);
+          dudz = D4(0, sz, cz); dvdz = D4(1, sz, cz);
+        } else {
+          dvdx = D2(1, sx, cx); dwdx = D2(2, sx, cx);
+          dudy = D2(0, sy, cy); dwdy = D2(2, sy, cy);
+          dudz = D2(0, sz, cz); dvdz = D2(1, sz, cz);
+        }
 #undef D4
+#undef D2
         double w0 = relax * (dwdy - dvdz), w1 = relax * (dudz - dwdx), w2 = relax * (dvdx - dudy);
         vort[o] += w0;
         vort[o + 1] += w1;

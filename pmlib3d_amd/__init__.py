@@ -12,9 +12,12 @@ from .core import (Patch, Mesh, Particles, Parameters, pmlib_write, mesh_extent,
                    pmlib_poisson_setup, pmlib_poisson_solve, PoissonSolver)
 from .particles import (cell_index, pmlib_interp_particle_mesh, pmlib_interp_mesh_particle,
                         interp_mesh_field, pmlib_remesh, pmlib_particles_advance, ParticleOps)
+from .penalisation import penalise, penalise_device, sphere_mask, brinkman_factor
+from .step import VortexStep
 from ._lib import PmgpuError, FP32, FP64
 
 __all__ = ["Patch", "Mesh", "Particles", "Parameters", "pmlib_write", "mesh_extent",
            "pmlib_poisson_setup", "pmlib_poisson_solve", "PoissonSolver", "PmgpuError",
            "FP32", "FP64", "cell_index", "pmlib_interp_particle_mesh", "pmlib_interp_mesh_particle",
-           "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps"]
+           "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps", "penalise",
+           "penalise_device", "sphere_mask", "brinkman_factor", "VortexStep"]

@@ -58,6 +58,8 @@ SIGNATURES = {
     "pmgpu_particles_advance_device": (C.c_int, [C.POINTER(Geom), _I, _I, C.c_double, _I,
                                                  C.POINTER(_P), C.c_int64, C.POINTER(C.c_int64), _P]),
     "pmgpu_cell_index": (C.c_int, [C.POINTER(Geom), _P, C.c_int64, _P]),
+    "pmgpu_penalise": (C.c_int, [C.POINTER(Geom), _P, _P, _P, _P, C.c_double, _I, _P, _P]),
+    "pmgpu_penalise_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P, _P]),
 }
 
 _lib = None
@@ -67,6 +69,14 @@ def load():
     """Load libpmgpu.so and bind every entry point (raises if missing)."""
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime (libamdhip64.so.7 + libhsa-runtime64)
+        # and a process must hold only one: loading torch first lets
+        # libpmgpu.so bind to that copy (same SONAME), so device pointers and
+        # streams are shared.  Without torch the system runtime is used.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise PmgpuError("libpmgpu.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
         lib = C.CDLL(LIB_PATH)

@@ -43,6 +43,7 @@ int pmgpu_init(int32_t device) {
 int pmgpu_finalize(void) {
   PM_HIP(hipDeviceSynchronize());
   pmgpu::particles_release();
+  pmgpu::penalise_release();
   return 0;
 }
 

@@ -24,10 +24,16 @@ struct PGeom {
 };
 
 // NINT((x - xmin) * c_1_dx): correctly rounded subtract and multiply (no
-// contraction), then round half away from zero.
+// contraction), then round half away from zero, done by hand: trunc is exact
+// and so is t - trunc(t), so the half-way test is exact too.
 __device__ __forceinline__ int cell_index(double x, double xmin, double c1) {
-  double t = __dmul_rn(__dsub_rn(x, xmin), c1);
-  return (int)round(t);
+  const double t = __dmul_rn(__dsub_rn(x, xmin), c1);
+  if (!(fabs(t) < 1073741824.0)) return t > 0 ? 1073741824 : (t < 0 ? -1073741824 : 0);  // far off the mesh
+  double r = trunc(t);
+  const double f = __dsub_rn(t, r);
+  if (f >= 0.5) r = __dadd_rn(r, 1.0);
+  else if (f <= -0.5) r = __dsub_rn(r, 1.0);
+  return (int)r;
 }
 
 // normalised offset d = (x - xmin - h (b - 1/2)) * nrm

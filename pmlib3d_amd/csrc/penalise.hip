@@ -1,0 +1,258 @@
+// Brinkman penalisation step (HIP, gfx950): one fused stencil pass
+//   f = chi (us - u),  vort += relax curl(f)   on the interior cells,
+// with the force-increment and enstrophy sums reduced deterministically.
+//
+// Restates one iteration of penalisation_solve
+// (vortex_codes/vortex_penalisation/source/penalisation/penalisation_solve.f:
+// 96-111 for f, :146-193 / :195-273 for the 2nd / 4th order curl).  f is never
+// stored: each thread forms it at its 12 stencil neighbours from chi, u and us
+// (the neighbours along i are the adjacent lanes, so the re-reads hit L1/L2).
+// Every floating-point operation is an explicitly rounded one, in the order
+// the reference writes it, so the FP64 result is bit-identical to it.
+#include <cstring>
+
+#include "../../include/pmgpu.h"
+#include "pm_common.h"
+
+namespace pmgpu {
+
+struct BGeom {
+  int E[3], N[3], ng[3];
+  double xmin[3], dx[3];
+};
+
+template <typename T> __device__ __forceinline__ T rmul(T a, T b);
+template <typename T> __device__ __forceinline__ T radd(T a, T b);
+template <typename T> __device__ __forceinline__ T rsub(T a, T b);
+template <> __device__ __forceinline__ double rmul(double a, double b) { return __dmul_rn(a, b); }
+template <> __device__ __forceinline__ double radd(double a, double b) { return __dadd_rn(a, b); }
+template <> __device__ __forceinline__ double rsub(double a, double b) { return __dsub_rn(a, b); }
+template <> __device__ __forceinline__ float rmul(float a, float b) { return __fmul_rn(a, b); }
+template <> __device__ __forceinline__ float radd(float a, float b) { return __fadd_rn(a, b); }
+template <> __device__ __forceinline__ float rsub(float a, float b) { return __fsub_rn(a, b); }
+
+template <typename T> struct PenArgs {
+  BGeom g;
+  const T *vel, *chi, *us;  // us: (3, E...) field or null (then us0..2)
+  T us0, us1, us2;
+  T relax, cx, cy, cz;       // cx = 1 / (2 dx)
+  T *vort;
+  double *partial;           // 5 doubles per block, or null (no sums)
+};
+
+template <typename T>
+__device__ __forceinline__ T f_at(const PenArgs<T> &a, int64_t q, int c) {
+  T u = a.vel[3 * q + c];
+  T s = a.us ? a.us[3 * q + c] : (c == 0 ? a.us0 : (c == 1 ? a.us1 : a.us2));
+  return rmul(a.chi[q], rsub(s, u));
+}
+
+// central difference of component c along a stride st (in cells)
+template <typename T, int FD>
+__device__ __forceinline__ T cdiff(const PenArgs<T> &a, int64_t q, int64_t st, int c, T cc) {
+  if (FD == 4) {
+    const T c1 = T(1.0 / 6.0), c2 = T(4.0 / 3.0);
+    T m2 = f_at(a, q - 2 * st, c), m1 = f_at(a, q - st, c), p1 = f_at(a, q + st, c), p2 = f_at(a, q + 2 * st, c);
+    T t = rsub(rmul(c1, m2), rmul(c2, m1));
+    t = radd(t, rmul(c2, p1));
+    t = rsub(t, rmul(c1, p2));
+    return rmul(t, cc);
+  } else {
+    T m1 = f_at(a, q - st, c), p1 = f_at(a, q + st, c);
+    return rmul(radd(-m1, p1), cc);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+constexpr int kTx = 64, kTy = 4;
+
+template <typename T, int FD>
+__global__ void __launch_bounds__(kTx *kTy) k_penalise(PenArgs<T> a) {
+  const BGeom &g = a.g;
+  const int i = blockIdx.x * kTx + threadIdx.x + 1;  // interior 1..N
+  const int j = blockIdx.y * kTy + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  double s[5] = {0, 0, 0, 0, 0};
+  if (i <= g.N[0] && j <= g.N[1]) {
+    const int64_t sx = 1, sy = g.E[0], sz = (int64_t)g.E[0] * g.E[1];
+    const int64_t q = (int64_t)(i - 1 + g.ng[0]) + sy * (j - 1 + g.ng[1]) + sz * (k - 1 + g.ng[2]);
+    T dvdx = cdiff<T, FD>(a, q, sx, 1, a.cx), dwdx = cdiff<T, FD>(a, q, sx, 2, a.cx);
+    T dudy = cdiff<T, FD>(a, q, sy, 0, a.cy), dwdy = cdiff<T, FD>(a, q, sy, 2, a.cy);
+    T dudz = cdiff<T, FD>(a, q, sz, 0, a.cz), dvdz = cdiff<T, FD>(a, q, sz, 1, a.cz);
+    T w0 = rmul(a.relax, rsub(dwdy, dvdz));
+    T w1 = rmul(a.relax, rsub(dudz, dwdx));
+    T w2 = rmul(a.relax, rsub(dvdx, dudy));
+    T *v = a.vort + 3 * q;
+    T v0 = radd(v[0], w0), v1 = radd(v[1], w1), v2 = radd(v[2], w2);
+    v[0] = v0;
+    v[1] = v1;
+    v[2] = v2;
+    if (a.partial) {
+      const double px = __dadd_rn(g.xmin[0], __dmul_rn(__dadd_rn((double)(i - 1), 0.5), g.dx[0]));
+      const double py = __dadd_rn(g.xmin[1], __dmul_rn(__dadd_rn((double)(j - 1), 0.5), g.dx[1]));
+      const double pz = __dadd_rn(g.xmin[2], __dmul_rn(__dadd_rn((double)(k - 1), 0.5), g.dx[2]));
+      const double d0 = w0, d1 = w1, d2 = w2, e0 = v0, e1 = v1, e2 = v2;
+      s[0] = __dsub_rn(__dmul_rn(py, d2), __dmul_rn(pz, d1));
+      s[1] = __dsub_rn(__dmul_rn(pz, d0), __dmul_rn(px, d2));
+      s[2] = __dsub_rn(__dmul_rn(px, d1), __dmul_rn(py, d0));
+      s[3] = __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)), __dmul_rn(d2, d2));
+      s[4] = __dadd_rn(__dadd_rn(__dmul_rn(e0, e0), __dmul_rn(e1, e1)), __dmul_rn(e2, e2));
+    }
+  }
+  if (!a.partial) return;
+  __shared__ double red[kTy][5];
+  for (int m = 0; m < 5; ++m) {
+    double t = wave_sum(s[m]);
+    if (threadIdx.x == 0) red[threadIdx.y][m] = t;
+  }
+  __syncthreads();
+  if (threadIdx.y == 0 && threadIdx.x < 5) {
+    double t = 0.0;
+    for (int w = 0; w < kTy; ++w) t += red[w][threadIdx.x];
+    const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+    a.partial[5 * b + threadIdx.x] = t;
+  }
+}
+
+// fixed-order reduction of the per-block sums (one block)
+__global__ void __launch_bounds__(256) k_penalise_sums(const double *partial, int64_t nb, double *out) {
+  __shared__ double sh[5][256];
+  for (int m = 0; m < 5; ++m) {
+    double t = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += 256) t += partial[5 * b + m];
+    sh[m][threadIdx.x] = t;
+  }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int m = 0; m < 5; ++m) sh[m][threadIdx.x] += sh[m][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+struct PenWS {
+  double *partial = nullptr;
+  int64_t cap = 0;
+  double *sums = nullptr;
+};
+static PenWS *g_pen = nullptr;
+
+void penalise_release() {
+  if (g_pen) {
+    if (g_pen->partial) (void)hipFree(g_pen->partial);
+    if (g_pen->sums) (void)hipFree(g_pen->sums);
+    delete g_pen;
+    g_pen = nullptr;
+  }
+}
+
+template <typename T>
+static int penalise_launch(const pmgpu_geom *geom, const void *vel, const void *chi, const void *us_field,
+                           const double *us_const, double relax, int fd_order, void *vort, double *diag,
+                           hipStream_t st) {
+  PM_CHECK(geom && vel && chi && vort, "pmgpu_penalise: null argument");
+  PM_CHECK(fd_order == 2 || fd_order == 4, "pmgpu_penalise: fd_order must be 2 or 4");
+  PenArgs<T> a;
+  int32_t ext[3];
+  if (pmgpu_mesh_extent(geom, ext)) return -1;
+  for (int d = 0; d < 3; ++d) {
+    a.g.E[d] = ext[d];
+    a.g.N[d] = geom->ncell[d];
+    a.g.ng[d] = (ext[d] - geom->ncell[d]) / 2;
+    a.g.xmin[d] = geom->xmin[d];
+    a.g.dx[d] = geom->dx[d];
+    PM_CHECK(a.g.ng[d] >= fd_order / 2, "pmgpu_penalise: the stencil needs fd_order/2 ghost layers");
+  }
+  a.vel = (const T *)vel;
+  a.chi = (const T *)chi;
+  a.us = (const T *)us_field;
+  a.us0 = T(us_const ? us_const[0] : 0.0);
+  a.us1 = T(us_const ? us_const[1] : 0.0);
+  a.us2 = T(us_const ? us_const[2] : 0.0);
+  a.relax = T(relax);
+  // c_1_2dx2 = 1/(2 dx) is formed in the working precision from dx
+  a.cx = T(1.0) / (T(2.0) * T(geom->dx[0]));
+  a.cy = T(1.0) / (T(2.0) * T(geom->dx[1]));
+  a.cz = T(1.0) / (T(2.0) * T(geom->dx[2]));
+  a.vort = (T *)vort;
+  PM_CHECK(geom->ncell[2] < 65536 && geom->ncell[1] < (1 << 20), "pmgpu_penalise: mesh too large");
+  dim3 grid((geom->ncell[0] + kTx - 1) / kTx, (geom->ncell[1] + kTy - 1) / kTy, geom->ncell[2]);
+  const int64_t nb = (int64_t)grid.x * grid.y * grid.z;
+  a.partial = nullptr;
+  if (diag) {
+    if (!g_pen) g_pen = new PenWS;
+    if (g_pen->cap < nb) {
+      if (g_pen->partial) (void)hipFree(g_pen->partial);
+      g_pen->partial = nullptr;
+      g_pen->cap = 0;
+      PM_HIP(hipMalloc(&g_pen->partial, sizeof(double) * 5 * nb));
+      g_pen->cap = nb;
+    }
+    if (!g_pen->sums) PM_HIP(hipMalloc(&g_pen->sums, sizeof(double) * 8));
+    a.partial = g_pen->partial;
+  }
+  if (fd_order == 4)
+    hipLaunchKernelGGL((k_penalise<T, 4>), grid, dim3(kTx, kTy), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_penalise<T, 2>), grid, dim3(kTx, kTy), 0, st, a);
+  PM_HIP(hipGetLastError());
+  if (diag) {
+    hipLaunchKernelGGL(k_penalise_sums, dim3(1), dim3(256), 0, st, (const double *)a.partial, nb, g_pen->sums);
+    PM_HIP(hipGetLastError());
+    double h[5];
+    PM_HIP(hipMemcpyAsync(h, g_pen->sums, sizeof(h), hipMemcpyDeviceToHost, st));
+    PM_HIP(hipStreamSynchronize(st));
+    std::memcpy(diag, h, sizeof(h));
+  }
+  return 0;
+}
+
+}  // namespace pmgpu
+
+using namespace pmgpu;
+
+extern "C" {
+
+int pmgpu_penalise(const pmgpu_geom *g, const double *vel, const double *chi, const double *solid_vel,
+                   const double us_const[3], double relax, int32_t fd_order, double *vort, double diag[5]) {
+  PM_CHECK(g && vel && chi && vort, "pmgpu_penalise: null argument");
+  int32_t e[3];
+  if (pmgpu_mesh_extent(g, e)) return -1;
+  const size_t n = (size_t)e[0] * e[1] * e[2];
+  double *dv = nullptr, *dc = nullptr, *ds = nullptr, *dw = nullptr;
+  int rc = -1;
+  if (hipMalloc(&dv, 24 * n) == hipSuccess && hipMalloc(&dc, 8 * n) == hipSuccess &&
+      hipMalloc(&dw, 24 * n) == hipSuccess && (!solid_vel || hipMalloc(&ds, 24 * n) == hipSuccess) &&
+      hipMemcpy(dv, vel, 24 * n, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(dc, chi, 8 * n, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(dw, vort, 24 * n, hipMemcpyHostToDevice) == hipSuccess &&
+      (!solid_vel || hipMemcpy(ds, solid_vel, 24 * n, hipMemcpyHostToDevice) == hipSuccess)) {
+    rc = penalise_launch<double>(g, dv, dc, ds, us_const, relax, fd_order, dw, diag, nullptr);
+    if (rc == 0 && hipMemcpy(vort, dw, 24 * n, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+  } else {
+    set_error("pmgpu_penalise: device allocation or copy failed");
+  }
+  if (dv) (void)hipFree(dv);
+  if (dc) (void)hipFree(dc);
+  if (ds) (void)hipFree(ds);
+  if (dw) (void)hipFree(dw);
+  return rc;
+}
+
+int pmgpu_penalise_device(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_chi,
+                          const void *d_solid_vel, const double us_const[3], double relax, int32_t fd_order,
+                          void *d_vort, double *diag, void *stream) {
+  if (precision == PMGPU_FP32)
+    return penalise_launch<float>(g, d_vel, d_chi, d_solid_vel, us_const, relax, fd_order, d_vort, diag,
+                                  (hipStream_t)stream);
+  PM_CHECK(precision == PMGPU_FP64, "pmgpu_penalise_device: precision must be PMGPU_FP64 or PMGPU_FP32");
+  return penalise_launch<double>(g, d_vel, d_chi, d_solid_vel, us_const, relax, fd_order, d_vort, diag,
+                                 (hipStream_t)stream);
+}
+
+}  // extern "C"

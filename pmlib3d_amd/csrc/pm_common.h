@@ -52,7 +52,8 @@ const std::string &get_error();
     }                                \
   } while (0)
 
-// frees the particle workspace (particles.hip)
+// free the workspaces of particles.hip and penalise.hip
 void particles_release();
+void penalise_release();
 
 }  // namespace pmgpu

@@ -20,12 +20,21 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const
   const V *in = reinterpret_cast<const V *>(a.in) + b * a.bs_in + col0;
   V *out = reinterpret_cast<V *>(a.out) + b * a.bs_out + col0;
   const int tid = threadIdx.x;
-  const int c_ld = tid % T;
+  const int c_ld = tid % T, r0 = tid / T;  // NT / T == P rows per sweep
   const bool col_ok = col0 + c_ld < a.ncols;
-  for (int row = tid / T; row < L; row += NT / T) {
-    V v = cmake<V>(0, 0);
-    if (row < a.nin && col_ok) v = in[(int64_t)row * a.rs_in + c_ld];
-    lds[c_ld * ST + pad(row)] = v;
+  constexpr int E = (L + P - 1) / P;
+  // issue every load of this thread before the first LDS write
+  V v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int row = r0 + i * P;
+    v[i] = cmake<V>(0, 0);
+    if (row < a.nin && col_ok) v[i] = in[(int64_t)row * a.rs_in + c_ld];
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int row = r0 + i * P;
+    if (row < L) lds[c_ld * ST + pad(row)] = v[i];
   }
   __syncthreads();
   const int line = tid / P, p = tid % P;
@@ -49,6 +58,30 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
   const int tid = threadIdx.x;
   // packed pairs x[2m] + i x[2m+1]; m >= ceil(Min/2) are zero
   const int half = (a.Min + 1) / 2;
+  if (a.cs == 1 && a.xs == NC) {
+    // the NC components of a line are one contiguous row of NC*Min words:
+    // read it word by word across the lanes and scatter into the LDS lines
+    constexpr int W = NC * 2 * NH, K = (T * W + NT - 1) / NT;
+    R *ldsr = reinterpret_cast<R *>(lds);
+    R v[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int idx = tid + i * NT;
+      const int t = idx / W, w = idx - t * W;
+      const int y = y0 + t;
+      v[i] = (R)0;
+      if (t < T && w < NC * a.Min && y < a.My) v[i] = in[(int64_t)y * a.ys + (int64_t)z * a.zs + w];
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int idx = tid + i * NT;
+      const int t = idx / W, w = idx - t * W;
+      if (t < T) {
+        const int c = w % NC, x = w / NC;
+        ldsr[2 * ((t * NC + c) * ST + pad(x >> 1)) + (x & 1)] = v[i];
+      }
+    }
+  } else
   for (int idx = tid; idx < NL * NH; idx += NT) {
     int l = idx / NH, m = idx - l * NH;
     int t = l / NC, c = l - t * NC;
@@ -122,6 +155,18 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
   R *out = reinterpret_cast<R *>(a.out);
   const R sc = (R)a.scale;
   const int halfo = (a.Mout + 1) / 2;
+  if (a.cs == 1 && a.xs == NC) {
+    constexpr int W = NC * 2 * NH;
+    const R *ldsr = reinterpret_cast<const R *>(lds);
+    for (int idx = tid; idx < T * W; idx += NT) {
+      const int t = idx / W, w = idx - t * W;
+      const int y = y0 + t;
+      if (w >= NC * a.Mout || y >= a.My) continue;
+      const int c = w % NC, x = w / NC;
+      out[(int64_t)y * a.ys + (int64_t)z * a.zs + w] = ldsr[2 * ((t * NC + c) * ST + pad(x >> 1)) + (x & 1)] * sc;
+    }
+    return;
+  }
   for (int idx = tid; idx < NL * halfo; idx += NT) {
     int l = idx / halfo, m = idx - l * halfo;
     int t = l / NC, c = l - t * NC;
@@ -162,12 +207,23 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
   V *B = reinterpret_cast<V *>(a.B);
   const int64_t tile = (int64_t)ky * a.kys + (int64_t)kx0 * 3;  // kys = padded row
   V *myl = lds + line * ST;
-  // load the 3*TC interleaved lines, zero rows >= Mz
-  for (int idx = tid; idx < L * NLN; idx += NT) {
-    int row = idx / NLN, cl = idx - row * NLN;
-    V v = cmake<V>(0, 0);
-    if (row < a.Mz && cl < ncl) v = B[tile + (int64_t)row * a.rs + cl];
-    lds[cl * ST + pad(row)] = v;
+  // load the 3*TC interleaved lines, zero rows >= Mz; thread tid owns line
+  // word tid % NLN of rows tid / NLN + i P (NT = P * NLN), and issues all of
+  // its loads before the first LDS write
+  {
+    const int cl = tid % NLN, r0 = tid / NLN;
+    V v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int row = r0 + i * P;
+      v[i] = cmake<V>(0, 0);
+      if (row < a.Mz && cl < ncl) v[i] = B[tile + (int64_t)row * a.rs + cl];
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int row = r0 + i * P;
+      if (row < L) lds[cl * ST + pad(row)] = v[i];
+    }
   }
   __syncthreads();
   fft<L, -1>(myl, p, tw);

@@ -1,0 +1,89 @@
+"""Device-resident vortex time step built from the library's routines.
+
+This is the caller side of the boundary: the client time loop of the
+reference's vortex codes (vortex_codes/*/source/main.f) calls, per RK stage,
+pmlib_interp_particle_mesh -> pmlib_poisson_solve -> pmlib_interp_mesh_particle
+-> pmlib_particles_advance, then deposits again and calls pmlib_remesh.
+VortexStep runs that sequence with every array resident in HBM (torch
+tensors as device memory, the HIP kernels through the C ABI); only the
+particle count of the remesh crosses to the host.
+"""
+import numpy as np
+
+from . import _lib
+from .core import PoissonSolver, mesh_extent
+from .particles import ParticleOps
+
+_FIELDS = ("pos", "vel", "vort", "dvort", "vel_rk", "dvort_rk")
+
+
+class VortexStep:
+    def __init__(self, patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3, capacity=None,
+                 device="cuda:0"):
+        import torch
+        self.torch = torch
+        self.dev = torch.device(device)
+        self.patch = patch
+        self.rk_order = rk_order
+        self.solver = PoissonSolver(patch, _lib.FP64, order=order, kernel=1, eps_radius=eps_radius,
+                                    interp_order=interp_order)
+        self.solver.setup()
+        self.ops = ParticleOps(patch, interp_order)
+        ext = mesh_extent(patch, interp_order)
+        nm = 3 * int(np.prod(ext))
+        self.ext = ext
+        self.mesh_vort = torch.zeros(nm, dtype=torch.float64, device=self.dev)
+        self.mesh_vel = torch.zeros(nm, dtype=torch.float64, device=self.dev)
+        self.cap = int(capacity if capacity is not None else np.prod(ext))
+        self.buf = [{k: torch.zeros(3 * self.cap, dtype=torch.float64, device=self.dev) for k in _FIELDS}
+                    for _ in range(2)]
+        self.cur = 0
+        self.np = 0
+
+    @property
+    def arrays(self):
+        return self.buf[self.cur]
+
+    def load(self, pos, vort):
+        pos = np.asfortranarray(pos, dtype=np.float64)
+        n = pos.shape[1]
+        if n > self.cap:
+            raise _lib.PmgpuError("more particles than the step's capacity")
+        a = self.arrays
+        for k in _FIELDS:
+            a[k].zero_()
+        a["pos"][:3 * n].copy_(self.torch.from_numpy(pos.ravel(order="F")))
+        a["vort"][:3 * n].copy_(self.torch.from_numpy(np.asfortranarray(vort).ravel(order="F")))
+        self.np = n
+
+    def fetch(self):
+        a = self.arrays
+        n = self.np
+        return tuple(a[k][:3 * n].cpu().numpy().reshape((3, n), order="F") for k in ("pos", "vort", "vel"))
+
+    def stage(self, s, dt):
+        a = self.arrays
+        n = self.np
+        self.ops.p2m(a["pos"].data_ptr(), a["vort"].data_ptr(), n, self.mesh_vort.data_ptr(), clear=True)
+        self.solver.solve_device(self.mesh_vort.data_ptr(), self.mesh_vel.data_ptr())
+        self.ops.m2p(self.mesh_vel.data_ptr(), a["pos"].data_ptr(), n, a["vel"].data_ptr())
+        a["dvort"][:3 * n].zero_()
+        self.np = self.ops.advance(self.rk_order, s, dt, [a[k].data_ptr() for k in _FIELDS], n)
+
+    def remesh(self, min_vort, min_dvort=0.0):
+        a = self.arrays
+        self.ops.p2m(a["pos"].data_ptr(), a["vort"].data_ptr(), self.np, self.mesh_vort.data_ptr(), clear=True)
+        nxt = self.buf[1 - self.cur]
+        n = self.ops.remesh(self.mesh_vort.data_ptr(), None, self.mesh_vel.data_ptr(), min_vort, min_dvort,
+                            out=[nxt[k].data_ptr() for k in ("pos", "vort", "vel", "dvort", "vel_rk",
+                                                             "dvort_rk")],
+                            cap=self.cap, dvort=False, vel=True)
+        if n > self.cap:
+            raise _lib.PmgpuError("remesh made %d particles, capacity %d" % (n, self.cap))
+        self.cur = 1 - self.cur
+        self.np = n
+
+    def step(self, dt, min_vort, min_dvort=0.0):
+        for s in range(1, self.rk_order + 1):
+            self.stage(s, dt)
+        self.remesh(min_vort, min_dvort)

@@ -9,6 +9,7 @@ Outputs (small, committed):
                          the pmlib_patch_adjust grids (order 6, eps 2h, ng 4)
   ref_solve_*.npz        inputs + reference outputs of small solves
   ref_particles_*.npz    inputs + reference outputs of P2M/M2P/remesh/advance
+  ref_step_*.npz         inputs + reference outputs of whole two-ring vortex steps
 """
 import json
 import os
@@ -97,6 +98,32 @@ def particle_case(name, ncell, ng, bc, io, seed):
     np.savez_compressed(os.path.join(HERE, "ref_particles_%s.npz" % name), **out)
 
 
+STEP_CASES = {
+    # name: (n, bound_cond, nsteps, min_vort)
+    "free16": (16, (0, 0, 0), 2, 1e-3),
+    "mixed16": (16, (1, 1, 0), 1, 1e-3),
+}
+
+
+def step_case(name, spec):
+    """The two-ring vortex step (P2M, order-6 solve, M2P, RK2 advance, then
+    P2M + remesh) run by the reference's own routines, nsteps times."""
+    from tests.helpers import ring_particles
+    n, bc, nsteps, min_vort = spec
+    h = 2.0 / n
+    g = O.make_geom((n, n, n), 0, [-1.0] * 3, h, bc, 3)
+    p = O.make_params(6, 1, 2.0)
+    pos, vort = ring_particles(n)
+    job = dict(O.geom_job(g, p), mode="step", npart=pos.shape[1], rk_order=2, dtime=0.5 * h,
+               nsteps=nsteps, min_vort=min_vort, min_dvort=0.0)
+    raw, _ = O.ref_run(job, [pos, vort])
+    npart = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    out = np.frombuffer(raw[4:]).reshape((3, 3 * npart), order="F")
+    np.savez_compressed(os.path.join(HERE, "ref_step_%s.npz" % name), pos_in=pos, vort_in=vort,
+                        pos=out[:, :npart], vort=out[:, npart:2 * npart], vel=out[:, 2 * npart:],
+                        spec=np.array(json.dumps(spec)))
+
+
 def kat():
     cases = []
     for (h, ncell, xmin, stored) in K.KAT_CASES:
@@ -118,6 +145,8 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref is not built: make -f oracle/Makefile.ref")
     for name, spec in SOLVE_CASES.items():
         solve_case(name, spec)
+    for name, spec in STEP_CASES.items():
+        step_case(name, spec)
     particle_case("m4_free", (10, 8, 12), 0, (0, 0, 0), 3, 11)
     particle_case("m4_ng2", (8, 10, 8), 2, (0, 0, 0), 3, 12)
     particle_case("m4_mixed", (8, 8, 10), 0, (1, 1, 0), 3, 13)

@@ -247,3 +247,89 @@ def test_oracle_matches_live_reference_random_case():
     vel, vo, _ = O.poisson_solve(g, p, O.poisson_setup(g, p), w, True)
     assert rel_linf(vel, ref[:w.size].reshape(w.shape, order="F")) <= 2e-15
     assert rel_linf(vo, ref[w.size:].reshape(w.shape, order="F")) <= 2e-15
+
+
+# ----------------------------------------------------------- penalisation
+def _poly_fields(g, deg):
+    """A velocity field whose components are polynomials of total degree
+    `deg`, its analytic curl, and the cell-centre coordinates."""
+    X, Y, Z = np.meshgrid(*cell_axes(g), indexing="ij")
+    rng = np.random.default_rng(deg)
+    a = rng.normal(size=(3, 4))
+    u = np.zeros((3,) + X.shape, order="F")
+    du = np.zeros((3, 3) + X.shape)   # du[c, d] = d u_c / d x_d
+    V = (X, Y, Z)
+    for c in range(3):
+        # u_c = a0 x^deg + a1 y^deg + a2 z^deg + a3 x y z^(deg-2)
+        u[c] = a[c, 0] * X ** deg + a[c, 1] * Y ** deg + a[c, 2] * Z ** deg + a[c, 3] * X * Y * Z ** (deg - 2)
+        du[c, 0] = deg * a[c, 0] * X ** (deg - 1) + a[c, 3] * Y * Z ** (deg - 2)
+        du[c, 1] = deg * a[c, 1] * Y ** (deg - 1) + a[c, 3] * X * Z ** (deg - 2)
+        du[c, 2] = deg * a[c, 2] * Z ** (deg - 1) + (deg - 2) * a[c, 3] * X * Y * Z ** max(deg - 3, 0)
+    curl = np.stack([du[2, 1] - du[1, 2], du[0, 2] - du[2, 0], du[1, 0] - du[0, 1]])
+    return u, curl, V
+
+
+@pytest.mark.parametrize("fd_order,deg", [(4, 4), (4, 3), (2, 2)])
+def test_penalisation_curl_exact_on_polynomials(fd_order, deg):
+    """Central differences of order p are exact on polynomials of degree p
+    (4th order: c1 = 1/6, c2 = 4/3 over 2h, penalisation_solve.f:127,195-273),
+    so with chi = 1 and a solid at rest the increment is -relax curl(u)
+    exactly up to rounding; a sign, index, coefficient or transposition
+    mistake breaks it.  The force and enstrophy sums follow from it."""
+    g = O.make_geom((10, 9, 8), 2, [-0.3, -0.2, -0.45], 1.0, (0, 0, 0))
+    g.dx[0], g.dx[1], g.dx[2] = 0.1, 0.12, 0.09
+    u, curl, (X, Y, Z) = _poly_fields(g, deg)
+    E = O.mesh_extent(g)
+    w0 = np.asfortranarray(np.random.default_rng(1).normal(size=(3,) + E))
+    relax = 0.7
+    w1, diag = O.penalise(g, u, np.ones(E), w0, relax=relax, fd_order=fd_order)
+    inner = (slice(None), slice(2, -2), slice(2, -2), slice(2, -2))
+    dw = -relax * curl[inner]
+    scale = np.abs(dw).max()
+    assert np.abs((w1 - w0)[inner] - dw).max() <= 1e-11 * scale
+    ghost = np.ones(E, dtype=bool)
+    ghost[2:-2, 2:-2, 2:-2] = False
+    assert np.array_equal(w1[:, ghost], w0[:, ghost])
+    px, py, pz = X[inner[1:]], Y[inner[1:]], Z[inner[1:]]
+    F = [np.sum(py * dw[2] - pz * dw[1]), np.sum(pz * dw[0] - px * dw[2]), np.sum(px * dw[1] - py * dw[0])]
+    assert np.allclose(diag[:3], F, rtol=1e-10, atol=1e-10 * scale * dw[0].size)
+    assert abs(diag[3] - np.sum(dw ** 2)) <= 1e-10 * np.sum(dw ** 2)
+    assert abs(diag[4] - np.sum(w1[inner] ** 2)) <= 1e-12 * np.sum(w1[inner] ** 2)
+
+
+def test_penalisation_brinkman_factor_and_solid_velocity():
+    """f = chi (us - u) is what gets differentiated (penalisation_solve.f:96-111):
+    the same increment comes from chi = 1, us = 0 and u = -f; and with chi = 1
+    a uniform solid velocity drops out of the curl."""
+    g = O.make_geom((9, 8, 10), 2, [-0.3, -0.2, -0.45], 0.1, (0, 0, 0))
+    u, _, (X, Y, Z) = _poly_fields(g, 3)
+    chi = 0.5 + 0.3 * X
+    us = np.zeros_like(u)
+    us[0] = 1.25   # free stream seen from the solid
+    E = O.mesh_extent(g)
+    w0 = np.asfortranarray(np.random.default_rng(3).normal(size=(3,) + E))
+    w_a, d_a = O.penalise(g, u, chi, w0, solid_vel=us, relax=0.9)
+    w_b, d_b = O.penalise(g, -(chi * (us - u)), np.ones(E), w0, relax=0.9)
+    assert np.array_equal(w_a, w_b) and np.array_equal(d_a, d_b)
+    inner = (slice(None), slice(2, -2), slice(2, -2), slice(2, -2))
+    w_c, _ = O.penalise(g, u, np.ones(E), w0, solid_vel=us, relax=0.9)
+    w_d, _ = O.penalise(g, u, np.ones(E), w0, relax=0.9)
+    assert np.abs(w_c[inner] - w_d[inner]).max() <= 1e-12 * np.abs(w_d[inner]).max()
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "ref_step_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[9:-4])
+def test_oracle_vortex_step_matches_reference_golden(path):
+    """Whole two-ring vortex steps (P2M, order-6 solve, M2P, RK2, remesh)
+    composed from the oracle's routines against the reference driver."""
+    from tests.helpers import oracle_vortex_step
+    d = np.load(path, allow_pickle=False)
+    n, bc, nsteps, min_vort = json.loads(str(d["spec"]))
+    h = 2.0 / n
+    g = O.make_geom((n, n, n), 0, [-1.0] * 3, h, bc, 3)
+    p = O.make_params(6, 1, 2.0)
+    pos, vort, vel = oracle_vortex_step(g, p, d["pos_in"], d["vort_in"], 0.5 * h, nsteps, min_vort)
+    assert pos.shape == d["pos"].shape
+    assert np.array_equal(pos, d["pos"])
+    assert rel_linf(vort, d["vort"]) <= 1e-13
+    assert rel_linf(vel, d["vel"]) <= 1e-13
