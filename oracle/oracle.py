@@ -227,10 +227,12 @@ def _fmt(v):
     return str(v)
 
 
-def ref_run(job, inputs, nproc=1, timeout=600):
+def ref_run(job, inputs, nproc=1, timeout=600, driver=None):
     """Run the reference harness with namelist `job` (dict); `inputs` is a list
     of arrays written back to back (Fortran order) as the input file.
-    Returns (raw float64 output, stdout)."""
+    `driver` replaces the harness binary (the same harness built on another
+    implementation of the subroutines).  Returns (raw float64 output, stdout)."""
+    drv = REF_DRIVER if driver is None else driver
     with tempfile.TemporaryDirectory() as td:
         inf = os.path.join(td, "in.bin")
         outf = os.path.join(td, "out.bin")
@@ -239,10 +241,10 @@ def ref_run(job, inputs, nproc=1, timeout=600):
                 np.asarray(a).ravel(order="F").tofile(f)
         job = dict(job, infile=inf, outfile=outf)
         nml = "&job " + ", ".join("%s=%s" % (k, _fmt(v)) for k, v in job.items()) + " /\n"
-        cmd = [REF_DRIVER] if nproc == 1 else [MPIEXEC, "-n", str(nproc), REF_DRIVER]
+        cmd = [drv] if nproc == 1 else [MPIEXEC, "-n", str(nproc), drv]
         r = subprocess.run(cmd, input=nml, text=True, capture_output=True, timeout=timeout)
         if r.returncode != 0:
-            raise RuntimeError("ref_driver failed: %s %s" % (r.stdout[-2000:], r.stderr[-2000:]))
+            raise RuntimeError("%s failed: %s %s" % (os.path.basename(drv), r.stdout[-2000:], r.stderr[-2000:]))
         raw = open(outf, "rb").read() if os.path.exists(outf) else b""
         return raw, r.stdout
 

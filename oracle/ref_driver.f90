@@ -9,6 +9,9 @@
 ! it would always grow the grid: patch/pmlib_patch_adjust.f:53).  The job is
 ! read as a namelist from stdin; raw arrays use stream I/O in Fortran order.
 !
+! Built with -DPMGPU_SHIM the same driver calls the GPU subroutines of
+! pmlib3d_amd/fortran/pmgpu_pmlib.f90 instead (tests/fortran/Makefile).
+!
 ! Modes
 !   solve    : pmlib_poisson_setup + pmlib_poisson_solve   (poisson/*.f)
 !   p2m      : pmlib_interp_particle_mesh                   (interpolation/)
@@ -23,10 +26,16 @@ PROGRAM ref_driver
   USE pmlib_mod_patch
   USE pmlib_mod_topology
   USE pmlib_mod_mesh
+#ifdef PMGPU_SHIM
+  ! the same client code on the GPU subroutines (pmlib3d_amd/fortran)
+  USE pmlib_mod_particles, ONLY: class_particles
+  USE pmgpu_pmlib
+#else
   USE pmlib_mod_particles
   USE pmlib_mod_poisson
   USE pmlib_mod_interpolation
   USE pmlib_mod_remesh
+#endif
   IMPLICIT NONE
 
   CHARACTER(LEN=16)  :: mode = 'solve'
@@ -250,6 +259,9 @@ PROGRAM ref_driver
     CALL die('unknown mode')
   END SELECT
 
+#ifdef PMGPU_SHIM
+  CALL pmgpu_release(ierr)
+#endif
   CALL MPI_FINALIZE(ierr)
 
 CONTAINS

@@ -1,0 +1,97 @@
+"""The Fortran drop-in layer on the GPU: the reference's own harness
+(oracle/ref_driver.f90) rebuilt with USE pmgpu_pmlib in place of the
+reference's Poisson / interpolation / remesh / advance modules
+(tests/fortran/Makefile), run on the inputs of the golden fixtures, must give
+the reference's outputs: the same client code, the same file format, the
+parity bars of the C ABI (solve and P2M <= 1e-12 relative L-inf; M2P,
+remesh, advance bit-identical; whole steps with the same particle set)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import rel_linf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SHIM = os.path.join(ROOT, "tests", "fortran", "_build", "shim_driver")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(SHIM),
+                                 reason="tests/fortran/_build/shim_driver not built (needs the reference "
+                                        "type modules of oracle/_ref)")]
+
+
+def run(job, inputs):
+    print("shim job", {k: v for k, v in job.items() if k in ("mode", "ncell", "bound_cond")}, flush=True)
+    return O.ref_run(job, inputs, driver=SHIM, timeout=300)
+
+
+SOLVES = sorted(f for f in os.listdir(GOLD) if f.startswith("ref_solve_"))
+
+
+@pytest.mark.parametrize("name", SOLVES)
+def test_shim_solve_matches_reference(name):
+    d = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    ncell, ng, bc, order, (rv, rp, cv) = json.loads(str(d["spec"]))
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+    p = O.make_params(order, 1, 2.0)
+    raw, _ = run(dict(O.geom_job(g, p), mode="solve", reg_vort=rv, reproj=rp, calc_vecpot=cv),
+                 [d["vort_in"]])
+    out = np.frombuffer(raw, dtype=np.float64)
+    n = d["vort_in"].size
+    shape = d["vort_in"].shape
+    assert rel_linf(out[:n].reshape(shape, order="F"), d["vel"]) <= 1e-12
+    q = 1
+    if rv or rp:
+        assert rel_linf(out[n:2 * n].reshape(shape, order="F"), d["vort"]) <= 1e-12
+        q = 2
+    if cv:
+        assert rel_linf(out[q * n:(q + 1) * n].reshape(shape, order="F"), d["psi"]) <= 1e-12
+
+
+PARTS = sorted(f for f in os.listdir(GOLD) if f.startswith("ref_particles_"))
+
+
+@pytest.mark.parametrize("name", PARTS)
+def test_shim_particles_match_reference(name):
+    d = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    ncell, ng, bc, io = json.loads(str(d["geom"]))
+    g = O.make_geom(ncell, ng, [-0.5, -0.5, -0.5], 1.0 / 8, bc, io)
+    npart = d["pos"].shape[1]
+    ext = d["p2m"].shape
+    raw, _ = run(dict(O.geom_job(g), mode="p2m", npart=npart, interp_order=io), [d["pos"], d["val"]])
+    assert rel_linf(np.frombuffer(raw).reshape(ext, order="F"), d["p2m"]) <= 1e-12
+    raw, _ = run(dict(O.geom_job(g), mode="m2p", npart=npart, interp_order=io), [d["mesh"], d["pos"]])
+    assert np.array_equal(np.frombuffer(raw).reshape((3, npart), order="F"), d["m2p"])
+    for ig in (0, 1):
+        raw, _ = run(dict(O.geom_job(g), mode="remesh", min_vort=1.2, min_dvort=0.15, incl_ghost=bool(ig),
+                          interp_order=io), [d["rm_vort"], d["rm_dvort"], d["rm_vel"]])
+        n = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+        assert np.array_equal(np.frombuffer(raw[4:]).reshape((3, 3 * n), order="F"), d["remesh%d" % ig])
+    arrs = [d[k] for k in ("a_pos", "a_vel", "a_vort", "a_dvort", "a_vel_rk", "a_dvort_rk")]
+    for st in (1, 2):
+        raw, _ = run(dict(O.geom_job(g), mode="advance", npart=npart, stage=st, dtime=0.2, rk_order=2,
+                          interp_order=io), arrs)
+        n = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+        assert np.array_equal(np.frombuffer(raw[4:]).reshape((3, 4 * n), order="F"), d["advance%d" % st])
+
+
+@pytest.mark.parametrize("name", ["free16", "mixed16"])
+def test_shim_vortex_step_matches_reference(name):
+    d = np.load(os.path.join(GOLD, "ref_step_%s.npz" % name), allow_pickle=False)
+    n, bc, nsteps, min_vort = json.loads(str(d["spec"]))
+    h = 2.0 / n
+    g = O.make_geom((n, n, n), 0, [-1.0] * 3, h, bc, 3)
+    p = O.make_params(6, 1, 2.0)
+    raw, out = run(dict(O.geom_job(g, p), mode="step", npart=d["pos_in"].shape[1], rk_order=2, dtime=0.5 * h,
+                        nsteps=nsteps, min_vort=min_vort, min_dvort=0.0), [d["pos_in"], d["vort_in"]])
+    npart = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    res = np.frombuffer(raw[4:]).reshape((3, 3 * npart), order="F")
+    assert npart == d["pos"].shape[1]
+    assert np.array_equal(res[:, :npart], d["pos"])
+    assert rel_linf(res[:, npart:2 * npart], d["vort"]) <= 1e-12
+    assert rel_linf(res[:, 2 * npart:], d["vel"]) <= 1e-12
+    print(out.strip().splitlines()[-1], flush=True)
