@@ -106,6 +106,61 @@ def cpu_reference_baseline(w64):
             "sample": "one 128^3 solve of the C oracle (1 thread), scaled by L^3 log L to 256^3"}
 
 
+def step_benchmark(steps, warmup, n=128, precision=0):
+    """The two-ring vortex step (config of 128^3 cells, 128^3 particles on the
+    jittered cell-centre lattice, order-6 solve, M'4, RK2, dt = h/2, remesh
+    with min_vort = 0) on the GPU, every array resident in HBM; precision 0
+    is FP64 throughout, 1 the FP32 mesh side (particles FP64).
+    Returns (Mparticle-steps/s, ms per step, particle count)."""
+    import torch
+    import pmlib3d_amd as pm
+    from tests.helpers import ring_particles
+    h = 2.0 / n
+    patch = pm.Patch(xmin=(-1.0,) * 3, xmax=(1.0,) * 3, ncell=(n,) * 3, dx=(h,) * 3)
+    pos, vort = ring_particles(n)
+    st = pm.VortexStep(patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3,
+                       capacity=max(pos.shape[1], n ** 3), precision=precision)
+    st.load(pos, vort)
+    for _ in range(warmup):
+        st.step(0.5 * h, 0.0)
+    torch.cuda.synchronize()
+    counted = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        counted += st.np
+        st.step(0.5 * h, 0.0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return counted / dt / 1e6, 1e3 * dt / steps, st.np
+
+
+def cpu_step_baseline(n=128):
+    """One two-ring vortex step of the compiled reference (one rank: its step
+    mode is single-rank), FP64."""
+    from oracle import oracle as O
+    from tests.helpers import ring_particles
+    if not (O.ref_available() and os.path.exists(O.MPIEXEC)):
+        return None
+    h = 2.0 / n
+    g = O.make_geom((n,) * 3, 0, [-1.0] * 3, h, (0, 0, 0), 3)
+    p = O.make_params(6, 1, 2.0)
+    pos, vort = ring_particles(n)
+    try:
+        _, out = O.ref_run(dict(O.geom_job(g, p), mode="step", npart=pos.shape[1], rk_order=2,
+                                dtime=0.5 * h, nsteps=1, min_vort=0.0, min_dvort=0.0),
+                           [pos, vort], timeout=400)
+    except Exception as e:  # pragma: no cover - reported, not hidden
+        log("cpu reference step failed (%s)" % e)
+        return None
+    line = [l for l in out.splitlines() if l.startswith("TIMING")][0]
+    step_s = float(line.split("step_s=")[1].split()[0])
+    log("cpu reference step: %s" % line)
+    return {"value": pos.shape[1] / step_s / 1e6, "unit": "Mparticle-steps/s", "cores": 1,
+            "kind": "reference",
+            "sample": "one two-ring vortex step at 128^3 (2.1M particles, order-6 solve, M'4, RK2, "
+                      "remesh), FP64, native Glassman FFT, 1 rank"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +168,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--step-steps", type=int, default=5,
+                    help="timed two-ring vortex steps at 128^3 (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -198,9 +255,23 @@ def main():
         (sps64, 1e3 / sps64, tot64_ms, z64_ms))
     log("fp32 vs fp64 relative L2 velocity error %.3e" % err)
 
+    step = None
+    if args.step_steps > 0:
+        del s32, s64
+        step = {"workload": "two-ring vortex step, 128^3 free, 128^3 jittered particles, order 6, "
+                            "eps 2h, M'4, RK2 dt=h/2, remesh min_vort=0; FP32 = FP32 meshes, solve and "
+                            "Kahan-summed interpolation, FP64 particles"}
+        for name, prec in (("fp64", 0), ("fp32", 1)):
+            mps, ms, npart = step_benchmark(args.step_steps, 1, precision=prec)
+            log("vortex step %s: %.1f Mparticle-steps/s (%.3f ms/step, %d particles)" % (name, mps, ms, npart))
+            step[name] = {"Mparticle_steps_per_s": mps, "ms_per_step": ms, "particles": npart}
+
     cpu = None
+    cpu_step = None
     if not args.no_cpu_baseline:
         cpu = cpu_reference_baseline(w64.cpu().numpy().reshape((3, N, N, N), order="F"))
+        if step is not None:
+            cpu_step = cpu_step_baseline()
 
     out = {
         "metric": "unbounded-Poisson solves/s + Mparticle-steps/s at 256³, FP32 vs FP64 GPU",
@@ -227,7 +298,9 @@ def main():
         "solve_algorithmic_bytes": {"fp32": bytes32, "fp64": bytes64},
         "solve_achieved_GBs": {"fp32": bytes32 / (1e-3 / sps32 * 1e3) / 1e9 * 1e3 / 1e3,
                                "fp64": bytes64 * sps64 / 1e9},
-        "roofline": {"bound": "hbm", "kernel": "k_z_convolve<float,8> (fused z pass)",
+        "vortex_step": step,
+        "cpu_baseline_step": cpu_step,
+        "roofline": {"bound": "hbm", "kernel": "k_z3_f<float,512,TC> (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": z_gbs / PEAK_HBM_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": z_bytes32, "avg_launch_ms": z32_ms},

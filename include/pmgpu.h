@@ -92,8 +92,12 @@ int pmgpu_poisson_last_timing(pmgpu_poisson_t h, double *zpass_ms, double *total
 int pmgpu_poisson_set_timing(pmgpu_poisson_t h, int32_t enable);
 
 /* ------------------------------------------------------------------------
- * Particles (FP64 throughout; the *_device variants take device pointers and
- * a hipStream_t, NULL for the default stream).  Particle arrays are (3, np).
+ * Particles.  Particle arrays are (3, np) FP64 everywhere (positions in FP64
+ * keep the particle -> cell index exact).  The host entry points use FP64
+ * meshes; the *_device variants take device pointers, a hipStream_t (NULL:
+ * the default stream) and the mesh precision: PMGPU_FP64 reproduces the
+ * reference's arithmetic, PMGPU_FP32 uses an FP32 mesh with FP32 kernel
+ * weights and Kahan-compensated sums.
  * ------------------------------------------------------------------------ */
 
 /* Replaces pmlib_interp_particle_mesh(topo, xpart, part, mesh, ierr, clear,
@@ -104,9 +108,9 @@ int pmgpu_poisson_set_timing(pmgpu_poisson_t h, int32_t enable);
  * ghost layers (:81-135).  Deterministic: no atomics touch the mesh. */
 int pmgpu_interp_particle_mesh(const pmgpu_geom *g, const double *xpart, const double *part, int64_t np,
                                double *mesh, int32_t clear, int32_t clear_ghost);
-int pmgpu_interp_particle_mesh_device(const pmgpu_geom *g, const void *d_xpart, const void *d_part,
-                                      int64_t np, void *d_mesh, int32_t clear, int32_t clear_ghost,
-                                      void *stream);
+int pmgpu_interp_particle_mesh_device(const pmgpu_geom *g, int32_t precision, const void *d_xpart,
+                                      const void *d_part, int64_t np, void *d_mesh, int32_t clear,
+                                      int32_t clear_ghost, void *stream);
 
 /* Replaces one field of pmlib_interp_mesh_particle(topo, mesh, part, ierr,
  * vort, dvort, vel, clear) (interpolation/pmlib_interp_mesh_particle.f:5-6):
@@ -114,8 +118,8 @@ int pmgpu_interp_particle_mesh_device(const pmgpu_geom *g, const void *d_xpart, 
  * reference's term order (bit-identical to it). */
 int pmgpu_interp_mesh_particle(const pmgpu_geom *g, const double *mesh, const double *pos, int64_t np,
                                double *out);
-int pmgpu_interp_mesh_particle_device(const pmgpu_geom *g, const void *d_mesh, const void *d_pos,
-                                      int64_t np, void *d_out, void *stream);
+int pmgpu_interp_mesh_particle_device(const pmgpu_geom *g, int32_t precision, const void *d_mesh,
+                                      const void *d_pos, int64_t np, void *d_out, void *stream);
 
 /* Replaces pmlib_remesh(patch, topo, mesh, part, min_vort, min_dvort, ierr,
  * dvort, vel, mask, incl_ghost) (remesh/pmlib_remesh.f:5-6): one particle at
@@ -131,8 +135,9 @@ int pmgpu_remesh(const pmgpu_geom *g, const double *vort_m, const double *dvort_
                  const uint8_t *mask, double min_vort, double min_dvort, int32_t dvort_flag,
                  int32_t vel_flag, int32_t incl_ghost, double *pos, double *vort, double *vel,
                  double *dvort, int64_t cap, int64_t *np_out);
-/* Device variant; also zeroes d_vel_rk and d_dvort_rk (either may be NULL). */
-int pmgpu_remesh_device(const pmgpu_geom *g, const void *d_vort_m, const void *d_dvort_m,
+/* Device variant (meshes in the given precision, particles FP64); also
+ * zeroes d_vel_rk and d_dvort_rk (either may be NULL). */
+int pmgpu_remesh_device(const pmgpu_geom *g, int32_t precision, const void *d_vort_m, const void *d_dvort_m,
                         const void *d_vel_m, const void *d_mask, double min_vort, double min_dvort,
                         int32_t dvort_flag, int32_t vel_flag, int32_t incl_ghost, void *d_pos,
                         void *d_vort, void *d_vel, void *d_dvort, void *d_vel_rk, void *d_dvort_rk,

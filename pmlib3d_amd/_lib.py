@@ -46,12 +46,12 @@ SIGNATURES = {
     "pmgpu_poisson_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pmgpu_poisson_set_timing": (C.c_int, [_P, _I]),
     "pmgpu_interp_particle_mesh": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_int64, _P, _I, _I]),
-    "pmgpu_interp_particle_mesh_device": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_int64, _P, _I, _I, _P]),
+    "pmgpu_interp_particle_mesh_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, C.c_int64, _P, _I, _I, _P]),
     "pmgpu_interp_mesh_particle": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_int64, _P]),
-    "pmgpu_interp_mesh_particle_device": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_int64, _P, _P]),
+    "pmgpu_interp_mesh_particle_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, C.c_int64, _P, _P]),
     "pmgpu_remesh": (C.c_int, [C.POINTER(Geom), _P, _P, _P, _P, C.c_double, C.c_double, _I, _I, _I,
                                _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
-    "pmgpu_remesh_device": (C.c_int, [C.POINTER(Geom), _P, _P, _P, _P, C.c_double, C.c_double, _I, _I,
+    "pmgpu_remesh_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, C.c_double, _I, _I,
                                       _I, _P, _P, _P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
     "pmgpu_particles_advance": (C.c_int, [C.POINTER(Geom), _I, _I, C.c_double, _I, _P, _P, _P, _P, _P,
                                           _P, C.c_int64, C.POINTER(C.c_int64)]),

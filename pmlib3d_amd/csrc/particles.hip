@@ -160,10 +160,14 @@ void particles_release() {
 }
 
 // ---------------------------------------------------------------- P2M
-int p2m_device(const pmgpu_geom *geom, const double *xpart, const double *part, int64_t np, double *mesh,
-               int clear, int clear_ghost, hipStream_t st) {
+// prec: PMGPU_FP64 (double mesh, reference arithmetic) or PMGPU_FP32 (float
+// mesh, FP32 weights with Kahan-compensated sums); particles are FP64.
+int p2m_device(const pmgpu_geom *geom, int prec, const double *xpart, const double *part, int64_t np, void *mesh,
+               int clear, hipStream_t st) {
   PGeom g;
   if (make_pgeom(geom, g)) return -1;
+  PM_CHECK(prec == PMGPU_FP64 || prec == PMGPU_FP32, "pmgpu: precision must be PMGPU_FP64 or PMGPU_FP32");
+  PM_CHECK(mesh && (np == 0 || (xpart && part)), "pmgpu_interp_particle_mesh: null argument");
   for (int d = 0; d < 3; ++d)
     PM_CHECK(g.bc[d] == 0 || g.ncell[d] >= 2 * g.ngl[d], "pmgpu: periodic axis shorter than two ghost layers");
   PartWS &w = ws();
@@ -191,20 +195,27 @@ int p2m_device(const pmgpu_geom *geom, const double *xpart, const double *part, 
     PM_HIP(rocprim::radix_sort_pairs(w.sort.p, sort_bytes, key_in, key_out, id_in, id_out, n, 0, end_bit, st));
   }
   if (scan_exclusive(w.count.as<int>(), w.start.as<int>(), nbins, w.scan, st)) return -1;
-  int accumulate = clear ? 0 : 1;
-  hipLaunchKernelGGL(k_p2m_gather, grid_for(nn), dim3(256), 0, st, g, xpart, part, w.start.as<int>(), id_out,
-                     mesh, accumulate);
+  const int accumulate = clear ? 0 : 1;
+  if (prec == PMGPU_FP64)
+    hipLaunchKernelGGL(k_p2m_gather, grid_for(nn), dim3(256), 0, st, g, xpart, part, w.start.as<int>(), id_out,
+                       (double *)mesh, accumulate);
+  else
+    hipLaunchKernelGGL(k_p2m_gather_f32, grid_for(nn), dim3(256), 0, st, g, xpart, part, w.start.as<int>(),
+                       id_out, (float *)mesh, accumulate);
   PM_HIP(hipGetLastError());
   for (int d = 0; d < 3; ++d)
     if (g.bc[d] != 0) {
-      hipLaunchKernelGGL(k_halo_add, grid_for(2LL * g.ngl[d] * 3 * g.E[(d + 1) % 3] * g.E[(d + 2) % 3]), dim3(256),
-                         0, st, mesh, g, d);
+      const dim3 grid = grid_for(2LL * g.ngl[d] * 3 * g.E[(d + 1) % 3] * g.E[(d + 2) % 3]);
+      if (prec == PMGPU_FP64)
+        hipLaunchKernelGGL(k_halo_add<double>, grid, dim3(256), 0, st, (double *)mesh, g, d);
+      else
+        hipLaunchKernelGGL(k_halo_add<float>, grid, dim3(256), 0, st, (float *)mesh, g, d);
       PM_HIP(hipGetLastError());
     }
   return 0;
 }
 
-__global__ void k_zero_ghosts(PGeom g, double *mesh) {
+template <typename T> __global__ void k_zero_ghosts(PGeom g, T *mesh) {
   const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
        idx += (int64_t)gridDim.x * blockDim.x) {
@@ -212,55 +223,90 @@ __global__ void k_zero_ghosts(PGeom g, double *mesh) {
     int j = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
     int k = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
     bool interior = i >= 1 && i <= g.ncell[0] && j >= 1 && j <= g.ncell[1] && k >= 1 && k <= g.ncell[2];
-    if (!interior) mesh[3 * idx] = mesh[3 * idx + 1] = mesh[3 * idx + 2] = 0.0;
+    if (!interior) mesh[3 * idx] = mesh[3 * idx + 1] = mesh[3 * idx + 2] = T(0);
   }
 }
 
-int p2m_device_full(const pmgpu_geom *geom, const double *xpart, const double *part, int64_t np, double *mesh,
-                    int clear, int clear_ghost, hipStream_t st) {
+int p2m_device_full(const pmgpu_geom *geom, int prec, const double *xpart, const double *part, int64_t np,
+                    void *mesh, int clear, int clear_ghost, hipStream_t st) {
   if (!clear && clear_ghost) {
     PGeom g;
     if (make_pgeom(geom, g)) return -1;
-    hipLaunchKernelGGL(k_zero_ghosts, grid_for(mesh_cells(g)), dim3(256), 0, st, g, mesh);
+    PM_CHECK(mesh, "pmgpu_interp_particle_mesh: null argument");
+    if (prec == PMGPU_FP32)
+      hipLaunchKernelGGL(k_zero_ghosts<float>, grid_for(mesh_cells(g)), dim3(256), 0, st, g, (float *)mesh);
+    else
+      hipLaunchKernelGGL(k_zero_ghosts<double>, grid_for(mesh_cells(g)), dim3(256), 0, st, g, (double *)mesh);
     PM_HIP(hipGetLastError());
   }
-  return p2m_device(geom, xpart, part, np, mesh, clear, 0, st);
+  return p2m_device(geom, prec, xpart, part, np, mesh, clear, st);
 }
 
 // ---------------------------------------------------------------- M2P
-int m2p_device(const pmgpu_geom *geom, const double *mesh, const double *pos, int64_t np, double *out,
+int m2p_device(const pmgpu_geom *geom, int prec, const void *mesh, const double *pos, int64_t np, double *out,
                hipStream_t st) {
   PGeom g;
   if (make_pgeom(geom, g)) return -1;
-  if (np > 0) hipLaunchKernelGGL(k_m2p, grid_for(np), dim3(256), 0, st, g, mesh, pos, np, out);
+  PM_CHECK(prec == PMGPU_FP64 || prec == PMGPU_FP32, "pmgpu: precision must be PMGPU_FP64 or PMGPU_FP32");
+  PM_CHECK(mesh && (np == 0 || (pos && out)), "pmgpu_interp_mesh_particle: null argument");
+  if (np > 0) {
+    if (prec == PMGPU_FP64)
+      hipLaunchKernelGGL(k_m2p, grid_for(np), dim3(256), 0, st, g, (const double *)mesh, pos, np, out);
+    else
+      hipLaunchKernelGGL(k_m2p_f32, grid_for(np), dim3(256), 0, st, g, (const float *)mesh, pos, np, out);
+  }
   PM_HIP(hipGetLastError());
   return 0;
 }
 
 // ------------------------------------------------------------- remesh
-int remesh_device(const pmgpu_geom *geom, const double *vort_m, const double *dvort_m, const double *vel_m,
+template <typename T>
+static void remesh_flags_launch(const PGeom &g, int64_t nn, const void *vort_m, const void *dvort_m,
+                                const uint8_t *mask, double min_vort, double min_dvort, int incl_ghost, int *flag,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_remesh_flags<T>, grid_for(nn), dim3(256), 0, st, g, (const T *)vort_m, (const T *)dvort_m,
+                     mask, min_vort, min_dvort, incl_ghost, flag);
+}
+template <typename T>
+static void remesh_create_launch(const PGeom &g, int64_t nn, const int *flag, const int *offs, const void *vort_m,
+                                 const void *dvort_m, const void *vel_m, double *pos, double *vort, double *vel,
+                                 double *dvort, double *vel_rk, double *dvort_rk, hipStream_t st) {
+  hipLaunchKernelGGL(k_remesh_create<T>, grid_for(nn), dim3(256), 0, st, g, flag, offs, (const T *)vort_m,
+                     (const T *)dvort_m, (const T *)vel_m, pos, vort, vel, dvort, vel_rk, dvort_rk);
+}
+
+int remesh_device(const pmgpu_geom *geom, int prec, const void *vort_m, const void *dvort_m, const void *vel_m,
                   const uint8_t *mask, double min_vort, double min_dvort, int copy_dvort, int copy_vel,
                   int incl_ghost, double *pos, double *vort, double *vel, double *dvort, double *vel_rk,
                   double *dvort_rk, int64_t cap, int64_t *np_out, hipStream_t st) {
   PGeom g;
   if (make_pgeom(geom, g)) return -1;
+  PM_CHECK(prec == PMGPU_FP64 || prec == PMGPU_FP32, "pmgpu: precision must be PMGPU_FP64 or PMGPU_FP32");
   PartWS &w = ws();
   const int64_t nn = mesh_cells(g);
   PM_CHECK(nn < (int64_t)1 << 31, "pmgpu: mesh too large for remeshing");
   if (w.flag.ensure(sizeof(int) * (nn + 1)) || w.offs.ensure(sizeof(int) * (nn + 1))) return -1;
   PM_CHECK(vort_m, "pmgpu_remesh: null vorticity mesh");
   PM_CHECK(!copy_vel || vel_m, "pmgpu_remesh: vel requested without a velocity mesh");
-  hipLaunchKernelGGL(k_remesh_flags, grid_for(nn), dim3(256), 0, st, g, vort_m, dvort_m, mask, min_vort, min_dvort,
-                     incl_ghost, w.flag.as<int>());
+  if (prec == PMGPU_FP64)
+    remesh_flags_launch<double>(g, nn, vort_m, dvort_m, mask, min_vort, min_dvort, incl_ghost, w.flag.as<int>(), st);
+  else
+    remesh_flags_launch<float>(g, nn, vort_m, dvort_m, mask, min_vort, min_dvort, incl_ghost, w.flag.as<int>(), st);
+  PM_HIP(hipGetLastError());
   if (scan_exclusive(w.flag.as<int>(), w.offs.as<int>(), nn, w.scan, st)) return -1;
   int total = 0;
   PM_HIP(hipMemcpyAsync(&total, w.offs.as<int>() + nn, sizeof(int), hipMemcpyDeviceToHost, st));
   PM_HIP(hipStreamSynchronize(st));
   *np_out = total;
   if (pos == nullptr || total > cap) return 0;  // count only
-  hipLaunchKernelGGL(k_remesh_create, grid_for(nn), dim3(256), 0, st, g, w.flag.as<int>(), w.offs.as<int>(),
-                     vort_m, copy_dvort ? dvort_m : nullptr, copy_vel ? vel_m : nullptr, pos, vort, vel, dvort,
-                     vel_rk, dvort_rk);
+  PM_CHECK(vort, "pmgpu_remesh: null particle vorticity");
+  const void *dv = copy_dvort ? dvort_m : nullptr, *vl = copy_vel ? vel_m : nullptr;
+  if (prec == PMGPU_FP64)
+    remesh_create_launch<double>(g, nn, w.flag.as<int>(), w.offs.as<int>(), vort_m, dv, vl, pos, vort, vel, dvort,
+                                 vel_rk, dvort_rk, st);
+  else
+    remesh_create_launch<float>(g, nn, w.flag.as<int>(), w.offs.as<int>(), vort_m, dv, vl, pos, vort, vel, dvort,
+                                vel_rk, dvort_rk, st);
   PM_HIP(hipGetLastError());
   return 0;
 }
@@ -417,14 +463,16 @@ int pmgpu_interp_particle_mesh(const pmgpu_geom *g, const double *xpart, const d
   if (to_dev(dp, xpart, sizeof(double) * 3 * np) || to_dev(dv, part, sizeof(double) * 3 * np) ||
       to_dev(dm, clear ? nullptr : mesh, mb))
     return -1;
-  if (p2m_device_full(g, (double *)dp.p, (double *)dv.p, np, (double *)dm.p, clear, clear_ghost, nullptr)) return -1;
+  if (p2m_device_full(g, PMGPU_FP64, (double *)dp.p, (double *)dv.p, np, dm.p, clear, clear_ghost, nullptr))
+    return -1;
   PM_HIP(hipMemcpy(mesh, dm.p, mb, hipMemcpyDeviceToHost));
   return 0;
 }
 
-int pmgpu_interp_particle_mesh_device(const pmgpu_geom *g, const void *d_xpart, const void *d_part, int64_t np,
-                                      void *d_mesh, int32_t clear, int32_t clear_ghost, void *stream) {
-  return p2m_device_full(g, (const double *)d_xpart, (const double *)d_part, np, (double *)d_mesh, clear,
+int pmgpu_interp_particle_mesh_device(const pmgpu_geom *g, int32_t precision, const void *d_xpart,
+                                      const void *d_part, int64_t np, void *d_mesh, int32_t clear,
+                                      int32_t clear_ghost, void *stream) {
+  return p2m_device_full(g, precision, (const double *)d_xpart, (const double *)d_part, np, d_mesh, clear,
                          clear_ghost, (hipStream_t)stream);
 }
 
@@ -435,14 +483,14 @@ int pmgpu_interp_mesh_particle(const pmgpu_geom *g, const double *mesh, const do
   Dev dm, dp, dout;
   if (to_dev(dm, mesh, mb) || to_dev(dp, pos, sizeof(double) * 3 * np) || to_dev(dout, nullptr, sizeof(double) * 3 * np))
     return -1;
-  if (m2p_device(g, (double *)dm.p, (double *)dp.p, np, (double *)dout.p, nullptr)) return -1;
+  if (m2p_device(g, PMGPU_FP64, dm.p, (double *)dp.p, np, (double *)dout.p, nullptr)) return -1;
   if (np) PM_HIP(hipMemcpy(out, dout.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
   return 0;
 }
 
-int pmgpu_interp_mesh_particle_device(const pmgpu_geom *g, const void *d_mesh, const void *d_pos, int64_t np,
-                                      void *d_out, void *stream) {
-  return m2p_device(g, (const double *)d_mesh, (const double *)d_pos, np, (double *)d_out, (hipStream_t)stream);
+int pmgpu_interp_mesh_particle_device(const pmgpu_geom *g, int32_t precision, const void *d_mesh,
+                                      const void *d_pos, int64_t np, void *d_out, void *stream) {
+  return m2p_device(g, precision, d_mesh, (const double *)d_pos, np, (double *)d_out, (hipStream_t)stream);
 }
 
 int pmgpu_remesh(const pmgpu_geom *g, const double *vort_m, const double *dvort_m, const double *vel_m,
@@ -457,9 +505,8 @@ int pmgpu_remesh(const pmgpu_geom *g, const double *vort_m, const double *dvort_
       (mask && to_dev(dmk, mask, mb / 24)))
     return -1;
   int64_t n = 0;
-  if (remesh_device(g, (double *)dw.p, (double *)ddw.p, (double *)dvl.p, (uint8_t *)dmk.p, min_vort, min_dvort,
-                    dvort_flag, vel_flag, incl_ghost, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, &n,
-                    nullptr))
+  if (remesh_device(g, PMGPU_FP64, dw.p, ddw.p, dvl.p, (uint8_t *)dmk.p, min_vort, min_dvort, dvort_flag, vel_flag,
+                    incl_ghost, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, &n, nullptr))
     return -1;
   *np_out = n;
   if (!pos || n > cap) return 0;
@@ -467,9 +514,9 @@ int pmgpu_remesh(const pmgpu_geom *g, const double *vort_m, const double *dvort_
   size_t pb = sizeof(double) * 3 * (size_t)n;
   if (to_dev(op, nullptr, pb) || to_dev(ov, nullptr, pb) || to_dev(ol, nullptr, pb) || to_dev(od, nullptr, pb))
     return -1;
-  if (remesh_device(g, (double *)dw.p, (double *)ddw.p, (double *)dvl.p, (uint8_t *)dmk.p, min_vort, min_dvort,
-                    dvort_flag, vel_flag, incl_ghost, (double *)op.p, (double *)ov.p, (double *)ol.p, (double *)od.p,
-                    nullptr, nullptr, n, &n, nullptr))
+  if (remesh_device(g, PMGPU_FP64, dw.p, ddw.p, dvl.p, (uint8_t *)dmk.p, min_vort, min_dvort, dvort_flag, vel_flag,
+                    incl_ghost, (double *)op.p, (double *)ov.p, (double *)ol.p, (double *)od.p, nullptr, nullptr, n,
+                    &n, nullptr))
     return -1;
   if (n) {
     PM_HIP(hipMemcpy(pos, op.p, pb, hipMemcpyDeviceToHost));
@@ -480,13 +527,14 @@ int pmgpu_remesh(const pmgpu_geom *g, const double *vort_m, const double *dvort_
   return 0;
 }
 
-int pmgpu_remesh_device(const pmgpu_geom *g, const void *d_vort_m, const void *d_dvort_m, const void *d_vel_m,
-                        const void *d_mask, double min_vort, double min_dvort, int32_t dvort_flag, int32_t vel_flag,
-                        int32_t incl_ghost, void *d_pos, void *d_vort, void *d_vel, void *d_dvort, void *d_vel_rk,
-                        void *d_dvort_rk, int64_t cap, int64_t *np_out, void *stream) {
+int pmgpu_remesh_device(const pmgpu_geom *g, int32_t precision, const void *d_vort_m, const void *d_dvort_m,
+                        const void *d_vel_m, const void *d_mask, double min_vort, double min_dvort,
+                        int32_t dvort_flag, int32_t vel_flag, int32_t incl_ghost, void *d_pos, void *d_vort,
+                        void *d_vel, void *d_dvort, void *d_vel_rk, void *d_dvort_rk, int64_t cap, int64_t *np_out,
+                        void *stream) {
   PM_CHECK(np_out, "pmgpu_remesh_device: null argument");
-  return remesh_device(g, (const double *)d_vort_m, (const double *)d_dvort_m, (const double *)d_vel_m,
-                       (const uint8_t *)d_mask, min_vort, min_dvort, dvort_flag, vel_flag, incl_ghost,
+  return remesh_device(g, precision, d_vort_m, d_dvort_m, d_vel_m, (const uint8_t *)d_mask, min_vort, min_dvort,
+                       dvort_flag, vel_flag, incl_ghost,
                        (double *)d_pos, (double *)d_vort, (double *)d_vel, (double *)d_dvort, (double *)d_vel_rk,
                        (double *)d_dvort_rk, cap, np_out, (hipStream_t)stream);
 }

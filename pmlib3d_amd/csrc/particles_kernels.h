@@ -81,6 +81,41 @@ __device__ __forceinline__ double weight(int m6, int s, double d) {
   return __dmul_rn(__dmul_rn(c3, __dsub_rn(x, 1.0)), p);
 }
 
+// FP32 form of the kernel weights (same polynomials, single precision)
+__device__ __forceinline__ float weight_f(int m6, int s, float d) {
+  if (!m6) {
+    switch (s) {
+      case 0: { float u = d + 1.0f, a = 2.0f - u; return 0.5f * (a * a) * (1.0f - u); }
+      case 1: return 1.0f - 2.5f * (d * d) + 1.5f * (d * d * d);
+      case 2: { float u = 1.0f - d; return 1.0f - 2.5f * (u * u) + 1.5f * (u * u * u); }
+      default: { float u = 2.0f - d, a = 2.0f - u; return 0.5f * (a * a) * (1.0f - u); }
+    }
+  }
+  const float c1 = -1.0f / 24.0f, c2 = 1.0f / 24.0f, c3 = -1.0f / 12.0f;
+  const float x = s == 0 ? d + 2.0f : (s == 1 ? d + 1.0f : (s == 2 ? d : (s == 3 ? 1.0f - d : (s == 4 ? 2.0f - d
+                                                                                                      : 3.0f - d))));
+  const float x2 = x * x, x3 = x2 * x, x4 = x3 * x;
+  if (s == 0 || s == 5) {
+    const float t = x - 3.0f;
+    return c1 * (x - 2.0f) * (t * t * t) * (5.0f * x - 8.0f);
+  }
+  if (s == 1 || s == 4) return c2 * (x - 1.0f) * (x - 2.0f) * (25.0f * x3 - 114.0f * x2 + 153.0f * x - 48.0f);
+  return c3 * (x - 1.0f) * (25.0f * x4 - 38.0f * x3 - 3.0f * x2 + 12.0f * x + 12.0f);
+}
+
+// compensated (Kahan) accumulation: sum + c carries the low-order bits the
+// float sum drops (this file is compiled without contraction, so the
+// compensation survives)
+struct Kahan {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
 // bin key of a particle (nbins: no stencil node touches the mesh, or the
 // particle lies outside a periodic axis, where the reference zeroes every
 // weight); ids[q] = q feeds the stable key sort
@@ -156,10 +191,61 @@ __global__ void k_p2m_gather(PGeom g, const double *pos, const double *val, cons
   }
 }
 
+// FP32 form: offsets from the FP64 positions (same bins and stencils),
+// weights and products in FP32, Kahan-compensated per-node sums
+__global__ void k_p2m_gather_f32(PGeom g, const double *pos, const double *val, const int *start, const int *ids,
+                                 float *mesh, int accumulate) {
+  const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int n[3];
+    n[0] = (int)(idx % g.E[0]) + g.lo[0];
+    n[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
+    n[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
+    Kahan a0, a1, a2;
+    for (int sz = g.ns - 1; sz >= 0; --sz) {
+      int bz = n[2] - sz + g.off;
+      int kz = bz - g.blo[2];
+      if (kz < 0 || kz >= g.nb[2]) continue;
+      for (int sy = g.ns - 1; sy >= 0; --sy) {
+        int by = n[1] - sy + g.off;
+        int ky = by - g.blo[1];
+        if (ky < 0 || ky >= g.nb[1]) continue;
+        for (int sx = g.ns - 1; sx >= 0; --sx) {
+          int bx = n[0] - sx + g.off;
+          int kx = bx - g.blo[0];
+          if (kx < 0 || kx >= g.nb[0]) continue;
+          int k = kx + g.nb[0] * (ky + g.nb[1] * kz);
+          for (int t = start[k]; t < start[k + 1]; ++t) {
+            int q = ids[t];
+            float wx = weight_f(g.m6, sx, (float)stencil_offset(pos[3 * q], g.xmin[0], g.dx[0], bx, g.c1dx[0]));
+            float wy = weight_f(g.m6, sy, (float)stencil_offset(pos[3 * q + 1], g.xmin[1], g.dx[1], by, g.c1dx[1]));
+            float wz = weight_f(g.m6, sz, (float)stencil_offset(pos[3 * q + 2], g.xmin[2], g.dx[2], bz, g.c1dx[2]));
+            float w = wx * wy * wz;
+            a0.add(w * (float)val[3 * q]);
+            a1.add(w * (float)val[3 * q + 1]);
+            a2.add(w * (float)val[3 * q + 2]);
+          }
+        }
+      }
+    }
+    float *m = mesh + 3 * idx;
+    if (accumulate) {
+      m[0] += a0.s;
+      m[1] += a1.s;
+      m[2] += a2.s;
+    } else {
+      m[0] = a0.s;
+      m[1] = a1.s;
+      m[2] = a2.s;
+    }
+  }
+}
+
 // Reverse periodic halo along axis d (mesh/pmlib_mesh_map_ghost.f with
 // incl_edges = .TRUE., op = add): cells [0, 2ng) and [N, N + 2ng) exchange
 // and add, both read before either is written.
-__global__ void k_halo_add(double *m, PGeom g, int d) {
+template <typename T> __global__ void k_halo_add(T *m, PGeom g, int d) {
   const int E0 = g.E[0], E1 = g.E[1];
   int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
   int64_t st[3] = {3, 3 * (int64_t)E0, 3 * (int64_t)E0 * E1};
@@ -174,7 +260,7 @@ __global__ void k_halo_add(double *m, PGeom g, int d) {
     int a1 = r % g.E[o1];
     int a2 = r / g.E[o1];
     int64_t base = c + st[o1] * a1 + st[o2] * a2;
-    double lo = m[base + st[d] * t], hi = m[base + st[d] * (N + t)];
+    T lo = m[base + st[d] * t], hi = m[base + st[d] * (N + t)];
     m[base + st[d] * t] = lo + hi;
     m[base + st[d] * (N + t)] = hi + lo;
   }
@@ -242,6 +328,54 @@ __global__ void k_m2p(PGeom g, const double *mesh, const double *pos, int64_t np
   }
 }
 
+// FP32 form of M2P: same stencil and cancellation, FP32 weights, Kahan sums
+__global__ void k_m2p_f32(PGeom g, const float *mesh, const double *pos, int64_t np, double *out) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    int idx[3][6];
+    float w[3][6];
+    for (int d = 0; d < 3; ++d) {
+      double x = pos[3 * q + d];
+      int b = cell_index(x, g.xmin[d], g.c1dx[d]);
+      double nrm = g.m6 ? g.c1dx[d] : g.c1dx[0];
+      float dd = (float)stencil_offset(x, g.xmin[d], g.dx[d], b, nrm);
+      for (int s = 0; s < g.ns; ++s) {
+        w[d][s] = weight_f(g.m6, s, dd);
+        idx[d][s] = b - g.off + s;
+      }
+    }
+    for (int d = 0; d < 3; ++d)
+      for (int s = 0; s < g.ns; ++s) {
+        if (idx[d][s] < g.lo[d]) {
+          if (g.m6 && d == 2 && s >= 4) {
+            w[1][s] = 0.0f;
+            idx[1][s] = g.lo[2];
+          }
+          w[d][s] = 0.0f;
+          idx[d][s] = g.lo[d];
+        }
+        if (idx[d][s] > g.hi[d]) {
+          w[d][s] = 0.0f;
+          idx[d][s] = g.hi[d];
+        }
+      }
+    for (int s = 0; s < g.ns; ++s) idx[1][s] = min(max(idx[1][s], g.lo[1]), g.hi[1]);
+    Kahan a0, a1, a2;
+    for (int sz = 0; sz < g.ns; ++sz)
+      for (int sy = 0; sy < g.ns; ++sy)
+        for (int sx = 0; sx < g.ns; ++sx) {
+          float wt = w[0][sx] * w[1][sy] * w[2][sz];
+          int64_t o = 3 * ((int64_t)(idx[0][sx] - g.lo[0]) +
+                           (int64_t)g.E[0] * ((idx[1][sy] - g.lo[1]) + (int64_t)g.E[1] * (idx[2][sz] - g.lo[2])));
+          a0.add(wt * mesh[o]);
+          a1.add(wt * mesh[o + 1]);
+          a2.add(wt * mesh[o + 2]);
+        }
+    out[3 * q] = a0.s;
+    out[3 * q + 1] = a1.s;
+    out[3 * q + 2] = a2.s;
+  }
+}
+
 __global__ void k_cell_index(PGeom g, const double *pos, int64_t np, int *out) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < 3 * np; q += (int64_t)gridDim.x * blockDim.x) {
     int d = q % 3;
@@ -251,7 +385,8 @@ __global__ void k_cell_index(PGeom g, const double *pos, int64_t np, int *out) {
 
 // ------------------------------------------------------------ remesh
 // flag of mesh cell idx (remesh/pmlib_remesh.f:104-125)
-__device__ __forceinline__ int remesh_flag(const PGeom &g, int64_t idx, const double *vort, const double *dvort,
+template <typename T>
+__device__ __forceinline__ int remesh_flag(const PGeom &g, int64_t idx, const T *vort, const T *dvort,
                                            const uint8_t *mask, double min_vort, double min_dvort,
                                            int incl_ghost) {
   int ii[3];
@@ -263,18 +398,19 @@ __device__ __forceinline__ int remesh_flag(const PGeom &g, int64_t idx, const do
     bool gh = incl_ghost && g.bc[d] == 0;
     if (!(in || gh)) return 0;
   }
-  const double *v = vort + 3 * idx;
-  double s = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(v[0], v[0]), __dmul_rn(v[1], v[1])), __dmul_rn(v[2], v[2])));
-  double ds = 0.0;
+  const T *v = vort + 3 * idx;
+  T s = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  T ds = 0;
   if (dvort) {
-    const double *u = dvort + 3 * idx;
-    ds = sqrt(__dadd_rn(__dadd_rn(__dmul_rn(u[0], u[0]), __dmul_rn(u[1], u[1])), __dmul_rn(u[2], u[2])));
+    const T *u = dvort + 3 * idx;
+    ds = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
   }
   if (!(s > min_vort || ds > min_dvort)) return 0;
   return mask ? (mask[idx] ? 1 : 0) : 1;
 }
 
-__global__ void k_remesh_flags(PGeom g, const double *vort, const double *dvort, const uint8_t *mask,
+template <typename T>
+__global__ void k_remesh_flags(PGeom g, const T *vort, const T *dvort, const uint8_t *mask,
                                double min_vort, double min_dvort, int incl_ghost, int *flag) {
   const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
@@ -284,8 +420,9 @@ __global__ void k_remesh_flags(PGeom g, const double *vort, const double *dvort,
 
 // create particles at cell centres in (k, j, i) order from the scanned flags;
 // vel / dvort are the fields copied (NULL: zero), per the vel / dvort flags
-__global__ void k_remesh_create(PGeom g, const int *flag, const int *offs, const double *vort, const double *dvort,
-                                const double *vel, double *pos, double *pvort, double *pvel, double *pdvort,
+template <typename T>
+__global__ void k_remesh_create(PGeom g, const int *flag, const int *offs, const T *vort, const T *dvort,
+                                const T *vel, double *pos, double *pvort, double *pvel, double *pdvort,
                                 double *vel_rk, double *dvort_rk) {
   const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
@@ -299,9 +436,9 @@ __global__ void k_remesh_create(PGeom g, const int *flag, const int *offs, const
     for (int d = 0; d < 3; ++d)
       pos[3 * p + d] = __dadd_rn(g.xmin[d], __dmul_rn(__dadd_rn((double)(ii[d] - 1), 0.5), g.dx[d]));
     for (int c = 0; c < 3; ++c) {
-      pvort[3 * p + c] = vort[3 * idx + c];
-      if (pvel) pvel[3 * p + c] = vel ? vel[3 * idx + c] : 0.0;
-      if (pdvort) pdvort[3 * p + c] = dvort ? dvort[3 * idx + c] : 0.0;
+      pvort[3 * p + c] = (double)vort[3 * idx + c];
+      if (pvel) pvel[3 * p + c] = vel ? (double)vel[3 * idx + c] : 0.0;
+      if (pdvort) pdvort[3 * p + c] = dvort ? (double)dvort[3 * idx + c] : 0.0;
       if (vel_rk) vel_rk[3 * p + c] = 0.0;
       if (dvort_rk) dvort_rk[3 * p + c] = 0.0;
     }

@@ -7,7 +7,8 @@
 // wavefront (P <= 64) synchronises with wave-level fences instead of block
 // barriers.  LDS indices are padded, i -> i + i/8 + i/64, which spreads the
 // stride-8 and stride-64 element writes of the first stages over the banks.
-// Twiddles: one table load per butterfly, higher powers by multiplication.
+// Twiddles: each thread loads the base twiddle of each of its butterflies
+// once (twid_load), higher powers by multiplication.
 #pragma once
 #include "pm_fft.h"
 
@@ -76,12 +77,48 @@ template <bool WAVE> __device__ __forceinline__ void line_sync() {
   }
 }
 
+// per-thread twiddles of every stage (the base power w1 of each butterfly),
+// loaded once per kernel and reused by forward and inverse transforms
+template <int L> constexpr int max_mb() {
+  int m = 1;
+  for (int q = 0; q < nstages(L); ++q) {
+    int nbf = L / radix_at(L, q);
+    int mb = (nbf + Geo<L>::P - 1) / Geo<L>::P;
+    if (mb > m) m = mb;
+  }
+  return m;
+}
+template <int L, typename V> struct Twid {
+  V w[nstages(L) > 0 ? nstages(L) : 1][max_mb<L>()];
+};
+
+template <int L, int I, typename V, bool DONE = (I >= nstages(L))> struct TwidLoad;
+template <int L, int I, typename V> struct TwidLoad<L, I, V, true> {
+  static __device__ __forceinline__ void run(Twid<L, V> &, int, const V *) {}
+};
+template <int L, int I, typename V> struct TwidLoad<L, I, V, false> {
+  static __device__ __forceinline__ void run(Twid<L, V> &t, int p, const V *tw) {
+    constexpr int R = radix_at(L, I), NS = ns_before(L, I), P = Geo<L>::P;
+    constexpr int NBF = L / R, MB = (NBF + P - 1) / P, TSTRIDE = L / (NS * R);
+    if (NS > 1) {
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        const int j = p + m * P;
+        t.w[I][m] = (MB * P == NBF || j < NBF) ? tw[(j % NS) * TSTRIDE] : tw[0];
+      }
+    }
+    TwidLoad<L, I + 1, V>::run(t, p, tw);
+  }
+};
+template <int L, typename V> __device__ __forceinline__ void twid_load(Twid<L, V> &t, int p, const V *tw) {
+  TwidLoad<L, 0, V>::run(t, p, tw);
+}
+
 template <int L, int S, int R, int NS, bool WAVE, typename V>
-__device__ __forceinline__ void stage(V *line, int p, const V *tw) {
+__device__ __forceinline__ void stage(V *line, int p, const V *twrow) {
   constexpr int P = Geo<L>::P;
   constexpr int NBF = L / R;
   constexpr int MB = (NBF + P - 1) / P;
-  constexpr int TSTRIDE = L / (NS * R);
   V v[MB][R];
 #pragma unroll
   for (int m = 0; m < MB; ++m) {
@@ -98,7 +135,7 @@ __device__ __forceinline__ void stage(V *line, int p, const V *tw) {
     if (MB * P == NBF || j < NBF) {
       const int k = NS == 1 ? 0 : j % NS;
       if (NS > 1) {
-        V w1 = tw[k * TSTRIDE];
+        V w1 = twrow[m];
         if (S > 0) w1.y = -w1.y;
         V w = w1;
 #pragma unroll
@@ -107,7 +144,7 @@ __device__ __forceinline__ void stage(V *line, int p, const V *tw) {
           if (r + 1 < R) w = cmul(w, w1);
         }
       }
-      DftAny<R, S, V>::run(v[m], tw, L);
+      Dft<R, S, V>::run(v[m]);
       const int base = (j - k) * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) line[pad(base + r * NS)] = v[m][r];
@@ -118,19 +155,20 @@ __device__ __forceinline__ void stage(V *line, int p, const V *tw) {
 
 template <int L, int S, int I, bool WAVE, typename V, bool DONE = (I >= nstages(L))> struct StageLoop;
 template <int L, int S, int I, bool WAVE, typename V> struct StageLoop<L, S, I, WAVE, V, true> {
-  static __device__ __forceinline__ void run(V *, int, const V *) {}
+  static __device__ __forceinline__ void run(V *, int, const Twid<L, V> &) {}
 };
 template <int L, int S, int I, bool WAVE, typename V> struct StageLoop<L, S, I, WAVE, V, false> {
-  static __device__ __forceinline__ void run(V *line, int p, const V *tw) {
-    stage<L, S, radix_at(L, I), ns_before(L, I), WAVE>(line, p, tw);
-    StageLoop<L, S, I + 1, WAVE, V>::run(line, p, tw);
+  static __device__ __forceinline__ void run(V *line, int p, const Twid<L, V> &t) {
+    stage<L, S, radix_at(L, I), ns_before(L, I), WAVE>(line, p, t.w[I]);
+    StageLoop<L, S, I + 1, WAVE, V>::run(line, p, t);
   }
 };
 
-// In-place transform of one padded line held in LDS (tw: length-L table).
-// The data must be visible to the line's threads (caller syncs before).
-template <int L, int S, typename V> __device__ __forceinline__ void fft(V *line, int p, const V *tw) {
-  StageLoop<L, S, 0, Geo<L>::WAVE, V>::run(line, p, tw);
+// In-place transform of one padded line held in LDS with twiddles preloaded
+// by twid_load from the length-L table.  The data must be visible to the
+// line's threads (caller syncs before).
+template <int L, int S, typename V> __device__ __forceinline__ void fft(V *line, int p, const Twid<L, V> &t) {
+  StageLoop<L, S, 0, Geo<L>::WAVE, V>::run(line, p, t);
 }
 
 }  // namespace fx

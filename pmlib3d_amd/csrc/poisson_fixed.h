@@ -20,6 +20,8 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const
   const V *in = reinterpret_cast<const V *>(a.in) + b * a.bs_in + col0;
   V *out = reinterpret_cast<V *>(a.out) + b * a.bs_out + col0;
   const int tid = threadIdx.x;
+  Twid<L, V> twd;
+  twid_load<L>(twd, tid % P, tw + a.plan.tw_off);
   const int c_ld = tid % T, r0 = tid / T;  // NT / T == P rows per sweep
   const bool col_ok = col0 + c_ld < a.ncols;
   constexpr int E = (L + P - 1) / P;
@@ -38,7 +40,7 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const
   }
   __syncthreads();
   const int line = tid / P, p = tid % P;
-  fft<L, S>(lds + line * ST, p, tw + a.plan.tw_off);
+  fft<L, S>(lds + line * ST, p, twd);
   __syncthreads();
   const R sc = (R)a.scale;
   if (col_ok)
@@ -56,6 +58,8 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
   const int y0 = blockIdx.x * T, z = blockIdx.y;
   const R *in = reinterpret_cast<const R *>(a.in);
   const int tid = threadIdx.x;
+  Twid<NH, V> twd;
+  twid_load<NH>(twd, tid % P, tw + a.plan.tw_off);
   // packed pairs x[2m] + i x[2m+1]; m >= ceil(Min/2) are zero
   const int half = (a.Min + 1) / 2;
   if (a.cs == 1 && a.xs == NC) {
@@ -97,7 +101,7 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
   }
   __syncthreads();
   const int line = tid / P, p = tid % P;
-  fft<NH, -1>(lds + line * ST, p, tw + a.plan.tw_off);
+  fft<NH, -1>(lds + line * ST, p, twd);
   __syncthreads();
   V *out = reinterpret_cast<V *>(a.out);
   const V *twL = tw + a.twL_off;
@@ -123,12 +127,13 @@ template <typename R, int NH, int T, int NC>
 __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const cplx_t<R> *tw) {
   using V = cplx_t<R>;
   constexpr int P = Geo<NH>::P, ST = Geo<NH>::ST, NL = T * NC, NT = P * NL;
-  constexpr int HX = NH + 1;
   __shared__ V lds[NL * ST];
   const int y0 = blockIdx.x * T, z = blockIdx.y;
   const V *in = reinterpret_cast<const V *>(a.in);
   const V *twL = tw + a.twL_off;
   const int tid = threadIdx.x;
+  Twid<NH, V> twd;
+  twid_load<NH>(twd, tid % P, tw + a.plan.tw_off);
   for (int idx = tid; idx < NL * NH; idx += NT) {
     int c = idx % NC, k = (idx / NC) % NH, t = idx / (NC * NH);
     int l = t * NC + c;
@@ -150,7 +155,7 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
   }
   __syncthreads();
   const int line = tid / P, p = tid % P;
-  fft<NH, 1>(lds + line * ST, p, tw + a.plan.tw_off);
+  fft<NH, 1>(lds + line * ST, p, twd);
   __syncthreads();
   R *out = reinterpret_cast<R *>(a.out);
   const R sc = (R)a.scale;
@@ -200,6 +205,8 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
   const int kx0 = (int)(lt % gridDim.x) * TC, ky = (int)(lt / gridDim.x);
   const int tid = threadIdx.x;
   const int line = tid / P, p = tid % P;
+  Twid<L, V> twd;
+  twid_load<L>(twd, p, tw);
   const int col = line / 3, comp = line - 3 * col;
   const int kx = kx0 + col;
   const bool col_ok = kx < a.Hx;
@@ -226,7 +233,7 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
     }
   }
   __syncthreads();
-  fft<L, -1>(myl, p, tw);
+  fft<L, -1>(myl, p, twd);
   __syncthreads();
   const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
   const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
@@ -237,7 +244,6 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
   const R *G = reinterpret_cast<const R *>(a.Ghat);
   const R scale = (R)1 / (R)L;
   // EXTRA = psi and/or filtered vorticity requested (else velocity only)
-  constexpr bool extra = EXTRA;
   const V *cl3 = lds + (3 * col) * ST;  // the three component lines of my column
   V own[EXTRA ? E : 1];                // my component's forward spectrum (extra outputs)
   const int nkind = EXTRA ? 1 + (a.want_psi ? 1 : 0) + (a.want_vort ? 1 : 0) : 1;
@@ -310,7 +316,7 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
       if (FULL || n < L) myl[pad(n)] = r[e];
     }
     line_sync<Geo<L>::WAVE>();
-    fft<L, 1>(myl, p, tw);
+    fft<L, 1>(myl, p, twd);
     __syncthreads();
     V *dst = (kind == 0 ? B : (kind == 1 ? reinterpret_cast<V *>(a.Bpsi) : reinterpret_cast<V *>(a.Bvort))) + tile;
     for (int idx = tid; idx < a.Mz * NLN; idx += NT) {

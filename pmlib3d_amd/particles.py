@@ -160,22 +160,24 @@ def pmlib_particles_advance(patch, part, itimestage, dtime, euler_vort=False, rk
 
 class ParticleOps:
     """Device entry points for one patch; arguments are device addresses
-    (ints, e.g. torch tensor.data_ptr()) of float64 arrays and an optional
-    hipStream_t (int)."""
+    (ints, e.g. torch tensor.data_ptr()) and an optional hipStream_t (int).
+    Particle arrays are float64; meshes are in `precision` (FP64 or FP32)."""
 
-    def __init__(self, patch, interp_order=None):
+    def __init__(self, patch, interp_order=None, precision=_lib.FP64):
         self.g = _geom(patch, interp_order)
         self.lib = _lib.load()
         self.ext = mesh_extent(patch, interp_order)
+        self.precision = precision
 
     def p2m(self, d_xpart, d_part, npart, d_mesh, clear=True, clear_ghost=False, stream=None):
-        _lib.check(self.lib.pmgpu_interp_particle_mesh_device(C.byref(self.g), d_xpart, d_part, npart, d_mesh,
-                                                              int(clear), int(clear_ghost), stream),
+        _lib.check(self.lib.pmgpu_interp_particle_mesh_device(C.byref(self.g), self.precision, d_xpart, d_part,
+                                                              npart, d_mesh, int(clear), int(clear_ghost), stream),
                    "pmgpu_interp_particle_mesh_device")
 
     def m2p(self, d_mesh, d_pos, npart, d_out, stream=None):
-        _lib.check(self.lib.pmgpu_interp_mesh_particle_device(C.byref(self.g), d_mesh, d_pos, npart, d_out,
-                                                              stream), "pmgpu_interp_mesh_particle_device")
+        _lib.check(self.lib.pmgpu_interp_mesh_particle_device(C.byref(self.g), self.precision, d_mesh, d_pos,
+                                                              npart, d_out, stream),
+                   "pmgpu_interp_mesh_particle_device")
 
     def remesh(self, d_vort_m, d_dvort_m, d_vel_m, min_vort, min_dvort, out=None, cap=0, d_mask=None,
                dvort=False, vel=False, incl_ghost=False, stream=None):
@@ -183,7 +185,8 @@ class ParticleOps:
         (3, cap) arrays, or None to count only; returns the particle count."""
         n = C.c_int64()
         o = list(out) if out is not None else [None] * 6
-        _lib.check(self.lib.pmgpu_remesh_device(C.byref(self.g), d_vort_m, d_dvort_m, d_vel_m, d_mask,
+        _lib.check(self.lib.pmgpu_remesh_device(C.byref(self.g), self.precision, d_vort_m, d_dvort_m, d_vel_m,
+                                                d_mask,
                                                 float(min_vort), float(min_dvort), int(dvort), int(vel),
                                                 int(incl_ghost), *o, int(cap), C.byref(n), stream),
                    "pmgpu_remesh_device")

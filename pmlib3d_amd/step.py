@@ -19,21 +19,25 @@ _FIELDS = ("pos", "vel", "vort", "dvort", "vel_rk", "dvort_rk")
 
 class VortexStep:
     def __init__(self, patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3, capacity=None,
-                 device="cuda:0"):
+                 device="cuda:0", precision=_lib.FP64):
+        """precision selects the mesh side (deposition, solve, gather,
+        remesh thresholds): FP64, or FP32 with Kahan-compensated interpolation
+        sums.  Particles are FP64 either way."""
         import torch
         self.torch = torch
         self.dev = torch.device(device)
         self.patch = patch
         self.rk_order = rk_order
-        self.solver = PoissonSolver(patch, _lib.FP64, order=order, kernel=1, eps_radius=eps_radius,
+        self.solver = PoissonSolver(patch, precision, order=order, kernel=1, eps_radius=eps_radius,
                                     interp_order=interp_order)
         self.solver.setup()
-        self.ops = ParticleOps(patch, interp_order)
+        self.ops = ParticleOps(patch, interp_order, precision)
         ext = mesh_extent(patch, interp_order)
         nm = 3 * int(np.prod(ext))
         self.ext = ext
-        self.mesh_vort = torch.zeros(nm, dtype=torch.float64, device=self.dev)
-        self.mesh_vel = torch.zeros(nm, dtype=torch.float64, device=self.dev)
+        mdt = torch.float64 if precision == _lib.FP64 else torch.float32
+        self.mesh_vort = torch.zeros(nm, dtype=mdt, device=self.dev)
+        self.mesh_vel = torch.zeros(nm, dtype=mdt, device=self.dev)
         self.cap = int(capacity if capacity is not None else np.prod(ext))
         self.buf = [{k: torch.zeros(3 * self.cap, dtype=torch.float64, device=self.dev) for k in _FIELDS}
                     for _ in range(2)]

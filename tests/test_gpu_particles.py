@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.helpers import rel_linf
+from tests.helpers import rel_l2, rel_linf
 
 pytestmark = pytest.mark.gpu
 
@@ -292,3 +292,42 @@ def test_device_entry_points_match_host_entry_points():
     m = d_mesh.cpu().numpy().reshape(ext, order="F")
     assert np.array_equal(m, _p2m(g, pos, val))
     assert np.array_equal(d_out.cpu().numpy().reshape(pos.shape, order="F"), _m2p(g, m, pos))
+
+
+# ---------------------------------------------------------- FP32 meshes
+@pytest.mark.parametrize("case", [GEOMS[0], GEOMS[2], GEOMS[5]], ids=["m4_free", "m4_periodic", "m6_periodic"])
+def test_fp32_mesh_interpolation_against_fp64(case):
+    """FP32 meshes (FP32 weights, Kahan sums) against the FP64 oracle:
+    relative L2 <= 1e-6 for deposition and gather; the FP32 remesh of an
+    FP32 mesh places the same particles as the FP64 remesh of its values."""
+    import torch
+    import pmlib3d_amd as pm
+    ncell, ng, bc, io = case
+    g = O.make_geom(ncell, ng, [-0.4, -0.3, -0.55], 1.0 / 10, bc, io)
+    pos, val = _particles(g, 20000, 77)
+    dev = torch.device("cuda:0")
+    ops = pm.ParticleOps(_patch(g), interp_order=io, precision=pm.FP32)
+    ext = (3,) + O.mesh_extent(g)
+    d_pos = torch.from_numpy(pos.ravel(order="F")).to(dev)
+    d_val = torch.from_numpy(val.ravel(order="F")).to(dev)
+    d_mesh = torch.empty(int(np.prod(ext)), dtype=torch.float32, device=dev)
+    ops.p2m(d_pos.data_ptr(), d_val.data_ptr(), pos.shape[1], d_mesh.data_ptr(), clear=True)
+    ref = O.p2m(g, pos, val)
+    got = d_mesh.double().cpu().numpy().reshape(ext, order="F")
+    assert rel_l2(got, ref) <= 1e-6, rel_l2(got, ref)
+    mesh = np.asfortranarray(np.random.default_rng(5).normal(size=ext))
+    d_m32 = torch.from_numpy(mesh.ravel(order="F")).to(dev).float()
+    d_out = torch.empty_like(d_pos)
+    ops.m2p(d_m32.data_ptr(), d_pos.data_ptr(), pos.shape[1], d_out.data_ptr())
+    out = d_out.cpu().numpy().reshape(pos.shape, order="F")
+    m2p_ref = O.m2p(g, d_m32.double().cpu().numpy().reshape(ext, order="F"), pos)
+    assert rel_l2(out, m2p_ref) <= 1e-6, rel_l2(out, m2p_ref)
+    # remesh from the FP32 mesh
+    m32 = d_m32.double().cpu().numpy().reshape(ext, order="F")
+    n = ops.remesh(d_m32.data_ptr(), None, None, 1.2, 0.0)
+    bufs = [torch.zeros(3 * n, dtype=torch.float64, device=dev) for _ in range(6)]
+    n2 = ops.remesh(d_m32.data_ptr(), None, None, 1.2, 0.0, out=[b.data_ptr() for b in bufs], cap=n)
+    P, W, V, D = O.remesh(g, m32, None, None, 1.2, 0.0, False, dvort=False, vel=False)
+    assert n2 == n == P.shape[1]
+    assert np.array_equal(bufs[0].cpu().numpy().reshape(3, n, order="F"), P)
+    assert np.array_equal(bufs[1].cpu().numpy().reshape(3, n, order="F"), W)
