@@ -134,6 +134,79 @@ def step_benchmark(steps, warmup, n=128, precision=0):
     return counted / dt / 1e6, 1e3 * dt / steps, st.np
 
 
+def patch_bytes(n, s, s_g, comps=1):
+    """Algorithmic bytes of one free-space solve of a patch n = (nx, ny, nz)
+    with the 5-pass implicit-padding schedule (comps components in and out,
+    G-hat as its real octant)."""
+    Mx, My, Mz = n
+    Lx, Ly, Lz = 2 * Mx, 2 * My, 2 * Mz
+    Hx, Hy, Hz = Lx // 2 + 1, Ly // 2 + 1, Lz // 2 + 1
+    c = 2 * s
+    b = comps * (Mx * My * Mz * s + Hx * My * Mz * c)           # x forward
+    b += comps * (Hx * My * Mz * c + Hx * Ly * Mz * c)          # y forward
+    b += comps * 2 * Hx * Ly * Mz * c + Hx * Hy * Hz * s_g      # fused z
+    b += comps * (Hx * Ly * Mz * c + Hx * My * Mz * c)          # y inverse
+    b += comps * (Hx * My * Mz * c + Mx * My * Mz * s)          # x inverse
+    return b
+
+
+def ring_patch(n, h, rng):
+    """x component of a Gaussian ring's vorticity on a patch centred at the
+    origin: R = 0.3 min extent, sigma = 0.15 R, Gamma = 1, axis uniform on
+    the sphere."""
+    import numpy as np
+    ext = [v * h for v in n]
+    R = 0.3 * min(ext)
+    sig = 0.15 * R
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    c = [-e / 2 + (np.arange(v) + 0.5) * h for e, v in zip(ext, n)]
+    X, Y, Z = np.meshgrid(*c, indexing="ij")
+    P = np.stack([X, Y, Z])
+    z = np.tensordot(axis, P, axes=1)
+    radial = P - axis[:, None, None, None] * z
+    rho = np.sqrt((radial ** 2).sum(axis=0))
+    wt = 1.0 / (np.pi * sig ** 2) * np.exp(-((rho - R) ** 2 + z ** 2) / sig ** 2)
+    eth = np.cross(axis[:, None, None, None], radial, axis=0) / np.maximum(rho, 1e-30)
+    return np.asfortranarray(wt * eth[0])
+
+
+def batch_benchmark(steps, warmup, npatch=24, seed=4):
+    """24 free-space patches, per-axis sizes drawn from {32,48,64,96,128,160}
+    (seeded), h = 1/128, single-component FP32 solves of order 4 (eps = 2h),
+    all patches replayed as one captured HIP graph.  Returns batches/s, the
+    algorithmic bytes per batch and the patch sizes."""
+    import numpy as np
+    import torch
+    import pmlib3d_amd as pm
+    rng = np.random.default_rng(seed)
+    h = 1.0 / 128
+    sizes = [tuple(int(v) for v in rng.choice([32, 48, 64, 96, 128, 160], 3)) for _ in range(npatch)]
+    dev = torch.device("cuda:0")
+    solvers, ins, outs = [], [], []
+    for n in sizes:
+        patch = pm.Patch(xmin=tuple(-v * h / 2 for v in n), xmax=tuple(v * h / 2 for v in n), ncell=n,
+                         dx=(h,) * 3)
+        sv = pm.PoissonSolver(patch, pm.FP32, order=4, kernel=1, eps_radius=2.0, interp_order=3)
+        sv.setup()
+        w = torch.from_numpy(ring_patch(n, h, rng).ravel(order="F")).to(dev).float()
+        solvers.append(sv)
+        ins.append(w)
+        outs.append(torch.zeros_like(w))
+    batch = pm.PoissonBatch(solvers)
+    pi, po = [t.data_ptr() for t in ins], [t.data_ptr() for t in outs]
+    for _ in range(warmup):
+        batch.solve_scalar(pi, po)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.solve_scalar(pi, po)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    nbytes = sum(patch_bytes(n, 4, 4) for n in sizes)
+    return 1.0 / dt, nbytes, sizes
+
+
 def cpu_step_baseline(n=128):
     """One two-ring vortex step of the compiled reference (one rank: its step
     mode is single-rank), FP64."""
@@ -170,6 +243,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--step-steps", type=int, default=5,
                     help="timed two-ring vortex steps at 128^3 (0 skips)")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="timed 24-patch batched single-component solves (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -266,6 +341,18 @@ def main():
             log("vortex step %s: %.1f Mparticle-steps/s (%.3f ms/step, %d particles)" % (name, mps, ms, npart))
             step[name] = {"Mparticle_steps_per_s": mps, "ms_per_step": ms, "particles": npart}
 
+    batch = None
+    if args.batch_steps > 0:
+        bps, nbytes, sizes = batch_benchmark(args.batch_steps, 2)
+        log("batched multipatch: %.1f batches/s (%.3f ms/batch, %.2f GB algorithmic, %.0f GB/s)" %
+            (bps, 1e3 / bps, nbytes / 1e9, nbytes * bps / 1e9))
+        batch = {"batches_per_s": bps, "ms_per_batch": 1e3 / bps, "algorithmic_bytes": nbytes,
+                 "achieved_GBs": nbytes * bps / 1e9, "frac_hbm": nbytes * bps / 1e9 / PEAK_HBM_GBS,
+                 "dtype": "f32", "patches": sizes,
+                 "workload": "24 free-space patches, sizes per axis from {32,48,64,96,128,160} (seed 4), "
+                             "h=1/128, one Gaussian ring each (R=0.3 min extent, sigma=0.15R, random axis), "
+                             "single-component psi solve, order 4, eps 2h, one captured HIP graph per batch"}
+
     cpu = None
     cpu_step = None
     if not args.no_cpu_baseline:
@@ -299,6 +386,7 @@ def main():
         "solve_achieved_GBs": {"fp32": bytes32 / (1e-3 / sps32 * 1e3) / 1e9 * 1e3 / 1e3,
                                "fp64": bytes64 * sps64 / 1e9},
         "vortex_step": step,
+        "batched_multipatch": batch,
         "cpu_baseline_step": cpu_step,
         "roofline": {"bound": "hbm", "kernel": "k_z3_f<float,512,TC> (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",

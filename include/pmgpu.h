@@ -45,6 +45,7 @@ typedef struct pmgpu_params {
 enum { PMGPU_FP64 = 0, PMGPU_FP32 = 1 };
 
 typedef struct pmgpu_poisson_s *pmgpu_poisson_t;
+typedef struct pmgpu_poisson_batch_s *pmgpu_poisson_batch_t;
 
 /* Library state (no reference counterpart; MPI_INIT plays this role). */
 int pmgpu_init(int32_t device);
@@ -90,6 +91,27 @@ int pmgpu_poisson_kernel_octant(pmgpu_poisson_t h, void *host, int64_t cap_bytes
  * over the solves recorded since (up to 512), or -1 if none. */
 int pmgpu_poisson_last_timing(pmgpu_poisson_t h, double *zpass_ms, double *total_ms);
 int pmgpu_poisson_set_timing(pmgpu_poisson_t h, int32_t enable);
+
+/* Single-component solve psi = G * w (the vector potential of one
+ * vorticity component, the psi-hat = G w-hat product of
+ * poisson/pmlib_poisson_solve.f:854-858 for Cin = Cout = 1).  Meshes are
+ * (E...) scalar fields; host FP64, or device in the handle's precision.
+ * The z-pass tile width is chosen by timing on first use
+ * (pmgpu_poisson_tune_scalar; cached per transform shape, and in the file
+ * named by PMGPU_TUNE_CACHE when set). */
+int pmgpu_poisson_tune_scalar(pmgpu_poisson_t h);
+int pmgpu_poisson_solve_scalar(pmgpu_poisson_t h, const double *w, double *psi);
+int pmgpu_poisson_solve_scalar_device(pmgpu_poisson_t h, const void *d_w, void *d_psi);
+
+/* Batched multipatch solve (no reference counterpart: the reference solves
+ * one patch per call).  A batch holds n set-up handles of any sizes;
+ * pmgpu_poisson_batch_solve_scalar runs all n single-component solves as one
+ * HIP graph (captured on the first call and whenever a pointer changes,
+ * replayed otherwise), each patch on its own branch, on the batch stream. */
+int pmgpu_poisson_batch_create(const pmgpu_poisson_t *handles, int32_t n, void *stream,
+                               pmgpu_poisson_batch_t *out);
+int pmgpu_poisson_batch_solve_scalar(pmgpu_poisson_batch_t b, const void *const *d_in, void *const *d_out);
+int pmgpu_poisson_batch_destroy(pmgpu_poisson_batch_t b);
 
 /* ------------------------------------------------------------------------
  * Particles.  Particle arrays are (3, np) FP64 everywhere (positions in FP64

@@ -9,7 +9,7 @@ libpmgpu.so through its C ABI; subroutines return ierr (0 or -1) and write
 "[rank](caller) : message" on failure, as pmlib_write does.
 """
 from .core import (Patch, Mesh, Particles, Parameters, params, pmlib_write, mesh_extent,
-                   pmlib_poisson_setup, pmlib_poisson_solve, PoissonSolver)
+                   pmlib_poisson_setup, pmlib_poisson_solve, PoissonSolver, PoissonBatch)
 from .particles import (cell_index, pmlib_interp_particle_mesh, pmlib_interp_mesh_particle,
                         interp_mesh_field, pmlib_remesh, pmlib_particles_advance, ParticleOps)
 from .penalisation import penalise, penalise_device, sphere_mask, brinkman_factor
@@ -17,7 +17,7 @@ from .step import VortexStep
 from ._lib import PmgpuError, FP32, FP64
 
 __all__ = ["Patch", "Mesh", "Particles", "Parameters", "params", "pmlib_write", "mesh_extent",
-           "pmlib_poisson_setup", "pmlib_poisson_solve", "PoissonSolver", "PmgpuError",
+           "pmlib_poisson_setup", "pmlib_poisson_solve", "PoissonSolver", "PoissonBatch", "PmgpuError",
            "FP32", "FP64", "cell_index", "pmlib_interp_particle_mesh", "pmlib_interp_mesh_particle",
            "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps", "penalise",
            "penalise_device", "sphere_mask", "brinkman_factor", "VortexStep"]

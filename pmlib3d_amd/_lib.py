@@ -45,6 +45,12 @@ SIGNATURES = {
     "pmgpu_poisson_kernel_octant": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int32)]),
     "pmgpu_poisson_last_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pmgpu_poisson_set_timing": (C.c_int, [_P, _I]),
+    "pmgpu_poisson_tune_scalar": (C.c_int, [_P]),
+    "pmgpu_poisson_solve_scalar": (C.c_int, [_P, _P, _P]),
+    "pmgpu_poisson_solve_scalar_device": (C.c_int, [_P, _P, _P]),
+    "pmgpu_poisson_batch_create": (C.c_int, [C.POINTER(_P), _I, _P, C.POINTER(_P)]),
+    "pmgpu_poisson_batch_solve_scalar": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "pmgpu_poisson_batch_destroy": (C.c_int, [_P]),
     "pmgpu_interp_particle_mesh": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_int64, _P, _I, _I]),
     "pmgpu_interp_particle_mesh_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, C.c_int64, _P, _I, _I, _P]),
     "pmgpu_interp_mesh_particle": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_int64, _P]),

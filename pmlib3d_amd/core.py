@@ -172,6 +172,21 @@ class PoissonSolver:
                    "pmgpu_poisson_kernel_octant")
         return buf.reshape((dims[1], dims[0], dims[2]))  # [ky', kx, kz']
 
+    def tune_scalar(self):
+        _lib.check(self.lib.pmgpu_poisson_tune_scalar(self.h), "pmgpu_poisson_tune_scalar")
+
+    def solve_scalar(self, w):
+        """psi = G * w for one scalar field w (Ex, Ey, Ez), FP64 host arrays."""
+        w = np.asfortranarray(w, dtype=np.float64)
+        assert w.shape == self.ext, (w.shape, self.ext)
+        psi = np.zeros_like(w, order="F")
+        _lib.check(self.lib.pmgpu_poisson_solve_scalar(self.h, _ptr(w), _ptr(psi)), "pmgpu_poisson_solve_scalar")
+        return psi
+
+    def solve_scalar_device(self, d_w, d_psi):
+        _lib.check(self.lib.pmgpu_poisson_solve_scalar_device(self.h, d_w, d_psi),
+                   "pmgpu_poisson_solve_scalar_device")
+
     def set_timing(self, on=True):
         _lib.check(self.lib.pmgpu_poisson_set_timing(self.h, int(on)), "pmgpu_poisson_set_timing")
 
@@ -185,6 +200,36 @@ class PoissonSolver:
             if getattr(self, "h", None):
                 self.lib.pmgpu_poisson_destroy(self.h)
                 self.h = None
+        except Exception:
+            pass
+
+
+class PoissonBatch:
+    """Several set-up solvers (patches of any sizes) whose single-component
+    solves run as one captured HIP graph, one branch per patch."""
+
+    def __init__(self, solvers, stream=None):
+        self.lib = _lib.load()
+        self.solvers = list(solvers)
+        hs = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        b = C.c_void_p()
+        _lib.check(self.lib.pmgpu_poisson_batch_create(hs, len(self.solvers), stream, C.byref(b)),
+                   "pmgpu_poisson_batch_create")
+        self.b = b
+
+    def solve_scalar(self, d_ins, d_outs):
+        """Device addresses of the input and output scalar meshes, one per
+        patch, in the solvers' precision; stream-ordered."""
+        n = len(self.solvers)
+        ins = (C.c_void_p * n)(*d_ins)
+        outs = (C.c_void_p * n)(*d_outs)
+        _lib.check(self.lib.pmgpu_poisson_batch_solve_scalar(self.b, ins, outs), "pmgpu_poisson_batch_solve_scalar")
+
+    def __del__(self):
+        try:
+            if getattr(self, "b", None):
+                self.lib.pmgpu_poisson_batch_destroy(self.b)
+                self.b = None
         except Exception:
             pass
 

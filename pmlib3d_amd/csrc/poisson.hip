@@ -2,8 +2,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <cstdio>
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "../../include/pmgpu.h"
@@ -20,6 +22,45 @@ int fx_strided(int L, int S, StridedArgs a, int nb_y, int nb_z, const cplx_t<R> 
 template <typename R> int fx_z(int L, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st);
 template <typename R> int fx_xfwd(int NH, int NC, XFwdArgs a, const cplx_t<R> *tw, hipStream_t st);
 template <typename R> int fx_xinv(int NH, XInvArgs a, const cplx_t<R> *tw, hipStream_t st);
+template <typename R> int fx_z1(int L, int T, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st);
+
+// Tile widths chosen by timing, per (pass, lengths, precision), shared by all
+// handles of the process and kept across runs in the file named by
+// PMGPU_TUNE_CACHE (one "key value" line per entry) when that is set.
+struct TuneCache {
+  std::map<std::string, int> m;
+  bool loaded = false;
+  void load() {
+    if (loaded) return;
+    loaded = true;
+    const char *f = getenv("PMGPU_TUNE_CACHE");
+    if (!f) return;
+    FILE *fp = fopen(f, "r");
+    if (!fp) return;
+    char key[256];
+    int v;
+    while (fscanf(fp, "%255s %d", key, &v) == 2) m[key] = v;
+    fclose(fp);
+  }
+  int get(const std::string &k) {
+    load();
+    auto it = m.find(k);
+    return it == m.end() ? 0 : it->second;
+  }
+  void put(const std::string &k, int v) {
+    m[k] = v;
+    const char *f = getenv("PMGPU_TUNE_CACHE");
+    if (!f) return;
+    FILE *fp = fopen(f, "a");
+    if (!fp) return;
+    fprintf(fp, "%s %d\n", k.c_str(), v);
+    fclose(fp);
+  }
+};
+static TuneCache &tune_cache() {
+  static TuneCache c;
+  return c;
+}
 
 // Factor L into the supported radices, largest first.
 static bool make_plan(int L, FftPlan &pl) {
@@ -62,6 +103,9 @@ struct PoissonBase {
   virtual int solve_host(const double *vort, double *vel, double *vort_out, double *vecpot, int reg,
                          int reproj, int psi) = 0;
   virtual int octant(void *host, int64_t cap, int32_t *dims) = 0;
+  virtual int solve_scalar_device(const void *w, void *psi) = 0;
+  virtual int solve_scalar_host(const double *w, double *psi) = 0;
+  virtual int tune_scalar() = 0;
   pmgpu_geom g;
   pmgpu_params prm;
   hipStream_t stream = nullptr;
@@ -387,7 +431,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
       for (int d = 0; d < 3; ++d)
         if (g.bound_cond[d] == 1) {
           hipLaunchKernelGGL(k_ghost_fill<R>, dim3(1024), dim3(256), 0, stream, reinterpret_cast<R *>(outs[f]),
-                             E[0], E[1], E[2], d, g.ncell[d], ngs[d]);
+                             E[0], E[1], E[2], d, g.ncell[d], ngs[d], 3);
           PM_HIP(hipGetLastError());
         }
     }
@@ -433,6 +477,128 @@ template <typename R> struct PoissonGPU : PoissonBase {
     }
     return 0;
   }
+
+  // ---- single-component solve: psi = G * w (Cin = Cout = 1) -----------
+  int tz1 = 0;  // scalar z-pass columns per block (0: not tuned yet)
+  std::string z1_key() const {
+    char k[128];
+    snprintf(k, sizeof(k), "z1:L%d:Ly%d:Hx%d:%s", L[2], L[1], Hx, sizeof(R) == 4 ? "f32" : "f64");
+    return k;
+  }
+  int64_t row1() const {
+    const int64_t align = 128 / (int64_t)sizeof(V);
+    return ((int64_t)Hx + align - 1) / align * align;
+  }
+  ZArgs z1_args() const {
+    ZArgs za{};
+    const int64_t H1 = row1();
+    za.B = B;
+    za.Ghat = Ghat;
+    za.rs = H1 * L[1];
+    za.cs = 0;
+    za.kxs = 1;
+    za.kys = H1;
+    za.Hx = Hx;
+    za.Ly = L[1];
+    za.Lz = L[2];
+    za.Lx = L[0];
+    za.Mz = M[2];
+    za.Hy = Hy;
+    za.Hz = Hz;
+    za.allper = allper;
+    for (int d = 0; d < 3; ++d) {
+      za.dk[d] = 1.0 / ((double)L[d] * g.dx[d]);
+      za.ksmp[d] = 1.0 / (2.0 * g.dx[d]);
+    }
+    za.h3 = g.dx[0] * g.dx[1] * g.dx[2];
+    za.plan = pLz;
+    return za;
+  }
+
+  // time the scalar z pass for each tile width on this handle's buffers
+  int tune_scalar() override {
+    if (tz1) return 0;
+    const std::string key = z1_key();
+    if (int v = tune_cache().get(key)) {
+      tz1 = v;
+      return 0;
+    }
+    PM_CHECK(allper || Ghat != nullptr, "pmgpu: pmgpu_poisson_setup was not called");
+    PM_HIP(hipMemsetAsync(B, 0, nB * sizeof(V), stream));
+    hipEvent_t e0, e1;
+    PM_HIP(hipEventCreate(&e0));
+    PM_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_t = 0;
+    const int cands[4] = {1, 2, 4, 8};
+    ZArgs za = z1_args();
+    for (int t : cands) {
+      if (fx_z1<R>(L[2], t, za, L[1], d_tw, stream) != 1) continue;  // warm-up / availability
+      PM_HIP(hipEventRecord(e0, stream));
+      for (int r = 0; r < 3; ++r) fx_z1<R>(L[2], t, za, L[1], d_tw, stream);
+      PM_HIP(hipEventRecord(e1, stream));
+      PM_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      PM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_t = t;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    PM_CHECK(best_t > 0, "pmgpu: no scalar z-pass kernel for this transform length");
+    tz1 = best_t;
+    tune_cache().put(key, best_t);
+    return 0;
+  }
+
+  // Stream-ordered; no host synchronisation and no memset, so it can be
+  // captured into a graph once tune_scalar has run.
+  int solve_scalar_device(const void *w, void *psi) override {
+    PM_CHECK(allper || Ghat != nullptr, "pmgpu: pmgpu_poisson_setup was not called");
+    PM_CHECK(w && psi, "pmgpu: null mesh pointer");
+    PM_CHECK(tz1 > 0, "pmgpu: scalar solve not tuned (pmgpu_poisson_tune_scalar)");
+    const int64_t H1 = row1();
+    const int64_t off = (int64_t)lo[0] + (int64_t)E[0] * (lo[1] + (int64_t)E[1] * lo[2]);
+    const int64_t ys = E[0], zs = (int64_t)E[0] * E[1];
+    const R *win = reinterpret_cast<const R *>(w) + off;
+    int e = 0;
+    e |= launch_xfwd<R>(win, A, 0, 1, ys, zs, 0, 1, H1, H1 * M[1], 1, M[0], M[1], M[2], Tx, pNh, twLx_off, d_tw);
+    e |= launch_strided<R, -1>(A, B, H1, H1, H1 * M[1], H1 * L[1], Hx, M[1], L[1], M[2], 1, 1.0, Ts, pLy, d_tw);
+    if (e) return -1;
+    ZArgs za = z1_args();
+    PM_CHECK(fx_z1<R>(L[2], tz1, za, L[1], d_tw, stream) == 1, "pmgpu: scalar z-pass launch failed");
+    e |= launch_strided<R, 1>(B, A, H1, H1, H1 * L[1], H1 * M[1], Hx, L[1], M[1], M[2], 1, 1.0 / (double)L[1], Ts,
+                              pLy, d_tw);
+    if (e) return -1;
+    XInvArgs xa{A, reinterpret_cast<R *>(psi) + off, 0, 1, ys, zs, 0, 1, H1, H1 * M[1], 1, M[0], M[1], M[2],
+                Hx, Tx, 1.0 / (double)L[0], pNh, twLx_off};
+    PM_CHECK(fx_xinv<R>(Nh, xa, d_tw, stream) == 1, "pmgpu: scalar solve needs a specialised x length");
+    for (int d = 0; d < 3; ++d)
+      if (g.bound_cond[d] == 1) {
+        hipLaunchKernelGGL(k_ghost_fill<R>, dim3(1024), dim3(256), 0, stream, reinterpret_cast<R *>(psi), E[0],
+                           E[1], E[2], d, g.ncell[d], ngs[d], 1);
+        PM_HIP(hipGetLastError());
+      }
+    return 0;
+  }
+
+  int solve_scalar_host(const double *w, double *psi) override {
+    if (tune_scalar()) return -1;
+    const size_t n1 = nmesh / 3;
+    if (!mesh_in) PM_HIP(hipMalloc(&mesh_in, nmesh * sizeof(R)));
+    if (!mesh_out) PM_HIP(hipMalloc(&mesh_out, 3 * nmesh * sizeof(R)));
+    std::vector<R> stage(n1);
+    for (size_t i = 0; i < n1; ++i) stage[i] = (R)w[i];
+    PM_HIP(hipMemcpyAsync(mesh_in, stage.data(), n1 * sizeof(R), hipMemcpyHostToDevice, stream));
+    PM_HIP(hipMemsetAsync(mesh_out, 0, n1 * sizeof(R), stream));
+    if (solve_scalar_device(mesh_in, mesh_out)) return -1;
+    PM_HIP(hipStreamSynchronize(stream));
+    PM_HIP(hipMemcpy(stage.data(), mesh_out, n1 * sizeof(R), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n1; ++i) psi[i] = (double)stage[i];
+    return 0;
+  }
 };
 
 }  // namespace pmgpu
@@ -441,6 +607,29 @@ using namespace pmgpu;
 
 struct pmgpu_poisson_s {
   std::unique_ptr<PoissonBase> impl;
+};
+
+// A set of solver handles (patches of any sizes) solved together: the
+// single-component solves of all patches are captured once into a HIP graph,
+// one branch per patch on its own stream so that small patches run side by
+// side, and the graph is replayed while the mesh pointers stay the same.
+struct pmgpu_poisson_batch_s {
+  std::vector<pmgpu_poisson_t> h;
+  hipStream_t stream = nullptr;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void *> in;
+  std::vector<void *> out;
+  ~pmgpu_poisson_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto st : side) (void)hipStreamDestroy(st);
+  }
 };
 
 extern "C" {
@@ -508,6 +697,93 @@ int pmgpu_poisson_set_timing(pmgpu_poisson_t h, int32_t enable) {
   PM_CHECK(h, "pmgpu_poisson_set_timing: null handle");
   h->impl->timing = enable != 0;
   h->impl->nrec = 0;
+  return 0;
+}
+
+int pmgpu_poisson_tune_scalar(pmgpu_poisson_t h) {
+  PM_CHECK(h, "pmgpu_poisson_tune_scalar: null handle");
+  return h->impl->tune_scalar();
+}
+
+int pmgpu_poisson_solve_scalar(pmgpu_poisson_t h, const double *w, double *psi) {
+  PM_CHECK(h && w && psi, "pmgpu_poisson_solve_scalar: null argument");
+  return h->impl->solve_scalar_host(w, psi);
+}
+
+int pmgpu_poisson_solve_scalar_device(pmgpu_poisson_t h, const void *d_w, void *d_psi) {
+  PM_CHECK(h, "pmgpu_poisson_solve_scalar_device: null handle");
+  if (h->impl->tune_scalar()) return -1;
+  return h->impl->solve_scalar_device(d_w, d_psi);
+}
+
+int pmgpu_poisson_batch_create(const pmgpu_poisson_t *handles, int32_t n, void *stream,
+                               pmgpu_poisson_batch_t *out) {
+  PM_CHECK(handles && out && n > 0, "pmgpu_poisson_batch_create: bad argument");
+  std::unique_ptr<pmgpu_poisson_batch_s> b(new pmgpu_poisson_batch_s);
+  b->h.assign(handles, handles + n);
+  b->stream = (hipStream_t)stream;
+  for (auto hh : b->h) {
+    PM_CHECK(hh, "pmgpu_poisson_batch_create: null handle");
+    if (hh->impl->tune_scalar()) return -1;  // tuning launches may not be captured
+  }
+  const int nside = n < 8 ? n : 8;
+  for (int i = 0; i < nside; ++i) {
+    hipStream_t st;
+    PM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    b->side.push_back(st);
+  }
+  for (int i = 0; i < n; ++i) {
+    hipEvent_t e;
+    PM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->ev.push_back(e);
+  }
+  PM_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  *out = b.release();
+  return 0;
+}
+
+int pmgpu_poisson_batch_destroy(pmgpu_poisson_batch_t b) {
+  delete b;
+  return 0;
+}
+
+int pmgpu_poisson_batch_solve_scalar(pmgpu_poisson_batch_t b, const void *const *d_in, void *const *d_out) {
+  PM_CHECK(b && d_in && d_out, "pmgpu_poisson_batch_solve_scalar: null argument");
+  const size_t n = b->h.size();
+  bool same = b->exec != nullptr;
+  for (size_t i = 0; same && i < n; ++i) same = b->in[i] == d_in[i] && b->out[i] == d_out[i];
+  if (!same) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->in.assign(d_in, d_in + n);
+    b->out.assign(d_out, d_out + n);
+    // capture: fork every patch onto a side stream, join back
+    PM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = hipEventRecord(b->fork, b->stream) == hipSuccess ? 0 : -1;
+    for (size_t i = 0; rc == 0 && i < n; ++i) {
+      hipStream_t st = b->side[i % b->side.size()];
+      PoissonBase *p = b->h[i]->impl.get();
+      hipStream_t own = p->stream;
+      if (hipStreamWaitEvent(st, b->fork, 0) != hipSuccess) rc = -1;
+      p->stream = st;
+      if (rc == 0 && p->solve_scalar_device(d_in[i], d_out[i])) rc = -1;
+      p->stream = own;
+      if (rc == 0 && hipEventRecord(b->ev[i], st) != hipSuccess) rc = -1;
+      if (rc == 0 && hipStreamWaitEvent(b->stream, b->ev[i], 0) != hipSuccess) rc = -1;
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+    if (rc != 0 || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      b->in.clear();
+      PM_CHECK(false, "pmgpu_poisson_batch_solve_scalar: graph capture failed");
+    }
+    b->graph = graph;
+    PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  PM_HIP(hipGraphLaunch(b->exec, b->stream));
   return 0;
 }
 

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
   twid_load<NH>(twd, tid % P, tw + a.plan.tw_off);
   // packed pairs x[2m] + i x[2m+1]; m >= ceil(Min/2) are zero
   const int half = (a.Min + 1) / 2;
-  if (a.cs == 1 && a.xs == NC) {
+  if ((NC == 1 && a.xs == 1) || (a.cs == 1 && a.xs == NC)) {
     // the NC components of a line are one contiguous row of NC*Min words:
     // read it word by word across the lanes and scatter into the LDS lines
     constexpr int W = NC * 2 * NH, K = (T * W + NT - 1) / NT;
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
   R *out = reinterpret_cast<R *>(a.out);
   const R sc = (R)a.scale;
   const int halfo = (a.Mout + 1) / 2;
-  if (a.cs == 1 && a.xs == NC) {
+  if ((NC == 1 && a.xs == 1) || (a.cs == 1 && a.xs == NC)) {
     constexpr int W = NC * 2 * NH;
     const R *ldsr = reinterpret_cast<const R *>(lds);
     for (int idx = tid; idx < T * W; idx += NT) {
@@ -325,6 +325,75 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
     }
     __syncthreads();
   }
+}
+
+// ------------------------------------------------------- scalar z pass
+// One component (psi-hat = G w-hat, Cin = Cout = 1): a block owns T
+// consecutive kx columns of one ky row, B[((z*Ly + ky)*rowp + kx)], and
+// transforms them forward, multiplies by G h^3 (or 1/(4 pi^2 |k|^2) when all
+// axes are periodic) and transforms back, in place.
+template <typename R, int L, int T>
+__global__ void __launch_bounds__(Geo<L>::P *T) k_z1_f(ZArgs a, const cplx_t<R> *tw_all) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<L>::P, ST = Geo<L>::ST, NT = P * T;
+  constexpr int E = (L + P - 1) / P;
+  constexpr bool FULL = E * P == L;
+  __shared__ V lds[T * ST];
+  const V *tw = tw_all + a.plan.tw_off;
+  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
+  const int kx0 = (int)(lt % gridDim.x) * T, ky = (int)(lt / gridDim.x);
+  const int tid = threadIdx.x;
+  const int line = tid / P, p = tid % P;
+  Twid<L, V> twd;
+  twid_load<L>(twd, p, tw);
+  const int kx = kx0 + line;
+  const int ncl = a.Hx - kx0 < T ? a.Hx - kx0 : T;
+  V *B = reinterpret_cast<V *>(a.B);
+  const int64_t tile = (int64_t)ky * a.kys + kx0;
+  const int cl = tid % T, r0 = tid / T;  // NT / T == P rows per sweep
+  {
+    V v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int row = r0 + i * P;
+      v[i] = cmake<V>(0, 0);
+      if (row < a.Mz && cl < ncl) v[i] = B[tile + (int64_t)row * a.rs + cl];
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int row = r0 + i * P;
+      if (row < L) lds[cl * ST + pad(row)] = v[i];
+    }
+  }
+  __syncthreads();
+  V *myl = lds + line * ST;
+  fft<L, -1>(myl, p, twd);
+  const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
+  const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
+  const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
+  const R pi = (R)3.14159265358979323846;
+  const R *G = reinterpret_cast<const R *>(a.Ghat);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = p + e * P;
+    if (!(FULL || n < L)) continue;
+    R g = (R)0;
+    if (a.allper) {
+      const R kk = wavenumber<R>(n, L, a.dk[2], a.ksmp[2]);
+      const R kmag = ki * ki + kj * kj + kk * kk;
+      g = kmag > (R)0 ? (R)1 / ((R)4 * (pi * pi) * kmag) : (R)0;
+    } else if (kx < a.Hx) {
+      const int kzf = n <= L - n ? n : L - n;
+      g = G[((int64_t)kyf * a.Hx + kx) * a.Hz + kzf] * (R)a.h3;
+    }
+    myl[pad(n)] = cscale(myl[pad(n)], g);
+  }
+  line_sync<Geo<L>::WAVE>();
+  fft<L, 1>(myl, p, twd);
+  __syncthreads();
+  const R scale = (R)1 / (R)L;
+  if (cl < ncl)
+    for (int row = r0; row < a.Mz; row += P) B[tile + (int64_t)row * a.rs + cl] = cscale(lds[cl * ST + pad(row)], scale);
 }
 
 }  // namespace fx

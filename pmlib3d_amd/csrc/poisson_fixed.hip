@@ -92,11 +92,38 @@ template <typename R, int NH> int xfwd_L(int NC, XFwdArgs a, const cplx_t<R> *tw
 }
 
 template <typename R, int NH> int xinv_L(XInvArgs a, const cplx_t<R> *tw, hipStream_t st) {
-  constexpr int T = tile_lines<R, NH, 3>();
-  a.lines_per_block = T;
-  dim3 grid((a.My + T - 1) / T, a.Mz);
-  hipLaunchKernelGGL((k_x_inv_f<R, NH, T, 3>), grid, dim3(Geo<NH>::P * T * 3), 0, st, a, tw);
+  if (a.ncomp == 1) {
+    constexpr int T = tile_lines<R, NH, 1>();
+    a.lines_per_block = T;
+    dim3 grid((a.My + T - 1) / T, a.Mz);
+    hipLaunchKernelGGL((k_x_inv_f<R, NH, T, 1>), grid, dim3(Geo<NH>::P * T), 0, st, a, tw);
+  } else {
+    constexpr int T = tile_lines<R, NH, 3>();
+    a.lines_per_block = T;
+    dim3 grid((a.My + T - 1) / T, a.Mz);
+    hipLaunchKernelGGL((k_x_inv_f<R, NH, T, 3>), grid, dim3(Geo<NH>::P * T * 3), 0, st, a, tw);
+  }
   return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+
+// scalar z pass: T columns per block, T in {1, 2, 4, 8} where the LDS fits
+template <typename R, int L, int T> int z1_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  if constexpr (T * Geo<L>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget || Geo<L>::P * T > 1024) {
+    return 0;
+  } else {
+    dim3 grid((a.Hx + T - 1) / T, Ly);
+    hipLaunchKernelGGL((k_z1_f<R, L, T>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+    return hipGetLastError() == hipSuccess ? 1 : -1;
+  }
+}
+template <typename R, int L> int z1_L(int T, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  switch (T) {
+    case 1: return z1_T<R, L, 1>(a, Ly, tw, st);
+    case 2: return z1_T<R, L, 2>(a, Ly, tw, st);
+    case 4: return z1_T<R, L, 4>(a, Ly, tw, st);
+    case 8: return z1_T<R, L, 8>(a, Ly, tw, st);
+    default: return 0;
+  }
 }
 
 #define PM_FX_LENGTHS(X) \
@@ -132,6 +159,15 @@ template <typename R> int fx_xfwd(int NH, int NC, XFwdArgs a, const cplx_t<R> *t
     default: return 0;
   }
 }
+template <typename R> int fx_z1(int L, int T, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  switch (L) {
+#define X(n) \
+  case n: return fx::z1_L<R, n>(T, a, Ly, tw, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
 template <typename R> int fx_xinv(int NH, XInvArgs a, const cplx_t<R> *tw, hipStream_t st) {
   switch (NH) {
 #define X(n) \
@@ -150,5 +186,7 @@ template int fx_xfwd<float>(int, int, XFwdArgs, const float2 *, hipStream_t);
 template int fx_xfwd<double>(int, int, XFwdArgs, const double2 *, hipStream_t);
 template int fx_xinv<float>(int, XInvArgs, const float2 *, hipStream_t);
 template int fx_xinv<double>(int, XInvArgs, const double2 *, hipStream_t);
+template int fx_z1<float>(int, int, ZArgs, int, const float2 *, hipStream_t);
+template int fx_z1<double>(int, int, ZArgs, int, const double2 *, hipStream_t);
 
 }  // namespace pmgpu
