@@ -171,5 +171,71 @@ template <int L, int S, typename V> __device__ __forceinline__ void fft(V *line,
   StageLoop<L, S, 0, Geo<L>::WAVE, V>::run(line, p, t);
 }
 
+// ------------------------------------------------ register-resident lines
+// Thread p of a line holds elements p + P m (m < L / P) in registers on entry
+// and on exit; the first stage reads its inputs from those registers and the
+// last stage leaves its outputs there, so the line buffer in LDS is used only
+// for the exchanges between stages.  Lines may span several wavefronts: every
+// thread of the block calls this together (block barriers).
+template <int L> constexpr bool regs_ok() {
+  constexpr int P = Geo<L>::P;
+  if (!supported(L) || nstages(L) < 2 || (L / P) * P != L) return false;
+  const int n0 = L / radix_at(L, 0), n1 = L / radix_at(L, nstages(L) - 1);
+  return n0 % P == 0 && n1 % P == 0;
+}
+
+template <int L, int S, int I, typename V, bool DONE = (I >= nstages(L) - 1)> struct MidStages;
+template <int L, int S, int I, typename V> struct MidStages<L, S, I, V, true> {
+  static __device__ __forceinline__ void run(V *, int, const Twid<L, V> &) {}
+};
+template <int L, int S, int I, typename V> struct MidStages<L, S, I, V, false> {
+  static __device__ __forceinline__ void run(V *line, int p, const Twid<L, V> &t) {
+    stage<L, S, radix_at(L, I), ns_before(L, I), false>(line, p, t.w[I]);
+    MidStages<L, S, I + 1, V>::run(line, p, t);
+  }
+};
+
+template <int L, int S, typename V>
+__device__ __forceinline__ void fft_regs(V (&x)[L / Geo<L>::P], V *line, int p, const Twid<L, V> &t) {
+  constexpr int P = Geo<L>::P;
+  {  // first stage: registers -> line (no twiddles, NS = 1)
+    constexpr int R = radix_at(L, 0), NBF = L / R, MB = NBF / P;
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      V v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = x[b + MB * r];
+      Dft<R, S, V>::run(v);
+      const int jj = p + b * P;
+#pragma unroll
+      for (int r = 0; r < R; ++r) line[pad(jj * R + r)] = v[r];
+    }
+  }
+  __syncthreads();
+  MidStages<L, S, 1, V>::run(line, p, t);
+  {  // last stage: line -> registers (NS = NBF, outputs at p + P m)
+    constexpr int I = nstages(L) - 1, R = radix_at(L, I), NBF = L / R, MB = NBF / P;
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int jj = p + b * P;
+      V v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = line[pad(jj + r * NBF)];
+      V w1 = t.w[I][b];
+      if (S > 0) w1.y = -w1.y;
+      V w = w1;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        v[r] = cmul(v[r], w);
+        if (r + 1 < R) w = cmul(w, w1);
+      }
+      Dft<R, S, V>::run(v);
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[b + MB * r] = v[r];
+    }
+  }
+  __syncthreads();  // the line buffer is free again
+}
+
 }  // namespace fx
 }  // namespace pmgpu

@@ -166,6 +166,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
   int Tx = 1, Ts = 1, Tz = 1;
   int64_t H3p = 0;   // padded row length (complex) of the interleaved spectra
   int tune_tc = 0;   // z-pass columns per block (0 = built-in default)
+  int zkernel = 0;   // 0: register-resident velocity z pass where available, 1: LDS-resident
   int lds_limit = 65536;
 
   ~PoissonGPU() override {
@@ -220,6 +221,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     nA = (size_t)H3p * M[1] * M[2];
     nB = (size_t)H3p * L[1] * M[2];
     if (const char *env = getenv("PMGPU_Z_TC")) tune_tc = atoi(env);
+    if (const char *env = getenv("PMGPU_Z_KERNEL")) zkernel = atoi(env);
     nmesh = (size_t)3 * E[0] * E[1] * E[2];
     PM_HIP(hipMalloc(&A, nA * sizeof(V)));
     PM_HIP(hipMalloc(&B, nB * sizeof(V)));
@@ -338,10 +340,17 @@ template <typename R> struct PoissonGPU : PoissonBase {
     dims[0] = Hx;
     dims[1] = Hy;
     dims[2] = Hz;
-    size_t n = (size_t)Hx * Hy * Hz * sizeof(R);
+    const size_t cnt = (size_t)Hx * Hy * Hz;
     PM_CHECK(Ghat != nullptr, "pmgpu: kernel not set up");
-    PM_CHECK((int64_t)n <= cap, "pmgpu: buffer too small for the kernel octant");
-    PM_HIP(hipMemcpy(host, Ghat, n, hipMemcpyDeviceToHost));
+    PM_CHECK((int64_t)(cnt * sizeof(R)) <= cap, "pmgpu: buffer too small for the kernel octant");
+    // device layout [(kz' * Hy + ky') * Hx + kx]; exported as [(ky' * Hx + kx) * Hz + kz']
+    std::vector<R> dev(cnt);
+    PM_HIP(hipMemcpy(dev.data(), Ghat, cnt * sizeof(R), hipMemcpyDeviceToHost));
+    R *out = reinterpret_cast<R *>(host);
+    for (int kz = 0; kz < Hz; ++kz)
+      for (int ky = 0; ky < Hy; ++ky)
+        for (int kx = 0; kx < Hx; ++kx)
+          out[((size_t)ky * Hx + kx) * Hz + kz] = dev[((size_t)kz * Hy + ky) * Hx + kx];
     return 0;
   }
 
@@ -392,6 +401,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     za.want_vort = want_vort;
     za.reproj = reproj;
     za.tune_tc = tune_tc;
+    za.zkernel = zkernel;
     double dmax = g.dx[0] > g.dx[1] ? g.dx[0] : g.dx[1];
     dmax = dmax > g.dx[2] ? dmax : g.dx[2];
     for (int d = 0; d < 3; ++d) {

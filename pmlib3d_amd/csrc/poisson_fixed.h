@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
         g = kmag > (R)0 ? zeta / ((R)4 * (pi * pi) * kmag) : (R)0;
       } else {
         const int kzf = n <= L - n ? n : L - n;
-        g = col_ok ? G[((int64_t)kyf * a.Hx + kx) * a.Hz + kzf] * (R)a.h3 : (R)0;
+        g = col_ok ? G[((int64_t)kzf * a.Hy + kyf) * a.Hx + kx] * (R)a.h3 : (R)0;
       }
       if (EXTRA && kind == 1) {
         r[e] = cscale(own[EXTRA ? e : 0], g);
@@ -324,6 +324,85 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
       if (cl < ncl) dst[(int64_t)row * a.rs + cl] = cscale(lds[cl * ST + pad(row)], scale);
     }
     __syncthreads();
+  }
+}
+
+// ------------------------------------------- register-resident fused z pass
+// Velocity only.  Thread (col, p) of a block owns kx column kx0 + col (of TC
+// consecutive ones) and the z rows p + P m of all three components: its loads
+// and stores are whole cache lines across the block's lanes (consecutive
+// columns, components interleaved), the first and last FFT stages run on the
+// registers, and the curl is formed in registers at the thread's own
+// wavenumbers; LDS only carries the exchanges between FFT stages, one
+// component at a time.
+template <typename R, int L, int TC>
+__global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R> *tw_all) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<L>::P, ST = Geo<L>::ST, E = L / P;
+  __shared__ V lds[TC * ST];
+  const V *tw = tw_all + a.plan.tw_off;
+  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
+  const int kx0 = (int)(lt % gridDim.x) * TC, ky = (int)(lt / gridDim.x);
+  const int tid = threadIdx.x;
+  const int col = tid % TC, p = tid / TC;
+  Twid<L, V> twd;
+  twid_load<L>(twd, p, tw);
+  const int kx = kx0 + col;
+  const bool col_ok = kx < a.Hx;
+  V *B = reinterpret_cast<V *>(a.B) + (int64_t)ky * a.kys + (int64_t)kx * 3;
+  V w[3][E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int row = p + P * m;
+    const bool ok = col_ok && row < a.Mz;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) w[c][m] = ok ? B[(int64_t)row * a.rs + c] : cmake<V>(0, 0);
+  }
+  V *line = lds + col * ST;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) fft_regs<L, -1>(w[c], line, p, twd);
+  // u-hat = G 2 pi i (k x w-hat); Nyquist wavenumbers drop out (Hermitian
+  // projection of the real inverse)
+  const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
+  const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
+  const R kx_ = (2 * kx == a.Lx) ? (R)0 : ki, ky_ = (2 * ky == a.Ly) ? (R)0 : kj;
+  const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
+  const R pi = (R)3.14159265358979323846, twopi = (R)6.28318530717958647692;
+  const R *G = reinterpret_cast<const R *>(a.Ghat);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = p + P * m;
+    const R kk = wavenumber<R>(n, L, a.dk[2], a.ksmp[2]);
+    R g = (R)0;
+    if (a.allper) {
+      const R kmag = ki * ki + kj * kj + kk * kk;
+      g = kmag > (R)0 ? (R)1 / ((R)4 * (pi * pi) * kmag) : (R)0;
+    } else if (col_ok) {
+      const int kzf = n <= L - n ? n : L - n;
+      g = G[((int64_t)kzf * a.Hy + kyf) * a.Hx + kx] * (R)a.h3;
+    }
+    const R kz_ = (2 * n == L) ? (R)0 : kk;
+    const V w0 = w[0][m], w1 = w[1][m], w2 = w[2][m];
+    const V c0 = csub(cscale(w2, ky_), cscale(w1, kz_));
+    const V c1 = csub(cscale(w0, kz_), cscale(w2, kx_));
+    const V c2 = csub(cscale(w1, kx_), cscale(w0, ky_));
+    const R f = g * twopi;
+    w[0][m] = cmake<V>(-c0.y * f, c0.x * f);
+    w[1][m] = cmake<V>(-c1.y * f, c1.x * f);
+    w[2][m] = cmake<V>(-c2.y * f, c2.x * f);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) fft_regs<L, 1>(w[c], line, p, twd);
+  const R scale = (R)1 / (R)L;
+  if (col_ok) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int row = p + P * m;
+      if (row < a.Mz) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) B[(int64_t)row * a.rs + c] = cscale(w[c][m], scale);
+      }
+    }
   }
 }
 
@@ -384,7 +463,7 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_z1_f(ZArgs a, const cplx_t<R> 
       g = kmag > (R)0 ? (R)1 / ((R)4 * (pi * pi) * kmag) : (R)0;
     } else if (kx < a.Hx) {
       const int kzf = n <= L - n ? n : L - n;
-      g = G[((int64_t)kyf * a.Hx + kx) * a.Hz + kzf] * (R)a.h3;
+      g = G[((int64_t)kzf * a.Hy + kyf) * a.Hx + kx] * (R)a.h3;
     }
     myl[pad(n)] = cscale(myl[pad(n)], g);
   }

@@ -191,6 +191,7 @@ struct ZArgs {
   int allper;         // all axes periodic: G = zeta / (4 pi^2 |k|^2)
   int want_psi, want_vort, reproj;
   int tune_tc;        // z-pass columns per block (0 = default)
+  int zkernel;        // velocity-only z pass: 0 register-resident if available, 1 LDS-resident
   double dk[3], ksmp[3], h3, eps2pi;
   FftPlan plan;       // length Lz
 };
@@ -281,7 +282,7 @@ __global__ void __launch_bounds__(1024) k_z_convolve(ZArgs a, const cplx_t<R> *t
         R kmag = ki * ki + kj * kj + kk * kk;
         g = kmag > (R)0 ? zeta / ((R)4 * (pi * pi) * kmag) : (R)0;
       } else {
-        g = col_ok ? G[((int64_t)kyf * a.Hx + kx) * a.Hz + kzf] * (R)a.h3 : (R)0;
+        g = col_ok ? G[((int64_t)kzf * a.Hy + kyf) * a.Hx + kx] * (R)a.h3 : (R)0;
       }
       V r;
       if (kind == 0) {
@@ -402,16 +403,17 @@ static __global__ void k_green(GreenArgs a) {
 }
 
 // Octant extraction of the real, even kernel spectrum:
-// Ghat[(ky' * Hx + kx) * Hz + kz'] = Re B[(kz' * Ly + ky') * Hx + kx].
+// Ghat[(kz' * Hy + ky') * Hx + kx] = Re B[(kz' * Ly + ky') * Hx + kx]: kx
+// fastest, as the z passes read it (consecutive lanes, consecutive kx).
 template <typename R>
 __global__ void k_g_octant(const double2 *B, R *G, int Hx, int Ly, int Hy, int Hz) {
   int64_t n = (int64_t)Hy * Hx * Hz;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    int kz = idx % Hz;
-    int64_t r = idx / Hz;
-    int kx = r % Hx;
-    int ky = r / Hx;
+    int kx = idx % Hx;
+    int64_t r = idx / Hx;
+    int ky = r % Hy;
+    int kz = r / Hy;
     G[idx] = (R)B[((int64_t)kz * Ly + ky) * Hx + kx].x;
   }
 }
