@@ -167,6 +167,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
   int64_t H3p = 0;   // padded row length (complex) of the interleaved spectra
   int tune_tc = 0;   // z-pass columns per block (0 = built-in default)
   int zkernel = 0;   // 0: register-resident velocity z pass where available, 1: LDS-resident
+  int skernel = 0;   // the same choice for the strided (y and z) passes
   int lds_limit = 65536;
 
   ~PoissonGPU() override {
@@ -222,6 +223,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     nB = (size_t)H3p * L[1] * M[2];
     if (const char *env = getenv("PMGPU_Z_TC")) tune_tc = atoi(env);
     if (const char *env = getenv("PMGPU_Z_KERNEL")) zkernel = atoi(env);
+    if (const char *env = getenv("PMGPU_STRIDED_KERNEL")) skernel = atoi(env);
     nmesh = (size_t)3 * E[0] * E[1] * E[2];
     PM_HIP(hipMalloc(&A, nA * sizeof(V)));
     PM_HIP(hipMalloc(&B, nB * sizeof(V)));
@@ -262,7 +264,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
   int launch_strided(const void *in, void *out, int64_t rs_in, int64_t rs_out, int64_t bs_in, int64_t bs_out,
                      int ncols, int nin, int nout, int nb_y, int nb_z, double scale, int T, const FftPlan &pl,
                      const cplx_t<RR> *twp) {
-    StridedArgs a{in, out, rs_in, rs_out, bs_in, bs_out, ncols, nin, nout, T, scale, pl};
+    StridedArgs a{in, out, rs_in, rs_out, bs_in, bs_out, ncols, nin, nout, T, scale, pl, skernel};
     int f = fx_strided<RR>(pl.L, S, a, nb_y, nb_z, twp, stream);
     PM_CHECK(f >= 0, "pmgpu: strided-pass launch failed");
     if (f == 1) return 0;
@@ -625,10 +627,11 @@ struct pmgpu_poisson_s {
 // side, and the graph is replayed while the mesh pointers stay the same.
 struct pmgpu_poisson_batch_s {
   std::vector<pmgpu_poisson_t> h;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // the caller's stream (may be the null stream)
+  hipStream_t cap = nullptr;      // the graph's own stream (the null stream cannot be captured)
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> ev;
-  hipEvent_t fork = nullptr;
+  hipEvent_t fork = nullptr, in_done = nullptr, out_done = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<const void *> in;
@@ -637,8 +640,11 @@ struct pmgpu_poisson_batch_s {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto e : ev) (void)hipEventDestroy(e);
-    if (fork) (void)hipEventDestroy(fork);
+    hipEvent_t es[3] = {fork, in_done, out_done};
+    for (auto e : es)
+      if (e) (void)hipEventDestroy(e);
     for (auto st : side) (void)hipStreamDestroy(st);
+    if (cap) (void)hipStreamDestroy(cap);
   }
 };
 
@@ -748,6 +754,9 @@ int pmgpu_poisson_batch_create(const pmgpu_poisson_t *handles, int32_t n, void *
     b->ev.push_back(e);
   }
   PM_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  PM_HIP(hipEventCreateWithFlags(&b->in_done, hipEventDisableTiming));
+  PM_HIP(hipEventCreateWithFlags(&b->out_done, hipEventDisableTiming));
+  PM_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
   *out = b.release();
   return 0;
 }
@@ -769,9 +778,10 @@ int pmgpu_poisson_batch_solve_scalar(pmgpu_poisson_batch_t b, const void *const 
     b->graph = nullptr;
     b->in.assign(d_in, d_in + n);
     b->out.assign(d_out, d_out + n);
-    // capture: fork every patch onto a side stream, join back
-    PM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = hipEventRecord(b->fork, b->stream) == hipSuccess ? 0 : -1;
+    // capture on the batch's own stream: fork every patch onto a side
+    // stream, join back
+    PM_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    int rc = hipEventRecord(b->fork, b->cap) == hipSuccess ? 0 : -1;
     for (size_t i = 0; rc == 0 && i < n; ++i) {
       hipStream_t st = b->side[i % b->side.size()];
       PoissonBase *p = b->h[i]->impl.get();
@@ -781,10 +791,11 @@ int pmgpu_poisson_batch_solve_scalar(pmgpu_poisson_batch_t b, const void *const 
       if (rc == 0 && p->solve_scalar_device(d_in[i], d_out[i])) rc = -1;
       p->stream = own;
       if (rc == 0 && hipEventRecord(b->ev[i], st) != hipSuccess) rc = -1;
-      if (rc == 0 && hipStreamWaitEvent(b->stream, b->ev[i], 0) != hipSuccess) rc = -1;
+      if (rc == 0 && hipStreamWaitEvent(b->cap, b->ev[i], 0) != hipSuccess) rc = -1;
     }
     hipGraph_t graph = nullptr;
-    hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+    hipError_t ec = hipStreamEndCapture(b->cap, &graph);
+    (void)hipGetLastError();  // a failed capture leaves its error behind
     if (rc != 0 || ec != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
       b->in.clear();
@@ -793,7 +804,12 @@ int pmgpu_poisson_batch_solve_scalar(pmgpu_poisson_batch_t b, const void *const 
     b->graph = graph;
     PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   }
-  PM_HIP(hipGraphLaunch(b->exec, b->stream));
+  // order the replay after the caller's earlier work and before its later work
+  PM_HIP(hipEventRecord(b->in_done, b->stream));
+  PM_HIP(hipStreamWaitEvent(b->cap, b->in_done, 0));
+  PM_HIP(hipGraphLaunch(b->exec, b->cap));
+  PM_HIP(hipEventRecord(b->out_done, b->cap));
+  PM_HIP(hipStreamWaitEvent(b->stream, b->out_done, 0));
   return 0;
 }
 

@@ -48,6 +48,42 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const
       out[(int64_t)row * a.rs_out + c_ld] = cscale(lds[c_ld * ST + pad(row)], sc);
 }
 
+// Register-resident form: thread (col, p) owns rows p + P m of column
+// col0 + col; loads and stores are T consecutive complex words per row across
+// the lanes; first and last FFT stages on registers, LDS for the exchanges.
+template <typename R, int L, int T, int S>
+__global__ void __launch_bounds__(Geo<L>::P *T) k_strided_r(StridedArgs a, const cplx_t<R> *tw) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<L>::P, ST = Geo<L>::ST, E = L / P;
+  __shared__ V lds[T * ST];
+  const int64_t nblk = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+  const int64_t lt = xcd_tile(block_linear(), nblk);
+  const int col0 = (int)(lt % gridDim.x) * T;
+  const int64_t b = lt / gridDim.x;
+  const int tid = threadIdx.x;
+  const int col = tid % T, p = tid / T;
+  Twid<L, V> twd;
+  twid_load<L>(twd, p, tw + a.plan.tw_off);
+  const bool col_ok = col0 + col < a.ncols;
+  const V *in = reinterpret_cast<const V *>(a.in) + b * a.bs_in + col0 + col;
+  V *out = reinterpret_cast<V *>(a.out) + b * a.bs_out + col0 + col;
+  V x[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int row = p + P * m;
+    x[m] = (col_ok && row < a.nin) ? in[(int64_t)row * a.rs_in] : cmake<V>(0, 0);
+  }
+  fft_regs<L, S>(x, lds + col * ST, p, twd);
+  const R sc = (R)a.scale;
+  if (col_ok) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int row = p + P * m;
+      if (row < a.nout) out[(int64_t)row * a.rs_out] = cscale(x[m], sc);
+    }
+  }
+}
+
 // -------------------------------------------------------------- x forward
 // Lines of ncomp (= 3 or 1) components; NH = half transform length.
 template <typename R, int NH, int T, int NC>

@@ -24,6 +24,18 @@ template <typename R, int NH, int NC> constexpr int tile_lines() {
 
 template <typename R, int L>
 int strided_L(int S, StridedArgs a, int nb_y, int nb_z, const cplx_t<R> *tw, hipStream_t st) {
+  if constexpr (regs_ok<L>()) {
+    if (a.kernel == 0) {
+      constexpr int T = tile_cols<R, L>();
+      a.T = T;
+      dim3 grid((a.ncols + T - 1) / T, nb_y, nb_z);
+      if (S < 0)
+        hipLaunchKernelGGL((k_strided_r<R, L, T, -1>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+      else
+        hipLaunchKernelGGL((k_strided_r<R, L, T, 1>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+      return hipGetLastError() == hipSuccess ? 1 : -1;
+    }
+  }
   constexpr int T = tile_cols<R, L>();
   a.T = T;
   dim3 grid((a.ncols + T - 1) / T, nb_y, nb_z);
@@ -61,7 +73,7 @@ template <typename R, int L, int TC> int zr_T(ZArgs a, int Ly, const cplx_t<R> *
   }
 }
 template <typename R, int L> int zr_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
-  const int want = a.tune_tc > 0 ? a.tune_tc : (sizeof(R) == 4 ? 8 : 4);
+  const int want = a.tune_tc > 0 ? a.tune_tc : (sizeof(R) == 4 ? 4 : 8);
   switch (want) {
     case 4: return zr_T<R, L, 4>(a, Ly, tw, st);
     case 8: return zr_T<R, L, 8>(a, Ly, tw, st);

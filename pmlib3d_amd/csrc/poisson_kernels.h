@@ -89,6 +89,7 @@ struct StridedArgs {
   int ncols, nin, nout, T;
   double scale;
   FftPlan plan;
+  int kernel;  // 0: register-resident where the length allows, 1: LDS-resident
 };
 
 template <typename R, int S>

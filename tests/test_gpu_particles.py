@@ -177,7 +177,13 @@ def test_p2m_deterministic_and_empty():
     z = _p2m(g, np.zeros((3, 0), order="F"), np.zeros((3, 0), order="F"))
     assert not z.any()
     m = np.asfortranarray(np.random.default_rng(0).normal(size=z.shape))
-    assert np.array_equal(_p2m(g, np.zeros((3, 0)), np.zeros((3, 0)), m, clear=False), m)
+    # with no particles the periodic ghost layers are still folded onto
+    # their images, as in the reference
+    got = _p2m(g, np.zeros((3, 0)), np.zeros((3, 0)), m, clear=False)
+    assert np.array_equal(got, O.p2m(g, np.zeros((3, 0)), np.zeros((3, 0)), m.copy(order="F"), clear=False))
+    gf = O.make_geom((16, 16, 16), 1, [0, 0, 0], 1.0 / 16, (0, 0, 0), 3)
+    mf = np.asfortranarray(np.random.default_rng(0).normal(size=(3,) + O.mesh_extent(gf)))
+    assert np.array_equal(_p2m(gf, np.zeros((3, 0)), np.zeros((3, 0)), mf, clear=False), mf)
 
 
 def test_p2m_conserves_circulation_periodic_large():
