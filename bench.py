@@ -391,7 +391,7 @@ def main():
         "vortex_step": step,
         "batched_multipatch": batch,
         "cpu_baseline_step": cpu_step,
-        "roofline": {"bound": "hbm", "kernel": "k_z3_f<float,512,TC> (fused z pass)",
+        "roofline": {"bound": "hbm", "kernel": "k_z3_r<float,512,TC> (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": z_gbs / PEAK_HBM_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": z_bytes32, "avg_launch_ms": z32_ms},

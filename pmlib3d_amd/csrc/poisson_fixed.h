@@ -364,14 +364,35 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
 }
 
 // ------------------------------------------- register-resident fused z pass
+// Buffer-resource access (vector memory, one 32-bit VGPR offset per thread
+// plus a scalar offset per row): the resource's size bounds every access.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff, float2) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ double2 buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff, double2) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_store(float2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  using U = unsigned int __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_store(double2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  using U = unsigned int __attribute__((ext_vector_type(4)));
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U, v), r, voff, soff, 0);
+}
+
 // Velocity only.  Thread (col, p) of a block owns kx column kx0 + col (of TC
 // consecutive ones) and the z rows p + P m of all three components: its loads
 // and stores are whole cache lines across the block's lanes (consecutive
 // columns, components interleaved), the first and last FFT stages run on the
 // registers, and the curl is formed in registers at the thread's own
 // wavenumbers; LDS only carries the exchanges between FFT stages, one
-// component at a time.
-template <typename R, int L, int TC>
+// component at a time.  The host guarantees Mz * rs complex words fit a
+// 31-bit byte offset.
+template <typename R, int L, int TC, bool ALLPER>
 __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R> *tw_all) {
   using V = cplx_t<R>;
   constexpr int P = Geo<L>::P, ST = Geo<L>::ST, E = L / P;
@@ -385,14 +406,21 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   twid_load<L>(twd, p, tw);
   const int kx = kx0 + col;
   const bool col_ok = kx < a.Hx;
-  V *B = reinterpret_cast<V *>(a.B) + (int64_t)ky * a.kys + (int64_t)kx * 3;
+  // this block's tile: rows [0, Mz) of columns kx0 .. kx0 + TC - 1 (3 words each)
+  V *base = reinterpret_cast<V *>(a.B) + (int64_t)ky * a.kys + (int64_t)kx0 * 3;
+  const int64_t rsb = a.rs * (int64_t)sizeof(V);
+  const int ncols_here = a.Hx - kx0 < TC ? a.Hx - kx0 : TC;
+  const uint32_t extent = (uint32_t)((a.Mz - 1) * rsb + (int64_t)ncols_here * 3 * sizeof(V));
+  const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(base, extent);
+  const int voff = (int)(p * rsb) + col * 3 * (int)sizeof(V);
   V w[3][E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const int row = p + P * m;
-    const bool ok = col_ok && row < a.Mz;
+    const bool ok = col_ok && p + P * m < a.Mz;
+    const int soff = (int)((int64_t)P * m * rsb);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) w[c][m] = ok ? B[(int64_t)row * a.rs + c] : cmake<V>(0, 0);
+    for (int c = 0; c < 3; ++c)
+      w[c][m] = ok ? buf_load(rsrc, voff + c * (int)sizeof(V), soff, V{}) : cmake<V>(0, 0);
   }
   V *line = lds + col * ST;
 #pragma unroll
@@ -403,19 +431,21 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
   const R kx_ = (2 * kx == a.Lx) ? (R)0 : ki, ky_ = (2 * ky == a.Ly) ? (R)0 : kj;
   const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
-  const R pi = (R)3.14159265358979323846, twopi = (R)6.28318530717958647692;
-  const R *G = reinterpret_cast<const R *>(a.Ghat);
+  const R twopi = (R)6.28318530717958647692;
+  const R *G = reinterpret_cast<const R *>(a.Ghat) + (int64_t)kyf * a.Hx + kx;
+  const int64_t gz = (int64_t)a.Hy * a.Hx;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int n = p + P * m;
     const R kk = wavenumber<R>(n, L, a.dk[2], a.ksmp[2]);
-    R g = (R)0;
-    if (a.allper) {
+    R g;
+    if constexpr (ALLPER) {
       const R kmag = ki * ki + kj * kj + kk * kk;
+      const R pi = (R)3.14159265358979323846;
       g = kmag > (R)0 ? (R)1 / ((R)4 * (pi * pi) * kmag) : (R)0;
-    } else if (col_ok) {
+    } else {
       const int kzf = n <= L - n ? n : L - n;
-      g = G[((int64_t)kzf * a.Hy + kyf) * a.Hx + kx] * (R)a.h3;
+      g = col_ok ? G[kzf * gz] * (R)a.h3 : (R)0;
     }
     const R kz_ = (2 * n == L) ? (R)0 : kk;
     const V w0 = w[0][m], w1 = w[1][m], w2 = w[2][m];
@@ -433,10 +463,10 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   if (col_ok) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int row = p + P * m;
-      if (row < a.Mz) {
+      if (p + P * m < a.Mz) {
+        const int soff = (int)((int64_t)P * m * rsb);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) B[(int64_t)row * a.rs + c] = cscale(w[c][m], scale);
+        for (int c = 0; c < 3; ++c) buf_store(cscale(w[c][m], scale), rsrc, voff + c * (int)sizeof(V), soff);
       }
     }
   }

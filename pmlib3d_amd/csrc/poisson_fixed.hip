@@ -63,27 +63,31 @@ template <typename R, int L, int TC> int z_tc(ZArgs a, int Ly, const cplx_t<R> *
 
 // register-resident velocity z pass with TC columns per block
 template <typename R, int L, int TC> int zr_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
-  if constexpr (!regs_ok<L>() || TC * Geo<L>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget ||
-                Geo<L>::P * TC > 1024) {
+  if constexpr (!regs_ok<L>() || TC * Geo<L>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget || Geo<L>::P * TC > 1024) {
     return 0;
   } else {
+    // buffer offsets of the tile are 31-bit
+    if ((double)a.Mz * (double)a.rs * sizeof(cplx_t<R>) >= 2147483647.0) return 0;
     dim3 grid((a.Hx + TC - 1) / TC, Ly);
-    hipLaunchKernelGGL((k_z3_r<R, L, TC>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    if (a.allper)
+      hipLaunchKernelGGL((k_z3_r<R, L, TC, true>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    else
+      hipLaunchKernelGGL((k_z3_r<R, L, TC, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
     return hipGetLastError() == hipSuccess ? 1 : -1;
   }
 }
 template <typename R, int L> int zr_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
   const int want = a.tune_tc > 0 ? a.tune_tc : (sizeof(R) == 4 ? 4 : 8);
   switch (want) {
+    case 2: return zr_T<R, L, 2>(a, Ly, tw, st);
     case 4: return zr_T<R, L, 4>(a, Ly, tw, st);
     case 8: return zr_T<R, L, 8>(a, Ly, tw, st);
-    case 16: return zr_T<R, L, 16>(a, Ly, tw, st);
     default: return 0;
   }
 }
 
 template <typename R, int L> int z_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
-  if (!(a.want_psi || a.want_vort) && a.zkernel == 0) {
+  if (!(a.want_psi || a.want_vort) && a.zkernel != 1) {
     const int r = zr_L<R, L>(a, Ly, tw, st);
     if (r != 0) return r;
   }
