@@ -144,7 +144,7 @@ static dim3 grid_for(int64_t n, int threads = 256) {
 
 // Workspace shared by the particle entry points of one process.
 struct PartWS {
-  Scratch key, ids, sort, count, start, scan, flag, offs, cls, tmp6;
+  Scratch key, ids, sort, count, start, scan, flag, offs, cls, tmp6, rec;
 };
 // Allocated on first use and released by pmgpu_finalize (never from a static
 // destructor, which may run after the HIP runtime has shut down).
@@ -196,12 +196,21 @@ int p2m_device(const pmgpu_geom *geom, int prec, const double *xpart, const doub
   }
   if (scan_exclusive(w.count.as<int>(), w.start.as<int>(), nbins, w.scan, st)) return -1;
   const int accumulate = clear ? 0 : 1;
-  if (prec == PMGPU_FP64)
-    hipLaunchKernelGGL(k_p2m_gather, grid_for(nn), dim3(256), 0, st, g, xpart, part, w.start.as<int>(), id_out,
-                       (double *)mesh, accumulate);
-  else
-    hipLaunchKernelGGL(k_p2m_gather_f32, grid_for(nn), dim3(256), 0, st, g, xpart, part, w.start.as<int>(),
-                       id_out, (float *)mesh, accumulate);
+  const size_t rec_bytes = (prec == PMGPU_FP64 ? sizeof(double) : sizeof(float)) * PRec<double>::kSlots;
+  if (w.rec.ensure(rec_bytes * ((size_t)np + 1))) return -1;
+  if (prec == PMGPU_FP64) {
+    if (np > 0)
+      hipLaunchKernelGGL(k_p2m_stage<double>, grid_for(np), dim3(256), 0, st, g, xpart, part, id_out,
+                         w.start.as<int>(), nbins, w.rec.as<double>());
+    hipLaunchKernelGGL(k_p2m_gather_staged<double>, grid_for(nn), dim3(256), 0, st, g, w.start.as<int>(),
+                       w.rec.as<double>(), (double *)mesh, accumulate);
+  } else {
+    if (np > 0)
+      hipLaunchKernelGGL(k_p2m_stage<float>, grid_for(np), dim3(256), 0, st, g, xpart, part, id_out,
+                         w.start.as<int>(), nbins, w.rec.as<float>());
+    hipLaunchKernelGGL(k_p2m_gather_staged<float>, grid_for(nn), dim3(256), 0, st, g, w.start.as<int>(),
+                       w.rec.as<float>(), (float *)mesh, accumulate);
+  }
   PM_HIP(hipGetLastError());
   for (int d = 0; d < 3; ++d)
     if (g.bc[d] != 0) {

@@ -3,10 +3,11 @@
 //
 // Deposition is a gather, not a scatter: particles are binned by the cell of
 // their stencil base index (bit-exact NINT), a stable radix sort orders
-// them by bin and, inside a bin, by particle id, and every mesh node sums
-// the contributions of the ns^3 bins whose stencils cover it, in a fixed
-// order.  No atomics touch the mesh, so the result is deterministic run to
-// run.
+// them by bin and, inside a bin, by particle id, each sorted particle is
+// staged once as a record of its kernel weights and strength, and every mesh
+// node sums the records of the ns^2 stencil rows that cover it (each row one
+// contiguous range of records), in a fixed order.  No atomics touch the
+// mesh, so the result is deterministic run to run.
 #pragma once
 #include "pm_common.h"
 
@@ -140,61 +141,49 @@ __global__ void k_p2m_keys(PGeom g, const double *pos, int64_t np, int nbins, in
   }
 }
 
-// one thread per mesh node: sum over the bins whose stencils cover it
-__global__ void k_p2m_gather(PGeom g, const double *pos, const double *val, const int *start, const int *ids,
-                             double *mesh, int accumulate) {
-  const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int n[3];
-    n[0] = (int)(idx % g.E[0]) + g.lo[0];
-    n[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
-    n[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-    // node n is slot s = n - b + off of base b, s in [0, ns)
-    for (int sz = g.ns - 1; sz >= 0; --sz) {
-      int bz = n[2] - sz + g.off;
-      int kz = bz - g.blo[2];
-      if (kz < 0 || kz >= g.nb[2]) continue;
-      for (int sy = g.ns - 1; sy >= 0; --sy) {
-        int by = n[1] - sy + g.off;
-        int ky = by - g.blo[1];
-        if (ky < 0 || ky >= g.nb[1]) continue;
-        for (int sx = g.ns - 1; sx >= 0; --sx) {
-          int bx = n[0] - sx + g.off;
-          int kx = bx - g.blo[0];
-          if (kx < 0 || kx >= g.nb[0]) continue;
-          int k = kx + g.nb[0] * (ky + g.nb[1] * kz);
-          for (int t = start[k]; t < start[k + 1]; ++t) {
-            int q = ids[t];
-            double wx = weight(g.m6, sx, stencil_offset(pos[3 * q], g.xmin[0], g.dx[0], bx, g.c1dx[0]));
-            double wy = weight(g.m6, sy, stencil_offset(pos[3 * q + 1], g.xmin[1], g.dx[1], by, g.c1dx[1]));
-            double wz = weight(g.m6, sz, stencil_offset(pos[3 * q + 2], g.xmin[2], g.dx[2], bz, g.c1dx[2]));
-            double w = __dmul_rn(__dmul_rn(wx, wy), wz);
-            acc0 = __dadd_rn(acc0, __dmul_rn(w, val[3 * q]));
-            acc1 = __dadd_rn(acc1, __dmul_rn(w, val[3 * q + 1]));
-            acc2 = __dadd_rn(acc2, __dmul_rn(w, val[3 * q + 2]));
-          }
-        }
+// Staged particle record, in bin order: the 3 x ns kernel weights, the 3
+// strength components and the base cell along x (a T-typed copy of the int).
+template <typename T> struct PRec {
+  static constexpr int kSlots = 24;  // 3 * 6 weights + 3 values + base x, padded
+};
+
+// Records of the sorted particles t < start[nbins]: weights from the FP64
+// offsets (FP64: the reference's arithmetic; FP32: weights in single
+// precision from the FP64 offset).
+template <typename T>
+__global__ void k_p2m_stage(PGeom g, const double *pos, const double *val, const int *ids, const int *start,
+                            int nbins, T *rec) {
+  const int nvalid = start[nbins];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nvalid; t += gridDim.x * blockDim.x) {
+    const int q = ids[t];
+    T *r = rec + (int64_t)t * PRec<T>::kSlots;
+    int bx = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double x = pos[3 * q + d];
+      const int b = cell_index(x, g.xmin[d], g.c1dx[d]);
+      if (d == 0) bx = b;
+      const double dd = stencil_offset(x, g.xmin[d], g.dx[d], b, g.c1dx[d]);
+      for (int s2 = 0; s2 < g.ns; ++s2) {
+        if constexpr (sizeof(T) == sizeof(double))
+          r[d * g.ns + s2] = weight(g.m6, s2, dd);
+        else
+          r[d * g.ns + s2] = weight_f(g.m6, s2, (float)dd);
       }
     }
-    double *m = mesh + 3 * idx;
-    if (accumulate) {
-      m[0] += acc0;
-      m[1] += acc1;
-      m[2] += acc2;
-    } else {
-      m[0] = acc0;
-      m[1] = acc1;
-      m[2] = acc2;
-    }
+    r[18] = (T)val[3 * q];
+    r[19] = (T)val[3 * q + 1];
+    r[20] = (T)val[3 * q + 2];
+    r[21] = (T)bx;
   }
 }
 
-// FP32 form: offsets from the FP64 positions (same bins and stencils),
-// weights and products in FP32, Kahan-compensated per-node sums
-__global__ void k_p2m_gather_f32(PGeom g, const double *pos, const double *val, const int *start, const int *ids,
-                                 float *mesh, int accumulate) {
+// One thread per mesh node: for each (sz, sy) stencil row the bins whose
+// x stencils cover the node are consecutive, so their particles are one
+// contiguous range of records.  Summation order: sz and sy descending, then
+// bins ascending (sx descending), particles by id within a bin.  FP64 sums
+// with the reference's rounding per term; FP32 sums are Kahan-compensated.
+template <typename T>
+__global__ void k_p2m_gather_staged(PGeom g, const int *start, const T *rec, T *mesh, int accumulate) {
   const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
        idx += (int64_t)gridDim.x * blockDim.x) {
@@ -202,42 +191,51 @@ __global__ void k_p2m_gather_f32(PGeom g, const double *pos, const double *val, 
     n[0] = (int)(idx % g.E[0]) + g.lo[0];
     n[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
     n[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
-    Kahan a0, a1, a2;
-    for (int sz = g.ns - 1; sz >= 0; --sz) {
-      int bz = n[2] - sz + g.off;
-      int kz = bz - g.blo[2];
-      if (kz < 0 || kz >= g.nb[2]) continue;
-      for (int sy = g.ns - 1; sy >= 0; --sy) {
-        int by = n[1] - sy + g.off;
-        int ky = by - g.blo[1];
-        if (ky < 0 || ky >= g.nb[1]) continue;
-        for (int sx = g.ns - 1; sx >= 0; --sx) {
-          int bx = n[0] - sx + g.off;
-          int kx = bx - g.blo[0];
-          if (kx < 0 || kx >= g.nb[0]) continue;
-          int k = kx + g.nb[0] * (ky + g.nb[1] * kz);
-          for (int t = start[k]; t < start[k + 1]; ++t) {
-            int q = ids[t];
-            float wx = weight_f(g.m6, sx, (float)stencil_offset(pos[3 * q], g.xmin[0], g.dx[0], bx, g.c1dx[0]));
-            float wy = weight_f(g.m6, sy, (float)stencil_offset(pos[3 * q + 1], g.xmin[1], g.dx[1], by, g.c1dx[1]));
-            float wz = weight_f(g.m6, sz, (float)stencil_offset(pos[3 * q + 2], g.xmin[2], g.dx[2], bz, g.c1dx[2]));
-            float w = wx * wy * wz;
-            a0.add(w * (float)val[3 * q]);
-            a1.add(w * (float)val[3 * q + 1]);
-            a2.add(w * (float)val[3 * q + 2]);
+    const int ns = g.ns;
+    int kx_lo = n[0] + g.off - ns + 1 - g.blo[0], kx_hi = n[0] + g.off - g.blo[0];
+    if (kx_lo < 0) kx_lo = 0;
+    if (kx_hi > g.nb[0] - 1) kx_hi = g.nb[0] - 1;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    Kahan f0, f1, f2;
+    if (kx_lo <= kx_hi) {
+      for (int sz = ns - 1; sz >= 0; --sz) {
+        const int kz = n[2] - sz + g.off - g.blo[2];
+        if (kz < 0 || kz >= g.nb[2]) continue;
+        for (int sy = ns - 1; sy >= 0; --sy) {
+          const int ky = n[1] - sy + g.off - g.blo[1];
+          if (ky < 0 || ky >= g.nb[1]) continue;
+          const int krow = g.nb[0] * (ky + g.nb[1] * kz);
+          const int t0 = start[krow + kx_lo], t1 = start[krow + kx_hi + 1];
+          for (int t = t0; t < t1; ++t) {
+            const T *r = rec + (int64_t)t * PRec<T>::kSlots;
+            const int sx = n[0] - (int)r[21] + g.off;
+            if constexpr (sizeof(T) == sizeof(double)) {
+              const double w = __dmul_rn(__dmul_rn(r[sx], r[ns + sy]), r[2 * ns + sz]);
+              d0 = __dadd_rn(d0, __dmul_rn(w, r[18]));
+              d1 = __dadd_rn(d1, __dmul_rn(w, r[19]));
+              d2 = __dadd_rn(d2, __dmul_rn(w, r[20]));
+            } else {
+              const float w = r[sx] * r[ns + sy] * r[2 * ns + sz];
+              f0.add(w * r[18]);
+              f1.add(w * r[19]);
+              f2.add(w * r[20]);
+            }
           }
         }
       }
     }
-    float *m = mesh + 3 * idx;
+    T *m = mesh + 3 * idx;
+    const T a0 = sizeof(T) == sizeof(double) ? (T)d0 : (T)f0.s;
+    const T a1 = sizeof(T) == sizeof(double) ? (T)d1 : (T)f1.s;
+    const T a2 = sizeof(T) == sizeof(double) ? (T)d2 : (T)f2.s;
     if (accumulate) {
-      m[0] += a0.s;
-      m[1] += a1.s;
-      m[2] += a2.s;
+      m[0] += a0;
+      m[1] += a1;
+      m[2] += a2;
     } else {
-      m[0] = a0.s;
-      m[1] = a1.s;
-      m[2] = a2.s;
+      m[0] = a0;
+      m[1] = a1;
+      m[2] = a2;
     }
   }
 }

@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
 template <typename R, int L, int T>
 __global__ void __launch_bounds__(Geo<L>::P *T) k_z1_f(ZArgs a, const cplx_t<R> *tw_all) {
   using V = cplx_t<R>;
-  constexpr int P = Geo<L>::P, ST = Geo<L>::ST, NT = P * T;
+  constexpr int P = Geo<L>::P, ST = Geo<L>::ST;
   constexpr int E = (L + P - 1) / P;
   constexpr bool FULL = E * P == L;
   __shared__ V lds[T * ST];
