@@ -237,6 +237,24 @@ def cpu_step_baseline(n=128):
                       "remesh), FP64, native Glassman FFT, 1 rank"}
 
 
+Z_KERNEL = "k_z3_r<float, 512, 4, false>"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the z pass from the committed PMC summary
+    (profiles/z_pass_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per
+    dispatch, each scaled by the copy-calibration factor), or None when that
+    summary was taken on another kernel than the one this build launches."""
+    path = os.path.join(ROOT, "profiles", "z_pass_traffic.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if d.get("kernel") != Z_KERNEL:
+        return None
+    return (d["fetch_kib"] * d["fetch_scale"] + d["write_kib"] * d["write_scale"]) * 1024.0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,9 +409,9 @@ def main():
         "vortex_step": step,
         "batched_multipatch": batch,
         "cpu_baseline_step": cpu_step,
-        "roofline": {"bound": "hbm", "kernel": "k_z3_r<float,512,TC> (fused z pass)",
+        "roofline": {"bound": "hbm", "kernel": Z_KERNEL + " (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": z_gbs / PEAK_HBM_GBS, "traffic": None,
+                     "frac": z_gbs / PEAK_HBM_GBS, "traffic": measured_traffic(),
                      "algorithmic_bytes_per_launch": z_bytes32, "avg_launch_ms": z32_ms},
         "cpu_baseline": cpu,
     }
