@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
 }
 
 // ------------------------------------------- register-resident fused z pass
-// Buffer-resource access (vector memory, one 32-bit VGPR offset per thread
+// Buffer-resource loads (vector memory, one 32-bit VGPR offset per thread
 // plus a scalar offset per row): the resource's size bounds every access.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
@@ -374,14 +374,6 @@ __device__ __forceinline__ float2 buf_load(__amdgpu_buffer_rsrc_t r, int voff, i
 }
 __device__ __forceinline__ double2 buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff, double2) {
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
-}
-__device__ __forceinline__ void buf_store(float2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  using U = unsigned int __attribute__((ext_vector_type(2)));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, voff, soff, 0);
-}
-__device__ __forceinline__ void buf_store(double2 v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  using U = unsigned int __attribute__((ext_vector_type(4)));
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U, v), r, voff, soff, 0);
 }
 
 // Velocity only.  Thread (col, p) of a block owns kx column kx0 + col (of TC
@@ -432,8 +424,19 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   const R kx_ = (2 * kx == a.Lx) ? (R)0 : ki, ky_ = (2 * ky == a.Ly) ? (R)0 : kj;
   const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
   const R twopi = (R)6.28318530717958647692;
-  const R *G = reinterpret_cast<const R *>(a.Ghat) + (int64_t)kyf * a.Hx + kx;
+  // G-hat of the thread's rows, all loads issued before any is used (a
+  // column past Hx reads column 0 and is masked)
+  const R *G = reinterpret_cast<const R *>(a.Ghat) + (int64_t)kyf * a.Hx + (col_ok ? kx : 0);
   const int64_t gz = (int64_t)a.Hy * a.Hx;
+  R gv[E];
+  if constexpr (!ALLPER) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = p + P * m;
+      const int kzf = n <= L - n ? n : L - n;
+      gv[m] = G[kzf * gz];
+    }
+  }
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int n = p + P * m;
@@ -444,8 +447,7 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
       const R pi = (R)3.14159265358979323846;
       g = kmag > (R)0 ? (R)1 / ((R)4 * (pi * pi) * kmag) : (R)0;
     } else {
-      const int kzf = n <= L - n ? n : L - n;
-      g = col_ok ? G[kzf * gz] * (R)a.h3 : (R)0;
+      g = col_ok ? gv[m] * (R)a.h3 : (R)0;
     }
     const R kz_ = (2 * n == L) ? (R)0 : kk;
     const V w0 = w[0][m], w1 = w[1][m], w2 = w[2][m];
@@ -459,14 +461,17 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) fft_regs<L, 1>(w[c], line, p, twd);
+  // Stores are plain global stores: buffer-resource stores of these
+  // 8-byte-aligned records, which the compiler merges into 16-byte stores
+  // across components, have returned corrupted first components.
   const R scale = (R)1 / (R)L;
   if (col_ok) {
+    V *dst = base + (int64_t)p * a.rs + col * 3;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       if (p + P * m < a.Mz) {
-        const int soff = (int)((int64_t)P * m * rsb);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) buf_store(cscale(w[c][m], scale), rsrc, voff + c * (int)sizeof(V), soff);
+        for (int c = 0; c < 3; ++c) dst[(int64_t)P * m * a.rs + c] = cscale(w[c][m], scale);
       }
     }
   }

@@ -77,7 +77,7 @@ template <typename R, int L, int TC> int zr_T(ZArgs a, int Ly, const cplx_t<R> *
   }
 }
 template <typename R, int L> int zr_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
-  const int want = a.tune_tc > 0 ? a.tune_tc : (sizeof(R) == 4 ? 4 : 8);
+  const int want = a.tune_tc > 0 ? a.tune_tc : 4;
   switch (want) {
     case 2: return zr_T<R, L, 2>(a, Ly, tw, st);
     case 4: return zr_T<R, L, 4>(a, Ly, tw, st);

@@ -133,3 +133,38 @@ def test_bad_inputs_report_errors():
     patch = pm.Patch(ncell=(8, 8, 8), dx=(0.1,) * 3, xmin=(0, 0, 0), xmax=(0.8,) * 3)
     mesh = pm.Mesh.allocate(patch)
     assert pm.pmlib_poisson_solve(patch, mesh) == -1     # setup not called
+
+
+@pytest.mark.parametrize("n,precision", [(32, 0), (64, 1), (128, 0), (256, 1)])
+def test_z_pass_tile_widths_agree(n, precision, monkeypatch):
+    """Every column-tile width of the register-resident z pass (2, 4, 8
+    columns per block) gives the LDS-resident z pass's velocity, on each of
+    three repeated solves (white-noise vorticity, device pointers)."""
+    import torch
+    import pmlib3d_amd as pm
+    h = 2.0 / n
+    patch = pm.Patch(xmin=(-1.0,) * 3, xmax=(1.0,) * 3, ncell=(n,) * 3, dx=(h,) * 3)
+    dt = torch.float64 if precision == 0 else torch.float32
+    w = torch.from_numpy(np.random.default_rng(n).normal(size=3 * n ** 3)).to("cuda:0", dt)
+
+    def solve(zkernel, tc, reps):
+        monkeypatch.setenv("PMGPU_Z_KERNEL", str(zkernel))
+        monkeypatch.setenv("PMGPU_Z_TC", str(tc))
+        s = pm.PoissonSolver(patch, precision, order=4, kernel=1, eps_radius=2.0, interp_order=3)
+        s.setup()
+        outs = []
+        for _ in range(reps):
+            out = torch.full_like(w, float("nan"))
+            s.solve_device(w.data_ptr(), out.data_ptr())
+            torch.cuda.synchronize()
+            outs.append(out)
+        return outs
+
+    ref = solve(1, 1, 1)[0]
+    scale = float(ref.abs().max())
+    tol = 1e-13 if precision == 0 else 2e-6
+    for tc in (2, 4, 8):
+        for r, out in enumerate(solve(0, tc, 3)):
+            e = float((out - ref).abs().max()) / scale
+            print("n %d precision %d tc %d rep %d rel max diff %.3e" % (n, precision, tc, r, e), flush=True)
+            assert e <= tol, (tc, r, e)
