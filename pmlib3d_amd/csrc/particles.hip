@@ -6,72 +6,9 @@
 #include "../../include/pmgpu.h"
 #include "particles_kernels.h"
 
-#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 namespace pmgpu {
-
-// ---------------------------------------------------------------- scan
-constexpr int kScanThreads = 256, kScanItems = 16, kScanTile = kScanThreads * kScanItems;
-
-__device__ __forceinline__ int block_exclusive(int v, int *sh, int &total) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int o = 1; o < kScanThreads; o <<= 1) {
-    int x = t >= o ? sh[t - o] : 0;
-    __syncthreads();
-    sh[t] += x;
-    __syncthreads();
-  }
-  total = sh[kScanThreads - 1];
-  int r = sh[t] - v;
-  __syncthreads();
-  return r;
-}
-
-__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int *in, int64_t n, int *tile_sums) {
-  __shared__ int sh[kScanThreads];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  int s = 0;
-  for (int i = 0; i < kScanItems; ++i)
-    if (base + i < n) s += in[base + i];
-  int tot;
-  block_exclusive(s, sh, tot);
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
-}
-
-// exclusive scan of tile sums in one block; writes the grand total at [nt]
-__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int *sums, int nt) {
-  __shared__ int sh[kScanThreads];
-  int carry = 0;
-  for (int b0 = 0; b0 < nt; b0 += kScanThreads) {
-    int i = b0 + threadIdx.x;
-    int v = i < nt ? sums[i] : 0;
-    int tot;
-    int ex = block_exclusive(v, sh, tot);
-    if (i < nt) sums[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) sums[nt] = carry;
-}
-
-__global__ void __launch_bounds__(kScanThreads) k_scan_final(const int *in, int64_t n, const int *tile_offs,
-                                                             int *out) {
-  __shared__ int sh[kScanThreads];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  int v[kScanItems];
-  int s = 0;
-  for (int i = 0; i < kScanItems; ++i) {
-    v[i] = base + i < n ? in[base + i] : 0;
-    s += v[i];
-  }
-  int tot;
-  int ex = block_exclusive(s, sh, tot) + tile_offs[blockIdx.x];
-  for (int i = 0; i < kScanItems; ++i) {
-    if (base + i < n) out[base + i] = ex;
-    ex += v[i];
-  }
-}
 
 struct Scratch {
   void *p = nullptr;
@@ -94,16 +31,18 @@ struct Scratch {
   template <typename T> T *as() { return reinterpret_cast<T *>(p); }
 };
 
-// exclusive scan of n ints (in -> out, may alias); out[n] receives the total
+// exclusive scan of n ints (in -> out, not aliased); out[n] receives the
+// total: out[0] = 0 and an inclusive scan of in into out + 1
+__global__ void k_zero_first(int *out) { out[0] = 0; }
 static int scan_exclusive(const int *in, int *out, int64_t n, Scratch &tmp, hipStream_t st) {
-  int nt = (int)((n + kScanTile - 1) / kScanTile);
-  if (nt < 1) nt = 1;
-  if (tmp.ensure((size_t)(nt + 1) * sizeof(int))) return -1;
-  int *sums = tmp.as<int>();
-  hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, in, n, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, st, sums, nt);
-  hipLaunchKernelGGL(k_scan_final, dim3(nt), dim3(kScanThreads), 0, st, in, n, sums, out);
-  PM_HIP(hipMemcpyAsync(out + n, sums + nt, sizeof(int), hipMemcpyDeviceToDevice, st));
+  PM_CHECK(in != out, "pmgpu: internal scan aliasing");
+  hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, st, out);
+  if (n > 0) {
+    size_t bytes = 0;
+    PM_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out + 1, (size_t)n, rocprim::plus<int>(), st));
+    if (tmp.ensure(bytes + 16)) return -1;
+    PM_HIP(rocprim::inclusive_scan(tmp.p, bytes, in, out + 1, (size_t)n, rocprim::plus<int>(), st));
+  }
   PM_HIP(hipGetLastError());
   return 0;
 }
@@ -162,6 +101,19 @@ void particles_release() {
 // ---------------------------------------------------------------- P2M
 // prec: PMGPU_FP64 (double mesh, reference arithmetic) or PMGPU_FP32 (float
 // mesh, FP32 weights with Kahan-compensated sums); particles are FP64.
+// Order each bin's particles and stage their records (one thread per bin),
+// then gather them line by line of 64 nodes along x (one wavefront per line).
+template <typename T, int NS>
+static void p2m_stage_gather(const PGeom &g, const double *xpart, const double *part, int64_t np, int *ids,
+                             PartWS &w, int nbins, T *mesh, int accumulate, hipStream_t st) {
+  if (np > 0)
+    hipLaunchKernelGGL((k_p2m_stage<T, NS>), grid_for(nbins), dim3(256), 0, st, g, xpart, part, ids,
+                       w.start.as<int>(), nbins, w.rec.as<T>());
+  const dim3 grid((g.E[0] + 63) / 64, g.E[1], g.E[2]);
+  hipLaunchKernelGGL((k_p2m_rows<T, NS>), grid, dim3(64), 0, st, g, w.start.as<int>(), w.rec.as<T>(), mesh,
+                     accumulate);
+}
+
 int p2m_device(const pmgpu_geom *geom, int prec, const double *xpart, const double *part, int64_t np, void *mesh,
                int clear, hipStream_t st) {
   PGeom g;
@@ -172,44 +124,36 @@ int p2m_device(const pmgpu_geom *geom, int prec, const double *xpart, const doub
     PM_CHECK(g.bc[d] == 0 || g.ncell[d] >= 2 * g.ngl[d], "pmgpu: periodic axis shorter than two ghost layers");
   PartWS &w = ws();
   const int nbins = g.nb[0] * g.nb[1] * g.nb[2];
-  const int64_t nn = mesh_cells(g);
   PM_CHECK(np >= 0 && np < (int64_t)1 << 31, "pmgpu: particle count out of range");
   PM_CHECK((int64_t)nbins < ((int64_t)1 << 30), "pmgpu: mesh too large for deposition");
-  int end_bit = 1;
-  while ((1 << end_bit) <= nbins) ++end_bit;
-  const int n = (int)np;
-  size_t sort_bytes = 0;
-  PM_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (int *)nullptr, (int *)nullptr, (int *)nullptr,
-                                   (int *)nullptr, n, 0, end_bit, st));
-  if (w.key.ensure(sizeof(int) * 2 * ((size_t)np + 1)) || w.ids.ensure(sizeof(int) * 2 * ((size_t)np + 1)) ||
-      w.count.ensure(sizeof(int) * (nbins + 1)) || w.start.ensure(sizeof(int) * (nbins + 1)) ||
-      w.sort.ensure(sort_bytes + 16))
+  // counting sort by bin: count and rank (keys), scan, scatter; the stage
+  // kernel restores id order inside each bin
+  if (w.key.ensure(sizeof(int) * ((size_t)np + 1)) || w.ids.ensure(sizeof(int) * 2 * ((size_t)np + 1)) ||
+      w.count.ensure(sizeof(int) * (nbins + 1)) || w.start.ensure(sizeof(int) * (nbins + 1)))
     return -1;
-  int *key_in = w.key.as<int>(), *key_out = key_in + np + 1;
-  int *id_in = w.ids.as<int>(), *id_out = id_in + np + 1;
+  int *key = w.key.as<int>();
+  int *rank = w.ids.as<int>(), *id_out = rank + np + 1;
   PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * (nbins + 1), st));
   if (np > 0) {
-    hipLaunchKernelGGL(k_p2m_keys, grid_for(np), dim3(256), 0, st, g, xpart, np, nbins, key_in, id_in,
+    hipLaunchKernelGGL(k_p2m_keys, grid_for(np), dim3(256), 0, st, g, xpart, np, nbins, key, rank,
                        w.count.as<int>());
     PM_HIP(hipGetLastError());
-    PM_HIP(rocprim::radix_sort_pairs(w.sort.p, sort_bytes, key_in, key_out, id_in, id_out, n, 0, end_bit, st));
   }
   if (scan_exclusive(w.count.as<int>(), w.start.as<int>(), nbins, w.scan, st)) return -1;
+  if (np > 0) {
+    hipLaunchKernelGGL(k_p2m_scatter, grid_for(np), dim3(256), 0, st, np, nbins, key, rank, w.start.as<int>(),
+                       id_out);
+    PM_HIP(hipGetLastError());
+  }
   const int accumulate = clear ? 0 : 1;
-  const size_t rec_bytes = (prec == PMGPU_FP64 ? sizeof(double) : sizeof(float)) * PRec<double>::kSlots;
+  const size_t rec_bytes = (prec == PMGPU_FP64 ? sizeof(double) : sizeof(float)) * PRec<double, 6>::kSlots;
   if (w.rec.ensure(rec_bytes * ((size_t)np + 1))) return -1;
   if (prec == PMGPU_FP64) {
-    if (np > 0)
-      hipLaunchKernelGGL(k_p2m_stage<double>, grid_for(np), dim3(256), 0, st, g, xpart, part, id_out,
-                         w.start.as<int>(), nbins, w.rec.as<double>());
-    hipLaunchKernelGGL(k_p2m_gather_staged<double>, grid_for(nn), dim3(256), 0, st, g, w.start.as<int>(),
-                       w.rec.as<double>(), (double *)mesh, accumulate);
+    if (g.ns == 4) p2m_stage_gather<double, 4>(g, xpart, part, np, id_out, w, nbins, (double *)mesh, accumulate, st);
+    else p2m_stage_gather<double, 6>(g, xpart, part, np, id_out, w, nbins, (double *)mesh, accumulate, st);
   } else {
-    if (np > 0)
-      hipLaunchKernelGGL(k_p2m_stage<float>, grid_for(np), dim3(256), 0, st, g, xpart, part, id_out,
-                         w.start.as<int>(), nbins, w.rec.as<float>());
-    hipLaunchKernelGGL(k_p2m_gather_staged<float>, grid_for(nn), dim3(256), 0, st, g, w.start.as<int>(),
-                       w.rec.as<float>(), (float *)mesh, accumulate);
+    if (g.ns == 4) p2m_stage_gather<float, 4>(g, xpart, part, np, id_out, w, nbins, (float *)mesh, accumulate, st);
+    else p2m_stage_gather<float, 6>(g, xpart, part, np, id_out, w, nbins, (float *)mesh, accumulate, st);
   }
   PM_HIP(hipGetLastError());
   for (int d = 0; d < 3; ++d)

@@ -119,8 +119,9 @@ struct Kahan {
 
 // bin key of a particle (nbins: no stencil node touches the mesh, or the
 // particle lies outside a periodic axis, where the reference zeroes every
-// weight); ids[q] = q feeds the stable key sort
-__global__ void k_p2m_keys(PGeom g, const double *pos, int64_t np, int nbins, int *key, int *ids, int *count) {
+// weight) and its arrival rank in the bin (an atomic count: the order within a
+// bin is restored by particle id when the bin is staged)
+__global__ void k_p2m_keys(PGeom g, const double *pos, int64_t np, int nbins, int *key, int *rank, int *count) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     int b[3];
     bool ok = true;
@@ -131,112 +132,253 @@ __global__ void k_p2m_keys(PGeom g, const double *pos, int64_t np, int nbins, in
       if (bb - g.off + g.ns - 1 < g.lo[d] || bb - g.off > g.hi[d]) ok = false;
       if (g.bc[d] != 0 && (x < g.xmin[d] || x > g.xmax[d])) ok = false;
     }
-    int k = nbins;
+    int k = nbins, rk = 0;
     if (ok) {
       k = (b[0] - g.blo[0]) + g.nb[0] * ((b[1] - g.blo[1]) + g.nb[1] * (b[2] - g.blo[2]));
-      atomicAdd(&count[k], 1);
+      rk = atomicAdd(&count[k], 1);
     }
     key[q] = k;
-    ids[q] = (int)q;
+    rank[q] = rk;
   }
 }
 
-// Staged particle record, in bin order: the 3 x ns kernel weights, the 3
-// strength components and the base cell along x (a T-typed copy of the int).
-template <typename T> struct PRec {
-  static constexpr int kSlots = 24;  // 3 * 6 weights + 3 values + base x, padded
+// counting-sort scatter: particle q to slot start[key] + rank of its bin
+__global__ void k_p2m_scatter(int64_t np, int nbins, const int *key, const int *rank, const int *start, int *ids) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    const int k = key[q];
+    if (k < nbins) ids[start[k] + rank[q]] = (int)q;
+  }
+}
+
+// ascending order of ids[0, n) in place: insertion sort for the usual bins of
+// a few particles, heap sort (n log n, no extra memory) for crowded ones
+__device__ inline void sort_ids(int *ids, int n) {
+  if (n <= 16) {
+    for (int i = 1; i < n; ++i) {
+      const int v = ids[i];
+      int j = i - 1;
+      while (j >= 0 && ids[j] > v) {
+        ids[j + 1] = ids[j];
+        --j;
+      }
+      ids[j + 1] = v;
+    }
+    return;
+  }
+  auto sift = [&](int root, int end) {
+    while (2 * root + 1 < end) {
+      int c = 2 * root + 1;
+      if (c + 1 < end && ids[c + 1] > ids[c]) ++c;
+      if (ids[root] >= ids[c]) return;
+      const int t = ids[root];
+      ids[root] = ids[c];
+      ids[c] = t;
+      root = c;
+    }
+  };
+  for (int i = n / 2 - 1; i >= 0; --i) sift(i, n);
+  for (int end = n - 1; end > 0; --end) {
+    const int t = ids[0];
+    ids[0] = ids[end];
+    ids[end] = t;
+    sift(0, end);
+  }
+}
+
+// Staged particle record, in bin order: the 3 x ns kernel weights in slots
+// [d * ns + s], the 3 strength components in slots 3 ns .. 3 ns + 2 and the
+// base cell along x (a T-typed copy of the int) in slot 3 ns + 3.  For M'4
+// (ns = 4) a record is 16 slots: one 128-byte line in FP64; for M6* 24.
+template <typename T, int NS> struct PRec {
+  static constexpr int kSlots = NS == 4 ? 16 : 24;
+  static constexpr int kVal = 3 * NS, kBx = 3 * NS + 3;
 };
 
-// Records of the sorted particles t < start[nbins]: weights from the FP64
-// offsets (FP64: the reference's arithmetic; FP32: weights in single
-// precision from the FP64 offset).
-template <typename T>
-__global__ void k_p2m_stage(PGeom g, const double *pos, const double *val, const int *ids, const int *start,
-                            int nbins, T *rec) {
-  const int nvalid = start[nbins];
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nvalid; t += gridDim.x * blockDim.x) {
-    const int q = ids[t];
-    T *r = rec + (int64_t)t * PRec<T>::kSlots;
-    int bx = 0;
-    for (int d = 0; d < 3; ++d) {
-      const double x = pos[3 * q + d];
-      const int b = cell_index(x, g.xmin[d], g.c1dx[d]);
-      if (d == 0) bx = b;
-      const double dd = stencil_offset(x, g.xmin[d], g.dx[d], b, g.c1dx[d]);
-      for (int s2 = 0; s2 < g.ns; ++s2) {
-        if constexpr (sizeof(T) == sizeof(double))
-          r[d * g.ns + s2] = weight(g.m6, s2, dd);
-        else
-          r[d * g.ns + s2] = weight_f(g.m6, s2, (float)dd);
+// One thread per bin: order the bin's particles by id (the order a stable
+// sort by bin would give), then write their records at the bin's slots:
+// weights from the FP64 offsets (FP64: the reference's arithmetic; FP32:
+// weights in single precision from the FP64 offset).
+template <typename T, int NS>
+__global__ void __launch_bounds__(256) k_p2m_stage(PGeom g, const double *pos, const double *val, int *ids,
+                                                   const int *start, int nbins, T *rec) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nbins; k += gridDim.x * blockDim.x) {
+    const int t0 = start[k], t1 = start[k + 1];
+    if (t1 == t0) continue;
+    if (t1 - t0 > 1) sort_ids(ids + t0, t1 - t0);
+    for (int t = t0; t < t1; ++t) {
+      const int q = ids[t];
+      T wt[3 * NS];
+      int bx = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double x = pos[3 * q + d];
+        const int b = cell_index(x, g.xmin[d], g.c1dx[d]);
+        if (d == 0) bx = b;
+        const double dd = stencil_offset(x, g.xmin[d], g.dx[d], b, g.c1dx[d]);
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) {
+          if constexpr (sizeof(T) == sizeof(double))
+            wt[d * NS + s2] = weight(NS == 6, s2, dd);
+          else
+            wt[d * NS + s2] = weight_f(NS == 6, s2, (float)dd);
+        }
       }
+      T *r = rec + (int64_t)t * PRec<T, NS>::kSlots;
+#pragma unroll
+      for (int i = 0; i < 3 * NS; ++i) r[i] = wt[i];
+      r[PRec<T, NS>::kVal] = (T)val[3 * q];
+      r[PRec<T, NS>::kVal + 1] = (T)val[3 * q + 1];
+      r[PRec<T, NS>::kVal + 2] = (T)val[3 * q + 2];
+      r[PRec<T, NS>::kBx] = (T)bx;
     }
-    r[18] = (T)val[3 * q];
-    r[19] = (T)val[3 * q + 1];
-    r[20] = (T)val[3 * q + 2];
-    r[21] = (T)bx;
   }
 }
 
-// One thread per mesh node: for each (sz, sy) stencil row the bins whose
-// x stencils cover the node are consecutive, so their particles are one
-// contiguous range of records.  Summation order: sz and sy descending, then
-// bins ascending (sx descending), particles by id within a bin.  FP64 sums
-// with the reference's rounding per term; FP32 sums are Kahan-compensated.
-template <typename T>
-__global__ void k_p2m_gather_staged(PGeom g, const int *start, const T *rec, T *mesh, int accumulate) {
-  const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int n[3];
-    n[0] = (int)(idx % g.E[0]) + g.lo[0];
-    n[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
-    n[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
-    const int ns = g.ns;
-    int kx_lo = n[0] + g.off - ns + 1 - g.blo[0], kx_hi = n[0] + g.off - g.blo[0];
-    if (kx_lo < 0) kx_lo = 0;
-    if (kx_hi > g.nb[0] - 1) kx_hi = g.nb[0] - 1;
-    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-    Kahan f0, f1, f2;
-    if (kx_lo <= kx_hi) {
-      for (int sz = ns - 1; sz >= 0; --sz) {
-        const int kz = n[2] - sz + g.off - g.blo[2];
-        if (kz < 0 || kz >= g.nb[2]) continue;
-        for (int sy = ns - 1; sy >= 0; --sy) {
-          const int ky = n[1] - sy + g.off - g.blo[1];
-          if (ky < 0 || ky >= g.nb[1]) continue;
-          const int krow = g.nb[0] * (ky + g.nb[1] * kz);
-          const int t0 = start[krow + kx_lo], t1 = start[krow + kx_hi + 1];
-          for (int t = t0; t < t1; ++t) {
-            const T *r = rec + (int64_t)t * PRec<T>::kSlots;
-            const int sx = n[0] - (int)r[21] + g.off;
-            if constexpr (sizeof(T) == sizeof(double)) {
-              const double w = __dmul_rn(__dmul_rn(r[sx], r[ns + sy]), r[2 * ns + sz]);
-              d0 = __dadd_rn(d0, __dmul_rn(w, r[18]));
-              d1 = __dadd_rn(d1, __dmul_rn(w, r[19]));
-              d2 = __dadd_rn(d2, __dmul_rn(w, r[20]));
-            } else {
-              const float w = r[sx] * r[ns + sy] * r[2 * ns + sz];
-              f0.add(w * r[18]);
-              f1.add(w * r[19]);
-              f2.add(w * r[20]);
-            }
-          }
-        }
+// Row-staged gather.  A workgroup is one wavefront and owns a line of 64
+// consecutive mesh nodes along x (node ix = 64 blockIdx.x + lane at node
+// row iy = blockIdx.y, iz = blockIdx.z); each lane sums its own node.  The
+// node line is reached by the ns x ns stencil rows (ky, kz) = (iy + a, iz + b),
+// a, b < ns, and within a row by the bins [64 blockIdx.x, 64 blockIdx.x + 63
+// + ns - 1], one contiguous range of records.  Row by row the wavefront
+// copies those records into LDS with coalesced loads (only the values the
+// row needs: the ns x weights, the row's y and z weights, the strength and
+// the bin), and each lane then sums the records of its own bins [ix, ix + ns)
+// from LDS.  The next row's records are loaded into registers while the
+// current row is summed.  Per node the terms arrive in the order kz
+// ascending (sz descending), ky ascending (sy descending), records
+// ascending (bins ascending, particles by id within a bin); FP64 terms are
+// ((wx wy) wz) v rounded per operation as the reference does, FP32 sums are
+// Kahan-compensated.
+constexpr int kRowCap = 128;   // records staged per row pass (two per lane)
+
+template <typename T, int NS> struct RowStage {
+  static constexpr int NV = NS + 6;   // wx[ns], wy, wz, v0, v1, v2, x slot base
+  T val[kRowCap / 64][NV];
+};
+
+template <typename T, int NS>
+__device__ __forceinline__ void row_fetch(RowStage<T, NS> &st, const T *rec, int t0, int n, int lane, int sy,
+                                          int sz, int x0, int blo0) {
+  constexpr int K = PRec<T, NS>::kSlots, KV = PRec<T, NS>::kVal, KB = PRec<T, NS>::kBx;
+#pragma unroll
+  for (int h = 0; h < kRowCap / 64; ++h) {
+    const int i = lane + 64 * h;
+    // unconditional loads (no branch for the wait counters to merge over):
+    // an empty row reads record t0 <= np, inside the allocation of np + 1
+    const int t = t0 + (i < n ? i : (n > 0 ? n - 1 : 0));
+    const T *r = static_cast<const T *>(__builtin_assume_aligned(rec + (int64_t)t * K, 16));
+#pragma unroll
+    for (int s = 0; s < NS; ++s) st.val[h][s] = r[s];
+    st.val[h][NS] = r[NS + sy];
+    st.val[h][NS + 1] = r[2 * NS + sz];
+    st.val[h][NS + 2] = r[KV];
+    st.val[h][NS + 3] = r[KV + 1];
+    st.val[h][NS + 4] = r[KV + 2];
+    // x slot of lane 0: sx = lane + (x0 + ns - 1 - kx)
+    st.val[h][NS + 5] = (T)(x0 + NS - 1 - ((int)r[KB] - blo0));
+  }
+}
+
+template <typename T, int NS>
+__global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, const T *rec, T *mesh, int accumulate) {
+  constexpr int NV = RowStage<T, NS>::NV, NR = NS * NS;
+  constexpr bool F64 = sizeof(T) == sizeof(double);
+  __shared__ T lds[NV][kRowCap];
+  const int lane = threadIdx.x;
+  const int x0 = blockIdx.x * 64, iy = blockIdx.y, iz = blockIdx.z;
+  const int ix = x0 + lane;
+  const bool node_ok = ix < g.E[0];
+  const int x1 = min(x0 + 64 + NS - 2, g.nb[0] - 1);
+  const int blo0 = g.blo[0];
+  // start[] offset of stencil row r: (ky, kz) = (iy + r % ns, iz + r / ns)
+  auto krow = [&](int r) { return g.nb[0] * ((iy + r % NS) + g.nb[1] * (iz + r / NS)); };
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  Kahan f0, f1, f2;
+  auto to_lds = [&](const RowStage<T, NS> &st, int m) {
+#pragma unroll
+    for (int h = 0; h < kRowCap / 64; ++h) {
+      const int i = lane + 64 * h;
+      if (i < m) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lds[v][i] = st.val[h][v];
       }
     }
-    T *m = mesh + 3 * idx;
-    const T a0 = sizeof(T) == sizeof(double) ? (T)d0 : (T)f0.s;
-    const T a1 = sizeof(T) == sizeof(double) ? (T)d1 : (T)f1.s;
-    const T a2 = sizeof(T) == sizeof(double) ? (T)d2 : (T)f2.s;
-    if (accumulate) {
-      m[0] += a0;
-      m[1] += a1;
-      m[2] += a2;
+  };
+  // sum the lane's records t in [lo, hi) staged from record `base` on: the
+  // first 8 as straight-line predicated code (no loop for the wait counters
+  // to drain pending loads in front of), the rest in a loop
+  auto term = [&](int i) {
+    const int sx = lane + (int)lds[NS + 5][i];   // in [0, ns) for the lane's bins
+    const T wx = lds[sx][i];
+    if constexpr (F64) {
+      const double w = __dmul_rn(__dmul_rn(wx, lds[NS][i]), lds[NS + 1][i]);
+      d0 = __dadd_rn(d0, __dmul_rn(w, lds[NS + 2][i]));
+      d1 = __dadd_rn(d1, __dmul_rn(w, lds[NS + 3][i]));
+      d2 = __dadd_rn(d2, __dmul_rn(w, lds[NS + 4][i]));
     } else {
-      m[0] = a0;
-      m[1] = a1;
-      m[2] = a2;
+      const float w = wx * lds[NS][i] * lds[NS + 1][i];
+      f0.add(w * lds[NS + 2][i]);
+      f1.add(w * lds[NS + 3][i]);
+      f2.add(w * lds[NS + 4][i]);
     }
+  };
+  auto sum = [&](int lo, int hi, int base) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (lo + u < hi) term(lo + u - base);
+    for (int t = lo + 8; t < hi; ++t) term(t - base);
+  };
+  // every row's record range, loaded up front: uniform [rt0, rt1) over the
+  // line's bins, per lane [ra0, ra1) over the lane's bins [ix, ix + ns)
+  int rt0[NR], rt1[NR], ra0[NR], ra1[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int k = krow(r);
+    // the line's range is the same for every lane: keep it in scalar registers
+    rt0[r] = __builtin_amdgcn_readfirstlane(start[k + x0]);
+    rt1[r] = __builtin_amdgcn_readfirstlane(start[k + x1 + 1]);
+    ra0[r] = node_ok ? start[k + ix] : 0;
+    ra1[r] = node_ok ? start[k + ix + NS] : 0;
+  }
+  int total = 0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) total += rt1[r] - rt0[r];
+  if (total > 0) {
+    // rows in order (the loop is unrolled: every index is static); row r + 1's
+    // first pass loads while row r is summed
+    RowStage<T, NS> cur, nxt;
+    row_fetch<T, NS>(cur, rec, rt0[0], min(rt1[0] - rt0[0], kRowCap), lane, NS - 1, NS - 1, x0, blo0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int T0 = rt0[r], n = rt1[r] - rt0[r], a0 = ra0[r], a1 = ra1[r];
+      const int sy = NS - 1 - r % NS, sz = NS - 1 - r / NS;
+      if (n > 0) {
+        to_lds(cur, min(n, kRowCap));
+        __syncthreads();
+      }
+      if (r + 1 < NR)
+        row_fetch<T, NS>(nxt, rec, rt0[min(r + 1, NR - 1)], min(rt1[min(r + 1, NR - 1)] - rt0[min(r + 1, NR - 1)], kRowCap),
+                         lane, NS - 1 - (r + 1) % NS, NS - 1 - (r + 1) / NS, x0, blo0);
+      if (n > 0) {
+        sum(a0, min(a1, T0 + kRowCap), T0);
+        __syncthreads();
+        for (int p = kRowCap; p < n; p += kRowCap) {   // rows of more than kRowCap records
+          const int m = min(n - p, kRowCap);
+          row_fetch<T, NS>(cur, rec, T0 + p, m, lane, sy, sz, x0, blo0);
+          to_lds(cur, m);
+          __syncthreads();
+          sum(max(a0, T0 + p), min(a1, T0 + p + m), T0 + p);
+          __syncthreads();
+        }
+      }
+      cur = nxt;
+    }
+  }
+  if (node_ok) {
+    T *m = mesh + 3 * ((int64_t)ix + (int64_t)g.E[0] * (iy + (int64_t)g.E[1] * iz));
+    const T a[3] = {F64 ? (T)d0 : (T)f0.s, F64 ? (T)d1 : (T)f1.s, F64 ? (T)d2 : (T)f2.s};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[c] = accumulate ? m[c] + a[c] : a[c];
   }
 }
 
