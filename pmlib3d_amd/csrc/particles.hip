@@ -203,10 +203,17 @@ int m2p_device(const pmgpu_geom *geom, int prec, const void *mesh, const double 
   PM_CHECK(prec == PMGPU_FP64 || prec == PMGPU_FP32, "pmgpu: precision must be PMGPU_FP64 or PMGPU_FP32");
   PM_CHECK(mesh && (np == 0 || (pos && out)), "pmgpu_interp_mesh_particle: null argument");
   if (np > 0) {
-    if (prec == PMGPU_FP64)
-      hipLaunchKernelGGL(k_m2p, grid_for(np), dim3(256), 0, st, g, (const double *)mesh, pos, np, out);
-    else
-      hipLaunchKernelGGL(k_m2p_f32, grid_for(np), dim3(256), 0, st, g, (const float *)mesh, pos, np, out);
+    if (prec == PMGPU_FP64) {
+      if (g.ns == 4)
+        hipLaunchKernelGGL((k_m2p<double, 4>), grid_for(np), dim3(256), 0, st, g, (const double *)mesh, pos, np, out);
+      else
+        hipLaunchKernelGGL((k_m2p<double, 6>), grid_for(np), dim3(256), 0, st, g, (const double *)mesh, pos, np, out);
+    } else {
+      if (g.ns == 4)
+        hipLaunchKernelGGL((k_m2p<float, 4>), grid_for(np), dim3(256), 0, st, g, (const float *)mesh, pos, np, out);
+      else
+        hipLaunchKernelGGL((k_m2p<float, 6>), grid_for(np), dim3(256), 0, st, g, (const float *)mesh, pos, np, out);
+    }
   }
   PM_HIP(hipGetLastError());
   return 0;

@@ -407,112 +407,89 @@ template <typename T> __global__ void k_halo_add(T *m, PGeom g, int d) {
 }
 
 // M2P: one thread per particle, the reference's term order
-// (interp_mesh_particle.f:87-409; the M'4 branch scales the y and z offsets
-// by 1/dx(1), the M6* branch reproduces the k5/k6 quirk of :587-590).
-__global__ void k_m2p(PGeom g, const double *mesh, const double *pos, int64_t np, double *out) {
+// (interp_mesh_particle.f:87-409: sz, sy, sx ascending; the M'4 branch scales
+// the y and z offsets by 1/dx(1), the M6* branch reproduces the k5/k6 quirk of
+// :587-590), FP64 rounded per operation as the reference does; the FP32 form
+// (FP32 mesh) takes FP32 weights and Kahan sums.  Stencil weights and indices
+// live in registers (the stencil width is a template parameter).
+template <typename T, int NS>
+__global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const double *pos, int64_t np, double *out) {
+  constexpr bool F64 = sizeof(T) == sizeof(double);
+  constexpr bool M6 = NS == 6;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    int idx[3][6];
-    double w[3][6];
+    int idx[3][NS];
+    T w[3][NS];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-      double x = pos[3 * q + d];
-      int b = cell_index(x, g.xmin[d], g.c1dx[d]);
-      double nrm = g.m6 ? g.c1dx[d] : g.c1dx[0];
-      double dd = stencil_offset(x, g.xmin[d], g.dx[d], b, nrm);
-      for (int s = 0; s < g.ns; ++s) {
-        w[d][s] = weight(g.m6, s, dd);
-        idx[d][s] = b - g.off + s;
+      const double x = pos[3 * q + d];
+      const int b = cell_index(x, g.xmin[d], g.c1dx[d]);
+      // the reference normalises every axis by 1/dx of x for M'4
+      const double nrm = M6 ? g.c1dx[d] : g.c1dx[0];
+      const double dd = stencil_offset(x, g.xmin[d], g.dx[d], b, nrm);
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) {
+        if constexpr (F64)
+          w[d][s2] = weight(M6, s2, dd);
+        else
+          w[d][s2] = weight_f(M6, s2, (float)dd);
+        idx[d][s2] = b - g.off + s2;
       }
     }
+#pragma unroll
     for (int d = 0; d < 3; ++d)
-      for (int s = 0; s < g.ns; ++s) {
-        if (idx[d][s] < g.lo[d]) {
-          if (g.m6 && d == 2 && s >= 4) {
-            w[1][s] = 0.0;
-            idx[1][s] = g.lo[2];
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) {
+        if (idx[d][s2] < g.lo[d]) {
+          if (M6 && d == 2 && s2 >= 4) {   // the reference's M6* quirk on the y slots
+            w[1][s2] = (T)0;
+            idx[1][s2] = g.lo[2];
           }
-          w[d][s] = 0.0;
-          idx[d][s] = g.lo[d];
+          w[d][s2] = (T)0;
+          idx[d][s2] = g.lo[d];
         }
-        if (idx[d][s] > g.hi[d]) {
-          w[d][s] = 0.0;
-          idx[d][s] = g.hi[d];
+        if (idx[d][s2] > g.hi[d]) {
+          w[d][s2] = (T)0;
+          idx[d][s2] = g.hi[d];
         }
       }
     // The quirk can leave a y index outside the y range (when the z axis has
     // more ghost layers); its weight is zero, so the read is clamped into the
     // mesh instead of leaving the allocation.
-    for (int s = 0; s < g.ns; ++s) idx[1][s] = min(max(idx[1][s], g.lo[1]), g.hi[1]);
+#pragma unroll
+    for (int s2 = 0; s2 < NS; ++s2) idx[1][s2] = min(max(idx[1][s2], g.lo[1]), g.hi[1]);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    bool first = true;
-    for (int sz = 0; sz < g.ns; ++sz)
-      for (int sy = 0; sy < g.ns; ++sy)
-        for (int sx = 0; sx < g.ns; ++sx) {
-          double wt = __dmul_rn(__dmul_rn(w[0][sx], w[1][sy]), w[2][sz]);
-          int64_t o = 3 * ((int64_t)(idx[0][sx] - g.lo[0]) +
-                           (int64_t)g.E[0] * ((idx[1][sy] - g.lo[1]) + (int64_t)g.E[1] * (idx[2][sz] - g.lo[2])));
-          double t0 = __dmul_rn(wt, mesh[o]), t1 = __dmul_rn(wt, mesh[o + 1]), t2 = __dmul_rn(wt, mesh[o + 2]);
-          if (first) {
-            a0 = t0;
-            a1 = t1;
-            a2 = t2;
-            first = false;
+    Kahan k0, k1, k2;
+#pragma unroll
+    for (int sz = 0; sz < NS; ++sz)
+#pragma unroll
+      for (int sy = 0; sy < NS; ++sy) {
+        const int64_t row = (int64_t)g.E[0] * ((idx[1][sy] - g.lo[1]) + (int64_t)g.E[1] * (idx[2][sz] - g.lo[2]));
+#pragma unroll
+        for (int sx = 0; sx < NS; ++sx) {
+          const T *m = mesh + 3 * ((int64_t)(idx[0][sx] - g.lo[0]) + row);
+          if constexpr (F64) {
+            const double wt = __dmul_rn(__dmul_rn(w[0][sx], w[1][sy]), w[2][sz]);
+            const double t0 = __dmul_rn(wt, m[0]), t1 = __dmul_rn(wt, m[1]), t2 = __dmul_rn(wt, m[2]);
+            if (sz == 0 && sy == 0 && sx == 0) {   // the first term starts the sums
+              a0 = t0;
+              a1 = t1;
+              a2 = t2;
+            } else {
+              a0 = __dadd_rn(a0, t0);
+              a1 = __dadd_rn(a1, t1);
+              a2 = __dadd_rn(a2, t2);
+            }
           } else {
-            a0 = __dadd_rn(a0, t0);
-            a1 = __dadd_rn(a1, t1);
-            a2 = __dadd_rn(a2, t2);
+            const float wt = w[0][sx] * w[1][sy] * w[2][sz];
+            k0.add(wt * m[0]);
+            k1.add(wt * m[1]);
+            k2.add(wt * m[2]);
           }
         }
-    out[3 * q] = a0;
-    out[3 * q + 1] = a1;
-    out[3 * q + 2] = a2;
-  }
-}
-
-// FP32 form of M2P: same stencil and cancellation, FP32 weights, Kahan sums
-__global__ void k_m2p_f32(PGeom g, const float *mesh, const double *pos, int64_t np, double *out) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    int idx[3][6];
-    float w[3][6];
-    for (int d = 0; d < 3; ++d) {
-      double x = pos[3 * q + d];
-      int b = cell_index(x, g.xmin[d], g.c1dx[d]);
-      double nrm = g.m6 ? g.c1dx[d] : g.c1dx[0];
-      float dd = (float)stencil_offset(x, g.xmin[d], g.dx[d], b, nrm);
-      for (int s = 0; s < g.ns; ++s) {
-        w[d][s] = weight_f(g.m6, s, dd);
-        idx[d][s] = b - g.off + s;
       }
-    }
-    for (int d = 0; d < 3; ++d)
-      for (int s = 0; s < g.ns; ++s) {
-        if (idx[d][s] < g.lo[d]) {
-          if (g.m6 && d == 2 && s >= 4) {
-            w[1][s] = 0.0f;
-            idx[1][s] = g.lo[2];
-          }
-          w[d][s] = 0.0f;
-          idx[d][s] = g.lo[d];
-        }
-        if (idx[d][s] > g.hi[d]) {
-          w[d][s] = 0.0f;
-          idx[d][s] = g.hi[d];
-        }
-      }
-    for (int s = 0; s < g.ns; ++s) idx[1][s] = min(max(idx[1][s], g.lo[1]), g.hi[1]);
-    Kahan a0, a1, a2;
-    for (int sz = 0; sz < g.ns; ++sz)
-      for (int sy = 0; sy < g.ns; ++sy)
-        for (int sx = 0; sx < g.ns; ++sx) {
-          float wt = w[0][sx] * w[1][sy] * w[2][sz];
-          int64_t o = 3 * ((int64_t)(idx[0][sx] - g.lo[0]) +
-                           (int64_t)g.E[0] * ((idx[1][sy] - g.lo[1]) + (int64_t)g.E[1] * (idx[2][sz] - g.lo[2])));
-          a0.add(wt * mesh[o]);
-          a1.add(wt * mesh[o + 1]);
-          a2.add(wt * mesh[o + 2]);
-        }
-    out[3 * q] = a0.s;
-    out[3 * q + 1] = a1.s;
-    out[3 * q + 2] = a2.s;
+    out[3 * q] = F64 ? a0 : (double)k0.s;
+    out[3 * q + 1] = F64 ? a1 : (double)k1.s;
+    out[3 * q + 2] = F64 ? a2 : (double)k2.s;
   }
 }
 
