@@ -237,7 +237,7 @@ def cpu_step_baseline(n=128):
                       "remesh), FP64, native Glassman FFT, 1 rank"}
 
 
-Z_KERNEL = "k_z3_r<float, 512, 4, false>"
+Z_KERNEL = "k_z3_r<float, 512, 4, false, true>"
 
 
 def measured_traffic():

@@ -54,6 +54,11 @@ template <int L> struct Geo {
   static constexpr bool WAVE = P <= 64;
 };
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3) + (i >> 6); }
+// Padding of the register-resident transforms (PM = 1): i + i / 8 keeps the
+// first stage's stride-8 writes, the unit-stride reads and the middle stage's
+// grouped writes on distinct banks (with a line stride of 8 mod 32 complex
+// FP32 words, 4 mod 16 FP64 words between the columns of a block).
+template <int PM> __device__ __forceinline__ int padm(int i) { return PM == 1 ? i + (i >> 3) : pad(i); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Map the linear block id to a logical tile id so that consecutive
@@ -114,7 +119,7 @@ template <int L, typename V> __device__ __forceinline__ void twid_load(Twid<L, V
   TwidLoad<L, 0, V>::run(t, p, tw);
 }
 
-template <int L, int S, int R, int NS, bool WAVE, typename V>
+template <int L, int S, int R, int NS, bool WAVE, typename V, int PM = 0>
 __device__ __forceinline__ void stage(V *line, int p, const V *twrow) {
   constexpr int P = Geo<L>::P;
   constexpr int NBF = L / R;
@@ -125,7 +130,7 @@ __device__ __forceinline__ void stage(V *line, int p, const V *twrow) {
     const int j = p + m * P;
     if (MB * P == NBF || j < NBF) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[m][r] = line[pad(j + r * NBF)];
+      for (int r = 0; r < R; ++r) v[m][r] = line[padm<PM>(j + r * NBF)];
     }
   }
   line_sync<WAVE>();
@@ -147,7 +152,7 @@ __device__ __forceinline__ void stage(V *line, int p, const V *twrow) {
       Dft<R, S, V>::run(v[m]);
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) line[pad(base + r * NS)] = v[m][r];
+      for (int r = 0; r < R; ++r) line[padm<PM>(base + r * NS)] = v[m][r];
     }
   }
   line_sync<WAVE>();
@@ -184,35 +189,38 @@ template <int L> constexpr bool regs_ok() {
   return n0 % P == 0 && n1 % P == 0;
 }
 
-template <int L, int S, int I, typename V, bool DONE = (I >= nstages(L) - 1)> struct MidStages;
-template <int L, int S, int I, typename V> struct MidStages<L, S, I, V, true> {
+template <int L, int S, int I, typename V, int PM = 0, bool DONE = (I >= nstages(L) - 1)> struct MidStages;
+template <int L, int S, int I, typename V, int PM> struct MidStages<L, S, I, V, PM, true> {
   static __device__ __forceinline__ void run(V *, int, const Twid<L, V> &) {}
 };
-template <int L, int S, int I, typename V> struct MidStages<L, S, I, V, false> {
+template <int L, int S, int I, typename V, int PM> struct MidStages<L, S, I, V, PM, false> {
   static __device__ __forceinline__ void run(V *line, int p, const Twid<L, V> &t) {
-    stage<L, S, radix_at(L, I), ns_before(L, I), false>(line, p, t.w[I]);
-    MidStages<L, S, I + 1, V>::run(line, p, t);
+    stage<L, S, radix_at(L, I), ns_before(L, I), false, V, PM>(line, p, t.w[I]);
+    MidStages<L, S, I + 1, V, PM>::run(line, p, t);
   }
 };
 
-template <int L, int S, typename V>
+// ZIN: the upper half of the line's elements (indices >= L / 2) are zero on
+// entry, so the first stage skips them; HALFOUT: only the lower half is
+// wanted on exit, so the last stage leaves the upper half unset.
+template <int L, int S, typename V, bool ZIN = false, bool HALFOUT = false, int PM = 0>
 __device__ __forceinline__ void fft_regs(V (&x)[L / Geo<L>::P], V *line, int p, const Twid<L, V> &t) {
-  constexpr int P = Geo<L>::P;
+  constexpr int P = Geo<L>::P, E = L / P;
   {  // first stage: registers -> line (no twiddles, NS = 1)
     constexpr int R = radix_at(L, 0), NBF = L / R, MB = NBF / P;
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
       V v[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = x[b + MB * r];
+      for (int r = 0; r < R; ++r) v[r] = (ZIN && b + MB * r >= E / 2) ? cmake<V>(0, 0) : x[b + MB * r];
       Dft<R, S, V>::run(v);
       const int jj = p + b * P;
 #pragma unroll
-      for (int r = 0; r < R; ++r) line[pad(jj * R + r)] = v[r];
+      for (int r = 0; r < R; ++r) line[padm<PM>(jj * R + r)] = v[r];
     }
   }
   __syncthreads();
-  MidStages<L, S, 1, V>::run(line, p, t);
+  MidStages<L, S, 1, V, PM>::run(line, p, t);
   {  // last stage: line -> registers (NS = NBF, outputs at p + P m)
     constexpr int I = nstages(L) - 1, R = radix_at(L, I), NBF = L / R, MB = NBF / P;
 #pragma unroll
@@ -220,7 +228,7 @@ __device__ __forceinline__ void fft_regs(V (&x)[L / Geo<L>::P], V *line, int p, 
       const int jj = p + b * P;
       V v[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = line[pad(jj + r * NBF)];
+      for (int r = 0; r < R; ++r) v[r] = line[padm<PM>(jj + r * NBF)];
       V w1 = t.w[I][b];
       if (S > 0) w1.y = -w1.y;
       V w = w1;
@@ -231,7 +239,8 @@ __device__ __forceinline__ void fft_regs(V (&x)[L / Geo<L>::P], V *line, int p, 
       }
       Dft<R, S, V>::run(v);
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[b + MB * r] = v[r];
+      for (int r = 0; r < R; ++r)
+        if (!(HALFOUT && b + MB * r >= E / 2)) x[b + MB * r] = v[r];
     }
   }
   __syncthreads();  // the line buffer is free again

@@ -364,6 +364,16 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
 }
 
 // ------------------------------------------- register-resident fused z pass
+// Line stride (complex words) of the register-resident kernels' LDS lines:
+// room for the padded line (i + i / 8), and 8 mod 32 words (8-byte FP32
+// complex) or 4 mod 16 words (16-byte FP64 complex) so that the columns of a
+// block fall on different banks.
+template <int L, int VB> constexpr int reg_line_stride() {
+  int st = L + L / 8;
+  while (VB == 8 ? (st % 32 != 8) : (st % 16 != 4)) ++st;
+  return st;
+}
+
 // Buffer-resource loads (vector memory, one 32-bit VGPR offset per thread
 // plus a scalar offset per row): the resource's size bounds every access.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
@@ -384,10 +394,15 @@ __device__ __forceinline__ double2 buf_load(__amdgpu_buffer_rsrc_t r, int voff, 
 // wavenumbers; LDS only carries the exchanges between FFT stages, one
 // component at a time.  The host guarantees Mz * rs complex words fit a
 // 31-bit byte offset.
-template <typename R, int L, int TC, bool ALLPER>
+// HALF (free-space z, Mz = L / 2): rows >= L / 2 are zero on the way in and
+// not wanted on the way out, so the first forward and the last inverse stage
+// skip them.
+template <typename R, int L, int TC, bool ALLPER, bool HALF>
 __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R> *tw_all) {
   using V = cplx_t<R>;
-  constexpr int P = Geo<L>::P, ST = Geo<L>::ST, E = L / P;
+  constexpr int P = Geo<L>::P, E = L / P;
+  // line stride between the block's columns for the i + i / 8 padding
+  constexpr int ST = reg_line_stride<L, sizeof(V)>();
   __shared__ V lds[TC * ST];
   const V *tw = tw_all + a.plan.tw_off;
   const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
@@ -408,7 +423,7 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   V w[3][E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const bool ok = col_ok && p + P * m < a.Mz;
+    const bool ok = col_ok && (HALF ? m < E / 2 : p + P * m < a.Mz);
     const int soff = (int)((int64_t)P * m * rsb);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -416,7 +431,7 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   }
   V *line = lds + col * ST;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) fft_regs<L, -1>(w[c], line, p, twd);
+  for (int c = 0; c < 3; ++c) fft_regs<L, -1, V, HALF, false, 1>(w[c], line, p, twd);
   // u-hat = G 2 pi i (k x w-hat); Nyquist wavenumbers drop out (Hermitian
   // projection of the real inverse)
   const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
@@ -460,7 +475,7 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
     w[2][m] = cmake<V>(-c2.y * f, c2.x * f);
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) fft_regs<L, 1>(w[c], line, p, twd);
+  for (int c = 0; c < 3; ++c) fft_regs<L, 1, V, false, HALF, 1>(w[c], line, p, twd);
   // Stores are plain global stores: buffer-resource stores of these
   // 8-byte-aligned records, which the compiler merges into 16-byte stores
   // across components, have returned corrupted first components.
@@ -469,7 +484,7 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
     V *dst = base + (int64_t)p * a.rs + col * 3;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      if (p + P * m < a.Mz) {
+      if (HALF ? m < E / 2 : p + P * m < a.Mz) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) dst[(int64_t)P * m * a.rs + c] = cscale(w[c][m], scale);
       }

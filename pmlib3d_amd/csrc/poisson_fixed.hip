@@ -63,16 +63,19 @@ template <typename R, int L, int TC> int z_tc(ZArgs a, int Ly, const cplx_t<R> *
 
 // register-resident velocity z pass with TC columns per block
 template <typename R, int L, int TC> int zr_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
-  if constexpr (!regs_ok<L>() || TC * Geo<L>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget || Geo<L>::P * TC > 1024) {
+  if constexpr (!regs_ok<L>() || TC * reg_line_stride<L, sizeof(cplx_t<R>)>() * (int)sizeof(cplx_t<R>) > kLdsBudget ||
+                Geo<L>::P * TC > 1024) {
     return 0;
   } else {
     // buffer offsets of the tile are 31-bit
     if ((double)a.Mz * (double)a.rs * sizeof(cplx_t<R>) >= 2147483647.0) return 0;
     dim3 grid((a.Hx + TC - 1) / TC, Ly);
     if (a.allper)
-      hipLaunchKernelGGL((k_z3_r<R, L, TC, true>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+      hipLaunchKernelGGL((k_z3_r<R, L, TC, true, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    else if (2 * a.Mz == L && (L / Geo<L>::P) % 2 == 0)
+      hipLaunchKernelGGL((k_z3_r<R, L, TC, false, true>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
     else
-      hipLaunchKernelGGL((k_z3_r<R, L, TC, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+      hipLaunchKernelGGL((k_z3_r<R, L, TC, false, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
     return hipGetLastError() == hipSuccess ? 1 : -1;
   }
 }
