@@ -58,7 +58,9 @@ __device__ __forceinline__ int pad(int i) { return i + (i >> 3) + (i >> 6); }
 // first stage's stride-8 writes, the unit-stride reads and the middle stage's
 // grouped writes on distinct banks (with a line stride of 8 mod 32 complex
 // FP32 words, 4 mod 16 FP64 words between the columns of a block).
-template <int PM> __device__ __forceinline__ int padm(int i) { return PM == 1 ? i + (i >> 3) : pad(i); }
+template <int PM> __device__ __forceinline__ int padm(int i) {
+  return PM == 1 ? i + (i >> 3) : (PM == 2 ? i + (i >> 4) : pad(i));
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Map the linear block id to a logical tile id so that consecutive
@@ -158,22 +160,23 @@ __device__ __forceinline__ void stage(V *line, int p, const V *twrow) {
   line_sync<WAVE>();
 }
 
-template <int L, int S, int I, bool WAVE, typename V, bool DONE = (I >= nstages(L))> struct StageLoop;
-template <int L, int S, int I, bool WAVE, typename V> struct StageLoop<L, S, I, WAVE, V, true> {
+template <int L, int S, int I, bool WAVE, typename V, int PM = 0, bool DONE = (I >= nstages(L))> struct StageLoop;
+template <int L, int S, int I, bool WAVE, typename V, int PM> struct StageLoop<L, S, I, WAVE, V, PM, true> {
   static __device__ __forceinline__ void run(V *, int, const Twid<L, V> &) {}
 };
-template <int L, int S, int I, bool WAVE, typename V> struct StageLoop<L, S, I, WAVE, V, false> {
+template <int L, int S, int I, bool WAVE, typename V, int PM> struct StageLoop<L, S, I, WAVE, V, PM, false> {
   static __device__ __forceinline__ void run(V *line, int p, const Twid<L, V> &t) {
-    stage<L, S, radix_at(L, I), ns_before(L, I), WAVE>(line, p, t.w[I]);
-    StageLoop<L, S, I + 1, WAVE, V>::run(line, p, t);
+    stage<L, S, radix_at(L, I), ns_before(L, I), WAVE, V, PM>(line, p, t.w[I]);
+    StageLoop<L, S, I + 1, WAVE, V, PM>::run(line, p, t);
   }
 };
 
 // In-place transform of one padded line held in LDS with twiddles preloaded
 // by twid_load from the length-L table.  The data must be visible to the
 // line's threads (caller syncs before).
-template <int L, int S, typename V> __device__ __forceinline__ void fft(V *line, int p, const Twid<L, V> &t) {
-  StageLoop<L, S, 0, Geo<L>::WAVE, V>::run(line, p, t);
+template <int L, int S, typename V, int PM = 0>
+__device__ __forceinline__ void fft(V *line, int p, const Twid<L, V> &t) {
+  StageLoop<L, S, 0, Geo<L>::WAVE, V, PM>::run(line, p, t);
 }
 
 // ------------------------------------------------ register-resident lines

@@ -7,6 +7,15 @@
 namespace pmgpu {
 namespace fx {
 
+// LDS layout of the x passes (padding mode, line stride in complex words),
+// chosen with a bank model of their staging and transform accesses: for the
+// hot half-lengths i + i / 16 padding with the modelled best stride; other
+// lengths keep the general padding.
+template <int NH, int VB> struct XLayout {
+  static constexpr int PM = (NH == 256 || NH == 128) ? 2 : 0;
+  static constexpr int ST = NH == 256 ? (VB == 8 ? 272 : 283) : (NH == 128 ? (VB == 8 ? 136 : 140) : Geo<NH>::ST);
+};
+
 // ---------------------------------------------------------------- strided
 template <typename R, int L, int T, int S>
 __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const cplx_t<R> *tw) {
@@ -89,7 +98,8 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_r(StridedArgs a, const
 template <typename R, int NH, int T, int NC>
 __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const cplx_t<R> *tw) {
   using V = cplx_t<R>;
-  constexpr int P = Geo<NH>::P, ST = Geo<NH>::ST, NL = T * NC, NT = P * NL;
+  constexpr int P = Geo<NH>::P, ST = XLayout<NH, sizeof(V)>::ST, PM = XLayout<NH, sizeof(V)>::PM, NL = T * NC,
+                NT = P * NL;
   __shared__ V lds[NL * ST];
   const int y0 = blockIdx.x * T, z = blockIdx.y;
   const R *in = reinterpret_cast<const R *>(a.in);
@@ -118,7 +128,7 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
       const int t = idx / W, w = idx - t * W;
       if (t < T) {
         const int c = w % NC, x = w / NC;
-        ldsr[2 * ((t * NC + c) * ST + pad(x >> 1)) + (x & 1)] = v[i];
+        ldsr[2 * ((t * NC + c) * ST + padm<PM>(x >> 1)) + (x & 1)] = v[i];
       }
     }
   } else
@@ -133,11 +143,11 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
       v.x = src[(int64_t)x0 * a.xs];
       v.y = x0 + 1 < a.Min ? src[(int64_t)(x0 + 1) * a.xs] : (R)0;
     }
-    lds[l * ST + pad(m)] = v;
+    lds[l * ST + padm<PM>(m)] = v;
   }
   __syncthreads();
   const int line = tid / P, p = tid % P;
-  fft<NH, -1>(lds + line * ST, p, twd);
+  fft<NH, -1, V, PM>(lds + line * ST, p, twd);
   __syncthreads();
   V *out = reinterpret_cast<V *>(a.out);
   const V *twL = tw + a.twL_off;
@@ -149,8 +159,8 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
     int y = y0 + t;
     if (y >= a.My) continue;
     const V *zl = lds + (t * NC + c) * ST;
-    V zk = zl[pad(k == NH ? 0 : k)];
-    V zm = cconj(zl[pad(k == 0 ? 0 : NH - k)]);
+    V zk = zl[padm<PM>(k == NH ? 0 : k)];
+    V zm = cconj(zl[padm<PM>(k == 0 ? 0 : NH - k)]);
     V e = cscale(cadd(zk, zm), (R)0.5);
     V o = cscale(csub(zk, zm), (R)0.5);
     V wo = cmul(twL[k], o);
@@ -162,7 +172,8 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
 template <typename R, int NH, int T, int NC>
 __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const cplx_t<R> *tw) {
   using V = cplx_t<R>;
-  constexpr int P = Geo<NH>::P, ST = Geo<NH>::ST, NL = T * NC, NT = P * NL;
+  constexpr int P = Geo<NH>::P, ST = XLayout<NH, sizeof(V)>::ST, PM = XLayout<NH, sizeof(V)>::PM, NL = T * NC,
+                NT = P * NL;
   __shared__ V lds[NL * ST];
   const int y0 = blockIdx.x * T, z = blockIdx.y;
   const V *in = reinterpret_cast<const V *>(a.in);
@@ -187,11 +198,11 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
       V wd = cmul(cconj(twL[k]), d);
       Zk = cadd(s, cmake<V>(-wd.y, wd.x));
     }
-    lds[l * ST + pad(k)] = Zk;
+    lds[l * ST + padm<PM>(k)] = Zk;
   }
   __syncthreads();
   const int line = tid / P, p = tid % P;
-  fft<NH, 1>(lds + line * ST, p, twd);
+  fft<NH, 1, V, PM>(lds + line * ST, p, twd);
   __syncthreads();
   R *out = reinterpret_cast<R *>(a.out);
   const R sc = (R)a.scale;
@@ -204,7 +215,7 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
       const int y = y0 + t;
       if (w >= NC * a.Mout || y >= a.My) continue;
       const int c = w % NC, x = w / NC;
-      out[(int64_t)y * a.ys + (int64_t)z * a.zs + w] = ldsr[2 * ((t * NC + c) * ST + pad(x >> 1)) + (x & 1)] * sc;
+      out[(int64_t)y * a.ys + (int64_t)z * a.zs + w] = ldsr[2 * ((t * NC + c) * ST + padm<PM>(x >> 1)) + (x & 1)] * sc;
     }
     return;
   }
@@ -213,7 +224,7 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
     int t = l / NC, c = l - t * NC;
     int y = y0 + t;
     if (y >= a.My) continue;
-    V v = lds[l * ST + pad(m)];
+    V v = lds[l * ST + padm<PM>(m)];
     R *dst = out + (int64_t)c * a.cs + (int64_t)y * a.ys + (int64_t)z * a.zs;
     int x0 = 2 * m;
     dst[(int64_t)x0 * a.xs] = v.x * sc;
