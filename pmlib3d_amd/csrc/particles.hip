@@ -101,14 +101,17 @@ void particles_release() {
 // ---------------------------------------------------------------- P2M
 // prec: PMGPU_FP64 (double mesh, reference arithmetic) or PMGPU_FP32 (float
 // mesh, FP32 weights with Kahan-compensated sums); particles are FP64.
-// Order each bin's particles and stage their records (one thread per bin),
-// then gather them line by line of 64 nodes along x (one wavefront per line).
+// Order each bin's particles by id (one thread per bin), stage their records
+// (one thread per sorted particle), then gather them line by line of 64 nodes
+// along x (one wavefront per line).
 template <typename T, int NS>
 static void p2m_stage_gather(const PGeom &g, const double *xpart, const double *part, int64_t np, int *ids,
                              PartWS &w, int nbins, T *mesh, int accumulate, hipStream_t st) {
-  if (np > 0)
-    hipLaunchKernelGGL((k_p2m_stage<T, NS>), grid_for(nbins), dim3(256), 0, st, g, xpart, part, ids,
+  if (np > 0) {
+    hipLaunchKernelGGL(k_p2m_binsort, grid_for(nbins), dim3(256), 0, st, w.start.as<int>(), nbins, ids);
+    hipLaunchKernelGGL((k_p2m_stage<T, NS>), grid_for(np), dim3(256), 0, st, g, xpart, part, ids,
                        w.start.as<int>(), nbins, w.rec.as<T>());
+  }
   const dim3 grid((g.E[0] + 63) / 64, g.E[1], g.E[2]);
   hipLaunchKernelGGL((k_p2m_rows<T, NS>), grid, dim3(64), 0, st, g, w.start.as<int>(), w.rec.as<T>(), mesh,
                      accumulate);

@@ -195,42 +195,46 @@ template <typename T, int NS> struct PRec {
 };
 
 // One thread per bin: order the bin's particles by id (the order a stable
-// sort by bin would give), then write their records at the bin's slots:
-// weights from the FP64 offsets (FP64: the reference's arithmetic; FP32:
+// sort by bin would give); bins of one particle, the usual case, do nothing.
+__global__ void __launch_bounds__(256) k_p2m_binsort(const int *start, int nbins, int *ids) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nbins; k += gridDim.x * blockDim.x) {
+    const int t0 = start[k], n = start[k + 1] - t0;
+    if (n > 1) sort_ids(ids + t0, n);
+  }
+}
+
+// One thread per sorted particle t < start[nbins]: its record at slot t
+// (weights from the FP64 offsets; FP64: the reference's arithmetic; FP32:
 // weights in single precision from the FP64 offset).
 template <typename T, int NS>
-__global__ void __launch_bounds__(256) k_p2m_stage(PGeom g, const double *pos, const double *val, int *ids,
+__global__ void __launch_bounds__(256) k_p2m_stage(PGeom g, const double *pos, const double *val, const int *ids,
                                                    const int *start, int nbins, T *rec) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nbins; k += gridDim.x * blockDim.x) {
-    const int t0 = start[k], t1 = start[k + 1];
-    if (t1 == t0) continue;
-    if (t1 - t0 > 1) sort_ids(ids + t0, t1 - t0);
-    for (int t = t0; t < t1; ++t) {
-      const int q = ids[t];
-      T wt[3 * NS];
-      int bx = 0;
+  const int nvalid = start[nbins];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nvalid; t += gridDim.x * blockDim.x) {
+    const int q = ids[t];
+    T wt[3 * NS];
+    int bx = 0;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double x = pos[3 * q + d];
-        const int b = cell_index(x, g.xmin[d], g.c1dx[d]);
-        if (d == 0) bx = b;
-        const double dd = stencil_offset(x, g.xmin[d], g.dx[d], b, g.c1dx[d]);
+    for (int d = 0; d < 3; ++d) {
+      const double x = pos[3 * q + d];
+      const int b = cell_index(x, g.xmin[d], g.c1dx[d]);
+      if (d == 0) bx = b;
+      const double dd = stencil_offset(x, g.xmin[d], g.dx[d], b, g.c1dx[d]);
 #pragma unroll
-        for (int s2 = 0; s2 < NS; ++s2) {
-          if constexpr (sizeof(T) == sizeof(double))
-            wt[d * NS + s2] = weight(NS == 6, s2, dd);
-          else
-            wt[d * NS + s2] = weight_f(NS == 6, s2, (float)dd);
-        }
+      for (int s2 = 0; s2 < NS; ++s2) {
+        if constexpr (sizeof(T) == sizeof(double))
+          wt[d * NS + s2] = weight(NS == 6, s2, dd);
+        else
+          wt[d * NS + s2] = weight_f(NS == 6, s2, (float)dd);
       }
-      T *r = rec + (int64_t)t * PRec<T, NS>::kSlots;
-#pragma unroll
-      for (int i = 0; i < 3 * NS; ++i) r[i] = wt[i];
-      r[PRec<T, NS>::kVal] = (T)val[3 * q];
-      r[PRec<T, NS>::kVal + 1] = (T)val[3 * q + 1];
-      r[PRec<T, NS>::kVal + 2] = (T)val[3 * q + 2];
-      r[PRec<T, NS>::kBx] = (T)bx;
     }
+    T *r = rec + (int64_t)t * PRec<T, NS>::kSlots;
+#pragma unroll
+    for (int i = 0; i < 3 * NS; ++i) r[i] = wt[i];
+    r[PRec<T, NS>::kVal] = (T)val[3 * q];
+    r[PRec<T, NS>::kVal + 1] = (T)val[3 * q + 1];
+    r[PRec<T, NS>::kVal + 2] = (T)val[3 * q + 2];
+    r[PRec<T, NS>::kBx] = (T)bx;
   }
 }
 

@@ -16,6 +16,20 @@ template <int NH, int VB> struct XLayout {
   static constexpr int ST = NH == 256 ? (VB == 8 ? 272 : 283) : (NH == 128 ? (VB == 8 ? 136 : 140) : Geo<NH>::ST);
 };
 
+// Line stride (complex words) of the register-resident kernels' LDS lines
+// for T columns per block (thread tid: column tid % T, position tid / T):
+// room for the padded line (i + i / 8), and g = 32 / T mod 32 words for
+// 8-byte FP32 complex (16 / T mod 16 for 16-byte FP64), g being the number
+// of positions a column has in one pass of the LDS (32 or 16 lanes), so the
+// T columns' runs of g consecutive positions tile the banks.
+template <int L, int VB, int T> constexpr int reg_line_stride() {
+  const int mod = VB == 8 ? 32 : 16;
+  const int g = (mod / T) % mod;
+  int st = L + L / 8;
+  while (st % mod != g) ++st;
+  return st;
+}
+
 // ---------------------------------------------------------------- strided
 template <typename R, int L, int T, int S>
 __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const cplx_t<R> *tw) {
@@ -60,10 +74,10 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const
 // Register-resident form: thread (col, p) owns rows p + P m of column
 // col0 + col; loads and stores are T consecutive complex words per row across
 // the lanes; first and last FFT stages on registers, LDS for the exchanges.
-template <typename R, int L, int T, int S>
+template <typename R, int L, int T, int S, bool HALF>
 __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_r(StridedArgs a, const cplx_t<R> *tw) {
   using V = cplx_t<R>;
-  constexpr int P = Geo<L>::P, ST = Geo<L>::ST, E = L / P;
+  constexpr int P = Geo<L>::P, ST = reg_line_stride<L, sizeof(V), T>(), E = L / P;
   __shared__ V lds[T * ST];
   const int64_t nblk = (int64_t)gridDim.x * gridDim.y * gridDim.z;
   const int64_t lt = xcd_tile(block_linear(), nblk);
@@ -82,7 +96,9 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_r(StridedArgs a, const
     const int row = p + P * m;
     x[m] = (col_ok && row < a.nin) ? in[(int64_t)row * a.rs_in] : cmake<V>(0, 0);
   }
-  fft_regs<L, S>(x, lds + col * ST, p, twd);
+  // HALF: forward with the upper half of the rows zero, or inverse with only
+  // the lower half wanted
+  fft_regs<L, S, V, HALF && (S < 0), HALF && (S > 0), 1>(x, lds + col * ST, p, twd);
   const R sc = (R)a.scale;
   if (col_ok) {
 #pragma unroll
@@ -375,16 +391,6 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
 }
 
 // ------------------------------------------- register-resident fused z pass
-// Line stride (complex words) of the register-resident kernels' LDS lines:
-// room for the padded line (i + i / 8), and 8 mod 32 words (8-byte FP32
-// complex) or 4 mod 16 words (16-byte FP64 complex) so that the columns of a
-// block fall on different banks.
-template <int L, int VB> constexpr int reg_line_stride() {
-  int st = L + L / 8;
-  while (VB == 8 ? (st % 32 != 8) : (st % 16 != 4)) ++st;
-  return st;
-}
-
 // Buffer-resource loads (vector memory, one 32-bit VGPR offset per thread
 // plus a scalar offset per row): the resource's size bounds every access.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
@@ -413,7 +419,7 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   using V = cplx_t<R>;
   constexpr int P = Geo<L>::P, E = L / P;
   // line stride between the block's columns for the i + i / 8 padding
-  constexpr int ST = reg_line_stride<L, sizeof(V)>();
+  constexpr int ST = reg_line_stride<L, sizeof(V), TC>();
   __shared__ V lds[TC * ST];
   const V *tw = tw_all + a.plan.tw_off;
   const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);

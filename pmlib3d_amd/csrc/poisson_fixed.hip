@@ -29,10 +29,15 @@ int strided_L(int S, StridedArgs a, int nb_y, int nb_z, const cplx_t<R> *tw, hip
       constexpr int T = tile_cols<R, L>();
       a.T = T;
       dim3 grid((a.ncols + T - 1) / T, nb_y, nb_z);
-      if (S < 0)
-        hipLaunchKernelGGL((k_strided_r<R, L, T, -1>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+      const bool half = (L / Geo<L>::P) % 2 == 0 && (S < 0 ? 2 * a.nin == L : 2 * a.nout == L);
+      if (S < 0 && half)
+        hipLaunchKernelGGL((k_strided_r<R, L, T, -1, true>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+      else if (S < 0)
+        hipLaunchKernelGGL((k_strided_r<R, L, T, -1, false>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+      else if (half)
+        hipLaunchKernelGGL((k_strided_r<R, L, T, 1, true>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
       else
-        hipLaunchKernelGGL((k_strided_r<R, L, T, 1>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
+        hipLaunchKernelGGL((k_strided_r<R, L, T, 1, false>), grid, dim3(Geo<L>::P * T), 0, st, a, tw);
       return hipGetLastError() == hipSuccess ? 1 : -1;
     }
   }
@@ -63,7 +68,7 @@ template <typename R, int L, int TC> int z_tc(ZArgs a, int Ly, const cplx_t<R> *
 
 // register-resident velocity z pass with TC columns per block
 template <typename R, int L, int TC> int zr_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
-  if constexpr (!regs_ok<L>() || TC * reg_line_stride<L, sizeof(cplx_t<R>)>() * (int)sizeof(cplx_t<R>) > kLdsBudget ||
+  if constexpr (!regs_ok<L>() || TC * reg_line_stride<L, sizeof(cplx_t<R>), TC>() * (int)sizeof(cplx_t<R>) > kLdsBudget ||
                 Geo<L>::P * TC > 1024) {
     return 0;
   } else {
