@@ -127,24 +127,48 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
   if ((NC == 1 && a.xs == 1) || (a.cs == 1 && a.xs == NC)) {
     // the NC components of a line are one contiguous row of NC*Min words:
     // read it word by word across the lanes and scatter into the LDS lines
-    constexpr int W = NC * 2 * NH, K = (T * W + NT - 1) / NT;
+    // When a row is W = J NT words, word j NT + tid of row t is component
+    // tid % NC of element x = j NT / NC + tid / NC: no division in the loop
+    // (NT is a multiple of NC).  Otherwise the general indexing.
+    constexpr int W = NC * 2 * NH;
     R *ldsr = reinterpret_cast<R *>(lds);
-    R v[K];
+    if constexpr (W % NT == 0) {
+      constexpr int J = W / NT, K = T * J;
+      const int c = NC == 1 ? 0 : tid % NC, xb = NC == 1 ? tid : tid / NC;
+      R v[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int idx = tid + i * NT;
-      const int t = idx / W, w = idx - t * W;
-      const int y = y0 + t;
-      v[i] = (R)0;
-      if (t < T && w < NC * a.Min && y < a.My) v[i] = in[(int64_t)y * a.ys + (int64_t)z * a.zs + w];
-    }
+      for (int i = 0; i < K; ++i) {
+        const int t = i / J, j = i % J;
+        const int w = j * NT + tid;
+        const int y = y0 + t;
+        v[i] = (R)0;
+        if (w < NC * a.Min && y < a.My) v[i] = in[(int64_t)y * a.ys + (int64_t)z * a.zs + w];
+      }
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int idx = tid + i * NT;
-      const int t = idx / W, w = idx - t * W;
-      if (t < T) {
-        const int c = w % NC, x = w / NC;
+      for (int i = 0; i < K; ++i) {
+        const int t = i / J, j = i % J;
+        const int x = j * (NT / NC) + xb;
         ldsr[2 * ((t * NC + c) * ST + padm<PM>(x >> 1)) + (x & 1)] = v[i];
+      }
+    } else {
+      constexpr int K = (T * W + NT - 1) / NT;
+      R v[K];
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int idx = tid + i * NT;
+        const int t = idx / W, w = idx - t * W;
+        const int y = y0 + t;
+        v[i] = (R)0;
+        if (t < T && w < NC * a.Min && y < a.My) v[i] = in[(int64_t)y * a.ys + (int64_t)z * a.zs + w];
+      }
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int idx = tid + i * NT;
+        const int t = idx / W, w = idx - t * W;
+        if (t < T) {
+          const int cc = w % NC, x = w / NC;
+          ldsr[2 * ((t * NC + cc) * ST + padm<PM>(x >> 1)) + (x & 1)] = v[i];
+        }
       }
     }
   } else
@@ -169,18 +193,21 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
   const V *twL = tw + a.twL_off;
   constexpr int HX = NH + 1;
   // c fastest: with the interleaved spectral layout consecutive threads
-  // write consecutive complex words
-  for (int idx = tid; idx < T * HX * NC; idx += NT) {
-    int c = idx % NC, k = (idx / NC) % HX, t = idx / (NC * HX);
-    int y = y0 + t;
-    if (y >= a.My) continue;
-    const V *zl = lds + (t * NC + c) * ST;
-    V zk = zl[padm<PM>(k == NH ? 0 : k)];
-    V zm = cconj(zl[padm<PM>(k == 0 ? 0 : NH - k)]);
-    V e = cscale(cadd(zk, zm), (R)0.5);
-    V o = cscale(csub(zk, zm), (R)0.5);
-    V wo = cmul(twL[k], o);
-    out[c * a.o_cs + k * a.o_ks + (int64_t)y * a.o_ys + (int64_t)z * a.o_zs] = cadd(e, cmake<V>(wo.y, -wo.x));
+  // write consecutive complex words (thread: component tid % NC, wavenumbers
+  // tid / NC + n NT / NC)
+  const int pc = NC == 1 ? 0 : tid % NC, pk = NC == 1 ? tid : tid / NC;
+  for (int t = 0; t < T; ++t) {
+    const int y = y0 + t;
+    if (y >= a.My) break;
+    const V *zl = lds + (t * NC + pc) * ST;
+    for (int k = pk; k < HX; k += NT / NC) {
+      V zk = zl[padm<PM>(k == NH ? 0 : k)];
+      V zm = cconj(zl[padm<PM>(k == 0 ? 0 : NH - k)]);
+      V e = cscale(cadd(zk, zm), (R)0.5);
+      V o = cscale(csub(zk, zm), (R)0.5);
+      V wo = cmul(twL[k], o);
+      out[pc * a.o_cs + k * a.o_ks + (int64_t)y * a.o_ys + (int64_t)z * a.o_zs] = cadd(e, cmake<V>(wo.y, -wo.x));
+    }
   }
 }
 
@@ -197,24 +224,27 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
   const int tid = threadIdx.x;
   Twid<NH, V> twd;
   twid_load<NH>(twd, tid % P, tw + a.plan.tw_off);
-  for (int idx = tid; idx < NL * NH; idx += NT) {
-    int c = idx % NC, k = (idx / NC) % NH, t = idx / (NC * NH);
-    int l = t * NC + c;
-    int y = y0 + t;
-    V Zk = cmake<V>(0, 0);
-    if (y < a.My) {
-      const V *xl = in + c * a.i_cs + (int64_t)y * a.i_ys + (int64_t)z * a.i_zs;
-      V xk = xl[k * a.i_ks], xm = xl[(NH - k) * a.i_ks];
-      if (k == 0) {
-        xk.y = 0;
-        xm.y = 0;
+  // thread: component tid % NC, wavenumbers tid / NC + n NT / NC
+  const int pc = NC == 1 ? 0 : tid % NC, pk = NC == 1 ? tid : tid / NC;
+  for (int t = 0; t < T; ++t) {
+    const int l = t * NC + pc;
+    const int y = y0 + t;
+    const V *xl = in + pc * a.i_cs + (int64_t)y * a.i_ys + (int64_t)z * a.i_zs;
+    for (int k = pk; k < NH; k += NT / NC) {
+      V Zk = cmake<V>(0, 0);
+      if (y < a.My) {
+        V xk = xl[k * a.i_ks], xm = xl[(NH - k) * a.i_ks];
+        if (k == 0) {
+          xk.y = 0;
+          xm.y = 0;
+        }
+        V s = cadd(xk, cconj(xm));
+        V d = csub(xk, cconj(xm));
+        V wd = cmul(cconj(twL[k]), d);
+        Zk = cadd(s, cmake<V>(-wd.y, wd.x));
       }
-      V s = cadd(xk, cconj(xm));
-      V d = csub(xk, cconj(xm));
-      V wd = cmul(cconj(twL[k]), d);
-      Zk = cadd(s, cmake<V>(-wd.y, wd.x));
+      lds[l * ST + padm<PM>(k)] = Zk;
     }
-    lds[l * ST + padm<PM>(k)] = Zk;
   }
   __syncthreads();
   const int line = tid / P, p = tid % P;
@@ -224,14 +254,30 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
   const R sc = (R)a.scale;
   const int halfo = (a.Mout + 1) / 2;
   if ((NC == 1 && a.xs == 1) || (a.cs == 1 && a.xs == NC)) {
+    // when row t is W = J NT words, word j NT + tid is component tid % NC
+    // of element j NT / NC + tid / NC (no division in the loop)
     constexpr int W = NC * 2 * NH;
     const R *ldsr = reinterpret_cast<const R *>(lds);
-    for (int idx = tid; idx < T * W; idx += NT) {
-      const int t = idx / W, w = idx - t * W;
-      const int y = y0 + t;
-      if (w >= NC * a.Mout || y >= a.My) continue;
-      const int c = w % NC, x = w / NC;
-      out[(int64_t)y * a.ys + (int64_t)z * a.zs + w] = ldsr[2 * ((t * NC + c) * ST + padm<PM>(x >> 1)) + (x & 1)] * sc;
+    if constexpr (W % NT == 0) {
+      constexpr int J = W / NT;
+      const int c = NC == 1 ? 0 : tid % NC, xb = NC == 1 ? tid : tid / NC;
+#pragma unroll
+      for (int i = 0; i < T * J; ++i) {
+        const int t = i / J, j = i % J;
+        const int w = j * NT + tid;
+        const int y = y0 + t;
+        if (w >= NC * a.Mout || y >= a.My) continue;
+        const int x = j * (NT / NC) + xb;
+        out[(int64_t)y * a.ys + (int64_t)z * a.zs + w] = ldsr[2 * ((t * NC + c) * ST + padm<PM>(x >> 1)) + (x & 1)] * sc;
+      }
+    } else {
+      for (int idx = tid; idx < T * W; idx += NT) {
+        const int t = idx / W, w = idx - t * W;
+        const int y = y0 + t;
+        if (w >= NC * a.Mout || y >= a.My) continue;
+        const int c = w % NC, x = w / NC;
+        out[(int64_t)y * a.ys + (int64_t)z * a.zs + w] = ldsr[2 * ((t * NC + c) * ST + padm<PM>(x >> 1)) + (x & 1)] * sc;
+      }
     }
     return;
   }
