@@ -15,23 +15,67 @@
 namespace pmgpu {
 namespace fx {
 
-constexpr int radix_at(int L, int s) {
-  int r = L, st = 0;
-  const int rads[5] = {8, 4, 2, 3, 5};
-  for (int q = 0; q < 5; ++q) {
-    while (r % rads[q] == 0 && r > 1) {
-      if (st == s) return rads[q];
-      r /= rads[q];
-      ++st;
-    }
-  }
-  return 0;
+// Radix sequence of a length.  Powers of two: radix 8 stages, the remaining
+// 4 or 2 last.  q 2^k with q in {3, 5} and k >= 6: q first, then 8s, the
+// remaining 4 or 2 last, so that the threads per line P (below) divide both
+// the first and the last stage's butterfly counts and a thread's first- and
+// last-stage elements are the ones it holds in registers; where that would
+// leave more than 12 elements per thread the last 8 (or 4) is split into
+// 4 (or 2) and 2.  Other lengths: 8, 4, 2, 3, 5 in that order.
+struct RadixSeq {
+  int r[16];
+  int n;
+  int P;
+};
+constexpr int pow2_part(int v) {
+  int p = 1;
+  while (v % (2 * p) == 0) p *= 2;
+  return p;
 }
-constexpr int nstages(int L) {
-  int s = 0;
-  while (radix_at(L, s) != 0) ++s;
+constexpr int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
+  return p;
+}
+constexpr RadixSeq radix_seq(int L) {
+  RadixSeq s{};
+  const int p2 = L > 0 ? pow2_part(L) : 1, q = L > 0 ? L / p2 : 1;
+  int k = 0;
+  while ((1 << k) < p2) ++k;
+  if ((q == 3 || q == 5) && k >= 6) {
+    s.r[s.n++] = q;
+    for (int i = 0; i < k / 3; ++i) s.r[s.n++] = 8;
+    if (k % 3) s.r[s.n++] = 1 << (k % 3);
+    auto threads = [&](const RadixSeq &t) {
+      const int a = pow2_part(L / t.r[0]), b = pow2_part(L / t.r[t.n - 1]);
+      const int np = next_pow2((L + 7) / 8);
+      int P = a < b ? a : b;
+      return P < np ? P : np;
+    };
+    s.P = threads(s);
+    if (L / s.P > 12 && s.r[s.n - 1] >= 4) {
+      s.r[s.n - 1] /= 2;
+      s.r[s.n++] = 2;
+      s.P = threads(s);
+    }
+    return s;
+  }
+  int r = L;
+  const int rads[5] = {8, 4, 2, 3, 5};
+  for (int i = 0; i < 5; ++i)
+    while (r % rads[i] == 0 && r > 1 && s.n < 16) {
+      s.r[s.n++] = rads[i];
+      r /= rads[i];
+    }
+  if (r != 1) s.n = 0;  // unsupported
+  s.P = next_pow2((L + 7) / 8);
   return s;
 }
+constexpr int radix_at(int L, int s) {
+  const RadixSeq q = radix_seq(L);
+  return s < q.n ? q.r[s] : 0;
+}
+constexpr int nstages(int L) { return radix_seq(L).n; }
 constexpr bool supported(int L) {
   int p = 1;
   for (int s = 0; s < nstages(L); ++s) p *= radix_at(L, s);
@@ -42,13 +86,8 @@ constexpr int ns_before(int L, int s) {
   for (int q = 0; q < s; ++q) p *= radix_at(L, q);
   return p;
 }
-constexpr int next_pow2(int v) {
-  int p = 1;
-  while (p < v) p *= 2;
-  return p;
-}
 template <int L> struct Geo {
-  static constexpr int P = next_pow2((L + 7) / 8);   // threads per line
+  static constexpr int P = radix_seq(L).P;           // threads per line
   static constexpr int PADDED = L + L / 8 + L / 64 + 1;
   static constexpr int ST = PADDED | 1;               // odd line stride
   static constexpr bool WAVE = P <= 64;

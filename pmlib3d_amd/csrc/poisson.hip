@@ -530,6 +530,11 @@ template <typename R> struct PoissonGPU : PoissonBase {
   // time the scalar z pass for each tile width on this handle's buffers
   int tune_scalar() override {
     if (tz1) return 0;
+    if (const char *env = getenv("PMGPU_Z1_KERNEL")) {  // fixed choice, no timing
+      tz1 = atoi(env);
+      PM_CHECK(tz1 > 0, "pmgpu: bad PMGPU_Z1_KERNEL");
+      return 0;
+    }
     const std::string key = z1_key();
     if (int v = tune_cache().get(key)) {
       tz1 = v;
@@ -542,21 +547,24 @@ template <typename R> struct PoissonGPU : PoissonBase {
     PM_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int best_t = 0;
-    const int cands[4] = {1, 2, 4, 8};
+    const int cands[] = {1, 2, 4, 8, 16, kZ1Reg + 4, kZ1Reg + 8, kZ1Reg + 16};
     ZArgs za = z1_args();
-    for (int t : cands) {
-      if (fx_z1<R>(L[2], t, za, L[1], d_tw, stream) != 1) continue;  // warm-up / availability
-      PM_HIP(hipEventRecord(e0, stream));
-      for (int r = 0; r < 3; ++r) fx_z1<R>(L[2], t, za, L[1], d_tw, stream);
-      PM_HIP(hipEventRecord(e1, stream));
-      PM_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      PM_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        best_t = t;
+    // two sweeps over the candidates, 5 launches each; a candidate's time is
+    // its faster sweep (short launches are noisy)
+    for (int sweep = 0; sweep < 2; ++sweep)
+      for (int t : cands) {
+        if (fx_z1<R>(L[2], t, za, L[1], d_tw, stream) != 1) continue;  // warm-up / availability
+        PM_HIP(hipEventRecord(e0, stream));
+        for (int r = 0; r < 5; ++r) fx_z1<R>(L[2], t, za, L[1], d_tw, stream);
+        PM_HIP(hipEventRecord(e1, stream));
+        PM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        PM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          best_t = t;
+        }
       }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     PM_CHECK(best_t > 0, "pmgpu: no scalar z-pass kernel for this transform length");

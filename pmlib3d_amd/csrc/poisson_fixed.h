@@ -196,17 +196,26 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
   // write consecutive complex words (thread: component tid % NC, wavenumbers
   // tid / NC + n NT / NC)
   const int pc = NC == 1 ? 0 : tid % NC, pk = NC == 1 ? tid : tid / NC;
-  for (int t = 0; t < T; ++t) {
-    const int y = y0 + t;
-    if (y >= a.My) break;
+  auto post = [&](int t, int k) {
     const V *zl = lds + (t * NC + pc) * ST;
-    for (int k = pk; k < HX; k += NT / NC) {
-      V zk = zl[padm<PM>(k == NH ? 0 : k)];
-      V zm = cconj(zl[padm<PM>(k == 0 ? 0 : NH - k)]);
-      V e = cscale(cadd(zk, zm), (R)0.5);
-      V o = cscale(csub(zk, zm), (R)0.5);
-      V wo = cmul(twL[k], o);
-      out[pc * a.o_cs + k * a.o_ks + (int64_t)y * a.o_ys + (int64_t)z * a.o_zs] = cadd(e, cmake<V>(wo.y, -wo.x));
+    V zk = zl[padm<PM>(k == NH ? 0 : k)];
+    V zm = cconj(zl[padm<PM>(k == 0 ? 0 : NH - k)]);
+    V e = cscale(cadd(zk, zm), (R)0.5);
+    V o = cscale(csub(zk, zm), (R)0.5);
+    V wo = cmul(twL[k], o);
+    out[pc * a.o_cs + k * a.o_ks + (int64_t)(y0 + t) * a.o_ys + (int64_t)z * a.o_zs] = cadd(e, cmake<V>(wo.y, -wo.x));
+  };
+  if constexpr (NT / NC > HX) {
+    // more threads per component than wavenumbers: walk (line, k) flattened
+    for (int idx = pk; idx < T * HX; idx += NT / NC) {
+      const int t = idx / HX, k = idx - t * HX;
+      if (y0 + t >= a.My) break;
+      post(t, k);
+    }
+  } else {
+    for (int t = 0; t < T; ++t) {
+      if (y0 + t >= a.My) break;
+      for (int k = pk; k < HX; k += NT / NC) post(t, k);
     }
   }
 }
@@ -226,25 +235,32 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
   twid_load<NH>(twd, tid % P, tw + a.plan.tw_off);
   // thread: component tid % NC, wavenumbers tid / NC + n NT / NC
   const int pc = NC == 1 ? 0 : tid % NC, pk = NC == 1 ? tid : tid / NC;
-  for (int t = 0; t < T; ++t) {
-    const int l = t * NC + pc;
+  auto pre = [&](int t, int k) {
     const int y = y0 + t;
-    const V *xl = in + pc * a.i_cs + (int64_t)y * a.i_ys + (int64_t)z * a.i_zs;
-    for (int k = pk; k < NH; k += NT / NC) {
-      V Zk = cmake<V>(0, 0);
-      if (y < a.My) {
-        V xk = xl[k * a.i_ks], xm = xl[(NH - k) * a.i_ks];
-        if (k == 0) {
-          xk.y = 0;
-          xm.y = 0;
-        }
-        V s = cadd(xk, cconj(xm));
-        V d = csub(xk, cconj(xm));
-        V wd = cmul(cconj(twL[k]), d);
-        Zk = cadd(s, cmake<V>(-wd.y, wd.x));
+    V Zk = cmake<V>(0, 0);
+    if (y < a.My) {
+      const V *xl = in + pc * a.i_cs + (int64_t)y * a.i_ys + (int64_t)z * a.i_zs;
+      V xk = xl[k * a.i_ks], xm = xl[(NH - k) * a.i_ks];
+      if (k == 0) {
+        xk.y = 0;
+        xm.y = 0;
       }
-      lds[l * ST + padm<PM>(k)] = Zk;
+      V s = cadd(xk, cconj(xm));
+      V d = csub(xk, cconj(xm));
+      V wd = cmul(cconj(twL[k]), d);
+      Zk = cadd(s, cmake<V>(-wd.y, wd.x));
     }
+    lds[(t * NC + pc) * ST + padm<PM>(k)] = Zk;
+  };
+  if constexpr (NT / NC > NH) {
+    // more threads per component than wavenumbers: walk (line, k) flattened
+    for (int idx = pk; idx < T * NH; idx += NT / NC) {
+      const int t = idx / NH;
+      pre(t, idx - t * NH);
+    }
+  } else {
+    for (int t = 0; t < T; ++t)
+      for (int k = pk; k < NH; k += NT / NC) pre(t, k);
   }
   __syncthreads();
   const int line = tid / P, p = tid % P;
@@ -622,6 +638,77 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_z1_f(ZArgs a, const cplx_t<R> 
   const R scale = (R)1 / (R)L;
   if (cl < ncl)
     for (int row = r0; row < a.Mz; row += P) B[tile + (int64_t)row * a.rs + cl] = cscale(lds[cl * ST + pad(row)], scale);
+}
+
+// Register-resident scalar z pass: thread (col, p) of a block owns column
+// kx0 + col (of TC consecutive ones, TC * sizeof(V) contiguous bytes per z
+// row) and the rows p + P m; the first and last FFT stages run on the
+// registers and G-hat multiplies in registers, LDS only carries the
+// exchanges between stages.  HALF (free-space z, Mz = L / 2) skips the zero
+// upper half in the first forward and the last inverse stage.  The host
+// guarantees Mz * rs complex words fit a 31-bit byte offset.
+template <typename R, int L, int TC, bool ALLPER, bool HALF>
+__global__ void __launch_bounds__(Geo<L>::P *TC) k_z1_r(ZArgs a, const cplx_t<R> *tw_all) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<L>::P, E = L / P;
+  constexpr int ST = reg_line_stride<L, sizeof(V), TC>();
+  __shared__ V lds[TC * ST];
+  const V *tw = tw_all + a.plan.tw_off;
+  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
+  const int kx0 = (int)(lt % gridDim.x) * TC, ky = (int)(lt / gridDim.x);
+  const int tid = threadIdx.x;
+  const int col = tid % TC, p = tid / TC;
+  Twid<L, V> twd;
+  twid_load<L>(twd, p, tw);
+  const int kx = kx0 + col;
+  const bool col_ok = kx < a.Hx;
+  V *base = reinterpret_cast<V *>(a.B) + (int64_t)ky * a.kys + kx0;
+  const int64_t rsb = a.rs * (int64_t)sizeof(V);
+  const int ncols_here = a.Hx - kx0 < TC ? a.Hx - kx0 : TC;
+  const uint32_t extent = (uint32_t)((a.Mz - 1) * rsb + (int64_t)ncols_here * sizeof(V));
+  const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(base, extent);
+  const int voff = (int)(p * rsb) + col * (int)sizeof(V);
+  V w[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const bool ok = col_ok && (HALF ? m < E / 2 : p + P * m < a.Mz);
+    w[m] = ok ? buf_load(rsrc, voff, (int)((int64_t)P * m * rsb), V{}) : cmake<V>(0, 0);
+  }
+  V *line = lds + col * ST;
+  fft_regs<L, -1, V, HALF, false, 1>(w, line, p, twd);
+  const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
+  if constexpr (ALLPER) {
+    const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
+    const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
+    const R pi = (R)3.14159265358979323846;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const R kk = wavenumber<R>(p + P * m, L, a.dk[2], a.ksmp[2]);
+      const R kmag = ki * ki + kj * kj + kk * kk;
+      w[m] = cscale(w[m], kmag > (R)0 ? (R)1 / ((R)4 * (pi * pi) * kmag) : (R)0);
+    }
+  } else {
+    // G-hat of the thread's rows, all loads issued before any is used (a
+    // column past Hx reads column 0; its results are not stored)
+    const R *G = reinterpret_cast<const R *>(a.Ghat) + (int64_t)kyf * a.Hx + (col_ok ? kx : 0);
+    const int64_t gz = (int64_t)a.Hy * a.Hx;
+    R gv[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = p + P * m;
+      gv[m] = G[(n <= L - n ? n : L - n) * gz];
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) w[m] = cscale(w[m], gv[m] * (R)a.h3);
+  }
+  fft_regs<L, 1, V, false, HALF, 1>(w, line, p, twd);
+  const R scale = (R)1 / (R)L;
+  if (col_ok) {
+    V *dst = base + (int64_t)p * a.rs + col;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      if (HALF ? m < E / 2 : p + P * m < a.Mz) dst[(int64_t)P * m * a.rs] = cscale(w[m], scale);
+  }
 }
 
 }  // namespace fx

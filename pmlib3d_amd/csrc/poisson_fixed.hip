@@ -155,7 +155,7 @@ template <typename R, int NH> int xinv_L(XInvArgs a, const cplx_t<R> *tw, hipStr
   return hipGetLastError() == hipSuccess ? 1 : -1;
 }
 
-// scalar z pass: T columns per block, T in {1, 2, 4, 8} where the LDS fits
+// scalar z pass: T columns per block, T in {1, 2, 4, 8, 16} where the LDS fits
 template <typename R, int L, int T> int z1_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
   if constexpr (T * Geo<L>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget || Geo<L>::P * T > 1024) {
     return 0;
@@ -165,12 +165,35 @@ template <typename R, int L, int T> int z1_T(ZArgs a, int Ly, const cplx_t<R> *t
     return hipGetLastError() == hipSuccess ? 1 : -1;
   }
 }
+// register-resident scalar z pass, TC columns per block
+template <typename R, int L, int TC> int z1r_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  if constexpr (!regs_ok<L>() || TC * reg_line_stride<L, sizeof(cplx_t<R>), TC>() * (int)sizeof(cplx_t<R>) > kLdsBudget ||
+                Geo<L>::P * TC > 1024) {
+    return 0;
+  } else {
+    if ((double)a.Mz * (double)a.rs * sizeof(cplx_t<R>) >= 2147483647.0) return 0;  // 31-bit buffer offsets
+    dim3 grid((a.Hx + TC - 1) / TC, Ly);
+    if (a.allper)
+      hipLaunchKernelGGL((k_z1_r<R, L, TC, true, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    else if (2 * a.Mz == L && (L / Geo<L>::P) % 2 == 0)
+      hipLaunchKernelGGL((k_z1_r<R, L, TC, false, true>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    else
+      hipLaunchKernelGGL((k_z1_r<R, L, TC, false, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    return hipGetLastError() == hipSuccess ? 1 : -1;
+  }
+}
+// T selects the kernel: 1..16 the LDS-resident one with T columns per
+// block, kZ1Reg + TC the register-resident one with TC columns per block
 template <typename R, int L> int z1_L(int T, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
   switch (T) {
     case 1: return z1_T<R, L, 1>(a, Ly, tw, st);
     case 2: return z1_T<R, L, 2>(a, Ly, tw, st);
     case 4: return z1_T<R, L, 4>(a, Ly, tw, st);
     case 8: return z1_T<R, L, 8>(a, Ly, tw, st);
+    case 16: return z1_T<R, L, 16>(a, Ly, tw, st);
+    case kZ1Reg + 4: return z1r_T<R, L, 4>(a, Ly, tw, st);
+    case kZ1Reg + 8: return z1r_T<R, L, 8>(a, Ly, tw, st);
+    case kZ1Reg + 16: return z1r_T<R, L, 16>(a, Ly, tw, st);
     default: return 0;
   }
 }

@@ -180,11 +180,15 @@ __global__ void __launch_bounds__(1024) k_x_inv(XInvArgs a, const cplx_t<R> *tw)
 // Fused third-direction pass: zero pad, forward FFT of the 3 vorticity
 // components, G-hat product + spectral curl (+ psi, + regularised /
 // reprojected vorticity), inverse FFTs, truncation to Mz rows, 1/Lz.
+// scalar z-pass choice: T < kZ1Reg the LDS-resident kernel with T columns
+// per block, kZ1Reg + TC the register-resident one with TC columns per block
+constexpr int kZ1Reg = 100;
+
 struct ZArgs {
   void *B;            // in: w-hat (3 comps), out: u-hat (in place)
   void *Bpsi;         // optional psi-hat output
   void *Bvort;        // optional regularised vorticity output
-  const void *Ghat;   // octant table [(ky' * Hx + kx) * Hz + kz'] (kernel 1)
+  const void *Ghat;   // octant table [(kz' * Hy + ky') * Hx + kx] (kernel 1)
   int64_t rs, cs;     // row (z) stride and component stride of B
   int64_t kxs, kys;   // kx and ky strides of B (interleaved layout: 3 and 3*Hx)
   int Hx, Ly, Lz, Lx, Mz, T, Hy, Hz;

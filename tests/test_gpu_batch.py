@@ -49,6 +49,38 @@ def test_scalar_solve_matches_oracle_psi(case):
     assert rel_l2(got32, psi[2]) <= 1e-5
 
 
+# every scalar z-pass kernel, forced: 1..16 the LDS-resident one with that
+# many columns per block, 100 + TC the register-resident one (transform
+# lengths 32 and 64, both register-capable)
+Z1_KERNELS = [1, 2, 4, 8, 16, 104, 108, 116]
+Z1_CASES = [
+    ((32, 24, 32), 0, (0, 0, 0), 4),   # free z, zero upper half skipped
+    ((28, 24, 28), 2, (0, 0, 0), 6),   # ghosts: Mz = 32 of Lz = 64
+    ((32, 16, 32), 0, (0, 0, 1), 4),   # periodic z, Mz = Lz
+    ((32, 16, 32), 0, (1, 1, 1), 2),   # all periodic: G formed in the pass
+]
+
+
+@pytest.mark.parametrize("kern", Z1_KERNELS)
+@pytest.mark.parametrize("case", Z1_CASES, ids=[str(i) for i in range(len(Z1_CASES))])
+def test_scalar_z_kernels_match_oracle_psi(case, kern, monkeypatch):
+    import pmlib3d_amd as pm
+    ncell, ng, bc, order = case
+    print("case", case, "z1 kernel", kern, flush=True)
+    monkeypatch.setenv("PMGPU_Z1_KERNEL", str(kern))
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+    p = O.make_params(order, 1, 2.0)
+    w3 = blob_field(g, seed=7 + sum(ncell))
+    w3[0] = 0.0
+    w3[1] = 0.0
+    _, _, psi = O.poisson_solve(g, p, O.poisson_setup(g, p), w3, False, False, True)
+    for rep in range(2):
+        got = _solver(g, p, pm.FP64).solve_scalar(w3[2])
+        assert rel_linf(got, psi[2]) <= 1e-12
+        got32 = _solver(g, p, pm.FP32).solve_scalar(w3[2])
+        assert rel_l2(got32, psi[2]) <= 1e-5
+
+
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
 def test_batch_graph_matches_individual_solves_on_every_replay(precision):
     import torch
