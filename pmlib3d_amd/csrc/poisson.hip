@@ -3,7 +3,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <cstdio>
+#include <algorithm>
 #include <map>
+#include <type_traits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -23,6 +25,18 @@ template <typename R> int fx_z(int L, ZArgs a, int Ly, const cplx_t<R> *tw, hipS
 template <typename R> int fx_xfwd(int NH, int NC, XFwdArgs a, const cplx_t<R> *tw, hipStream_t st);
 template <typename R> int fx_xinv(int NH, XInvArgs a, const cplx_t<R> *tw, hipStream_t st);
 template <typename R> int fx_z1(int L, int T, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st);
+// group launches over several patches (same return convention) and the
+// tile sizes their tables are built with (0: length not specialised)
+template <typename R> int fx_tile_x1(int NH);
+template <typename R> int fx_tile_cols(int L);
+int fx_even_rows(int L);
+template <typename R> int fx_grp_xfwd(int NH, const GrpEntry<XFwdArgs> *t, int n, int64_t nb, hipStream_t st);
+template <typename R> int fx_grp_xinv(int NH, const GrpEntry<XInvArgs> *t, int n, int64_t nb, hipStream_t st);
+template <typename R>
+int fx_grp_strided(int L, int S, int kernel, bool half, const GrpEntry<StridedArgs> *t, int n, int64_t nb,
+                   hipStream_t st);
+template <typename R>
+int fx_grp_z1(int L, int T, bool allper, bool half, const GrpEntry<ZArgs> *t, int n, int64_t nb, hipStream_t st);
 
 // Tile widths chosen by timing, per (pass, lengths, precision), shared by all
 // handles of the process and kept across runs in the file named by
@@ -96,7 +110,26 @@ template <typename R> struct TwTable {
   }
 };
 
+// The five passes of one handle's single-component solve as group-launch
+// entries (arguments, twiddles, tiles) plus what selects their kernel
+// instance: patches whose keys match share one launch per pass.
+struct ScalarPasses {
+  bool ok = false;   // every pass has a length-specialised kernel
+  bool fp64 = false;
+  bool ghost = false;  // periodic axes: ghost fill after the x inverse
+  int NH = 0, Ly = 0, Lz = 0, tz1 = 0, skernel = 0;
+  bool y_half = false, z_half = false, allper = false;
+  GrpEntry<XFwdArgs> xf{};
+  GrpEntry<StridedArgs> yf{};
+  GrpEntry<ZArgs> z{};
+  GrpEntry<StridedArgs> yi{};
+  GrpEntry<XInvArgs> xi{};
+  int64_t tiles[5] = {0, 0, 0, 0, 0};  // per pass: x fwd, y fwd, z, y inv, x inv
+};
+
 struct PoissonBase {
+  virtual int scalar_passes(const void *w, void *psi, ScalarPasses &sp) = 0;
+  virtual int scalar_ghost_fill(void *psi) = 0;
   virtual int setup() = 0;
   virtual int solve_device(const void *vort, void *vel, void *vort_out, void *vecpot, int reg, int reproj,
                            int psi) = 0;
@@ -595,12 +628,63 @@ template <typename R> struct PoissonGPU : PoissonBase {
     XInvArgs xa{A, reinterpret_cast<R *>(psi) + off, 0, 1, ys, zs, 0, 1, H1, H1 * M[1], 1, M[0], M[1], M[2],
                 Hx, Tx, 1.0 / (double)L[0], pNh, twLx_off};
     PM_CHECK(fx_xinv<R>(Nh, xa, d_tw, stream) == 1, "pmgpu: scalar solve needs a specialised x length");
+    return scalar_ghost_fill(psi);
+  }
+
+  int scalar_ghost_fill(void *psi) override {
     for (int d = 0; d < 3; ++d)
       if (g.bound_cond[d] == 1) {
         hipLaunchKernelGGL(k_ghost_fill<R>, dim3(1024), dim3(256), 0, stream, reinterpret_cast<R *>(psi), E[0],
                            E[1], E[2], d, g.ncell[d], ngs[d], 1);
         PM_HIP(hipGetLastError());
       }
+    return 0;
+  }
+
+  // The passes of solve_scalar_device as group entries: the same arguments,
+  // the same kernels (the tile sizes of the length-specialised launchers).
+  int scalar_passes(const void *w, void *psi, ScalarPasses &sp) override {
+    PM_CHECK(allper || Ghat != nullptr, "pmgpu: pmgpu_poisson_setup was not called");
+    PM_CHECK(w && psi, "pmgpu: null mesh pointer");
+    PM_CHECK(tz1 > 0, "pmgpu: scalar solve not tuned (pmgpu_poisson_tune_scalar)");
+    sp = ScalarPasses{};
+    const int tx = fx_tile_x1<R>(Nh), ty = fx_tile_cols<R>(L[1]);
+    const int tz = tz1 < kZ1Reg ? tz1 : tz1 - kZ1Reg;
+    sp.ok = tx > 0 && ty > 0 && fx_tile_cols<R>(L[2]) > 0;
+    if (!sp.ok) return 0;
+    sp.fp64 = sizeof(R) == 8;
+    sp.NH = Nh;
+    sp.Ly = L[1];
+    sp.Lz = L[2];
+    sp.tz1 = tz1;
+    sp.skernel = skernel;
+    sp.allper = allper != 0;
+    sp.y_half = fx_even_rows(L[1]) == 1 && 2 * M[1] == L[1];
+    sp.z_half = !sp.allper && fx_even_rows(L[2]) == 1 && 2 * M[2] == L[2];
+    for (int d = 0; d < 3; ++d) sp.ghost = sp.ghost || g.bound_cond[d] == 1;
+    const int64_t H1 = row1();
+    const int64_t off = (int64_t)lo[0] + (int64_t)E[0] * (lo[1] + (int64_t)E[1] * lo[2]);
+    const int64_t ys = E[0], zs = (int64_t)E[0] * E[1];
+    const R *win = reinterpret_cast<const R *>(w) + off;
+    R *pout = reinterpret_cast<R *>(psi) + off;
+    auto ceil_div = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
+    sp.xf.a = XFwdArgs{win, A, 0, 1, ys, zs, 0, 1, H1, H1 * M[1], 1, M[0], M[1], M[2], pNh.L + 1, tx, pNh, twLx_off};
+    sp.xf.gx = ceil_div(M[1], tx);
+    sp.tiles[0] = sp.xf.gx * M[2];
+    sp.yf.a = StridedArgs{A, B, H1, H1, H1 * M[1], H1 * L[1], Hx, M[1], L[1], ty, 1.0, pLy, skernel};
+    sp.yf.gx = ceil_div(Hx, ty);
+    sp.tiles[1] = sp.yf.gx * M[2];
+    sp.z.a = z1_args();
+    sp.z.gx = ceil_div(Hx, tz);
+    sp.tiles[2] = sp.z.gx * L[1];
+    sp.yi.a = StridedArgs{B, A, H1, H1, H1 * L[1], H1 * M[1], Hx, L[1], M[1], ty, 1.0 / (double)L[1], pLy, skernel};
+    sp.yi.gx = sp.yf.gx;
+    sp.tiles[3] = sp.tiles[1];
+    sp.xi.a = XInvArgs{A, pout, 0, 1, ys, zs, 0, 1, H1, H1 * M[1], 1, M[0], M[1], M[2], Hx, tx, 1.0 / (double)L[0], pNh,
+                       twLx_off};
+    sp.xi.gx = sp.xf.gx;
+    sp.tiles[4] = sp.tiles[0];
+    sp.xf.tw = sp.yf.tw = sp.z.tw = sp.yi.tw = sp.xi.tw = d_tw;
     return 0;
   }
 
@@ -630,9 +714,22 @@ struct pmgpu_poisson_s {
 };
 
 // A set of solver handles (patches of any sizes) solved together: the
-// single-component solves of all patches are captured once into a HIP graph,
-// one branch per patch on its own stream so that small patches run side by
-// side, and the graph is replayed while the mesh pointers stay the same.
+// single-component solves of all patches are captured once into a HIP graph
+// and the graph is replayed while the mesh pointers stay the same.  Grouped
+// form (every length specialised): per pass, the patches that share a
+// kernel instance run as one group launch (a table of their arguments), the
+// groups of a pass side by side on their own streams, the passes in order.
+// Otherwise one branch per patch, each on its own stream.
+struct BatchGroup {
+  int pass;        // 0 x fwd, 1 y fwd, 2 z, 3 y inv, 4 x inv
+  bool fp64;
+  int len, tz1, skernel;
+  bool half, allper;
+  std::vector<int> members;
+  size_t tab_off = 0;
+  int64_t nb = 0;  // tiles (blocks) of the launch
+};
+
 struct pmgpu_poisson_batch_s {
   std::vector<pmgpu_poisson_t> h;
   hipStream_t stream = nullptr;   // the caller's stream (may be the null stream)
@@ -644,7 +741,13 @@ struct pmgpu_poisson_batch_s {
   hipGraphExec_t exec = nullptr;
   std::vector<const void *> in;
   std::vector<void *> out;
+  std::vector<BatchGroup> groups;
+  std::vector<hipEvent_t> gev;
+  void *d_tab = nullptr;  // the groups' entry tables
   ~pmgpu_poisson_batch_s() {
+    if (cap) (void)hipStreamSynchronize(cap);
+    if (d_tab) (void)hipFree(d_tab);
+    for (auto e : gev) (void)hipEventDestroy(e);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto e : ev) (void)hipEventDestroy(e);
@@ -750,7 +853,10 @@ int pmgpu_poisson_batch_create(const pmgpu_poisson_t *handles, int32_t n, void *
     PM_CHECK(hh, "pmgpu_poisson_batch_create: null handle");
     if (hh->impl->tune_scalar()) return -1;  // tuning launches may not be captured
   }
-  const int nside = n < 8 ? n : 8;
+  // side streams of the captured graph (PMGPU_BATCH_STREAMS, default 8)
+  int nside = 8;
+  if (const char *env = getenv("PMGPU_BATCH_STREAMS")) nside = atoi(env) > 0 ? atoi(env) : 8;
+  nside = n < nside ? n : nside;
   for (int i = 0; i < nside; ++i) {
     hipStream_t st;
     PM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -774,43 +880,194 @@ int pmgpu_poisson_batch_destroy(pmgpu_poisson_batch_t b) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace pmgpu {
+
+// Group the patches' passes by kernel instance and upload the entry tables.
+static int batch_build_groups(pmgpu_poisson_batch_s *b, const std::vector<ScalarPasses> &sp) {
+  b->groups.clear();
+  std::map<std::string, int> index;
+  for (int pass = 0; pass < 5; ++pass)
+    for (size_t i = 0; i < sp.size(); ++i) {
+      const ScalarPasses &q = sp[i];
+      BatchGroup G{};
+      G.pass = pass;
+      G.fp64 = q.fp64;
+      G.len = (pass == 0 || pass == 4) ? q.NH : (pass == 2 ? q.Lz : q.Ly);
+      G.tz1 = pass == 2 ? q.tz1 : 0;
+      G.skernel = (pass == 1 || pass == 3) ? q.skernel : 0;
+      G.half = (pass == 1 || pass == 3) ? q.y_half : (pass == 2 ? q.z_half : false);
+      G.allper = pass == 2 && q.allper;
+      char key[128];
+      snprintf(key, sizeof(key), "%d:%d:%d:%d:%d:%d:%d", G.pass, (int)G.fp64, G.len, G.tz1, G.skernel, (int)G.half,
+               (int)G.allper);
+      auto it = index.find(key);
+      if (it == index.end()) {
+        index[key] = (int)b->groups.size();
+        G.members.push_back((int)i);
+        b->groups.push_back(G);
+      } else {
+        b->groups[it->second].members.push_back((int)i);
+      }
+    }
+  // tables: one per group, 256-byte aligned, in one device buffer
+  std::vector<unsigned char> host;
+  auto put = [&](BatchGroup &G, auto entry_of) {
+    using E = std::remove_reference_t<decltype(entry_of(sp[0]))>;
+    G.tab_off = (host.size() + 255) / 256 * 256;
+    host.resize(G.tab_off + G.members.size() * sizeof(E));
+    int64_t off = 0;
+    E *t = reinterpret_cast<E *>(host.data() + G.tab_off);
+    for (size_t k = 0; k < G.members.size(); ++k) {
+      const ScalarPasses &q = sp[G.members[k]];
+      t[k] = entry_of(q);
+      t[k].off = off;
+      off += q.tiles[G.pass];
+    }
+    G.nb = off;
+  };
+  for (auto &G : b->groups) {
+    switch (G.pass) {
+      case 0: put(G, [](const ScalarPasses &q) -> GrpEntry<XFwdArgs> { return q.xf; }); break;
+      case 1: put(G, [](const ScalarPasses &q) -> GrpEntry<StridedArgs> { return q.yf; }); break;
+      case 2: put(G, [](const ScalarPasses &q) -> GrpEntry<ZArgs> { return q.z; }); break;
+      case 3: put(G, [](const ScalarPasses &q) -> GrpEntry<StridedArgs> { return q.yi; }); break;
+      default: put(G, [](const ScalarPasses &q) -> GrpEntry<XInvArgs> { return q.xi; }); break;
+    }
+    PM_CHECK(G.nb > 0 && G.nb < 2147483647LL, "pmgpu_poisson_batch_solve_scalar: group launch too large");
+  }
+  if (b->d_tab) PM_HIP(hipFree(b->d_tab));
+  b->d_tab = nullptr;
+  PM_HIP(hipMalloc(&b->d_tab, host.size()));
+  PM_HIP(hipMemcpy(b->d_tab, host.data(), host.size(), hipMemcpyHostToDevice));
+  while (b->gev.size() < b->groups.size()) {
+    hipEvent_t e;
+    PM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->gev.push_back(e);
+  }
+  return 0;
+}
+
+static int batch_launch_group(const pmgpu_poisson_batch_s *b, const BatchGroup &G, hipStream_t st) {
+  const void *t = static_cast<const unsigned char *>(b->d_tab) + G.tab_off;
+  const int n = (int)G.members.size();
+  switch (G.pass) {
+    case 0: {
+      auto tt = static_cast<const GrpEntry<XFwdArgs> *>(t);
+      return G.fp64 ? fx_grp_xfwd<double>(G.len, tt, n, G.nb, st) : fx_grp_xfwd<float>(G.len, tt, n, G.nb, st);
+    }
+    case 1:
+    case 3: {
+      auto tt = static_cast<const GrpEntry<StridedArgs> *>(t);
+      const int S = G.pass == 1 ? -1 : 1;
+      return G.fp64 ? fx_grp_strided<double>(G.len, S, G.skernel, G.half, tt, n, G.nb, st)
+                    : fx_grp_strided<float>(G.len, S, G.skernel, G.half, tt, n, G.nb, st);
+    }
+    case 2: {
+      auto tt = static_cast<const GrpEntry<ZArgs> *>(t);
+      return G.fp64 ? fx_grp_z1<double>(G.len, G.tz1, G.allper, G.half, tt, n, G.nb, st)
+                    : fx_grp_z1<float>(G.len, G.tz1, G.allper, G.half, tt, n, G.nb, st);
+    }
+    default: {
+      auto tt = static_cast<const GrpEntry<XInvArgs> *>(t);
+      return G.fp64 ? fx_grp_xinv<double>(G.len, tt, n, G.nb, st) : fx_grp_xinv<float>(G.len, tt, n, G.nb, st);
+    }
+  }
+}
+
+}  // namespace pmgpu
+
+extern "C" {
+
 int pmgpu_poisson_batch_solve_scalar(pmgpu_poisson_batch_t b, const void *const *d_in, void *const *d_out) {
   PM_CHECK(b && d_in && d_out, "pmgpu_poisson_batch_solve_scalar: null argument");
   const size_t n = b->h.size();
   bool same = b->exec != nullptr;
   for (size_t i = 0; same && i < n; ++i) same = b->in[i] == d_in[i] && b->out[i] == d_out[i];
   if (!same) {
+    // a replay of the previous graph may still read its tables
+    PM_HIP(hipStreamSynchronize(b->cap));
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
-    b->in.assign(d_in, d_in + n);
-    b->out.assign(d_out, d_out + n);
-    // capture on the batch's own stream: fork every patch onto a side
-    // stream, join back
+    b->in.clear();
+    b->out.clear();
+    // grouped form (PMGPU_BATCH_GROUPED=1) when every patch's passes have
+    // group kernels; per-patch branches otherwise
+    std::vector<ScalarPasses> sp(n);
+    bool grouped = false;
+    if (const char *env = getenv("PMGPU_BATCH_GROUPED")) grouped = atoi(env) != 0;
+    for (size_t i = 0; grouped && i < n; ++i) {
+      if (b->h[i]->impl->scalar_passes(d_in[i], d_out[i], sp[i])) return -1;
+      grouped = sp[i].ok;
+    }
+    if (grouped && batch_build_groups(b, sp)) return -1;
+    // capture on the batch's own stream, forking onto the side streams
     PM_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
     int rc = hipEventRecord(b->fork, b->cap) == hipSuccess ? 0 : -1;
-    for (size_t i = 0; rc == 0 && i < n; ++i) {
-      hipStream_t st = b->side[i % b->side.size()];
-      PoissonBase *p = b->h[i]->impl.get();
-      hipStream_t own = p->stream;
-      if (hipStreamWaitEvent(st, b->fork, 0) != hipSuccess) rc = -1;
-      p->stream = st;
-      if (rc == 0 && p->solve_scalar_device(d_in[i], d_out[i])) rc = -1;
-      p->stream = own;
-      if (rc == 0 && hipEventRecord(b->ev[i], st) != hipSuccess) rc = -1;
-      if (rc == 0 && hipStreamWaitEvent(b->cap, b->ev[i], 0) != hipSuccess) rc = -1;
+    if (grouped) {
+      size_t k = 0;
+      for (int pass = 0; rc == 0 && pass < 5; ++pass) {
+        int gi = 0;
+        for (const BatchGroup &G : b->groups) {
+          if (G.pass != pass || rc != 0) continue;
+          hipStream_t st = b->side[gi++ % b->side.size()];
+          if (hipStreamWaitEvent(st, b->fork, 0) != hipSuccess) rc = -1;
+          if (rc == 0 && batch_launch_group(b, G, st) != 1) rc = -1;
+          if (rc == 0 && hipEventRecord(b->gev[k], st) != hipSuccess) rc = -1;
+          if (rc == 0 && hipStreamWaitEvent(b->cap, b->gev[k], 0) != hipSuccess) rc = -1;
+          ++k;
+        }
+        if (rc == 0 && hipEventRecord(b->fork, b->cap) != hipSuccess) rc = -1;  // the next pass starts here
+      }
+      for (size_t i = 0; rc == 0 && i < n; ++i) {
+        if (!sp[i].ghost) continue;
+        PoissonBase *p = b->h[i]->impl.get();
+        hipStream_t own = p->stream;
+        p->stream = b->cap;
+        if (p->scalar_ghost_fill(d_out[i])) rc = -1;
+        p->stream = own;
+      }
+    } else {
+      // largest patches first, each onto the side stream with the least
+      // work so far (mesh cells), so that the branches end together
+      std::vector<double> cost(n), load(b->side.size(), 0.0);
+      std::vector<size_t> order(n);
+      for (size_t i = 0; i < n; ++i) {
+        const pmgpu_geom &gg = b->h[i]->impl->g;
+        cost[i] = 1.0;
+        for (int d = 0; d < 3; ++d) cost[i] *= (double)(gg.ncell[d] + 2 * gg.nghost);
+        order[i] = i;
+      }
+      std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+      for (size_t oi = 0; rc == 0 && oi < n; ++oi) {
+        const size_t i = order[oi];
+        const size_t j = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[j] += cost[i];
+        hipStream_t st = b->side[j];
+        PoissonBase *p = b->h[i]->impl.get();
+        hipStream_t own = p->stream;
+        if (hipStreamWaitEvent(st, b->fork, 0) != hipSuccess) rc = -1;
+        p->stream = st;
+        if (rc == 0 && p->solve_scalar_device(d_in[i], d_out[i])) rc = -1;
+        p->stream = own;
+        if (rc == 0 && hipEventRecord(b->ev[i], st) != hipSuccess) rc = -1;
+        if (rc == 0 && hipStreamWaitEvent(b->cap, b->ev[i], 0) != hipSuccess) rc = -1;
+      }
     }
     hipGraph_t graph = nullptr;
     hipError_t ec = hipStreamEndCapture(b->cap, &graph);
     (void)hipGetLastError();  // a failed capture leaves its error behind
     if (rc != 0 || ec != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
-      b->in.clear();
       PM_CHECK(false, "pmgpu_poisson_batch_solve_scalar: graph capture failed");
     }
     b->graph = graph;
     PM_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->in.assign(d_in, d_in + n);
+    b->out.assign(d_out, d_out + n);
   }
   // order the replay after the caller's earlier work and before its later work
   PM_HIP(hipEventRecord(b->in_done, b->stream));

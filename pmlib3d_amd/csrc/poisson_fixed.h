@@ -31,15 +31,16 @@ template <int L, int VB, int T> constexpr int reg_line_stride() {
 }
 
 // ---------------------------------------------------------------- strided
+// Kernel bodies take the block's logical tile lt and the tiles per grid row
+// gx: the single-patch kernels derive them from the block index, the group
+// kernels (end of file) from a table of patches.
 template <typename R, int L, int T, int S>
-__global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const cplx_t<R> *tw) {
+__device__ __forceinline__ void strided_f_body(const StridedArgs &a, const cplx_t<R> *tw, int64_t lt, int gx) {
   using V = cplx_t<R>;
   constexpr int P = Geo<L>::P, ST = Geo<L>::ST, NT = P * T;
   __shared__ V lds[T * ST];
-  const int64_t nblk = (int64_t)gridDim.x * gridDim.y * gridDim.z;
-  const int64_t lt = xcd_tile(block_linear(), nblk);
-  const int col0 = (int)(lt % gridDim.x) * T;
-  const int64_t b = lt / gridDim.x;
+  const int col0 = (int)(lt % gx) * T;
+  const int64_t b = lt / gx;
   const V *in = reinterpret_cast<const V *>(a.in) + b * a.bs_in + col0;
   V *out = reinterpret_cast<V *>(a.out) + b * a.bs_out + col0;
   const int tid = threadIdx.x;
@@ -70,19 +71,21 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const
     for (int row = tid / T; row < a.nout; row += NT / T)
       out[(int64_t)row * a.rs_out + c_ld] = cscale(lds[c_ld * ST + pad(row)], sc);
 }
+template <typename R, int L, int T, int S>
+__global__ void __launch_bounds__(Geo<L>::P *T) k_strided_f(StridedArgs a, const cplx_t<R> *tw) {
+  strided_f_body<R, L, T, S>(a, tw, xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y * gridDim.z), gridDim.x);
+}
 
 // Register-resident form: thread (col, p) owns rows p + P m of column
 // col0 + col; loads and stores are T consecutive complex words per row across
 // the lanes; first and last FFT stages on registers, LDS for the exchanges.
 template <typename R, int L, int T, int S, bool HALF>
-__global__ void __launch_bounds__(Geo<L>::P *T) k_strided_r(StridedArgs a, const cplx_t<R> *tw) {
+__device__ __forceinline__ void strided_r_body(const StridedArgs &a, const cplx_t<R> *tw, int64_t lt, int gx) {
   using V = cplx_t<R>;
   constexpr int P = Geo<L>::P, ST = reg_line_stride<L, sizeof(V), T>(), E = L / P;
   __shared__ V lds[T * ST];
-  const int64_t nblk = (int64_t)gridDim.x * gridDim.y * gridDim.z;
-  const int64_t lt = xcd_tile(block_linear(), nblk);
-  const int col0 = (int)(lt % gridDim.x) * T;
-  const int64_t b = lt / gridDim.x;
+  const int col0 = (int)(lt % gx) * T;
+  const int64_t b = lt / gx;
   const int tid = threadIdx.x;
   const int col = tid % T, p = tid / T;
   Twid<L, V> twd;
@@ -108,16 +111,21 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_strided_r(StridedArgs a, const
     }
   }
 }
+template <typename R, int L, int T, int S, bool HALF>
+__global__ void __launch_bounds__(Geo<L>::P *T) k_strided_r(StridedArgs a, const cplx_t<R> *tw) {
+  strided_r_body<R, L, T, S, HALF>(a, tw, xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y * gridDim.z),
+                                   gridDim.x);
+}
 
 // -------------------------------------------------------------- x forward
 // Lines of ncomp (= 3 or 1) components; NH = half transform length.
 template <typename R, int NH, int T, int NC>
-__global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const cplx_t<R> *tw) {
+__device__ __forceinline__ void x_fwd_body(const XFwdArgs &a, const cplx_t<R> *tw, int64_t lt, int gx) {
   using V = cplx_t<R>;
   constexpr int P = Geo<NH>::P, ST = XLayout<NH, sizeof(V)>::ST, PM = XLayout<NH, sizeof(V)>::PM, NL = T * NC,
                 NT = P * NL;
   __shared__ V lds[NL * ST];
-  const int y0 = blockIdx.x * T, z = blockIdx.y;
+  const int y0 = (int)(lt % gx) * T, z = (int)(lt / gx);
   const R *in = reinterpret_cast<const R *>(a.in);
   const int tid = threadIdx.x;
   Twid<NH, V> twd;
@@ -219,15 +227,19 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const
     }
   }
 }
+template <typename R, int NH, int T, int NC>
+__global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_fwd_f(XFwdArgs a, const cplx_t<R> *tw) {
+  x_fwd_body<R, NH, T, NC>(a, tw, blockIdx.x + (int64_t)gridDim.x * blockIdx.y, gridDim.x);
+}
 
 // -------------------------------------------------------------- x inverse
 template <typename R, int NH, int T, int NC>
-__global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const cplx_t<R> *tw) {
+__device__ __forceinline__ void x_inv_body(const XInvArgs &a, const cplx_t<R> *tw, int64_t lt, int gx) {
   using V = cplx_t<R>;
   constexpr int P = Geo<NH>::P, ST = XLayout<NH, sizeof(V)>::ST, PM = XLayout<NH, sizeof(V)>::PM, NL = T * NC,
                 NT = P * NL;
   __shared__ V lds[NL * ST];
-  const int y0 = blockIdx.x * T, z = blockIdx.y;
+  const int y0 = (int)(lt % gx) * T, z = (int)(lt / gx);
   const V *in = reinterpret_cast<const V *>(a.in);
   const V *twL = tw + a.twL_off;
   const int tid = threadIdx.x;
@@ -308,6 +320,10 @@ __global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const
     dst[(int64_t)x0 * a.xs] = v.x * sc;
     if (x0 + 1 < a.Mout) dst[(int64_t)(x0 + 1) * a.xs] = v.y * sc;
   }
+}
+template <typename R, int NH, int T, int NC>
+__global__ void __launch_bounds__(Geo<NH>::P *T *NC) k_x_inv_f(XInvArgs a, const cplx_t<R> *tw) {
+  x_inv_body<R, NH, T, NC>(a, tw, blockIdx.x + (int64_t)gridDim.x * blockIdx.y, gridDim.x);
 }
 
 // ------------------------------------------------------------ fused z pass
@@ -577,15 +593,14 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
 // transforms them forward, multiplies by G h^3 (or 1/(4 pi^2 |k|^2) when all
 // axes are periodic) and transforms back, in place.
 template <typename R, int L, int T>
-__global__ void __launch_bounds__(Geo<L>::P *T) k_z1_f(ZArgs a, const cplx_t<R> *tw_all) {
+__device__ __forceinline__ void z1_f_body(const ZArgs &a, const cplx_t<R> *tw_all, int64_t lt, int gx) {
   using V = cplx_t<R>;
   constexpr int P = Geo<L>::P, ST = Geo<L>::ST;
   constexpr int E = (L + P - 1) / P;
   constexpr bool FULL = E * P == L;
   __shared__ V lds[T * ST];
   const V *tw = tw_all + a.plan.tw_off;
-  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
-  const int kx0 = (int)(lt % gridDim.x) * T, ky = (int)(lt / gridDim.x);
+  const int kx0 = (int)(lt % gx) * T, ky = (int)(lt / gx);
   const int tid = threadIdx.x;
   const int line = tid / P, p = tid % P;
   Twid<L, V> twd;
@@ -639,6 +654,10 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_z1_f(ZArgs a, const cplx_t<R> 
   if (cl < ncl)
     for (int row = r0; row < a.Mz; row += P) B[tile + (int64_t)row * a.rs + cl] = cscale(lds[cl * ST + pad(row)], scale);
 }
+template <typename R, int L, int T>
+__global__ void __launch_bounds__(Geo<L>::P *T) k_z1_f(ZArgs a, const cplx_t<R> *tw_all) {
+  z1_f_body<R, L, T>(a, tw_all, xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y), gridDim.x);
+}
 
 // Register-resident scalar z pass: thread (col, p) of a block owns column
 // kx0 + col (of TC consecutive ones, TC * sizeof(V) contiguous bytes per z
@@ -648,14 +667,13 @@ __global__ void __launch_bounds__(Geo<L>::P *T) k_z1_f(ZArgs a, const cplx_t<R> 
 // upper half in the first forward and the last inverse stage.  The host
 // guarantees Mz * rs complex words fit a 31-bit byte offset.
 template <typename R, int L, int TC, bool ALLPER, bool HALF>
-__global__ void __launch_bounds__(Geo<L>::P *TC) k_z1_r(ZArgs a, const cplx_t<R> *tw_all) {
+__device__ __forceinline__ void z1_r_body(const ZArgs &a, const cplx_t<R> *tw_all, int64_t lt, int gx) {
   using V = cplx_t<R>;
   constexpr int P = Geo<L>::P, E = L / P;
   constexpr int ST = reg_line_stride<L, sizeof(V), TC>();
   __shared__ V lds[TC * ST];
   const V *tw = tw_all + a.plan.tw_off;
-  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
-  const int kx0 = (int)(lt % gridDim.x) * TC, ky = (int)(lt / gridDim.x);
+  const int kx0 = (int)(lt % gx) * TC, ky = (int)(lt / gx);
   const int tid = threadIdx.x;
   const int col = tid % TC, p = tid / TC;
   Twid<L, V> twd;
@@ -709,6 +727,70 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z1_r(ZArgs a, const cplx_t<R>
     for (int m = 0; m < E; ++m)
       if (HALF ? m < E / 2 : p + P * m < a.Mz) dst[(int64_t)P * m * a.rs] = cscale(w[m], scale);
   }
+}
+template <typename R, int L, int TC, bool ALLPER, bool HALF>
+__global__ void __launch_bounds__(Geo<L>::P *TC) k_z1_r(ZArgs a, const cplx_t<R> *tw_all) {
+  z1_r_body<R, L, TC, ALLPER, HALF>(a, tw_all, xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y), gridDim.x);
+}
+
+// ------------------------------------------------------------ group kernels
+// One launch runs one kernel instance over several patches: entry e of the
+// table covers global tiles [off, off + tiles) of patch e's own grid (gx
+// tiles per row) with its own arguments and twiddle table.  The global tile
+// comes from the XCD-aware block mapping; the entry by binary search over
+// the offsets (uniform per block).
+// the block's entry, its arguments and its tile within the patch's grid
+template <typename A> struct GrpTile {
+  A a;
+  const void *tw;
+  int64_t lt;
+  int gx;
+};
+template <typename A>
+__device__ __forceinline__ GrpTile<A> grp_tile(const GrpEntry<A> *__restrict__ t, int n, int64_t nb) {
+  const int64_t g = xcd_tile(block_linear(), nb);
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t[mid].off <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return GrpTile<A>{t[lo].a, t[lo].tw, g - t[lo].off, (int)t[lo].gx};
+}
+
+template <typename R, int L, int T, int S>
+__global__ void __launch_bounds__(Geo<L>::P *T)
+    kg_strided_f(const GrpEntry<StridedArgs> *__restrict__ t, int n, int64_t nb) {
+  const GrpTile<StridedArgs> g = grp_tile(t, n, nb);
+  strided_f_body<R, L, T, S>(g.a, reinterpret_cast<const cplx_t<R> *>(g.tw), g.lt, g.gx);
+}
+template <typename R, int L, int T, int S, bool HALF>
+__global__ void __launch_bounds__(Geo<L>::P *T)
+    kg_strided_r(const GrpEntry<StridedArgs> *__restrict__ t, int n, int64_t nb) {
+  const GrpTile<StridedArgs> g = grp_tile(t, n, nb);
+  strided_r_body<R, L, T, S, HALF>(g.a, reinterpret_cast<const cplx_t<R> *>(g.tw), g.lt, g.gx);
+}
+template <typename R, int NH, int T, int NC>
+__global__ void __launch_bounds__(Geo<NH>::P *T *NC)
+    kg_x_fwd_f(const GrpEntry<XFwdArgs> *__restrict__ t, int n, int64_t nb) {
+  const GrpTile<XFwdArgs> g = grp_tile(t, n, nb);
+  x_fwd_body<R, NH, T, NC>(g.a, reinterpret_cast<const cplx_t<R> *>(g.tw), g.lt, g.gx);
+}
+template <typename R, int NH, int T, int NC>
+__global__ void __launch_bounds__(Geo<NH>::P *T *NC)
+    kg_x_inv_f(const GrpEntry<XInvArgs> *__restrict__ t, int n, int64_t nb) {
+  const GrpTile<XInvArgs> g = grp_tile(t, n, nb);
+  x_inv_body<R, NH, T, NC>(g.a, reinterpret_cast<const cplx_t<R> *>(g.tw), g.lt, g.gx);
+}
+template <typename R, int L, int T>
+__global__ void __launch_bounds__(Geo<L>::P *T) kg_z1_f(const GrpEntry<ZArgs> *__restrict__ t, int n, int64_t nb) {
+  const GrpTile<ZArgs> g = grp_tile(t, n, nb);
+  z1_f_body<R, L, T>(g.a, reinterpret_cast<const cplx_t<R> *>(g.tw), g.lt, g.gx);
+}
+template <typename R, int L, int TC, bool ALLPER, bool HALF>
+__global__ void __launch_bounds__(Geo<L>::P *TC) kg_z1_r(const GrpEntry<ZArgs> *__restrict__ t, int n, int64_t nb) {
+  const GrpTile<ZArgs> g = grp_tile(t, n, nb);
+  z1_r_body<R, L, TC, ALLPER, HALF>(g.a, reinterpret_cast<const cplx_t<R> *>(g.tw), g.lt, g.gx);
 }
 
 }  // namespace fx

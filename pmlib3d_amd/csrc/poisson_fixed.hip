@@ -198,6 +198,82 @@ template <typename R, int L> int z1_L(int T, ZArgs a, int Ly, const cplx_t<R> *t
   }
 }
 
+// ---- group launches (one kernel instance over several patches; nb tiles)
+template <typename R, int NH> int grp_xfwd_L(const GrpEntry<XFwdArgs> *t, int n, int64_t nb, hipStream_t st) {
+  constexpr int T = tile_lines<R, NH, 1>();
+  hipLaunchKernelGGL((kg_x_fwd_f<R, NH, T, 1>), dim3((unsigned)nb), dim3(Geo<NH>::P * T), 0, st, t, n, nb);
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+template <typename R, int NH> int grp_xinv_L(const GrpEntry<XInvArgs> *t, int n, int64_t nb, hipStream_t st) {
+  constexpr int T = tile_lines<R, NH, 1>();
+  hipLaunchKernelGGL((kg_x_inv_f<R, NH, T, 1>), dim3((unsigned)nb), dim3(Geo<NH>::P * T), 0, st, t, n, nb);
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+// the same kernel choice as strided_L
+template <typename R, int L>
+int grp_strided_L(int S, int kernel, bool half, const GrpEntry<StridedArgs> *t, int n, int64_t nb, hipStream_t st) {
+  constexpr int T = tile_cols<R, L>();
+  const dim3 grid((unsigned)nb), block(Geo<L>::P * T);
+  if constexpr (regs_ok<L>()) {
+    if (kernel == 0) {
+      if (S < 0 && half)
+        hipLaunchKernelGGL((kg_strided_r<R, L, T, -1, true>), grid, block, 0, st, t, n, nb);
+      else if (S < 0)
+        hipLaunchKernelGGL((kg_strided_r<R, L, T, -1, false>), grid, block, 0, st, t, n, nb);
+      else if (half)
+        hipLaunchKernelGGL((kg_strided_r<R, L, T, 1, true>), grid, block, 0, st, t, n, nb);
+      else
+        hipLaunchKernelGGL((kg_strided_r<R, L, T, 1, false>), grid, block, 0, st, t, n, nb);
+      return hipGetLastError() == hipSuccess ? 1 : -1;
+    }
+  }
+  if (S < 0)
+    hipLaunchKernelGGL((kg_strided_f<R, L, T, -1>), grid, block, 0, st, t, n, nb);
+  else
+    hipLaunchKernelGGL((kg_strided_f<R, L, T, 1>), grid, block, 0, st, t, n, nb);
+  return hipGetLastError() == hipSuccess ? 1 : -1;
+}
+// the same kernel choice and availability as z1_T / z1r_T
+template <typename R, int L, int T>
+int grp_z1_T(const GrpEntry<ZArgs> *t, int n, int64_t nb, hipStream_t st) {
+  if constexpr (T * Geo<L>::ST * (int)sizeof(cplx_t<R>) > kLdsBudget || Geo<L>::P * T > 1024) {
+    return 0;
+  } else {
+    hipLaunchKernelGGL((kg_z1_f<R, L, T>), dim3((unsigned)nb), dim3(Geo<L>::P * T), 0, st, t, n, nb);
+    return hipGetLastError() == hipSuccess ? 1 : -1;
+  }
+}
+template <typename R, int L, int TC>
+int grp_z1r_T(bool allper, bool half, const GrpEntry<ZArgs> *t, int n, int64_t nb, hipStream_t st) {
+  if constexpr (!regs_ok<L>() || TC * reg_line_stride<L, sizeof(cplx_t<R>), TC>() * (int)sizeof(cplx_t<R>) > kLdsBudget ||
+                Geo<L>::P * TC > 1024) {
+    return 0;
+  } else {
+    const dim3 grid((unsigned)nb), block(Geo<L>::P * TC);
+    if (allper)
+      hipLaunchKernelGGL((kg_z1_r<R, L, TC, true, false>), grid, block, 0, st, t, n, nb);
+    else if (half)
+      hipLaunchKernelGGL((kg_z1_r<R, L, TC, false, true>), grid, block, 0, st, t, n, nb);
+    else
+      hipLaunchKernelGGL((kg_z1_r<R, L, TC, false, false>), grid, block, 0, st, t, n, nb);
+    return hipGetLastError() == hipSuccess ? 1 : -1;
+  }
+}
+template <typename R, int L>
+int grp_z1_L(int T, bool allper, bool half, const GrpEntry<ZArgs> *t, int n, int64_t nb, hipStream_t st) {
+  switch (T) {
+    case 1: return grp_z1_T<R, L, 1>(t, n, nb, st);
+    case 2: return grp_z1_T<R, L, 2>(t, n, nb, st);
+    case 4: return grp_z1_T<R, L, 4>(t, n, nb, st);
+    case 8: return grp_z1_T<R, L, 8>(t, n, nb, st);
+    case 16: return grp_z1_T<R, L, 16>(t, n, nb, st);
+    case kZ1Reg + 4: return grp_z1r_T<R, L, 4>(allper, half, t, n, nb, st);
+    case kZ1Reg + 8: return grp_z1r_T<R, L, 8>(allper, half, t, n, nb, st);
+    case kZ1Reg + 16: return grp_z1r_T<R, L, 16>(allper, half, t, n, nb, st);
+    default: return 0;
+  }
+}
+
 #define PM_FX_LENGTHS(X) \
   X(16) X(32) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024)
 
@@ -250,6 +326,88 @@ template <typename R> int fx_xinv(int NH, XInvArgs a, const cplx_t<R> *tw, hipSt
   }
 }
 
+// lines per block of the one-component x passes, columns per block of the
+// strided passes (0: length not specialised); whether L / P is even (the
+// zero-half skipping of the register kernels applies)
+template <typename R> int fx_tile_x1(int NH) {
+  switch (NH) {
+#define X(n) \
+  case n: return fx::tile_lines<R, n, 1>();
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+template <typename R> int fx_tile_cols(int L) {
+  switch (L) {
+#define X(n) \
+  case n: return fx::tile_cols<R, n>();
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+int fx_even_rows(int L) {
+  switch (L) {
+#define X(n) \
+  case n: return (n / fx::Geo<n>::P) % 2 == 0 ? 1 : 0;
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+template <typename R> int fx_grp_xfwd(int NH, const GrpEntry<XFwdArgs> *t, int n, int64_t nb, hipStream_t st) {
+  switch (NH) {
+#define X(v) \
+  case v: return fx::grp_xfwd_L<R, v>(t, n, nb, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+template <typename R> int fx_grp_xinv(int NH, const GrpEntry<XInvArgs> *t, int n, int64_t nb, hipStream_t st) {
+  switch (NH) {
+#define X(v) \
+  case v: return fx::grp_xinv_L<R, v>(t, n, nb, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+template <typename R>
+int fx_grp_strided(int L, int S, int kernel, bool half, const GrpEntry<StridedArgs> *t, int n, int64_t nb,
+                   hipStream_t st) {
+  switch (L) {
+#define X(v) \
+  case v: return fx::grp_strided_L<R, v>(S, kernel, half, t, n, nb, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+template <typename R>
+int fx_grp_z1(int L, int T, bool allper, bool half, const GrpEntry<ZArgs> *t, int n, int64_t nb, hipStream_t st) {
+  switch (L) {
+#define X(v) \
+  case v: return fx::grp_z1_L<R, v>(T, allper, half, t, n, nb, st);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
+
+template int fx_tile_x1<float>(int);
+template int fx_tile_x1<double>(int);
+template int fx_tile_cols<float>(int);
+template int fx_tile_cols<double>(int);
+template int fx_grp_xfwd<float>(int, const GrpEntry<XFwdArgs> *, int, int64_t, hipStream_t);
+template int fx_grp_xfwd<double>(int, const GrpEntry<XFwdArgs> *, int, int64_t, hipStream_t);
+template int fx_grp_xinv<float>(int, const GrpEntry<XInvArgs> *, int, int64_t, hipStream_t);
+template int fx_grp_xinv<double>(int, const GrpEntry<XInvArgs> *, int, int64_t, hipStream_t);
+template int fx_grp_strided<float>(int, int, int, bool, const GrpEntry<StridedArgs> *, int, int64_t, hipStream_t);
+template int fx_grp_strided<double>(int, int, int, bool, const GrpEntry<StridedArgs> *, int, int64_t, hipStream_t);
+template int fx_grp_z1<float>(int, int, bool, bool, const GrpEntry<ZArgs> *, int, int64_t, hipStream_t);
+template int fx_grp_z1<double>(int, int, bool, bool, const GrpEntry<ZArgs> *, int, int64_t, hipStream_t);
 template int fx_strided<float>(int, int, StridedArgs, int, int, const float2 *, hipStream_t);
 template int fx_strided<double>(int, int, StridedArgs, int, int, const double2 *, hipStream_t);
 template int fx_z<float>(int, ZArgs, int, const float2 *, hipStream_t);

@@ -180,6 +180,16 @@ __global__ void __launch_bounds__(1024) k_x_inv(XInvArgs a, const cplx_t<R> *tw)
 // Fused third-direction pass: zero pad, forward FFT of the 3 vorticity
 // components, G-hat product + spectral curl (+ psi, + regularised /
 // reprojected vorticity), inverse FFTs, truncation to Mz rows, 1/Lz.
+// One patch's share of a group launch (one kernel instance run over several
+// patches): its arguments, its twiddle table, its first tile in the launch's
+// global tile order and its tiles per grid row.
+template <typename A> struct GrpEntry {
+  A a;
+  const void *tw;
+  int64_t off;
+  int64_t gx;
+};
+
 // scalar z-pass choice: T < kZ1Reg the LDS-resident kernel with T columns
 // per block, kZ1Reg + TC the register-resident one with TC columns per block
 constexpr int kZ1Reg = 100;

@@ -81,10 +81,13 @@ def test_scalar_z_kernels_match_oracle_psi(case, kern, monkeypatch):
         assert rel_l2(got32, psi[2]) <= 1e-5
 
 
+@pytest.mark.parametrize("grouped", [1, 0], ids=["grouped", "per_patch"])
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
-def test_batch_graph_matches_individual_solves_on_every_replay(precision):
+def test_batch_graph_matches_individual_solves_on_every_replay(precision, grouped, monkeypatch):
     import torch
     import pmlib3d_amd as pm
+    print("precision", precision, "grouped", grouped, flush=True)
+    monkeypatch.setenv("PMGPU_BATCH_GROUPED", str(grouped))
     rng = np.random.default_rng(4)
     sizes = [tuple(int(v) for v in rng.choice([16, 32, 48], 3)) for _ in range(7)]
     dev = torch.device("cuda:0")
@@ -119,3 +122,48 @@ def test_batch_graph_matches_individual_solves_on_every_replay(precision):
     torch.cuda.synchronize()
     for o, r in zip(outs2, refs):
         assert torch.equal(o, r)
+
+
+@pytest.mark.parametrize("grouped", [1, 0], ids=["grouped", "per_patch"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_batch_mixed_patches_match_oracle(precision, grouped, monkeypatch):
+    """Batch over free, ghosted, mixed and periodic patches of several
+    lengths, in both forms (grouped: groups of one and of several patches,
+    periodic ghost fills after the group launches): each patch's psi against
+    the oracle."""
+    import torch
+    import pmlib3d_amd as pm
+    print("precision", precision, "grouped", grouped, flush=True)
+    monkeypatch.setenv("PMGPU_BATCH_GROUPED", str(grouped))
+    dev = torch.device("cuda:0")
+    dt = torch.float64 if precision == 0 else torch.float32
+    cases = [((32, 32, 32), 0, (0, 0, 0), 4), ((32, 24, 32), 0, (0, 0, 0), 4), ((28, 24, 28), 2, (0, 0, 0), 4),
+             ((32, 32, 32), 0, (0, 0, 1), 4), ((32, 16, 32), 0, (1, 1, 1), 4), ((48, 32, 16), 0, (0, 0, 0), 4),
+             ((32, 32, 32), 0, (0, 0, 0), 4)]
+    solvers, ins, outs, refs = [], [], [], []
+    for i, (ncell, ng, bc, order) in enumerate(cases):
+        print("patch", i, ncell, ng, bc, flush=True)
+        g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+        p = O.make_params(order, 1, 2.0)
+        w3 = blob_field(g, seed=11 + i)
+        w3[0] = 0.0
+        w3[1] = 0.0
+        _, _, psi = O.poisson_solve(g, p, O.poisson_setup(g, p), w3, False, False, True)
+        solvers.append(_solver(g, p, precision))
+        w = torch.from_numpy(np.asfortranarray(w3[2]).ravel(order="F")).to(dev).to(dt)
+        ins.append(w)
+        outs.append(torch.full_like(w, float("nan")))
+        refs.append(psi[2])
+    batch = pm.PoissonBatch(solvers)
+    for rep in range(3):
+        batch.solve_scalar([t.data_ptr() for t in ins], [t.data_ptr() for t in outs])
+        torch.cuda.synchronize()
+        for i, (o, r) in enumerate(zip(outs, refs)):
+            got = o.cpu().numpy().astype(np.float64).reshape(r.shape, order="F")
+            if precision == 0:
+                assert rel_linf(got, r) <= 1e-12, (rep, i)
+            else:
+                assert rel_l2(got, r) <= 1e-5, (rep, i)
+        for o in outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
