@@ -336,6 +336,27 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
   PM_HIP(hipGetLastError());
   PartWS &w = ws();
   int64_t n = np;
+  if (n > 0) {
+    // one pass over the three axes first: when no particle is outside the
+    // patch on any axis, no axis below moves or drops one (its classification
+    // sees the same positions), so one read-back decides for all three
+    PBox box;
+    for (int d = 0; d < 3; ++d) {
+      box.lo[d] = geom->xmin[d];
+      box.hi[d] = geom->xmax[d];
+    }
+    if (w.count.ensure(sizeof(int) * 8)) return -1;
+    PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_any_outside, grid_for(n), dim3(256), 0, st, n, arr[0], box, w.count.as<int>());
+    PM_HIP(hipGetLastError());
+    int any = 0;
+    PM_HIP(hipMemcpyAsync(&any, w.count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    PM_HIP(hipStreamSynchronize(st));
+    if (any == 0) {
+      *np_out = n;
+      return 0;
+    }
+  }
   for (int d = 0; d < 3 && n > 0; ++d) {
     const double lo = geom->xmin[d], hi = geom->xmax[d], len = geom->xmax[d] - geom->xmin[d];
     const int per = geom->bound_cond[d] != 0;

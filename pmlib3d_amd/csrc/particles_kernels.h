@@ -584,6 +584,22 @@ __global__ void k_rk_update(RkCoef k, double dt, int64_t np, double *pos, const 
 }
 
 // migration class along axis d: 0 stay, 1 west (wrap), 2 east (wrap), 3 drop
+// any particle outside [lo, hi) on any axis (flag set to 1)
+struct PBox {
+  double lo[3], hi[3];
+};
+__global__ void k_any_outside(int64_t np, const double *pos, PBox b, int *flag) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    bool out = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double x = pos[3 * q + d];
+      out = out || x < b.lo[d] || x >= b.hi[d];
+    }
+    if (out) atomicOr(flag, 1);
+  }
+}
+
 __global__ void k_classify(int64_t np, const double *pos, int d, double lo, double hi, int per, int *cls,
                            int *counts) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {

@@ -279,6 +279,35 @@ def test_advance_multiple_wraps_and_empty():
     assert _advance(g, 2, 1, 0.1, empty)[0].shape == (3, 0)
 
 
+@pytest.mark.parametrize("bc", [(0, 0, 0), (1, 1, 1)])
+def test_advance_nobody_leaves_and_last_axis_only(bc):
+    """The single all-axes check: no particle outside (arrays as the RK
+    update leaves them), and particles leaving through z only (the per-axis
+    migration must still run for z)."""
+    g = O.make_geom((12, 10, 8), 1, [-0.5, -0.4, -0.3], 1.0 / 10, bc, 3)
+    rng = np.random.default_rng(31 + sum(bc))
+    n = 4000
+    lo, hi = np.array(g.xmin), np.array(g.xmax)
+    pos = np.asfortranarray(rng.uniform(lo + 0.05, hi - 0.05, size=(n, 3)).T)
+    rest = [np.asfortranarray(rng.normal(size=(3, n)) * s) for s in (1.0, 1.0, 0.5, 0.3, 0.2)]
+    arrs = [pos] + rest
+    got = _advance(g, 1, 1, 1e-3, arrs)
+    ref = O.advance(g, 1, 1, 1e-3, *arrs)
+    for a, b in zip(got, ref):
+        assert a.shape == (3, n) and np.array_equal(a, b)
+    vel = np.zeros((3, n), order="F")
+    vel[2] = np.where(np.arange(n) % 7 == 0, 400.0, 0.0)
+    arrs = [pos, vel] + rest[1:]
+    got = _advance(g, 1, 1, 1e-3, arrs)
+    ref = O.advance(g, 1, 1, 1e-3, *arrs)
+    for a, b in zip(got, ref):
+        assert a.shape == b.shape and np.array_equal(a, b)
+    if bc[2]:
+        assert got[0].shape[1] == n
+    else:
+        assert 0 < got[0].shape[1] < n
+
+
 # ------------------------------------------------------------ device API
 def test_device_entry_points_match_host_entry_points():
     import torch
