@@ -583,7 +583,6 @@ __global__ void k_rk_update(RkCoef k, double dt, int64_t np, double *pos, const 
   }
 }
 
-// migration class along axis d: 0 stay, 1 west (wrap), 2 east (wrap), 3 drop
 // any particle outside [lo, hi) on any axis (flag set to 1)
 struct PBox {
   double lo[3], hi[3];
@@ -600,6 +599,7 @@ __global__ void k_any_outside(int64_t np, const double *pos, PBox b, int *flag) 
   }
 }
 
+// migration class along axis d: 0 stay, 1 west (wrap), 2 east (wrap), 3 drop
 __global__ void k_classify(int64_t np, const double *pos, int d, double lo, double hi, int per, int *cls,
                            int *counts) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
