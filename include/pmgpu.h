@@ -206,6 +206,26 @@ int pmgpu_penalise_device(const pmgpu_geom *g, int32_t precision, const void *d_
                           const void *d_solid_vel, const double us_const[3], double relax,
                           int32_t fd_order, void *d_vort, double *diag, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Vorticity right-hand side (stretching and diffusion)
+ * ------------------------------------------------------------------------ */
+
+/* part_deriv_vorticity(topo, mesh, ierr)
+ * (vortex_codes/vortex_penalisation/source/particle_derivatives/
+ * part_deriv_vorticity.f:6, body :57-208): on the interior cells
+ * dvort_l = visc (d2 w_l/dx2 + d2 w_l/dy2 + d2 w_l/dz2) + w1 du_l/dx +
+ * w2 du_l/dy + w3 du_l/dz with central differences of order fd_order
+ * (2 or 4; pmlib_fd_order), visc being the client's viscosity; then the
+ * periodic ghost map of dvort.  vort, vel and dvort are (3, E...) meshes;
+ * dvort's ghost cells on free sides keep their values.  Needs fd_order / 2
+ * ghost layers on every side. */
+int pmgpu_vorticity_rhs(const pmgpu_geom *g, const double *vort, const double *vel, double visc,
+                        int32_t fd_order, double *dvort);
+/* Device variant in FP64 or FP32 (precision), stream-ordered; dvort must not
+ * alias vort or vel. */
+int pmgpu_vorticity_rhs_device(const pmgpu_geom *g, int32_t precision, const void *d_vort, const void *d_vel,
+                               double visc, int32_t fd_order, void *d_dvort, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

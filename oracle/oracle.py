@@ -88,6 +88,8 @@ def lib():
         L.or_advance.restype = C.c_int64
         L.or_penalise.argtypes = [G, P, P, P, C.c_double, C.c_int, P, P]
         L.or_penalise.restype = C.c_int
+        L.or_vort_rhs.argtypes = [G, P, P, C.c_double, C.c_int, P]
+        L.or_vort_rhs.restype = C.c_int
         _lib = L
     return _lib
 
@@ -205,6 +207,18 @@ def penalise(g, vel, chi, vort, solid_vel=None, relax=1.0, fd_order=4):
     if lib().or_penalise(g, _p(vel), _p(chi), _p(sv), relax, int(fd_order), _p(vort), _p(diag)):
         raise ValueError("penalise needs fd_order 2 or 4 and fd_order/2 ghost layers")
     return vort, diag
+
+
+def vort_rhs(g, vort, vel, visc, fd_order=4, dvort=None):
+    """Vorticity right-hand side visc lap(w) + (w . grad) u on the interior
+    cells, periodic ghosts mapped; free-side ghosts of dvort (zeros when not
+    given) are left as they are."""
+    vort = np.asfortranarray(vort, dtype=np.float64)
+    vel = np.asfortranarray(vel, dtype=np.float64)
+    out = np.zeros_like(vort, order="F") if dvort is None else np.array(dvort, dtype=np.float64, order="F", copy=True)
+    if lib().or_vort_rhs(g, _p(vort), _p(vel), float(visc), int(fd_order), _p(out)):
+        raise ValueError("vort_rhs needs fd_order 2 or 4 and fd_order/2 ghost layers")
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -24,6 +24,9 @@
  *   or_penalise        vortex_codes/vortex_penalisation/source/penalisation/
  *                      penalisation_solve.f:96-111,146-273 (one iteration,
  *                      2nd/4th-order curl, Brinkman factor per cell)
+ *   or_vort_rhs        vortex_codes/vortex_penalisation/source/
+ *                      particle_derivatives/part_deriv_vorticity.f:57-208
+ *                      (stretching + diffusion, 2nd/4th-order FD, ghost map)
  *
  * Array layout is the reference's: meshes are (3, i, j, k) with the component
  * fastest and ghost layers included, particles are (3, np).  Fixed-size fp64
@@ -31,6 +34,7 @@
  */
 #include <complex.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -850,5 +854,78 @@ int or_penalise(const or_geom *g, const double *vel, const double *chi, const do
   }
   free(f);
   diag[0] = F0; diag[1] = F1; diag[2] = F2; diag[3] = re; diag[4] = se;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Vorticity right-hand side                                                 */
+
+/* periodic forward ghost fill of a (3, E...) mesh, x then y then z
+ * (mesh/pmlib_mesh_map_ghost.f, incl_edges = .FALSE., op = replace) */
+static void ghost_fill_periodic(const or_geom *g, const int *E, double *m) {
+  for (int d = 0; d < 3; ++d) {
+    if (g->bound_cond[d] == 0) continue;
+    int ng = ng_side(g, d), N = g->ncell[d];
+    size_t st[3] = {3, 3 * (size_t)E[0], 3 * (size_t)E[0] * E[1]};
+    int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+    for (int b = 0; b < E[o2]; ++b)
+      for (int a = 0; a < E[o1]; ++a)
+        for (int c = 0; c < 3; ++c) {
+          size_t base = (size_t)c + st[o1] * a + st[o2] * b;
+          for (int t = 0; t < ng; ++t) {
+            m[base + st[d] * t] = m[base + st[d] * (N + t)];
+            m[base + st[d] * (ng + N + t)] = m[base + st[d] * (ng + t)];
+          }
+        }
+  }
+}
+
+/* Right-hand side of the vorticity equation on the interior cells
+ * (part_deriv_vorticity.f:57-196), for each component l:
+ *   dvort_l = visc (d2w_l/dx2 + d2w_l/dy2 + d2w_l/dz2)
+ *             + w1 du_l/dx + w2 du_l/dy + w3 du_l/dz
+ * fd_order 2: (1, -2, 1) / dx^2 and (-1, 1) / (2 dx) (:57-115);
+ * fd_order 4: (c1, c2, -2.5, c2, c1) / dx^2 with c1 = -1/12, c2 = 4/3 and
+ * (c3, -c4, c4, -c3) / (2 dx) with c3 = 1/6, c4 = 4/3 (:117-196); the
+ * reference's operation order, left to right.  Then the periodic ghost map of
+ * dvort (:201-208); free-side ghost cells of dvort are left unchanged.
+ * Needs fd_order / 2 ghost layers on every side. */
+int or_vort_rhs(const or_geom *g, const double *vort, const double *vel, double visc, int fd_order,
+                double *dvort) {
+  int E[3], ngl[3];
+  for (int d = 0; d < 3; ++d) { E[d] = mesh_ext(g, d); ngl[d] = ng_side(g, d); }
+  if (fd_order != 2 && fd_order != 4) return -1;
+  const int need = fd_order / 2;
+  if (ngl[0] < need || ngl[1] < need || ngl[2] < need) return -1;
+  double cdx2[3], c2dx[3];
+  for (int d = 0; d < 3; ++d) {
+    cdx2[d] = 1.0 / (g->dx[d] * g->dx[d]);
+    c2dx[d] = 1.0 / (2.0 * g->dx[d]);
+  }
+  const double c1 = -1.0 / 12.0, c2 = 4.0 / 3.0, c3 = 1.0 / 6.0, c4 = 4.0 / 3.0;
+  const size_t st[3] = {3, 3 * (size_t)E[0], 3 * (size_t)E[0] * E[1]};
+  for (int k = 0; k < g->ncell[2]; ++k)
+    for (int j = 0; j < g->ncell[1]; ++j)
+      for (int i = 0; i < g->ncell[0]; ++i) {
+        size_t o = 3 * ((size_t)(i + ngl[0]) + (size_t)E[0] * ((j + ngl[1]) + (size_t)E[1] * (k + ngl[2])));
+        double w1 = vort[o], w2 = vort[o + 1], w3 = vort[o + 2];
+        for (int l = 0; l < 3; ++l) {
+          double dd[3], du[3];
+          for (int d = 0; d < 3; ++d) {
+            const double *a = vort + o + l, *u = vel + o + l;
+            size_t s = st[d];
+            if (fd_order == 2) {
+              dd[d] = (1.0 * a[-(ptrdiff_t)s] - 2.0 * a[0] + 1.0 * a[s]) * cdx2[d];
+              du[d] = (-1.0 * u[-(ptrdiff_t)s] + 1.0 * u[s]) * c2dx[d];
+            } else {
+              dd[d] = (c1 * a[-2 * (ptrdiff_t)s] + c2 * a[-(ptrdiff_t)s] - 2.5 * a[0] + c2 * a[s] + c1 * a[2 * s]) *
+                      cdx2[d];
+              du[d] = (c3 * u[-2 * (ptrdiff_t)s] - c4 * u[-(ptrdiff_t)s] + c4 * u[s] - c3 * u[2 * s]) * c2dx[d];
+            }
+          }
+          dvort[o + l] = visc * (dd[0] + dd[1] + dd[2]) + w1 * du[0] + w2 * du[1] + w3 * du[2];
+        }
+      }
+  ghost_fill_periodic(g, E, dvort);
   return 0;
 }

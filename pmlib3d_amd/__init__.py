@@ -3,8 +3,8 @@
 Python mirror of the reference's Fortran interface for the hot path
 (the types class_patch / class_mesh / class_particles and the subroutines
 pmlib_poisson_setup, pmlib_poisson_solve, pmlib_interp_particle_mesh,
-pmlib_interp_mesh_particle, pmlib_remesh, pmlib_particles_advance and the
-Brinkman penalisation step).  Every routine runs the HIP kernels of
+pmlib_interp_mesh_particle, pmlib_remesh, pmlib_particles_advance, the
+Brinkman penalisation step and the vorticity right-hand side).  Every routine runs the HIP kernels of
 libpmgpu.so through its C ABI; subroutines return ierr (0 or -1) and write
 "[rank](caller) : message" on failure, as pmlib_write does.
 """
@@ -13,6 +13,7 @@ from .core import (Patch, Mesh, Particles, Parameters, params, pmlib_write, mesh
 from .particles import (cell_index, pmlib_interp_particle_mesh, pmlib_interp_mesh_particle,
                         interp_mesh_field, pmlib_remesh, pmlib_particles_advance, ParticleOps)
 from .penalisation import penalise, penalise_device, sphere_mask, brinkman_factor
+from .derivatives import vorticity_rhs, vorticity_rhs_device, part_deriv_vorticity
 from .step import VortexStep
 from ._lib import PmgpuError, FP32, FP64
 
@@ -20,4 +21,5 @@ __all__ = ["Patch", "Mesh", "Particles", "Parameters", "params", "pmlib_write", 
            "pmlib_poisson_setup", "pmlib_poisson_solve", "PoissonSolver", "PoissonBatch", "PmgpuError",
            "FP32", "FP64", "cell_index", "pmlib_interp_particle_mesh", "pmlib_interp_mesh_particle",
            "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps", "penalise",
-           "penalise_device", "sphere_mask", "brinkman_factor", "VortexStep"]
+           "penalise_device", "sphere_mask", "brinkman_factor", "vorticity_rhs", "vorticity_rhs_device",
+           "part_deriv_vorticity", "VortexStep"]

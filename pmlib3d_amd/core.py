@@ -28,6 +28,7 @@ class Parameters:
     pmlib_poisson_order: int = -99
     pmlib_interpolation_order: int = -99
     pmlib_time_integration_order: int = -99
+    pmlib_fd_order: int = -99
     mpi_rank: int = 0
 
 

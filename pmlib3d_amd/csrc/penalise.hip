@@ -21,16 +21,6 @@ struct BGeom {
   double xmin[3], dx[3];
 };
 
-template <typename T> __device__ __forceinline__ T rmul(T a, T b);
-template <typename T> __device__ __forceinline__ T radd(T a, T b);
-template <typename T> __device__ __forceinline__ T rsub(T a, T b);
-template <> __device__ __forceinline__ double rmul(double a, double b) { return __dmul_rn(a, b); }
-template <> __device__ __forceinline__ double radd(double a, double b) { return __dadd_rn(a, b); }
-template <> __device__ __forceinline__ double rsub(double a, double b) { return __dsub_rn(a, b); }
-template <> __device__ __forceinline__ float rmul(float a, float b) { return __fmul_rn(a, b); }
-template <> __device__ __forceinline__ float radd(float a, float b) { return __fadd_rn(a, b); }
-template <> __device__ __forceinline__ float rsub(float a, float b) { return __fsub_rn(a, b); }
-
 template <typename T> struct PenArgs {
   BGeom g;
   const T *vel, *chi, *us;  // us: (3, E...) field or null (then us0..2)

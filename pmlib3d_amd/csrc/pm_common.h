@@ -52,6 +52,18 @@ const std::string &get_error();
     }                                \
   } while (0)
 
+// explicitly rounded arithmetic (no contraction into multiply-adds): the
+// stencil kernels that reproduce the reference's rounding use these
+template <typename T> __device__ __forceinline__ T rmul(T a, T b);
+template <typename T> __device__ __forceinline__ T radd(T a, T b);
+template <typename T> __device__ __forceinline__ T rsub(T a, T b);
+template <> __device__ __forceinline__ double rmul(double a, double b) { return __dmul_rn(a, b); }
+template <> __device__ __forceinline__ double radd(double a, double b) { return __dadd_rn(a, b); }
+template <> __device__ __forceinline__ double rsub(double a, double b) { return __dsub_rn(a, b); }
+template <> __device__ __forceinline__ float rmul(float a, float b) { return __fmul_rn(a, b); }
+template <> __device__ __forceinline__ float radd(float a, float b) { return __fadd_rn(a, b); }
+template <> __device__ __forceinline__ float rsub(float a, float b) { return __fsub_rn(a, b); }
+
 // free the workspaces of particles.hip and penalise.hip
 void particles_release();
 void penalise_release();

@@ -1,0 +1,154 @@
+// Right-hand side of the vorticity equation on the mesh (HIP, gfx950):
+//   dvort_l = visc lap(w_l) + w . grad(u_l)   on the interior cells,
+// then the periodic ghost map of dvort.
+//
+// Restates part_deriv_vorticity (vortex_codes/vortex_penalisation/source/
+// particle_derivatives/part_deriv_vorticity.f:57-115 for the 2nd-order and
+// :117-196 for the 4th-order stencils, :201-208 for the ghost map).  One
+// thread per interior cell forms all three components; the stencil
+// neighbours along i belong to the adjacent lanes, so the re-reads hit L1/L2.
+// Every floating-point operation is an explicitly rounded one, in the order
+// the reference writes it, so the FP64 result is bit-identical to a
+// left-to-right evaluation of the reference's expressions.  Ghost cells of
+// dvort on free sides are not written.
+#include "../../include/pmgpu.h"
+#include "pm_common.h"
+#include "poisson_kernels.h"
+
+namespace pmgpu {
+
+template <typename T> struct RhsArgs {
+  const T *w, *u;
+  T *dw;
+  int E[3], N[3], ng[3];
+  T visc, cdx2[3], c2dx[3];  // 1 / dx^2 and 1 / (2 dx) per axis
+};
+
+template <typename T, int FD>
+__global__ void __launch_bounds__(256) k_vort_rhs(RhsArgs<T> a) {
+  const int64_t n = (int64_t)a.N[0] * a.N[1] * a.N[2];
+  const int64_t st[3] = {3, 3 * (int64_t)a.E[0], 3 * (int64_t)a.E[0] * a.E[1]};
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % a.N[0]);
+    const int64_t r = c / a.N[0];
+    const int j = (int)(r % a.N[1]), k = (int)(r / a.N[1]);
+    const int64_t o =
+        3 * ((int64_t)(i + a.ng[0]) + (int64_t)a.E[0] * ((j + a.ng[1]) + (int64_t)a.E[1] * (k + a.ng[2])));
+    const T w1 = a.w[o], w2 = a.w[o + 1], w3 = a.w[o + 2];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      T dd[3], du[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T *p = a.w + o + l, *q = a.u + o + l;
+        const int64_t s = st[d];
+        if (FD == 2) {
+          // (w(-1) - 2 w(0) + w(+1)) / dx^2 and (-u(-1) + u(+1)) / (2 dx)
+          dd[d] = rmul(radd(rsub(p[-s], rmul(T(2), p[0])), p[s]), a.cdx2[d]);
+          du[d] = rmul(radd(-q[-s], q[s]), a.c2dx[d]);
+        } else {
+          // (c1, c2, -2.5, c2, c1) / dx^2 and (c3, -c4, c4, -c3) / (2 dx)
+          const T c1 = T(-1.0 / 12.0), c2 = T(4.0 / 3.0), c3 = T(1.0 / 6.0), c4 = T(4.0 / 3.0);
+          T t = radd(rmul(c1, p[-2 * s]), rmul(c2, p[-s]));
+          t = rsub(t, rmul(T(2.5), p[0]));
+          t = radd(t, rmul(c2, p[s]));
+          t = radd(t, rmul(c1, p[2 * s]));
+          dd[d] = rmul(t, a.cdx2[d]);
+          T v = rsub(rmul(c3, q[-2 * s]), rmul(c4, q[-s]));
+          v = radd(v, rmul(c4, q[s]));
+          v = rsub(v, rmul(c3, q[2 * s]));
+          du[d] = rmul(v, a.c2dx[d]);
+        }
+      }
+      T res = radd(rmul(a.visc, radd(radd(dd[0], dd[1]), dd[2])), rmul(w1, du[0]));
+      res = radd(res, rmul(w2, du[1]));
+      res = radd(res, rmul(w3, du[2]));
+      a.dw[o + l] = res;
+    }
+  }
+}
+
+template <typename T>
+static int vort_rhs_launch(const pmgpu_geom *geom, const void *w, const void *u, double visc, int fd_order,
+                           void *dw, hipStream_t st) {
+  PM_CHECK(geom && w && u && dw, "pmgpu_vorticity_rhs: null argument");
+  PM_CHECK(fd_order == 2 || fd_order == 4, "pmgpu_vorticity_rhs: fd_order must be 2 or 4");
+  PM_CHECK(w != dw && u != dw, "pmgpu_vorticity_rhs: dvort must not alias the vorticity or velocity mesh");
+  RhsArgs<T> a;
+  int32_t ext[3];
+  if (pmgpu_mesh_extent(geom, ext)) return -1;
+  for (int d = 0; d < 3; ++d) {
+    a.E[d] = ext[d];
+    a.N[d] = geom->ncell[d];
+    a.ng[d] = (ext[d] - geom->ncell[d]) / 2;
+    PM_CHECK(a.ng[d] >= fd_order / 2, "pmgpu_vorticity_rhs: the stencil needs fd_order/2 ghost layers");
+    // part_deriv_vorticity.f:67-73: 1/dx**2 and 1/(2 dx), in the working precision
+    a.cdx2[d] = T(1.0) / (T(geom->dx[d]) * T(geom->dx[d]));
+    a.c2dx[d] = T(1.0) / (T(2.0) * T(geom->dx[d]));
+  }
+  a.w = (const T *)w;
+  a.u = (const T *)u;
+  a.dw = (T *)dw;
+  a.visc = T(visc);
+  const int64_t n = (int64_t)a.N[0] * a.N[1] * a.N[2];
+  if (n > 0) {
+    const int64_t nb = (n + 255) / 256;
+    const dim3 grid((unsigned)(nb < 65536 ? nb : 65536));
+    if (fd_order == 4)
+      hipLaunchKernelGGL((k_vort_rhs<T, 4>), grid, dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_vort_rhs<T, 2>), grid, dim3(256), 0, st, a);
+    PM_HIP(hipGetLastError());
+  }
+  // ghost map of dvort, x then y then z (periodic sides)
+  for (int d = 0; d < 3; ++d)
+    if (geom->bound_cond[d] == 1) {
+      PM_CHECK(a.N[d] >= a.ng[d], "pmgpu_vorticity_rhs: periodic axis shorter than its ghost layers");
+      hipLaunchKernelGGL(k_ghost_fill<T>, dim3(1024), dim3(256), 0, st, (T *)dw, a.E[0], a.E[1], a.E[2], d,
+                         a.N[d], a.ng[d], 3);
+      PM_HIP(hipGetLastError());
+    }
+  return 0;
+}
+
+}  // namespace pmgpu
+
+using namespace pmgpu;
+
+extern "C" {
+
+int pmgpu_vorticity_rhs(const pmgpu_geom *g, const double *vort, const double *vel, double visc, int32_t fd_order,
+                        double *dvort) {
+  PM_CHECK(g && vort && vel && dvort, "pmgpu_vorticity_rhs: null argument");
+  int32_t e[3];
+  if (pmgpu_mesh_extent(g, e)) return -1;
+  const size_t bytes = sizeof(double) * 3 * (size_t)e[0] * e[1] * e[2];
+  double *dw = nullptr, *du = nullptr, *dd = nullptr;
+  int rc = -1;
+  if (hipMalloc(&dw, bytes) == hipSuccess && hipMalloc(&du, bytes) == hipSuccess &&
+      hipMalloc(&dd, bytes) == hipSuccess && hipMemcpy(dw, vort, bytes, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(du, vel, bytes, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(dd, dvort, bytes, hipMemcpyHostToDevice) == hipSuccess) {
+    rc = vort_rhs_launch<double>(g, dw, du, visc, fd_order, dd, nullptr);
+    if (rc == 0 && hipMemcpy(dvort, dd, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+      set_error("pmgpu_vorticity_rhs: device copy failed");
+      rc = -1;
+    }
+  } else {
+    set_error("pmgpu_vorticity_rhs: device allocation or copy failed");
+  }
+  if (dw) (void)hipFree(dw);
+  if (du) (void)hipFree(du);
+  if (dd) (void)hipFree(dd);
+  return rc;
+}
+
+int pmgpu_vorticity_rhs_device(const pmgpu_geom *g, int32_t precision, const void *d_vort, const void *d_vel,
+                               double visc, int32_t fd_order, void *d_dvort, void *stream) {
+  if (precision == PMGPU_FP32)
+    return vort_rhs_launch<float>(g, d_vort, d_vel, visc, fd_order, d_dvort, (hipStream_t)stream);
+  PM_CHECK(precision == PMGPU_FP64, "pmgpu_vorticity_rhs_device: precision must be PMGPU_FP64 or PMGPU_FP32");
+  return vort_rhs_launch<double>(g, d_vort, d_vel, visc, fd_order, d_dvort, (hipStream_t)stream);
+}
+
+}  // extern "C"

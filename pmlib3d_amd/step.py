@@ -2,8 +2,9 @@
 
 This is the caller side of the boundary: the client time loop of the
 reference's vortex codes (vortex_codes/*/source/main.f) calls, per RK stage,
-pmlib_interp_particle_mesh -> pmlib_poisson_solve -> pmlib_interp_mesh_particle
--> pmlib_particles_advance, then deposits again and calls pmlib_remesh.
+pmlib_interp_particle_mesh -> pmlib_poisson_solve -> (part_deriv_vorticity) ->
+pmlib_interp_mesh_particle -> pmlib_particles_advance, then deposits again and
+calls pmlib_remesh.
 VortexStep runs that sequence with every array resident in HBM (torch
 tensors as device memory, the HIP kernels through the C ABI); only the
 particle count of the remesh crosses to the host.
@@ -12,6 +13,7 @@ import numpy as np
 
 from . import _lib
 from .core import PoissonSolver, mesh_extent
+from .derivatives import vorticity_rhs_device
 from .particles import ParticleOps
 
 _FIELDS = ("pos", "vel", "vort", "dvort", "vel_rk", "dvort_rk")
@@ -19,10 +21,14 @@ _FIELDS = ("pos", "vel", "vort", "dvort", "vel_rk", "dvort_rk")
 
 class VortexStep:
     def __init__(self, patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3, capacity=None,
-                 device="cuda:0", precision=_lib.FP64):
+                 device="cuda:0", precision=_lib.FP64, visc=None, fd_order=4):
         """precision selects the mesh side (deposition, solve, gather,
         remesh thresholds): FP64, or FP32 with Kahan-compensated interpolation
-        sums.  Particles are FP64 either way."""
+        sums.  Particles are FP64 either way.  visc: each stage also forms
+        the vorticity right-hand side visc lap(w) + (w . grad) u on the mesh
+        (finite differences of order fd_order; needs fd_order / 2 ghost
+        layers) and interpolates it to the particles' dvort; None: dvort = 0
+        (advection only)."""
         import torch
         self.torch = torch
         self.dev = torch.device(device)
@@ -38,6 +44,11 @@ class VortexStep:
         mdt = torch.float64 if precision == _lib.FP64 else torch.float32
         self.mesh_vort = torch.zeros(nm, dtype=mdt, device=self.dev)
         self.mesh_vel = torch.zeros(nm, dtype=mdt, device=self.dev)
+        self.precision = precision
+        self.interp_order = interp_order
+        self.visc, self.fd_order = visc, fd_order
+        # free-side ghost cells of the right-hand side mesh stay zero
+        self.mesh_dvort = torch.zeros(nm, dtype=mdt, device=self.dev) if visc is not None else None
         self.cap = int(capacity if capacity is not None else np.prod(ext))
         self.buf = [{k: torch.zeros(3 * self.cap, dtype=torch.float64, device=self.dev) for k in _FIELDS}
                     for _ in range(2)]
@@ -71,7 +82,13 @@ class VortexStep:
         self.ops.p2m(a["pos"].data_ptr(), a["vort"].data_ptr(), n, self.mesh_vort.data_ptr(), clear=True)
         self.solver.solve_device(self.mesh_vort.data_ptr(), self.mesh_vel.data_ptr())
         self.ops.m2p(self.mesh_vel.data_ptr(), a["pos"].data_ptr(), n, a["vel"].data_ptr())
-        a["dvort"][:3 * n].zero_()
+        if self.visc is None:
+            a["dvort"][:3 * n].zero_()
+        else:
+            vorticity_rhs_device(self.patch, self.precision, self.mesh_vort.data_ptr(), self.mesh_vel.data_ptr(),
+                                 self.mesh_dvort.data_ptr(), self.visc, fd_order=self.fd_order,
+                                 interp_order=self.interp_order)
+            self.ops.m2p(self.mesh_dvort.data_ptr(), a["pos"].data_ptr(), n, a["dvort"].data_ptr())
         self.np = self.ops.advance(self.rk_order, s, dt, [a[k].data_ptr() for k in _FIELDS], n)
 
     def remesh(self, min_vort, min_dvort=0.0):

@@ -76,10 +76,13 @@ def ring_particles(n, seed=2, jitter=0.1):
     return np.asfortranarray(pos), np.asfortranarray(w)
 
 
-def oracle_vortex_step(g, p, pos, vort, dt, nsteps, min_vort, Gh=None):
+def oracle_vortex_step(g, p, pos, vort, dt, nsteps, min_vort, Gh=None, visc=None, fd_order=4):
     """The two-ring vortex step composed from the oracle's routines, in the
     order of the reference driver's step mode: per RK2 stage P2M (clear),
-    solve, M2P(vel), advance; then P2M and remesh(vel=.TRUE.)."""
+    solve, M2P(vel), advance; then P2M and remesh(vel=.TRUE.).  With visc,
+    each stage also forms the vorticity right-hand side on the mesh and
+    interpolates it to the particles' dvort (the client's order,
+    vortex_codes/vortex_structures/source/main.f:453-471)."""
     if Gh is None:
         Gh = O.poisson_setup(g, p)
     z = np.zeros_like(pos)
@@ -89,7 +92,10 @@ def oracle_vortex_step(g, p, pos, vort, dt, nsteps, min_vort, Gh=None):
             mesh = O.p2m(g, pos, vort)
             vel_m = O.poisson_solve(g, p, Gh, mesh)[0]
             vel = O.m2p(g, vel_m, pos)
-            dvort = np.zeros_like(pos)
+            if visc is None:
+                dvort = np.zeros_like(pos)
+            else:
+                dvort = O.m2p(g, O.vort_rhs(g, mesh, vel_m, visc, fd_order), pos)
             pos, vel, vort, dvort, vel_rk, dvort_rk = O.advance(g, 2, s, dt, pos, vel, vort, dvort, vel_rk,
                                                                 dvort_rk)
         mesh = O.p2m(g, pos, vort)

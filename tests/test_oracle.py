@@ -333,3 +333,123 @@ def test_oracle_vortex_step_matches_reference_golden(path):
     assert np.array_equal(pos, d["pos"])
     assert rel_linf(vort, d["vort"]) <= 1e-13
     assert rel_linf(vel, d["vel"]) <= 1e-13
+
+
+# ------------------------------------------------------ vorticity right-hand side
+def _poly_vort(g, deg, seed):
+    """w_l = b0 x^deg + b1 y^deg + b2 z^deg + b3 x y z^(deg-2) and its
+    Laplacian, per component."""
+    X, Y, Z = np.meshgrid(*cell_axes(g), indexing="ij")
+    b = np.random.default_rng(seed).normal(size=(3, 4))
+    w = np.zeros((3,) + X.shape, order="F")
+    lap = np.zeros((3,) + X.shape)
+    for c in range(3):
+        w[c] = b[c, 0] * X ** deg + b[c, 1] * Y ** deg + b[c, 2] * Z ** deg + b[c, 3] * X * Y * Z ** (deg - 2)
+        lap[c] = deg * (deg - 1) * (b[c, 0] * X ** (deg - 2) + b[c, 1] * Y ** (deg - 2) + b[c, 2] * Z ** (deg - 2))
+        if deg >= 4:
+            lap[c] += (deg - 2) * (deg - 3) * b[c, 3] * X * Y * Z ** (deg - 4)
+    return w, lap
+
+
+def _poly_jacobian(g, deg):
+    """_poly_fields' velocity and its Jacobian du[c, d] = d u_c / d x_d."""
+    X, Y, Z = np.meshgrid(*cell_axes(g), indexing="ij")
+    a = np.random.default_rng(deg).normal(size=(3, 4))
+    u = np.zeros((3,) + X.shape, order="F")
+    du = np.zeros((3, 3) + X.shape)
+    for c in range(3):
+        u[c] = a[c, 0] * X ** deg + a[c, 1] * Y ** deg + a[c, 2] * Z ** deg + a[c, 3] * X * Y * Z ** (deg - 2)
+        du[c, 0] = deg * a[c, 0] * X ** (deg - 1) + a[c, 3] * Y * Z ** (deg - 2)
+        du[c, 1] = deg * a[c, 1] * Y ** (deg - 1) + a[c, 3] * X * Z ** (deg - 2)
+        du[c, 2] = deg * a[c, 2] * Z ** (deg - 1) + (deg - 2) * a[c, 3] * X * Y * Z ** max(deg - 3, 0)
+    return u, du
+
+
+@pytest.mark.parametrize("fd_order,deg", [(4, 4), (4, 3), (2, 2)])
+def test_vort_rhs_exact_on_polynomials(fd_order, deg):
+    """part_deriv_vorticity.f:57-196: the 4th-order stencils are exact on
+    quartics, the 2nd-order ones on quadratics, so dvort_l = visc lap(w_l) +
+    sum_m w_m du_l/dx_m holds to rounding on the interior; the velocity
+    Jacobian is not symmetric, so a transposed du_m/dx_l, a sign, a
+    coefficient or a dropped term fails.  Ghost cells stay as given."""
+    g = O.make_geom((10, 9, 8), 2, [-0.3, -0.2, -0.45], 1.0, (0, 0, 0))
+    g.dx[0], g.dx[1], g.dx[2] = 0.1, 0.12, 0.09
+    w, lap = _poly_vort(g, deg, 5)
+    u, du = _poly_jacobian(g, deg)
+    visc = 0.013
+    want = visc * lap + np.einsum("mxyz,lmxyz->lxyz", w, du)
+    E = O.mesh_extent(g)
+    sentinel = np.full((3,) + tuple(E), 7.5, order="F")
+    got = O.vort_rhs(g, w, u, visc, fd_order, dvort=sentinel)
+    inner = (slice(None), slice(2, -2), slice(2, -2), slice(2, -2))
+    assert np.abs(got[inner] - want[inner]).max() <= 1e-10 * np.abs(want[inner]).max()
+    ghost = np.ones(E, dtype=bool)
+    ghost[2:-2, 2:-2, 2:-2] = False
+    assert (got[:, ghost] == 7.5).all()
+
+
+def test_vort_rhs_second_order_is_second_order():
+    """On quartics the 2nd-order stencils are not exact (the error is the
+    O(h^2) term), while the 4th-order ones are: the two paths differ."""
+    g = O.make_geom((10, 9, 8), 2, [-0.3, -0.2, -0.45], 0.1, (0, 0, 0))
+    w, lap = _poly_vort(g, 4, 6)
+    u, du = _poly_jacobian(g, 4)
+    want = 0.02 * lap + np.einsum("mxyz,lmxyz->lxyz", w, du)
+    inner = (slice(None), slice(2, -2), slice(2, -2), slice(2, -2))
+    e2 = np.abs(O.vort_rhs(g, w, u, 0.02, 2)[inner] - want[inner]).max()
+    e4 = np.abs(O.vort_rhs(g, w, u, 0.02, 4)[inner] - want[inner]).max()
+    assert e4 <= 1e-10 * np.abs(want[inner]).max() < e2
+
+
+@pytest.mark.parametrize("fd_order", [2, 4])
+def test_vort_rhs_periodic_modified_wavenumbers(fd_order):
+    """All-periodic patch with single Fourier modes: central differences of
+    sin and cos give the modified wavenumbers in closed form,
+    2nd order: -(2 - 2 cos kh) / h^2 and sin(kh) / h, 4th order:
+    -(30 - 32 cos kh + 2 cos 2kh) / (12 h^2) and (8 sin kh - sin 2kh) / (6 h);
+    the ghost layers of dvort are the interior's periodic images."""
+    n = (12, 10, 8)
+    g = O.make_geom(n, 0, [0.0, 0.0, 0.0], 1.0, (1, 1, 1), 3)
+    g.dx[0], g.dx[1], g.dx[2] = 0.1, 0.125, 0.15
+    for d in range(3):
+        g.xmax[d] = g.xmin[d] + n[d] * g.dx[d]
+    X, Y, Z = np.meshgrid(*cell_axes(g), indexing="ij")
+    V = (X, Y, Z)
+    k = [2 * np.pi / (n[d] * g.dx[d]) for d in range(3)]
+    h = [g.dx[d] for d in range(3)]
+    if fd_order == 2:
+        k2 = [-(2 - 2 * np.cos(k[d] * h[d])) / h[d] ** 2 for d in range(3)]
+        k1 = [np.sin(k[d] * h[d]) / h[d] for d in range(3)]
+    else:
+        k2 = [-(30 - 32 * np.cos(k[d] * h[d]) + 2 * np.cos(2 * k[d] * h[d])) / (12 * h[d] ** 2) for d in range(3)]
+        k1 = [(8 * np.sin(k[d] * h[d]) - np.sin(2 * k[d] * h[d])) / (6 * h[d]) for d in range(3)]
+    # w_l = sin(k_l x_l) (one mode per component, along its own axis)
+    # u_l = cos(k_m x_m) along axis m = l + 1 (mod 3)
+    w = np.zeros((3,) + X.shape, order="F")
+    u = np.zeros((3,) + X.shape, order="F")
+    want = np.zeros((3,) + X.shape)
+    visc = 0.05
+    for l in range(3):
+        w[l] = np.sin(k[l] * V[l])
+        u[l] = np.cos(k[(l + 1) % 3] * V[(l + 1) % 3])
+    for l in range(3):
+        m = (l + 1) % 3
+        # lap(w_l) = k2_l sin(k_l x_l); du_l/dx_m = -k1_m sin(k_m x_m)
+        want[l] = visc * k2[l] * np.sin(k[l] * V[l]) + w[m] * (-k1[m] * np.sin(k[m] * V[m]))
+    got = O.vort_rhs(g, w, u, visc, fd_order)
+    ng = 2
+    inner = (slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
+    assert np.abs(got[inner] - want[inner]).max() <= 1e-12 * np.abs(want[inner]).max()
+    # ghosts: mapped images of the interior (x, then y, then z)
+    for d in range(3):
+        N = n[d]
+        lo = [slice(None)] * 4
+        src = [slice(None)] * 4
+        lo[d + 1] = slice(0, ng)
+        src[d + 1] = slice(N, N + ng)
+        assert np.array_equal(got[tuple(lo)], got[tuple(src)])
+        hi = [slice(None)] * 4
+        src2 = [slice(None)] * 4
+        hi[d + 1] = slice(ng + N, ng + N + ng)
+        src2[d + 1] = slice(ng, 2 * ng)
+        assert np.array_equal(got[tuple(hi)], got[tuple(src2)])
