@@ -210,6 +210,44 @@ def batch_benchmark(steps, warmup, npatch=24, seed=4):
     return 1.0 / dt, nbytes, sizes
 
 
+def rhs_benchmark(reps, n=256):
+    """Vorticity right-hand side (stretching + diffusion, 4th-order
+    differences) on a 256^3 free patch with two ghost layers, FP64 and FP32,
+    timed with HIP events over `reps` launches on the null stream.
+    Algorithmic bytes per launch: read w and u, write dw on the interior
+    cells, 9 s bytes per cell (the stencil re-reads are cache traffic)."""
+    import numpy as np
+    import torch
+    import pmlib3d_amd as pm
+    h = 2.0 / n
+    patch = pm.Patch(xmin=(-1.0,) * 3, xmax=(1.0,) * 3, ncell=(n,) * 3, dx=(h,) * 3, nghost=2)
+    ext = (3,) + pm.mesh_extent(patch, 3)
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(7)
+    res = {"workload": "256^3 free patch, 2 ghost layers, 4th-order differences, visc 0.01, "
+                       "seeded normal w and u"}
+    for name, prec, dt, s in (("fp64", pm.FP64, torch.float64, 8), ("fp32", pm.FP32, torch.float32, 4)):
+        w = torch.randn(int(np.prod(ext)), dtype=dt, device=dev, generator=g)
+        u = torch.randn_like(w)
+        dw = torch.zeros_like(w)
+        run = lambda: pm.vorticity_rhs_device(patch, prec, w.data_ptr(), u.data_ptr(), dw.data_ptr(), 0.01,
+                                              fd_order=4, interp_order=3)
+        for _ in range(3):
+            run()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = 9 * s * n ** 3
+        res[name] = {"ms": ms, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / ms / 1e6,
+                     "frac_hbm": nbytes / ms / 1e6 / PEAK_HBM_GBS}
+    return res
+
+
 def cpu_step_baseline(n=128):
     """One two-ring vortex step of the compiled reference (one rank: its step
     mode is single-rank), FP64."""
@@ -266,6 +304,8 @@ def main():
                     help="timed two-ring vortex steps at 128^3 (0 skips)")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="timed 24-patch batched single-component solves (0 skips)")
+    ap.add_argument("--rhs-steps", type=int, default=20,
+                    help="timed vorticity right-hand-side launches at 256^3 (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -374,6 +414,12 @@ def main():
                              "h=1/128, one Gaussian ring each (R=0.3 min extent, sigma=0.15R, random axis), "
                              "single-component psi solve, order 4, eps 2h, one captured HIP graph per batch"}
 
+    rhs = None
+    if args.rhs_steps > 0:
+        rhs = rhs_benchmark(args.rhs_steps)
+        log("vorticity rhs 256^3: fp64 %.3f ms (%.0f GB/s), fp32 %.3f ms (%.0f GB/s)" %
+            (rhs["fp64"]["ms"], rhs["fp64"]["achieved_GBs"], rhs["fp32"]["ms"], rhs["fp32"]["achieved_GBs"]))
+
     cpu = None
     cpu_step = None
     if not args.no_cpu_baseline:
@@ -408,6 +454,7 @@ def main():
                                "fp64": bytes64 * sps64 / 1e9},
         "vortex_step": step,
         "batched_multipatch": batch,
+        "vorticity_rhs": rhs,
         "cpu_baseline_step": cpu_step,
         "roofline": {"bound": "hbm", "kernel": Z_KERNEL + " (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",

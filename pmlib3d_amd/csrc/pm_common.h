@@ -52,6 +52,18 @@ const std::string &get_error();
     }                                \
   } while (0)
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Map the linear block id to a logical tile id so that consecutive
+// tiles, which share cache lines at their edges, run on the same XCD.
+__device__ __forceinline__ int64_t xcd_tile(int64_t lin, int64_t n) {
+  const int64_t per = n / 8, rem = n % 8;
+  const int64_t x = lin % 8, k = lin / 8;
+  return x * per + (x < rem ? x : rem) + k;
+}
+__device__ __forceinline__ int64_t block_linear() {
+  return blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+}
+
 // explicitly rounded arithmetic (no contraction into multiply-adds): the
 // stencil kernels that reproduce the reference's rounding use these
 template <typename T> __device__ __forceinline__ T rmul(T a, T b);
@@ -63,6 +75,28 @@ template <> __device__ __forceinline__ double rsub(double a, double b) { return 
 template <> __device__ __forceinline__ float rmul(float a, float b) { return __fmul_rn(a, b); }
 template <> __device__ __forceinline__ float radd(float a, float b) { return __fadd_rn(a, b); }
 template <> __device__ __forceinline__ float rsub(float a, float b) { return __fsub_rn(a, b); }
+
+// Periodic forward ghost fill of an (nc, E...) mesh along axis d
+// (mesh/pmlib_mesh_map_ghost.f with incl_edges = .FALSE., op = replace).
+template <typename R>
+__global__ void k_ghost_fill(R *m, int E0, int E1, int E2, int d, int N, int ng, int nc) {
+  int E[3] = {E0, E1, E2};
+  int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+  int64_t st[3] = {nc, nc * (int64_t)E0, nc * (int64_t)E0 * E1};
+  int64_t n = (int64_t)E[o1] * E[o2] * nc * ng;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int c = idx % nc;
+    int64_t r = idx / nc;
+    int t = r % ng;
+    r /= ng;
+    int a1 = r % E[o1];
+    int a2 = r / E[o1];
+    int64_t base = c + st[o1] * a1 + st[o2] * a2;
+    m[base + st[d] * t] = m[base + st[d] * (N + t)];
+    m[base + st[d] * (ng + N + t)] = m[base + st[d] * (ng + t)];
+  }
+}
 
 // free the workspaces of particles.hip and penalise.hip
 void particles_release();

@@ -101,17 +101,6 @@ template <int PM> __device__ __forceinline__ int padm(int i) {
   return PM == 1 ? i + (i >> 3) : (PM == 2 ? i + (i >> 4) : pad(i));
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Map the linear block id to a logical tile id so that consecutive
-// tiles, which share cache lines at their edges, run on the same XCD.
-__device__ __forceinline__ int64_t xcd_tile(int64_t lin, int64_t n) {
-  const int64_t per = n / 8, rem = n % 8;
-  const int64_t x = lin % 8, k = lin / 8;
-  return x * per + (x < rem ? x : rem) + k;
-}
-__device__ __forceinline__ int64_t block_linear() {
-  return blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-}
 
 template <bool WAVE> __device__ __forceinline__ void line_sync() {
   if (WAVE) {

@@ -433,26 +433,4 @@ __global__ void k_g_octant(const double2 *B, R *G, int Hx, int Ly, int Hy, int H
   }
 }
 
-// Periodic forward ghost fill of an (nc, E...) mesh along axis d
-// (mesh/pmlib_mesh_map_ghost.f with incl_edges = .FALSE., op = replace).
-template <typename R>
-__global__ void k_ghost_fill(R *m, int E0, int E1, int E2, int d, int N, int ng, int nc) {
-  int E[3] = {E0, E1, E2};
-  int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
-  int64_t st[3] = {nc, nc * (int64_t)E0, nc * (int64_t)E0 * E1};
-  int64_t n = (int64_t)E[o1] * E[o2] * nc * ng;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int c = idx % nc;
-    int64_t r = idx / nc;
-    int t = r % ng;
-    r /= ng;
-    int a1 = r % E[o1];
-    int a2 = r / E[o1];
-    int64_t base = c + st[o1] * a1 + st[o2] * a2;
-    m[base + st[d] * t] = m[base + st[d] * (N + t)];
-    m[base + st[d] * (ng + N + t)] = m[base + st[d] * (ng + t)];
-  }
-}
-
 }  // namespace pmgpu

@@ -5,15 +5,14 @@
 // Restates part_deriv_vorticity (vortex_codes/vortex_penalisation/source/
 // particle_derivatives/part_deriv_vorticity.f:57-115 for the 2nd-order and
 // :117-196 for the 4th-order stencils, :201-208 for the ghost map).  One
-// thread per interior cell forms all three components; the stencil
-// neighbours along i belong to the adjacent lanes, so the re-reads hit L1/L2.
+// thread per component of an interior cell; the stencil neighbours along i
+// belong to nearby lanes, so the re-reads hit L1/L2.
 // Every floating-point operation is an explicitly rounded one, in the order
 // the reference writes it, so the FP64 result is bit-identical to a
 // left-to-right evaluation of the reference's expressions.  Ghost cells of
 // dvort on free sides are not written.
 #include "../../include/pmgpu.h"
 #include "pm_common.h"
-#include "poisson_kernels.h"
 
 namespace pmgpu {
 
@@ -24,47 +23,56 @@ template <typename T> struct RhsArgs {
   T visc, cdx2[3], c2dx[3];  // 1 / dx^2 and 1 / (2 dx) per axis
 };
 
+// One thread per component of a row's interior cells (consecutive threads own
+// consecutive words of a row, components fastest, so each stencil load of a
+// wavefront is one contiguous run); a block covers kRowWords words of row j
+// and marches through kPlanes planes, so the neighbours two planes up and
+// down were read by the same block a few planes earlier (L1/L2 hits).
+constexpr int kRowWords = 256, kPlanes = 16;
+
 template <typename T, int FD>
-__global__ void __launch_bounds__(256) k_vort_rhs(RhsArgs<T> a) {
-  const int64_t n = (int64_t)a.N[0] * a.N[1] * a.N[2];
+__global__ void __launch_bounds__(kRowWords) k_vort_rhs(RhsArgs<T> a) {
+  // XCD-aware tile order: consecutive rows (which read each other's rows as
+  // y neighbours) run on one XCD and share its L2
+  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y * gridDim.z);
+  const int bx = (int)(lt % gridDim.x), j = (int)((lt / gridDim.x) % gridDim.y);
+  const int kz = (int)(lt / ((int64_t)gridDim.x * gridDim.y));
+  const int x3 = bx * kRowWords + threadIdx.x;  // word of the row's interior
+  if (x3 >= 3 * a.N[0]) return;
+  const int l = x3 % 3, i = x3 / 3;
+  const int k0 = kz * kPlanes, k1 = k0 + kPlanes < a.N[2] ? k0 + kPlanes : a.N[2];
   const int64_t st[3] = {3, 3 * (int64_t)a.E[0], 3 * (int64_t)a.E[0] * a.E[1]};
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(c % a.N[0]);
-    const int64_t r = c / a.N[0];
-    const int j = (int)(r % a.N[1]), k = (int)(r / a.N[1]);
+  for (int k = k0; k < k1; ++k) {
     const int64_t o =
         3 * ((int64_t)(i + a.ng[0]) + (int64_t)a.E[0] * ((j + a.ng[1]) + (int64_t)a.E[1] * (k + a.ng[2])));
     const T w1 = a.w[o], w2 = a.w[o + 1], w3 = a.w[o + 2];
+    const T *p = a.w + o + l, *q = a.u + o + l;
+    T dd[3], du[3];
 #pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      T dd[3], du[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const T *p = a.w + o + l, *q = a.u + o + l;
-        const int64_t s = st[d];
-        if (FD == 2) {
-          // (w(-1) - 2 w(0) + w(+1)) / dx^2 and (-u(-1) + u(+1)) / (2 dx)
-          dd[d] = rmul(radd(rsub(p[-s], rmul(T(2), p[0])), p[s]), a.cdx2[d]);
-          du[d] = rmul(radd(-q[-s], q[s]), a.c2dx[d]);
-        } else {
-          // (c1, c2, -2.5, c2, c1) / dx^2 and (c3, -c4, c4, -c3) / (2 dx)
-          const T c1 = T(-1.0 / 12.0), c2 = T(4.0 / 3.0), c3 = T(1.0 / 6.0), c4 = T(4.0 / 3.0);
-          T t = radd(rmul(c1, p[-2 * s]), rmul(c2, p[-s]));
-          t = rsub(t, rmul(T(2.5), p[0]));
-          t = radd(t, rmul(c2, p[s]));
-          t = radd(t, rmul(c1, p[2 * s]));
-          dd[d] = rmul(t, a.cdx2[d]);
-          T v = rsub(rmul(c3, q[-2 * s]), rmul(c4, q[-s]));
-          v = radd(v, rmul(c4, q[s]));
-          v = rsub(v, rmul(c3, q[2 * s]));
-          du[d] = rmul(v, a.c2dx[d]);
-        }
+    for (int d = 0; d < 3; ++d) {
+      const int64_t s = st[d];
+      if (FD == 2) {
+        // (w(-1) - 2 w(0) + w(+1)) / dx^2 and (-u(-1) + u(+1)) / (2 dx)
+        dd[d] = rmul(radd(rsub(p[-s], rmul(T(2), p[0])), p[s]), a.cdx2[d]);
+        du[d] = rmul(radd(-q[-s], q[s]), a.c2dx[d]);
+      } else {
+        // (c1, c2, -2.5, c2, c1) / dx^2 and (c3, -c4, c4, -c3) / (2 dx)
+        const T c1 = T(-1.0 / 12.0), c2 = T(4.0 / 3.0), c3 = T(1.0 / 6.0), c4 = T(4.0 / 3.0);
+        T v = radd(rmul(c1, p[-2 * s]), rmul(c2, p[-s]));
+        v = rsub(v, rmul(T(2.5), p[0]));
+        v = radd(v, rmul(c2, p[s]));
+        v = radd(v, rmul(c1, p[2 * s]));
+        dd[d] = rmul(v, a.cdx2[d]);
+        T x = rsub(rmul(c3, q[-2 * s]), rmul(c4, q[-s]));
+        x = radd(x, rmul(c4, q[s]));
+        x = rsub(x, rmul(c3, q[2 * s]));
+        du[d] = rmul(x, a.c2dx[d]);
       }
-      T res = radd(rmul(a.visc, radd(radd(dd[0], dd[1]), dd[2])), rmul(w1, du[0]));
-      res = radd(res, rmul(w2, du[1]));
-      res = radd(res, rmul(w3, du[2]));
-      a.dw[o + l] = res;
     }
+    T res = radd(rmul(a.visc, radd(radd(dd[0], dd[1]), dd[2])), rmul(w1, du[0]));
+    res = radd(res, rmul(w2, du[1]));
+    res = radd(res, rmul(w3, du[2]));
+    a.dw[o + l] = res;
   }
 }
 
@@ -90,14 +98,13 @@ static int vort_rhs_launch(const pmgpu_geom *geom, const void *w, const void *u,
   a.u = (const T *)u;
   a.dw = (T *)dw;
   a.visc = T(visc);
-  const int64_t n = (int64_t)a.N[0] * a.N[1] * a.N[2];
-  if (n > 0) {
-    const int64_t nb = (n + 255) / 256;
-    const dim3 grid((unsigned)(nb < 65536 ? nb : 65536));
+  if ((int64_t)a.N[0] * a.N[1] * a.N[2] > 0) {
+    PM_CHECK(a.N[1] < 65536 && a.N[2] / kPlanes < 65536, "pmgpu_vorticity_rhs: mesh too large");
+    const dim3 grid((3 * a.N[0] + kRowWords - 1) / kRowWords, a.N[1], (a.N[2] + kPlanes - 1) / kPlanes);
     if (fd_order == 4)
-      hipLaunchKernelGGL((k_vort_rhs<T, 4>), grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_vort_rhs<T, 4>), grid, dim3(kRowWords), 0, st, a);
     else
-      hipLaunchKernelGGL((k_vort_rhs<T, 2>), grid, dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_vort_rhs<T, 2>), grid, dim3(kRowWords), 0, st, a);
     PM_HIP(hipGetLastError());
   }
   // ghost map of dvort, x then y then z (periodic sides)
