@@ -132,6 +132,16 @@ MODULE pmgpu_iface
       REAL(C_DOUBLE), VALUE :: relax
       INTEGER(C_INT32_T), VALUE :: fd_order
     END FUNCTION pmgpu_penalise
+    ! part_deriv_vorticity (vortex_codes/*/source/particle_derivatives/
+    ! part_deriv_vorticity.f:57-208): dvort from vort and vel, (3, E...) meshes
+    INTEGER(C_INT) FUNCTION pmgpu_vorticity_rhs(g, vort, vel, visc, fd_order, dvort) &
+        & BIND(C, NAME='pmgpu_vorticity_rhs')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR, pmgpu_geom
+      TYPE(pmgpu_geom), INTENT(IN) :: g
+      TYPE(C_PTR), VALUE :: vort, vel, dvort
+      REAL(C_DOUBLE), VALUE :: visc
+      INTEGER(C_INT32_T), VALUE :: fd_order
+    END FUNCTION pmgpu_vorticity_rhs
   END INTERFACE
 
 CONTAINS
