@@ -26,8 +26,8 @@ template <typename T> struct RhsArgs {
 // One thread per component of a row's interior cells (consecutive threads own
 // consecutive words of a row, components fastest, so each stencil load of a
 // wavefront is one contiguous run); a block covers kRowWords words of row j
-// and marches through kPlanes planes, so the neighbours two planes up and
-// down were read by the same block a few planes earlier (L1/L2 hits).
+// and marches through kPlanes planes, each thread keeping its word's values
+// on the planes within two of the current one in registers.
 constexpr int kRowWords = 256, kPlanes = 16;
 
 template <typename T, int FD>
@@ -42,30 +42,57 @@ __global__ void __launch_bounds__(kRowWords) k_vort_rhs(RhsArgs<T> a) {
   const int l = x3 % 3, i = x3 / 3;
   const int k0 = kz * kPlanes, k1 = k0 + kPlanes < a.N[2] ? k0 + kPlanes : a.N[2];
   const int64_t st[3] = {3, 3 * (int64_t)a.E[0], 3 * (int64_t)a.E[0] * a.E[1]};
+  // this word's w_l and u_l on planes k - H .. k + H, H = FD / 2: each plane
+  // is loaded once as the block marches, the z stencil reads the queue
+  constexpr int H = FD / 2;
+  T zw[2 * H + 1], zu[2 * H + 1];
+  const int64_t row0 =
+      3 * ((int64_t)(i + a.ng[0]) + (int64_t)a.E[0] * ((j + a.ng[1]) + (int64_t)a.E[1] * (k0 + a.ng[2]))) + l;
+#pragma unroll
+  for (int m = 0; m < 2 * H; ++m) {
+    zw[m + 1] = a.w[row0 + (m - H) * st[2]];
+    zu[m + 1] = a.u[row0 + (m - H) * st[2]];
+  }
   for (int k = k0; k < k1; ++k) {
-    const int64_t o =
-        3 * ((int64_t)(i + a.ng[0]) + (int64_t)a.E[0] * ((j + a.ng[1]) + (int64_t)a.E[1] * (k + a.ng[2])));
+    const int64_t o = row0 - l + (int64_t)(k - k0) * st[2];
+#pragma unroll
+    for (int m = 0; m < 2 * H; ++m) {
+      zw[m] = zw[m + 1];
+      zu[m] = zu[m + 1];
+    }
+    zw[2 * H] = a.w[o + l + H * st[2]];
+    zu[2 * H] = a.u[o + l + H * st[2]];
     const T w1 = a.w[o], w2 = a.w[o + 1], w3 = a.w[o + 2];
     const T *p = a.w + o + l, *q = a.u + o + l;
     T dd[3], du[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const int64_t s = st[d];
+      // neighbours at -2, -1, 0, +1, +2 along d (z: the queue)
+      const T wm2 = d == 2 ? zw[H - 2 < 0 ? 0 : H - 2] : (H == 2 ? p[-2 * s] : T(0));
+      const T wm1 = d == 2 ? zw[H - 1] : p[-s];
+      const T w0 = d == 2 ? zw[H] : p[0];
+      const T wp1 = d == 2 ? zw[H + 1] : p[s];
+      const T wp2 = d == 2 ? zw[H + 2 > 2 * H ? 2 * H : H + 2] : (H == 2 ? p[2 * s] : T(0));
+      const T um2 = d == 2 ? zu[H - 2 < 0 ? 0 : H - 2] : (H == 2 ? q[-2 * s] : T(0));
+      const T um1 = d == 2 ? zu[H - 1] : q[-s];
+      const T up1 = d == 2 ? zu[H + 1] : q[s];
+      const T up2 = d == 2 ? zu[H + 2 > 2 * H ? 2 * H : H + 2] : (H == 2 ? q[2 * s] : T(0));
       if (FD == 2) {
         // (w(-1) - 2 w(0) + w(+1)) / dx^2 and (-u(-1) + u(+1)) / (2 dx)
-        dd[d] = rmul(radd(rsub(p[-s], rmul(T(2), p[0])), p[s]), a.cdx2[d]);
-        du[d] = rmul(radd(-q[-s], q[s]), a.c2dx[d]);
+        dd[d] = rmul(radd(rsub(wm1, rmul(T(2), w0)), wp1), a.cdx2[d]);
+        du[d] = rmul(radd(-um1, up1), a.c2dx[d]);
       } else {
         // (c1, c2, -2.5, c2, c1) / dx^2 and (c3, -c4, c4, -c3) / (2 dx)
         const T c1 = T(-1.0 / 12.0), c2 = T(4.0 / 3.0), c3 = T(1.0 / 6.0), c4 = T(4.0 / 3.0);
-        T v = radd(rmul(c1, p[-2 * s]), rmul(c2, p[-s]));
-        v = rsub(v, rmul(T(2.5), p[0]));
-        v = radd(v, rmul(c2, p[s]));
-        v = radd(v, rmul(c1, p[2 * s]));
+        T v = radd(rmul(c1, wm2), rmul(c2, wm1));
+        v = rsub(v, rmul(T(2.5), w0));
+        v = radd(v, rmul(c2, wp1));
+        v = radd(v, rmul(c1, wp2));
         dd[d] = rmul(v, a.cdx2[d]);
-        T x = rsub(rmul(c3, q[-2 * s]), rmul(c4, q[-s]));
-        x = radd(x, rmul(c4, q[s]));
-        x = rsub(x, rmul(c3, q[2 * s]));
+        T x = rsub(rmul(c3, um2), rmul(c4, um1));
+        x = radd(x, rmul(c4, up1));
+        x = rsub(x, rmul(c3, up2));
         du[d] = rmul(x, a.c2dx[d]);
       }
     }
