@@ -23,83 +23,115 @@ template <typename T> struct RhsArgs {
   T visc, cdx2[3], c2dx[3];  // 1 / dx^2 and 1 / (2 dx) per axis
 };
 
-// One thread per component of a row's interior cells (consecutive threads own
-// consecutive words of a row, components fastest, so each stencil load of a
-// wavefront is one contiguous run); a block covers kRowWords words of row j
-// and marches through kPlanes planes, each thread keeping its word's values
-// on the planes within two of the current one in registers.
-constexpr int kRowWords = 256, kPlanes = 16;
+// One thread per component word of kRows rows (consecutive threads own
+// consecutive words of a row, components fastest); a block covers kRowWords
+// words of kRows consecutive rows and marches through kPlanes planes.  Each
+// thread keeps its words' values on the planes within H = FD / 2 of the
+// current one in registers (one new plane loaded per step); the current
+// plane's tile with an H-cell halo in x and y is staged in LDS (the centre
+// from those registers, the halo from memory), and the x and y stencils read
+// it there.
+constexpr int kRowWords = 256, kRows = 4, kPlanes = 16;
 
 template <typename T, int FD>
 __global__ void __launch_bounds__(kRowWords) k_vort_rhs(RhsArgs<T> a) {
-  // XCD-aware tile order: consecutive rows (which read each other's rows as
-  // y neighbours) run on one XCD and share its L2
+  constexpr int H = FD / 2, HW = 3 * H, TW = kRowWords + 2 * HW, TR = kRows + 2 * H;
+  __shared__ T lw[TR][TW], lu[TR][TW];
+  // XCD-aware tile order: consecutive row groups (which read each other's
+  // rows as y neighbours) run on one XCD and share its L2
   const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y * gridDim.z);
-  const int bx = (int)(lt % gridDim.x), j = (int)((lt / gridDim.x) % gridDim.y);
+  const int bx = (int)(lt % gridDim.x), j0 = (int)((lt / gridDim.x) % gridDim.y) * kRows;
   const int kz = (int)(lt / ((int64_t)gridDim.x * gridDim.y));
-  const int x3 = bx * kRowWords + threadIdx.x;  // word of the row's interior
-  if (x3 >= 3 * a.N[0]) return;
-  const int l = x3 % 3, i = x3 / 3;
+  const int tid = threadIdx.x;
+  const int x0 = bx * kRowWords;                 // first interior word of the tile
+  const int x3 = x0 + tid, nw = 3 * a.N[0];      // this thread's word, words per row
+  const bool mine = x3 < nw;
+  const int l = x3 % 3;
   const int k0 = kz * kPlanes, k1 = k0 + kPlanes < a.N[2] ? k0 + kPlanes : a.N[2];
-  const int64_t st[3] = {3, 3 * (int64_t)a.E[0], 3 * (int64_t)a.E[0] * a.E[1]};
-  // this word's w_l and u_l on planes k - H .. k + H, H = FD / 2: each plane
-  // is loaded once as the block marches, the z stencil reads the queue
-  constexpr int H = FD / 2;
-  T zw[2 * H + 1], zu[2 * H + 1];
-  const int64_t row0 =
-      3 * ((int64_t)(i + a.ng[0]) + (int64_t)a.E[0] * ((j + a.ng[1]) + (int64_t)a.E[1] * (k0 + a.ng[2]))) + l;
+  const int64_t sy = 3 * (int64_t)a.E[0], sz = 3 * (int64_t)a.E[0] * a.E[1];
+  // word wd (interior-relative, may be a halo word), row jj, plane kk
+  auto at = [&](int wd, int jj, int kk) {
+    return (int64_t)(3 * a.ng[0] + wd) + sy * (jj + a.ng[1]) + sz * (kk + a.ng[2]);
+  };
+  // the queues also hold the ghost words and rows next to the interior (the
+  // tile centre positions the neighbours of interior words read)
+  T zw[kRows][2 * H + 1], zu[kRows][2 * H + 1];
 #pragma unroll
-  for (int m = 0; m < 2 * H; ++m) {
-    zw[m + 1] = a.w[row0 + (m - H) * st[2]];
-    zu[m + 1] = a.u[row0 + (m - H) * st[2]];
-  }
-  for (int k = k0; k < k1; ++k) {
-    const int64_t o = row0 - l + (int64_t)(k - k0) * st[2];
+  for (int r = 0; r < kRows; ++r)
 #pragma unroll
     for (int m = 0; m < 2 * H; ++m) {
-      zw[m] = zw[m + 1];
-      zu[m] = zu[m + 1];
+      const bool ok = x3 < nw + HW && j0 + r < a.N[1] + H;
+      zw[r][m + 1] = ok ? a.w[at(x3, j0 + r, k0 + m - H)] : T(0);
+      zu[r][m + 1] = ok ? a.u[at(x3, j0 + r, k0 + m - H)] : T(0);
     }
-    zw[2 * H] = a.w[o + l + H * st[2]];
-    zu[2 * H] = a.u[o + l + H * st[2]];
-    const T w1 = a.w[o], w2 = a.w[o + 1], w3 = a.w[o + 2];
-    const T *p = a.w + o + l, *q = a.u + o + l;
-    T dd[3], du[3];
+  for (int k = k0; k < k1; ++k) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const int64_t s = st[d];
-      // neighbours at -2, -1, 0, +1, +2 along d (z: the queue)
-      const T wm2 = d == 2 ? zw[H - 2 < 0 ? 0 : H - 2] : (H == 2 ? p[-2 * s] : T(0));
-      const T wm1 = d == 2 ? zw[H - 1] : p[-s];
-      const T w0 = d == 2 ? zw[H] : p[0];
-      const T wp1 = d == 2 ? zw[H + 1] : p[s];
-      const T wp2 = d == 2 ? zw[H + 2 > 2 * H ? 2 * H : H + 2] : (H == 2 ? p[2 * s] : T(0));
-      const T um2 = d == 2 ? zu[H - 2 < 0 ? 0 : H - 2] : (H == 2 ? q[-2 * s] : T(0));
-      const T um1 = d == 2 ? zu[H - 1] : q[-s];
-      const T up1 = d == 2 ? zu[H + 1] : q[s];
-      const T up2 = d == 2 ? zu[H + 2 > 2 * H ? 2 * H : H + 2] : (H == 2 ? q[2 * s] : T(0));
-      if (FD == 2) {
-        // (w(-1) - 2 w(0) + w(+1)) / dx^2 and (-u(-1) + u(+1)) / (2 dx)
-        dd[d] = rmul(radd(rsub(wm1, rmul(T(2), w0)), wp1), a.cdx2[d]);
-        du[d] = rmul(radd(-um1, up1), a.c2dx[d]);
-      } else {
-        // (c1, c2, -2.5, c2, c1) / dx^2 and (c3, -c4, c4, -c3) / (2 dx)
-        const T c1 = T(-1.0 / 12.0), c2 = T(4.0 / 3.0), c3 = T(1.0 / 6.0), c4 = T(4.0 / 3.0);
-        T v = radd(rmul(c1, wm2), rmul(c2, wm1));
-        v = rsub(v, rmul(T(2.5), w0));
-        v = radd(v, rmul(c2, wp1));
-        v = radd(v, rmul(c1, wp2));
-        dd[d] = rmul(v, a.cdx2[d]);
-        T x = rsub(rmul(c3, um2), rmul(c4, um1));
-        x = radd(x, rmul(c4, up1));
-        x = rsub(x, rmul(c3, up2));
-        du[d] = rmul(x, a.c2dx[d]);
+    for (int r = 0; r < kRows; ++r) {
+#pragma unroll
+      for (int m = 0; m < 2 * H; ++m) {
+        zw[r][m] = zw[r][m + 1];
+        zu[r][m] = zu[r][m + 1];
+      }
+      const bool ok = x3 < nw + HW && j0 + r < a.N[1] + H;
+      zw[r][2 * H] = ok ? a.w[at(x3, j0 + r, k + H)] : T(0);
+      zu[r][2 * H] = ok ? a.u[at(x3, j0 + r, k + H)] : T(0);
+      lw[H + r][HW + tid] = zw[r][H];
+      lu[H + r][HW + tid] = zu[r][H];
+    }
+    // halo: H rows above and below (whole tile width), HW words left and
+    // right of the centre rows; positions outside the mesh are not needed
+    for (int idx = tid; idx < TR * TW; idx += kRowWords) {
+      const int rr = idx / TW, c = idx - rr * TW;
+      if (rr >= H && rr < H + kRows && c >= HW && c < HW + kRowWords) continue;  // centre
+      const int wd = x0 - HW + c, jj = j0 - H + rr;
+      if (wd >= -HW && wd < nw + HW && jj >= -H && jj < a.N[1] + H) {
+        lw[rr][c] = a.w[at(wd, jj, k)];
+        lu[rr][c] = a.u[at(wd, jj, k)];
       }
     }
-    T res = radd(rmul(a.visc, radd(radd(dd[0], dd[1]), dd[2])), rmul(w1, du[0]));
-    res = radd(res, rmul(w2, du[1]));
-    res = radd(res, rmul(w3, du[2]));
-    a.dw[o + l] = res;
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        if (j0 + r >= a.N[1]) break;
+        const int rr = H + r, c = HW + tid;
+        const T w1 = lw[rr][c - l], w2 = lw[rr][c - l + 1], w3 = lw[rr][c - l + 2];
+        T dd[3], du[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          // neighbours at -2, -1, 0, +1, +2 along d: x and y from the tile,
+          // z from the register queue
+          auto W = [&](int o) {
+            return d == 0 ? lw[rr][c + 3 * o] : (d == 1 ? lw[rr + o][c] : zw[r][H + o]);
+          };
+          auto U = [&](int o) {
+            return d == 0 ? lu[rr][c + 3 * o] : (d == 1 ? lu[rr + o][c] : zu[r][H + o]);
+          };
+          if (FD == 2) {
+            // (w(-1) - 2 w(0) + w(+1)) / dx^2 and (-u(-1) + u(+1)) / (2 dx)
+            dd[d] = rmul(radd(rsub(W(-1), rmul(T(2), W(0))), W(1)), a.cdx2[d]);
+            du[d] = rmul(radd(-U(-1), U(1)), a.c2dx[d]);
+          } else {
+            // (c1, c2, -2.5, c2, c1) / dx^2 and (c3, -c4, c4, -c3) / (2 dx)
+            const T c1 = T(-1.0 / 12.0), c2 = T(4.0 / 3.0), c3 = T(1.0 / 6.0), c4 = T(4.0 / 3.0);
+            T v = radd(rmul(c1, W(-2)), rmul(c2, W(-1)));
+            v = rsub(v, rmul(T(2.5), W(0)));
+            v = radd(v, rmul(c2, W(1)));
+            v = radd(v, rmul(c1, W(2)));
+            dd[d] = rmul(v, a.cdx2[d]);
+            T x = rsub(rmul(c3, U(-2)), rmul(c4, U(-1)));
+            x = radd(x, rmul(c4, U(1)));
+            x = rsub(x, rmul(c3, U(2)));
+            du[d] = rmul(x, a.c2dx[d]);
+          }
+        }
+        T res = radd(rmul(a.visc, radd(radd(dd[0], dd[1]), dd[2])), rmul(w1, du[0]));
+        res = radd(res, rmul(w2, du[1]));
+        res = radd(res, rmul(w3, du[2]));
+        a.dw[at(x3, j0 + r, k)] = res;
+      }
+    }
+    __syncthreads();  // the tile is rewritten for the next plane
   }
 }
 
@@ -126,8 +158,9 @@ static int vort_rhs_launch(const pmgpu_geom *geom, const void *w, const void *u,
   a.dw = (T *)dw;
   a.visc = T(visc);
   if ((int64_t)a.N[0] * a.N[1] * a.N[2] > 0) {
-    PM_CHECK(a.N[1] < 65536 && a.N[2] / kPlanes < 65536, "pmgpu_vorticity_rhs: mesh too large");
-    const dim3 grid((3 * a.N[0] + kRowWords - 1) / kRowWords, a.N[1], (a.N[2] + kPlanes - 1) / kPlanes);
+    PM_CHECK(a.N[1] / kRows < 65535 && a.N[2] / kPlanes < 65535, "pmgpu_vorticity_rhs: mesh too large");
+    const dim3 grid((3 * a.N[0] + kRowWords - 1) / kRowWords, (a.N[1] + kRows - 1) / kRows,
+                    (a.N[2] + kPlanes - 1) / kPlanes);
     if (fd_order == 4)
       hipLaunchKernelGGL((k_vort_rhs<T, 4>), grid, dim3(kRowWords), 0, st, a);
     else
