@@ -16,6 +16,8 @@ GEOMS = [
     ((16, 12, 20), 0, (1, 1, 1), 3),
     ((16, 14, 12), 2, (1, 0, 1), 4),
     ((24, 20, 18), 1, (0, 0, 0), 3),   # one ghost layer: 2nd order only
+    ((90, 11, 19), 2, (0, 0, 0), 3),   # rows of 270 words: two tiles, a cell across the tile edge
+    ((172, 7, 5), 0, (1, 1, 1), 3),    # 516 words: three tiles; partial row groups
 ]
 
 
