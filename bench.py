@@ -1,23 +1,32 @@
-"""Benchmark of the unbounded Poisson solve on one MI355X.
+"""Benchmark of the particle-mesh hot path on one MI355X.
 
-Workload (one "step" = one full velocity solve): 256^3 cells on [-1,1]^3
-(h = 1/128), free-space boundaries, doubled FFT grid 512^3 handled with
-implicit zero padding, three vorticity components in one solve, regularised
-Green's function of order 4 with eps = 2h built in FP64.  Vorticity: 32
-seeded Gaussian blobs (centres U[-0.6,0.6]^3, sigma U[2h,6h], strengths
-N(0,1)^3, seed 3), projected divergence-free with the solver's spectral
-reprojection.  Inputs and outputs stay resident in HBM; the timed region
-contains only solves.
+Headline (`value`, one "step" = one full velocity solve): 256^3 cells on
+[-1,1]^3 (h = 1/128), free-space boundaries, doubled FFT grid 512^3 handled
+with implicit zero padding, three vorticity components in one solve,
+regularised Green's function of order 4 with eps = 2h built in FP64.
+Vorticity: 32 seeded Gaussian blobs (centres U[-0.6,0.6]^3, sigma U[2h,6h],
+strengths N(0,1)^3, seed 3), projected divergence-free with the solver's
+spectral reprojection.  Inputs and outputs stay resident in HBM; the timed
+region contains only solves.  FP32 (value) beside FP64.
 
-Prints one JSON line: FP32 solves/s (value), the FP64 rate beside it, the
-roofline of the dominant kernel (the fused z pass) from HIP events on the
-solver's stream, and the compiled reference CPU solve rate (cpu_baseline).
+Beside it, on the same GPU (each with its FP32-vs-FP64 error):
+  ring64     64^3 Gaussian ring, FP64 stream function + velocity solve
+  step128    two-ring vortex step at 128^3 on all 128^3 jittered particles
+             (the particle set is re-loaded before every timed step)
+  batch24    24 free-space patches of uneven sizes, one captured graph
+  mixed384   384^2 x 384 periodic-x,y / free-z solve of a turbulence-like
+             field, then remeshed FP32 vortex steps on that grid
+  rhs256     the vorticity right-hand side at 256^3
+and the compiled reference on the host's cores (cpu_baseline).
+
+Algorithmic bytes follow the 5-pass implicit-padding schedule (X R2C, Y,
+fused Z, Y inverse, X inverse), G-hat counted as a real half-spectrum of
+Hx * Ly * Lz values.  Prints one JSON line.
 """
 import argparse
 import json
 import math
 import os
-import subprocess
 import sys
 import time
 
@@ -28,6 +37,7 @@ N = 256
 ORDER = 4
 EPS_RADIUS = 2.0
 PEAK_HBM_GBS = 8000.0
+STEP_MIN_VORT = 1e-6   # remesh threshold of the two-ring step (peak |omega| is ~41)
 
 
 def log(*a):
@@ -55,102 +65,229 @@ def blob_vorticity(n, seed=3):
     return w
 
 
-def algorithmic_bytes(n, s, s_g):
-    """Bytes one solve must move with the 5-pass implicit-padding schedule,
-    3 components in and out, G-hat stored as its real octant."""
-    M = n
-    L = 2 * n
-    Hx = L // 2 + 1
-    Hy = Hz = L // 2 + 1
+def solve_bytes(M, L, cin, cout, s, s_g):
+    """Bytes one solve must move with the 5-pass implicit-padding schedule:
+    M, L solve-domain and transform lengths per axis, cin / cout components
+    in and out, s the mesh word, s_g the G-hat word (Hx * Ly * Lz values).
+    Returns (total, fused z pass)."""
+    Mx, My, Mz = M
+    Lx, Ly, Lz = L
+    Hx = Lx // 2 + 1
     c = 2 * s
-    x_f = 3 * M ** 3 * s + 3 * Hx * M * M * c
-    y_f = 3 * Hx * M * M * c + 3 * Hx * L * M * c
-    z_p = 3 * Hx * L * M * c + Hx * Hy * Hz * s_g + 3 * Hx * L * M * c
-    y_i = 3 * Hx * L * M * c + 3 * Hx * M * M * c
-    x_i = 3 * Hx * M * M * c + 3 * M ** 3 * s
+    x_f = cin * Mx * My * Mz * s + cin * Hx * My * Mz * c
+    y_f = cin * Hx * My * Mz * c + cin * Hx * Ly * Mz * c
+    z_p = cin * Hx * Ly * Mz * c + Hx * Ly * Lz * s_g + cout * Hx * Ly * Mz * c
+    y_i = cout * Hx * Ly * Mz * c + cout * Hx * My * Mz * c
+    x_i = cout * Hx * My * Mz * c + cout * Mx * My * Mz * s
     return x_f + y_f + z_p + y_i + x_i, z_p
 
 
-def cpu_reference_baseline(w64):
-    """The compiled reference (oracle/_ref, FP64, native Glassman FFT) on one
-    full 256^3 solve under MPI; falls back to the C oracle on 128^3."""
+def free_bytes(n, cin, cout, s, s_g):
+    M = tuple(n) if isinstance(n, (tuple, list)) else (n,) * 3
+    return solve_bytes(M, tuple(2 * m for m in M), cin, cout, s, s_g)
+
+
+def rel_l2_t(a, b):
+    return float(((a.double() - b.double()) ** 2).sum().sqrt() / (b.double() ** 2).sum().sqrt())
+
+
+# --------------------------------------------------------------- CPU reference
+def cpu_ranks():
+    """MPI ranks of the CPU reference: the cores this process may run on,
+    at most 16 (a one-GPU job's CPU share on the benchmark machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _reps(out):
+    line = [l for l in out.splitlines() if l.startswith("REPS")][0]
+    return [float(v) for v in line.split()[1:]]
+
+
+def cpu_reference_solve(w64, n, order, calc_vecpot=False, nrep=6):
+    """The compiled reference (oracle/_ref, FP64, native Glassman FFT) on a
+    free-space n^3 solve under MPI with cpu_ranks() ranks: one warm-up
+    repetition, then the median of the remaining nrep - 1; G-hat set-up
+    timed separately and not counted.  Falls back to the C oracle (port,
+    1 thread) when the reference is not built."""
+    import numpy as np
     from oracle import oracle as O
-    g = O.make_geom((N,) * 3, 0, [-1.0] * 3, 2.0 / N, (0, 0, 0))
-    p = O.make_params(ORDER, 1, EPS_RADIUS)
-    ranks = max(1, min(16, os.cpu_count() or 1))
+    g = O.make_geom((n,) * 3, 0, [-1.0] * 3, 2.0 / n, (0, 0, 0))
+    p = O.make_params(order, 1, EPS_RADIUS)
+    ranks = cpu_ranks()
+    what = "psi and u" if calc_vecpot else "u"
     if O.ref_available() and os.path.exists(O.MPIEXEC):
         try:
             t = time.time()
-            _, out = O.ref_run(dict(O.geom_job(g, p), mode="solve", nrep=1), [w64], nproc=ranks,
-                               timeout=240)
-            line = [l for l in out.splitlines() if l.startswith("TIMING")][0]
-            solve_s = float(line.split("solve_min_s=")[1].split()[0])
-            setup_s = float(line.split("setup_s=")[1].split()[0])
-            log("cpu reference: %s (wall %.1fs)" % (line, time.time() - t))
-            return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": ranks, "kind": "reference",
-                    "sample": "one 256^3 free-space solve (3 components, order 4, FP64, native "
-                              "Glassman FFT; FFTW absent) with mpiexec -n %d; setup %.2fs not "
-                              "counted" % (ranks, setup_s)}
+            _, out = O.ref_run(dict(O.geom_job(g, p), mode="solve", nrep=nrep, calc_vecpot=bool(calc_vecpot)),
+                               [w64], nproc=ranks, timeout=400)
+            reps = _reps(out)
+            med = float(np.median(reps[1:]))
+            setup_s = float([l for l in out.splitlines() if l.startswith("TIMING")][0]
+                            .split("setup_s=")[1].split()[0])
+            log("cpu reference %d^3 solve: ranks %d, reps %s s, setup %.2f s (wall %.1f s)" %
+                (n, ranks, " ".join("%.3f" % r for r in reps), setup_s, time.time() - t))
+            return {"value": 1.0 / med, "unit": "solves/s", "cores": ranks, "kind": "reference",
+                    "sample": "%d^3 free-space solve (3 components, %s, order %d, FP64, native Glassman FFT; "
+                              "FFTW absent) with mpiexec -n %d (the job's CPU cores); 1 warm-up + median "
+                              "of %d; setup %.2fs not counted" % (n, what, order, ranks, nrep - 1, setup_s)}
         except Exception as e:  # pragma: no cover - reported, not hidden
             log("cpu reference failed (%s); timing the C oracle instead" % e)
-    import numpy as np
-    n2 = 128
+    n2 = min(n, 128)
     g2 = O.make_geom((n2,) * 3, 0, [-1.0] * 3, 2.0 / n2, (0, 0, 0))
-    w2 = np.asfortranarray(w64[:, ::2, ::2, ::2])
+    step = n // n2
+    w2 = np.asfortranarray(w64[:, ::step, ::step, ::step])
     Gh = O.poisson_setup(g2, p)
     t = time.time()
-    O.poisson_solve(g2, p, Gh, w2)
+    O.poisson_solve(g2, p, Gh, w2, False, False, calc_vecpot)
     dt = time.time() - t
-    scale = (2 * N) ** 3 * math.log(2 * N) / ((2 * n2) ** 3 * math.log(2 * n2))
+    scale = (2 * n) ** 3 * math.log(2 * n) / ((2 * n2) ** 3 * math.log(2 * n2))
     return {"value": 1.0 / (dt * scale), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "one 128^3 solve of the C oracle (1 thread), scaled by L^3 log L to 256^3"}
+            "sample": "one %d^3 solve of the C oracle (1 thread), scaled by L^3 log L to %d^3" % (n2, n)}
 
 
-STEP_MIN_VORT = 1e-6   # remesh threshold of the vortex step (peak |omega| is ~41)
+def cpu_reference_step(n=128, nsteps=6):
+    """The two-ring vortex step of the compiled reference at n^3 with all
+    n^3 particles in every step (the harness re-loads them before each),
+    under MPI with cpu_ranks() ranks (each keeps its subdomain's
+    particles): one warm-up step, the median of the rest."""
+    import numpy as np
+    from oracle import oracle as O
+    from tests.helpers import ring_particles
+    if not (O.ref_available() and os.path.exists(O.MPIEXEC)):
+        return None
+    h = 2.0 / n
+    g = O.make_geom((n,) * 3, 0, [-1.0] * 3, h, (0, 0, 0), 3)
+    p = O.make_params(6, 1, 2.0)
+    pos, vort = ring_particles(n)
+    ranks = cpu_ranks()
+    try:
+        t = time.time()
+        _, out = O.ref_run(dict(O.geom_job(g, p), mode="step", npart=pos.shape[1], rk_order=2,
+                                dtime=0.5 * h, nsteps=nsteps, min_vort=STEP_MIN_VORT, min_dvort=0.0,
+                                reload=True), [pos, vort], nproc=ranks, timeout=400)
+    except Exception as e:  # pragma: no cover - reported, not hidden
+        log("cpu reference step failed (%s)" % e)
+        return None
+    reps = _reps(out)
+    med = float(np.median(reps[1:]))
+    log("cpu reference step: ranks %d, steps %s s (wall %.1f s); %s" %
+        (ranks, " ".join("%.3f" % r for r in reps), time.time() - t,
+         [l for l in out.splitlines() if l.startswith("TIMING")][0]))
+    return {"value": pos.shape[1] / med / 1e6, "unit": "Mparticle-steps/s", "cores": ranks,
+            "kind": "reference",
+            "sample": "two-ring vortex step at %d^3 on all %d particles (order-6 solve, M'4, RK2, remesh), "
+                      "FP64, native Glassman FFT, mpiexec -n %d (the job's CPU cores); 1 warm-up + median "
+                      "of %d" % (n, pos.shape[1], ranks, nsteps - 1)}
 
 
-def step_benchmark(steps, warmup, n=128, precision=0):
-    """The two-ring vortex step (config of 128^3 cells, 128^3 particles on the
-    jittered cell-centre lattice, order-6 solve, M'4, RK2, dt = h/2, remesh
-    with min_vort = 1e-6) on the GPU, every array resident in HBM; precision
-    0 is FP64 throughout, 1 the FP32 mesh side (particles FP64).
-    Returns (Mparticle-steps/s, ms per step, particle count)."""
+# ------------------------------------------------------------------ GPU parts
+def time_solves(s, d_in, d_out, k, warm=2, **kw):
+    """Seconds per solve over k stream-ordered solves (after warm-up)."""
+    import torch
+    for _ in range(warm):
+        s.solve_device(d_in.data_ptr(), d_out.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        s.solve_device(d_in.data_ptr(), d_out.data_ptr(), **kw)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / k
+
+
+def ring64_benchmark(reps):
+    """64^3 Gaussian ring on [-1,1]^3 (R = 0.5, sigma = 0.1, Gamma = 1, axis
+    z), order 4, eps = 2h, free space: stream function and velocity
+    (calc_vecpot) per solve, FP64 (the plain form) and FP32."""
+    import numpy as np
+    import torch
+    import pmlib3d_amd as pm
+    from oracle import oracle as O
+    from tests.helpers import gaussian_ring
+    n = 64
+    h = 2.0 / n
+    g = O.make_geom((n,) * 3, 0, [-1.0] * 3, h, (0, 0, 0))
+    w = gaussian_ring(g)
+    patch = pm.Patch(xmin=(-1.0,) * 3, xmax=(1.0,) * 3, ncell=(n,) * 3, dx=(h,) * 3)
+    dev = torch.device("cuda:0")
+    res = {"workload": "64^3 free-space solve of a Gaussian ring (R=0.5, sigma=0.1, Gamma=1, axis z) on "
+                       "[-1,1]^3, order 4, eps 2h, stream function + velocity (calc_vecpot)"}
+    outs = {}
+    for name, prec, dt, sz in (("fp64", pm.FP64, torch.float64, 8), ("fp32", pm.FP32, torch.float32, 4)):
+        s = pm.PoissonSolver(patch, prec, order=4, kernel=1, eps_radius=2.0, interp_order=3)
+        s.setup()
+        d_w = torch.from_numpy(w.ravel(order="F")).to(dev).to(dt)
+        d_v, d_p = torch.empty_like(d_w), torch.empty_like(d_w)
+        sec = time_solves(s, d_w, d_v, reps, d_vecpot=d_p.data_ptr(), calc_vecpot=True)
+        nbytes = free_bytes(n, 3, 6, sz, sz)[0]
+        res[name] = {"solves_per_s": 1.0 / sec, "ms": 1e3 * sec, "algorithmic_bytes": nbytes,
+                     "achieved_GBs": nbytes / sec / 1e9, "frac_hbm": nbytes / sec / 1e9 / PEAK_HBM_GBS}
+        outs[name] = (d_v.clone(), d_p.clone())
+        del s
+    res["fp32"]["rel_l2_vs_fp64"] = {"vel": rel_l2_t(outs["fp32"][0], outs["fp64"][0]),
+                                     "psi": rel_l2_t(outs["fp32"][1], outs["fp64"][1])}
+    log("ring64: fp64 %.0f solves/s (%.3f ms), fp32 %.0f solves/s; fp32 err vel %.2e psi %.2e" %
+        (res["fp64"]["solves_per_s"], res["fp64"]["ms"], res["fp32"]["solves_per_s"],
+         res["fp32"]["rel_l2_vs_fp64"]["vel"], res["fp32"]["rel_l2_vs_fp64"]["psi"]))
+    return res, np.asfortranarray(w)
+
+
+def step_benchmark(steps, n=128):
+    """The two-ring vortex step at n^3 (128^3 particles on the jittered
+    cell-centre lattice, order-6 solve, M'4, RK2, dt = h/2, remesh with
+    min_vort = 1e-6), every array resident in HBM.  Each timed step starts
+    from the full lattice (re-loaded device to device outside the timed
+    region), so every step advances n^3 particles.  FP64 throughout, and the
+    FP32 mesh side (particles FP64); the FP32 error is that of one step's
+    velocities and vorticities against FP64, particles matched by position."""
+    import numpy as np
     import torch
     import pmlib3d_amd as pm
     from tests.helpers import ring_particles
     h = 2.0 / n
     patch = pm.Patch(xmin=(-1.0,) * 3, xmax=(1.0,) * 3, ncell=(n,) * 3, dx=(h,) * 3)
     pos, vort = ring_particles(n)
-    st = pm.VortexStep(patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3,
-                       capacity=max(pos.shape[1], n ** 3), precision=precision)
-    st.load(pos, vort)
-    for _ in range(warmup):
-        st.step(0.5 * h, STEP_MIN_VORT)
-    torch.cuda.synchronize()
-    counted = 0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        counted += st.np
-        st.step(0.5 * h, STEP_MIN_VORT)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    return counted / dt / 1e6, 1e3 * dt / steps, st.np
-
-
-def patch_bytes(n, s, s_g, comps=1):
-    """Algorithmic bytes of one free-space solve of a patch n = (nx, ny, nz)
-    with the 5-pass implicit-padding schedule (comps components in and out,
-    G-hat as its real octant)."""
-    Mx, My, Mz = n
-    Lx, Ly, Lz = 2 * Mx, 2 * My, 2 * Mz
-    Hx, Hy, Hz = Lx // 2 + 1, Ly // 2 + 1, Lz // 2 + 1
-    c = 2 * s
-    b = comps * (Mx * My * Mz * s + Hx * My * Mz * c)           # x forward
-    b += comps * (Hx * My * Mz * c + Hx * Ly * Mz * c)          # y forward
-    b += comps * 2 * Hx * Ly * Mz * c + Hx * Hy * Hz * s_g      # fused z
-    b += comps * (Hx * Ly * Mz * c + Hx * My * Mz * c)          # y inverse
-    b += comps * (Hx * My * Mz * c + Mx * My * Mz * s)          # x inverse
-    return b
+    npart = pos.shape[1]
+    dev = torch.device("cuda:0")
+    d_pos = torch.from_numpy(pos.ravel(order="F")).to(dev)
+    d_vort = torch.from_numpy(vort.ravel(order="F")).to(dev)
+    res = {"workload": "two-ring vortex step, 128^3 free, all 128^3 = %d jittered particles every step "
+                       "(re-loaded before each), order 6, eps 2h, M'4, RK2 dt=h/2, remesh min_vort=1e-6; "
+                       "FP32 = FP32 meshes, solve and Kahan-summed interpolation, FP64 particles" % npart}
+    fetched = {}
+    for name, prec in (("fp64", pm.FP64), ("fp32", pm.FP32)):
+        st = pm.VortexStep(patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3,
+                           capacity=max(npart, n ** 3), precision=prec)
+        st.load_device(d_pos, d_vort, npart)
+        st.step(0.5 * h, STEP_MIN_VORT)  # warm-up
+        fetched[name] = st.fetch()
+        kept = st.np
+        total = 0.0
+        for _ in range(steps):
+            st.load_device(d_pos, d_vort, npart)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st.step(0.5 * h, STEP_MIN_VORT)
+            torch.cuda.synchronize()
+            total += time.perf_counter() - t0
+        sec = total / steps
+        res[name] = {"Mparticle_steps_per_s": npart / sec / 1e6, "ms_per_step": 1e3 * sec, "particles": npart,
+                     "particles_after_remesh": kept}
+        del st
+    (p64, w64, v64), (p32, w32, v32) = fetched["fp64"], fetched["fp32"]
+    key = {tuple(c): i for i, c in enumerate(p64.T)}
+    pairs = [(key[tuple(c)], j) for j, c in enumerate(p32.T) if tuple(c) in key]
+    i64, i32 = (np.array(x) for x in zip(*pairs))
+    rl2 = lambda a, b: float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+    res["fp32"]["rel_l2_vs_fp64"] = {"vel": rl2(v32[:, i32], v64[:, i64]), "vort": rl2(w32[:, i32], w64[:, i64]),
+                                     "matched_particles": len(pairs)}
+    log("step128: fp64 %.1f, fp32 %.1f Mparticle-steps/s (%.3f / %.3f ms, %d particles); fp32 err vel %.2e" %
+        (res["fp64"]["Mparticle_steps_per_s"], res["fp32"]["Mparticle_steps_per_s"], res["fp64"]["ms_per_step"],
+         res["fp32"]["ms_per_step"], npart, res["fp32"]["rel_l2_vs_fp64"]["vel"]))
+    return res
 
 
 def ring_patch(n, h, rng):
@@ -176,38 +313,123 @@ def ring_patch(n, h, rng):
 
 def batch_benchmark(steps, warmup, npatch=24, seed=4):
     """24 free-space patches, per-axis sizes drawn from {32,48,64,96,128,160}
-    (seeded), h = 1/128, single-component FP32 solves of order 4 (eps = 2h),
-    all patches replayed as one captured HIP graph.  Returns batches/s, the
-    algorithmic bytes per batch and the patch sizes."""
+    (seeded), h = 1/128, single-component solves of order 4 (eps = 2h), all
+    patches replayed as one captured HIP graph; FP32 timed, FP64 beside it
+    for the rate and the error."""
     import numpy as np
     import torch
     import pmlib3d_amd as pm
     rng = np.random.default_rng(seed)
     h = 1.0 / 128
     sizes = [tuple(int(v) for v in rng.choice([32, 48, 64, 96, 128, 160], 3)) for _ in range(npatch)]
+    fields = [ring_patch(n, h, rng) for n in sizes]
     dev = torch.device("cuda:0")
-    solvers, ins, outs = [], [], []
-    for n in sizes:
-        patch = pm.Patch(xmin=tuple(-v * h / 2 for v in n), xmax=tuple(v * h / 2 for v in n), ncell=n,
-                         dx=(h,) * 3)
-        sv = pm.PoissonSolver(patch, pm.FP32, order=4, kernel=1, eps_radius=2.0, interp_order=3)
-        sv.setup()
-        w = torch.from_numpy(ring_patch(n, h, rng).ravel(order="F")).to(dev).float()
-        solvers.append(sv)
-        ins.append(w)
-        outs.append(torch.zeros_like(w))
-    batch = pm.PoissonBatch(solvers)
-    pi, po = [t.data_ptr() for t in ins], [t.data_ptr() for t in outs]
-    for _ in range(warmup):
-        batch.solve_scalar(pi, po)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch.solve_scalar(pi, po)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
-    nbytes = sum(patch_bytes(n, 4, 4) for n in sizes)
-    return 1.0 / dt, nbytes, sizes
+    res = {"workload": "24 free-space patches, sizes per axis from {32,48,64,96,128,160} (seed 4), h=1/128, "
+                       "one Gaussian ring each (R=0.3 min extent, sigma=0.15R, random axis), single-component psi "
+                       "solve, order 4, eps 2h, one captured HIP graph per batch", "patches": sizes}
+    outs = {}
+    for name, prec, dt, sz in (("fp32", pm.FP32, torch.float32, 4), ("fp64", pm.FP64, torch.float64, 8)):
+        solvers, ins, o = [], [], []
+        for n, f in zip(sizes, fields):
+            patch = pm.Patch(xmin=tuple(-v * h / 2 for v in n), xmax=tuple(v * h / 2 for v in n), ncell=n,
+                             dx=(h,) * 3)
+            sv = pm.PoissonSolver(patch, prec, order=4, kernel=1, eps_radius=2.0, interp_order=3)
+            sv.setup()
+            solvers.append(sv)
+            ins.append(torch.from_numpy(f.ravel(order="F")).to(dev).to(dt))
+            o.append(torch.zeros_like(ins[-1]))
+        batch = pm.PoissonBatch(solvers)
+        pi, po = [t.data_ptr() for t in ins], [t.data_ptr() for t in o]
+        for _ in range(warmup):
+            batch.solve_scalar(pi, po)
+        torch.cuda.synchronize()
+        k = steps if name == "fp32" else max(2, steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            batch.solve_scalar(pi, po)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / k
+        nbytes = sum(free_bytes(n, 1, 1, sz, sz)[0] for n in sizes)
+        res[name] = {"batches_per_s": 1.0 / sec, "ms_per_batch": 1e3 * sec, "algorithmic_bytes": nbytes,
+                     "achieved_GBs": nbytes / sec / 1e9, "frac_hbm": nbytes / sec / 1e9 / PEAK_HBM_GBS}
+        outs[name] = o
+        del batch, solvers
+    res["fp32"]["rel_l2_vs_fp64_max_over_patches"] = max(rel_l2_t(a, b) for a, b in zip(outs["fp32"], outs["fp64"]))
+    log("batch24: fp32 %.1f batches/s (%.3f ms, %.0f GB/s), fp64 %.1f; fp32 err %.2e" %
+        (res["fp32"]["batches_per_s"], res["fp32"]["ms_per_batch"], res["fp32"]["achieved_GBs"],
+         res["fp64"]["batches_per_s"], res["fp32"]["rel_l2_vs_fp64_max_over_patches"]))
+    return res
+
+
+def mixed384_benchmark(reps, nsteps, n=384):
+    """384^2 x 384 on [-1,1]^3 (h = 2/384), periodic x and y, free z (FFT
+    384 x 384 x 768 with implicit padding in z), three components, order 4,
+    eps 2h.  Vorticity: seeded random solenoidal Fourier modes with
+    E(k) ~ k^4 exp(-2 (k/8)^2) times exp(-z^2 / 0.3^2)
+    (tests.helpers.turbulence_vorticity).  FP32 and FP64 solve rates and
+    the FP32 error against FP64; then nsteps remeshed FP32 vortex steps on
+    that grid (particles seeded on every cell with |omega| > 1e-6 max|omega|,
+    M'4, RK2, dt = h/2)."""
+    import torch
+    import pmlib3d_amd as pm
+    from tests.helpers import turbulence_vorticity
+    h = 2.0 / n
+    patch = pm.Patch(xmin=(-1.0,) * 3, xmax=(1.0,) * 3, ncell=(n,) * 3, dx=(h,) * 3, bound_cond=(1, 1, 0))
+    E = pm.mesh_extent(patch, 3)
+    dev = torch.device("cuda:0")
+    t = time.time()
+    w_int = turbulence_vorticity(n, torch_device=dev)   # (3, n, n, n) flat, Fortran order
+    w64 = torch.zeros(3 * E[0] * E[1] * E[2], dtype=torch.float64, device=dev)
+    lo = [(E[d] - n) // 2 for d in range(3)]
+    w64.view(E[2], E[1], E[0], 3)[lo[2]:lo[2] + n, lo[1]:lo[1] + n, lo[0]:lo[0] + n, :] = \
+        w_int.view(n, n, n, 3)
+    del w_int
+    wmax = float(w64.abs().max())
+    log("mixed384: field built in %.1f s, max |omega| %.3f" % (time.time() - t, wmax))
+    res = {"workload": "384^2 x 384 on [-1,1]^3, periodic x,y, free z (FFT 384x384x768, implicit z padding), 3 "
+                       "components, order 4, eps 2h; seeded solenoidal random Fourier modes E(k)~k^4 exp(-2(k/8)^2) "
+                       "times exp(-z^2/0.3^2)"}
+    M, L = (n, n, n), (n, n, 2 * n)
+    vel = {}
+    for name, prec, dt, sz in (("fp32", pm.FP32, torch.float32, 4), ("fp64", pm.FP64, torch.float64, 8)):
+        s = pm.PoissonSolver(patch, prec, order=4, kernel=1, eps_radius=2.0, interp_order=3)
+        s.setup()
+        d_w = w64.to(dt)
+        d_v = torch.zeros_like(d_w)
+        sec = time_solves(s, d_w, d_v, reps if name == "fp32" else max(2, reps // 2))
+        nbytes = solve_bytes(M, L, 3, 3, sz, sz)[0]
+        res[name] = {"solves_per_s": 1.0 / sec, "ms": 1e3 * sec, "algorithmic_bytes": nbytes,
+                     "achieved_GBs": nbytes / sec / 1e9, "frac_hbm": nbytes / sec / 1e9 / PEAK_HBM_GBS}
+        vel[name] = d_v
+        del s, d_w
+    res["fp32"]["rel_l2_vs_fp64"] = rel_l2_t(vel["fp32"], vel["fp64"])
+    log("mixed384 solve: fp32 %.1f solves/s (%.3f ms), fp64 %.1f; fp32 err %.2e" %
+        (res["fp32"]["solves_per_s"], res["fp32"]["ms"], res["fp64"]["solves_per_s"], res["fp32"]["rel_l2_vs_fp64"]))
+    del vel
+    if nsteps > 0:
+        torch.cuda.empty_cache()
+        st = pm.VortexStep(patch, order=4, eps_radius=2.0, rk_order=2, interp_order=3,
+                           capacity=E[0] * E[1] * E[2], precision=pm.FP32)
+        w32 = w64.float()
+        npart0 = st.seed_from_mesh(w32.data_ptr(), 1e-6 * wmax)
+        del w32
+        torch.cuda.synchronize()
+        counted = 0
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            counted += st.np
+            st.step(0.5 * h, 1e-6 * wmax)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        res["steps"] = {"steps": nsteps, "seconds": sec, "ms_per_step": 1e3 * sec / nsteps,
+                        "particles_initial": npart0, "particles_final": st.np,
+                        "Mparticle_steps_per_s": counted / sec / 1e6, "dtype": "f32 mesh side, f64 particles",
+                        "workload": "remeshed vortex steps on the mixed-BC grid: M'4, RK2, dt=h/2, order 4, "
+                                    "remesh |omega| > 1e-6 max|omega|"}
+        log("mixed384 steps: %d steps in %.2f s (%.1f ms/step), %d -> %d particles, %.1f Mparticle-steps/s" %
+            (nsteps, sec, 1e3 * sec / nsteps, npart0, st.np, counted / sec / 1e6))
+        del st
+    return res
 
 
 def rhs_benchmark(reps, n=256):
@@ -226,9 +448,13 @@ def rhs_benchmark(reps, n=256):
     g = torch.Generator(device=dev).manual_seed(7)
     res = {"workload": "256^3 free patch, 2 ghost layers, 4th-order differences, visc 0.01, "
                        "seeded normal w and u"}
+    outs = {}
     for name, prec, dt, s in (("fp64", pm.FP64, torch.float64, 8), ("fp32", pm.FP32, torch.float32, 4)):
-        w = torch.randn(int(np.prod(ext)), dtype=dt, device=dev, generator=g)
-        u = torch.randn_like(w)
+        if name == "fp64":
+            w = torch.randn(int(np.prod(ext)), dtype=dt, device=dev, generator=g)
+            u = torch.randn_like(w)
+        else:
+            w, u = outs["w"].float(), outs["u"].float()
         dw = torch.zeros_like(w)
         run = lambda: pm.vorticity_rhs_device(patch, prec, w.data_ptr(), u.data_ptr(), dw.data_ptr(), 0.01,
                                               fd_order=4, interp_order=3)
@@ -245,34 +471,11 @@ def rhs_benchmark(reps, n=256):
         nbytes = 9 * s * n ** 3
         res[name] = {"ms": ms, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / ms / 1e6,
                      "frac_hbm": nbytes / ms / 1e6 / PEAK_HBM_GBS}
+        if name == "fp64":
+            outs.update(w=w, u=u, dw=dw)
+        else:
+            res[name]["rel_l2_vs_fp64"] = rel_l2_t(dw, outs["dw"])
     return res
-
-
-def cpu_step_baseline(n=128):
-    """One two-ring vortex step of the compiled reference (one rank: its step
-    mode is single-rank), FP64."""
-    from oracle import oracle as O
-    from tests.helpers import ring_particles
-    if not (O.ref_available() and os.path.exists(O.MPIEXEC)):
-        return None
-    h = 2.0 / n
-    g = O.make_geom((n,) * 3, 0, [-1.0] * 3, h, (0, 0, 0), 3)
-    p = O.make_params(6, 1, 2.0)
-    pos, vort = ring_particles(n)
-    try:
-        _, out = O.ref_run(dict(O.geom_job(g, p), mode="step", npart=pos.shape[1], rk_order=2,
-                                dtime=0.5 * h, nsteps=1, min_vort=STEP_MIN_VORT, min_dvort=0.0),
-                           [pos, vort], timeout=400)
-    except Exception as e:  # pragma: no cover - reported, not hidden
-        log("cpu reference step failed (%s)" % e)
-        return None
-    line = [l for l in out.splitlines() if l.startswith("TIMING")][0]
-    step_s = float(line.split("step_s=")[1].split()[0])
-    log("cpu reference step: %s" % line)
-    return {"value": pos.shape[1] / step_s / 1e6, "unit": "Mparticle-steps/s", "cores": 1,
-            "kind": "reference",
-            "sample": "one two-ring vortex step at 128^3 (2.1M particles, order-6 solve, M'4, RK2, "
-                      "remesh), FP64, native Glassman FFT, 1 rank"}
 
 
 Z_KERNEL = "k_z3_r<float, 512, 4, false, true>"
@@ -300,10 +503,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--ring-steps", type=int, default=50, help="timed 64^3 ring solves (0 skips)")
     ap.add_argument("--step-steps", type=int, default=5,
-                    help="timed two-ring vortex steps at 128^3 (0 skips)")
+                    help="timed two-ring vortex steps at 128^3 on 2.1M particles (0 skips)")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="timed 24-patch batched single-component solves (0 skips)")
+    ap.add_argument("--mixed-steps", type=int, default=10, help="timed 384^2x384 mixed-BC solves (0 skips)")
+    ap.add_argument("--mixed-vortex-steps", type=int, default=100,
+                    help="remeshed FP32 vortex steps on the mixed-BC grid")
     ap.add_argument("--rhs-steps", type=int, default=20,
                     help="timed vorticity right-hand-side launches at 256^3 (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -314,7 +521,6 @@ def main():
               "(multi-GPU pencil decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
     import pmlib3d_amd as pm
 
@@ -332,6 +538,7 @@ def main():
     d_wp = torch.empty_like(d_w)
     s64.solve_device(d_w.data_ptr(), d_v.data_ptr(), d_wp.data_ptr(), None, reg_vort=False, reproj=True)
     torch.cuda.synchronize()
+    del d_w, d_v
     w64 = d_wp
     w32 = w64.to(torch.float32)
     v32 = torch.empty_like(w32)
@@ -374,58 +581,48 @@ def main():
         for _ in range(3):
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
+        del src, dst
         log("copy calibration: 3 x torch.neg over 1 GiB (1 GiB read + 1 GiB written)")
 
-    a = v32.double()
-    b = v64
-    err = float(torch.sqrt(torch.sum((a - b) ** 2) / torch.sum(b * b)))
-
+    err = rel_l2_t(v32, v64)
     sps32 = args.steps / t32
     sps64 = args.fp64_steps / t64
-    bytes32, z_bytes32 = algorithmic_bytes(N, 4, 4)
-    bytes64, z_bytes64 = algorithmic_bytes(N, 8, 8)
+    bytes32, z_bytes32 = free_bytes(N, 3, 3, 4, 4)
+    bytes64, z_bytes64 = free_bytes(N, 3, 3, 8, 8)
     z_gbs = z_bytes32 / (z32_ms * 1e-3) / 1e9
     log("fp32: %.1f solves/s (%.3f ms, events %.3f ms/solve, z pass %.3f ms)" %
         (sps32, 1e3 / sps32, tot32_ms, z32_ms))
     log("fp64: %.1f solves/s (%.3f ms, events %.3f ms/solve, z pass %.3f ms)" %
         (sps64, 1e3 / sps64, tot64_ms, z64_ms))
     log("fp32 vs fp64 relative L2 velocity error %.3e" % err)
+    w_cpu = w64.cpu().numpy().reshape((3, N, N, N), order="F")
+    del s32, s64, w32, v32, v64, w64
+    torch.cuda.empty_cache()
 
-    step = None
-    if args.step_steps > 0:
-        del s32, s64
-        step = {"workload": "two-ring vortex step, 128^3 free, 128^3 jittered particles, order 6, "
-                            "eps 2h, M'4, RK2 dt=h/2, remesh min_vort=1e-6; FP32 = FP32 meshes, solve and "
-                            "Kahan-summed interpolation, FP64 particles"}
-        for name, prec in (("fp64", 0), ("fp32", 1)):
-            mps, ms, npart = step_benchmark(args.step_steps, 1, precision=prec)
-            log("vortex step %s: %.1f Mparticle-steps/s (%.3f ms/step, %d particles)" % (name, mps, ms, npart))
-            step[name] = {"Mparticle_steps_per_s": mps, "ms_per_step": ms, "particles": npart}
-
-    batch = None
-    if args.batch_steps > 0:
-        bps, nbytes, sizes = batch_benchmark(args.batch_steps, 2)
-        log("batched multipatch: %.1f batches/s (%.3f ms/batch, %.2f GB algorithmic, %.0f GB/s)" %
-            (bps, 1e3 / bps, nbytes / 1e9, nbytes * bps / 1e9))
-        batch = {"batches_per_s": bps, "ms_per_batch": 1e3 / bps, "algorithmic_bytes": nbytes,
-                 "achieved_GBs": nbytes * bps / 1e9, "frac_hbm": nbytes * bps / 1e9 / PEAK_HBM_GBS,
-                 "dtype": "f32", "patches": sizes,
-                 "workload": "24 free-space patches, sizes per axis from {32,48,64,96,128,160} (seed 4), "
-                             "h=1/128, one Gaussian ring each (R=0.3 min extent, sigma=0.15R, random axis), "
-                             "single-component psi solve, order 4, eps 2h, one captured HIP graph per batch"}
-
+    ring = ring_w = None
+    if args.ring_steps > 0:
+        ring, ring_w = ring64_benchmark(args.ring_steps)
+    step = step_benchmark(args.step_steps) if args.step_steps > 0 else None
+    batch = batch_benchmark(args.batch_steps, 2) if args.batch_steps > 0 else None
+    mixed = None
+    if args.mixed_steps > 0:
+        mixed = mixed384_benchmark(args.mixed_steps, args.mixed_vortex_steps)
+        torch.cuda.empty_cache()
     rhs = None
     if args.rhs_steps > 0:
         rhs = rhs_benchmark(args.rhs_steps)
         log("vorticity rhs 256^3: fp64 %.3f ms (%.0f GB/s), fp32 %.3f ms (%.0f GB/s)" %
             (rhs["fp64"]["ms"], rhs["fp64"]["achieved_GBs"], rhs["fp32"]["ms"], rhs["fp32"]["achieved_GBs"]))
 
-    cpu = None
-    cpu_step = None
+    cpu = cpu_ring = cpu_step = None
     if not args.no_cpu_baseline:
-        cpu = cpu_reference_baseline(w64.cpu().numpy().reshape((3, N, N, N), order="F"))
+        cpu = cpu_reference_solve(w_cpu, N, ORDER)
+        if ring is not None:
+            cpu_ring = cpu_reference_solve(ring_w, 64, 4, calc_vecpot=True)
+            ring["cpu_baseline"] = cpu_ring
         if step is not None:
-            cpu_step = cpu_step_baseline()
+            cpu_step = cpu_reference_step()
+            step["cpu_baseline"] = cpu_step
 
     out = {
         "metric": "unbounded-Poisson solves/s + Mparticle-steps/s at 256³, FP32 vs FP64 GPU",
@@ -450,19 +647,20 @@ def main():
         "fp32_over_fp64": sps32 / sps64,
         "fp32_rel_l2_vs_fp64": err,
         "solve_algorithmic_bytes": {"fp32": bytes32, "fp64": bytes64},
-        "solve_achieved_GBs": {"fp32": bytes32 / (1e-3 / sps32 * 1e3) / 1e9 * 1e3 / 1e3,
-                               "fp64": bytes64 * sps64 / 1e9},
+        "solve_achieved_GBs": {"fp32": bytes32 * sps32 / 1e9, "fp64": bytes64 * sps64 / 1e9},
+        "solve_frac_hbm": {"fp32": bytes32 * sps32 / 1e9 / PEAK_HBM_GBS,
+                           "fp64": bytes64 * sps64 / 1e9 / PEAK_HBM_GBS},
+        "ring64": ring,
         "vortex_step": step,
         "batched_multipatch": batch,
+        "mixed384": mixed,
         "vorticity_rhs": rhs,
-        "cpu_baseline_step": cpu_step,
         "roofline": {"bound": "hbm", "kernel": Z_KERNEL + " (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": z_gbs / PEAK_HBM_GBS, "traffic": measured_traffic(),
                      "algorithmic_bytes_per_launch": z_bytes32, "avg_launch_ms": z32_ms},
         "cpu_baseline": cpu,
     }
-    out["solve_achieved_GBs"]["fp32"] = bytes32 * sps32 / 1e9
     print(json.dumps(out), flush=True)
 
 

@@ -18,7 +18,13 @@
 !   m2p      : pmlib_interp_mesh_particle (vel)             (interpolation/)
 !   remesh   : pmlib_remesh                                 (remesh/)
 !   advance  : pmlib_particles_advance                      (particles/)
-!   step     : 2 x (P2M, solve, M2P, advance) + (P2M, remesh), timed
+!   step     : 2 x (P2M, solve, M2P, advance) + (P2M, remesh), timed; with
+!              reload=.true. every step restarts from the input particles
+!              (timing a fixed particle count); on several ranks each rank
+!              keeps the input particles inside its own subdomain
+!
+! Timing lines: TIMING (summary) and REPS (every repetition / step, seconds,
+! slowest rank).
 !-------------------------------------------------------------------------------
 PROGRAM ref_driver
   USE pmlib_mod_parameters
@@ -45,13 +51,14 @@ PROGRAM ref_driver
   REAL(MK)           :: eps_radius = 2.0_MK
   LOGICAL            :: reg_vort = .FALSE., reproj = .FALSE., calc_vecpot = .FALSE.
   LOGICAL            :: clear = .TRUE., incl_ghost = .FALSE., euler_vort = .FALSE.
+  LOGICAL            :: reload = .FALSE.
   INTEGER            :: npart = 0, stage = 1, nrep = 1, nsteps = 1
   REAL(MK)           :: dtime = 0.0_MK, min_vort = 0.0_MK, min_dvort = 0.0_MK
   CHARACTER(LEN=512) :: infile = 'in.bin', outfile = 'out.bin'
   NAMELIST /job/ mode, ncell, nghost, bound_cond, fft_seq, xmin, xmax, dx, order, kernel, &
        & interp_order, rk_order, eps_radius, reg_vort, reproj, calc_vecpot, clear, &
        & incl_ghost, euler_vort, npart, stage, nrep, nsteps, dtime, min_vort, min_dvort, &
-       & infile, outfile
+       & infile, outfile, reload
 
   TYPE(class_patch)        :: patch
   TYPE(class_topology_all) :: topo_all
@@ -62,6 +69,8 @@ PROGRAM ref_driver
   REAL(MK), ALLOCATABLE    :: vort_in(:,:,:,:)
   REAL(MK)                 :: t0, t1, tsetup, tmin, tsum
   REAL(MK)                 :: tp2m, tsolve, tm2p, tadv, tremesh
+  REAL(MK), ALLOCATABLE    :: treps(:), pos0(:,:), vort0(:,:)
+  INTEGER                  :: np0, ntot
 
   CALL MPI_INIT(ierr)
   CALL MPI_COMM_RANK(MPI_COMM_WORLD, rank, ierr)
@@ -97,7 +106,8 @@ PROGRAM ref_driver
   ng6 = topo_all%cuboid(rank)%nghost
   lo = 1 - ng6(1:5:2)
   hi = topo_all%cuboid(rank)%ncell + ng6(2:6:2)
-  IF (nproc .GT. 1 .AND. mode .NE. 'solve') CALL die('only solve runs on several ranks')
+  IF (nproc .GT. 1 .AND. mode .NE. 'solve' .AND. mode .NE. 'step') &
+       & CALL die('only solve and step run on several ranks')
 
   CALL pmlib_mesh_allocate(topo_all%cuboid, mesh, ierr, vort=.TRUE., dvort=.TRUE., vel=.TRUE., &
        & mask=.FALSE.)
@@ -126,6 +136,7 @@ PROGRAM ref_driver
     vort_in = mesh%vort
     tmin = HUGE(1.0_MK)
     tsum = 0.0_MK
+    ALLOCATE(treps(nrep))
     DO irep = 1, nrep
       IF (.NOT. ASSOCIATED(mesh%vort)) ALLOCATE(mesh%vort(3, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)))
       mesh%vort = vort_in
@@ -138,9 +149,11 @@ PROGRAM ref_driver
       IF (ierr .NE. 0) CALL die('poisson solve')
       tmin = MIN(tmin, t1)
       tsum = tsum + t1
+      treps(irep) = t1
     END DO
     IF (rank .EQ. 0) WRITE(*,'(A,I4,A,E16.8,A,E16.8,A,E16.8)') 'TIMING ranks=', nproc, &
          & ' setup_s=', tsetup, ' solve_min_s=', tmin, ' solve_mean_s=', tsum/nrep
+    IF (rank .EQ. 0) WRITE(*,'(A,1000E16.8)') 'REPS', treps
     IF (nproc .EQ. 1) THEN
       OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
       WRITE(20) mesh%vel
@@ -207,14 +220,24 @@ PROGRAM ref_driver
     ! One vortex time step of the particle-mesh method, repeated nsteps times:
     ! nstage x (P2M -> solve -> M2P(vel) -> advance) then P2M -> remesh.
     CALL read_particles(.TRUE.)
+    IF (reload) THEN
+      np0 = topo_all%cuboid(rank)%npart
+      ALLOCATE(pos0(3, np0), vort0(3, np0))
+      pos0 = part%pos(:, 1:np0)
+      vort0 = part%vort(:, 1:np0)
+    END IF
     t0 = MPI_WTIME()
     CALL pmlib_poisson_setup(patch, topo_all, mesh, ierr)
     IF (ierr .NE. 0) CALL die('poisson setup')
     tsetup = MPI_WTIME() - t0
     nstage = rk_order
     tp2m = 0; tsolve = 0; tm2p = 0; tadv = 0; tremesh = 0
-    t1 = MPI_WTIME()
+    ALLOCATE(treps(nsteps))
+    t1 = 0.0_MK
     DO istep = 1, nsteps
+      IF (reload .AND. istep .GT. 1) CALL restore_particles()
+      CALL MPI_BARRIER(MPI_COMM_WORLD, ierr)
+      treps(istep) = MPI_WTIME()
       DO s = 1, nstage
         t0 = MPI_WTIME()
         CALL pmlib_interp_particle_mesh(topo_all%cuboid, part%pos, part%vort, mesh%vort, &
@@ -248,12 +271,19 @@ PROGRAM ref_driver
            & dvort=.FALSE., vel=.TRUE., mask=.FALSE., incl_ghost=.FALSE.)
       IF (ierr .NE. 0) CALL die('step remesh')
       tremesh = tremesh + MPI_WTIME() - t0
+      CALL MPI_BARRIER(MPI_COMM_WORLD, ierr)
+      treps(istep) = MPI_WTIME() - treps(istep)
+      t1 = t1 + treps(istep)
     END DO
-    t1 = MPI_WTIME() - t1
-    WRITE(*,'(A,E16.8,A,E16.8,A,5E16.8)') 'TIMING setup_s=', tsetup, ' step_s=', t1/nsteps, &
-         & ' parts(p2m,solve,m2p,adv,remesh)=', tp2m/nsteps, tsolve/nsteps, tm2p/nsteps, &
-         & tadv/nsteps, tremesh/nsteps
-    CALL write_particles()
+    CALL MPI_REDUCE(topo_all%cuboid(rank)%npart, ntot, 1, MPI_INTEGER, MPI_SUM, 0, MPI_COMM_WORLD, ierr)
+    IF (rank .EQ. 0) THEN
+      WRITE(*,'(A,E16.8,A,E16.8,A,5E16.8)') 'TIMING setup_s=', tsetup, ' step_s=', t1/nsteps, &
+           & ' parts(p2m,solve,m2p,adv,remesh)=', tp2m/nsteps, tsolve/nsteps, tm2p/nsteps, &
+           & tadv/nsteps, tremesh/nsteps
+      WRITE(*,'(A,1000E16.8)') 'REPS', treps
+      WRITE(*,'(A,I12,A,I4)') 'PARTICLES', ntot, ' ranks=', nproc
+    END IF
+    IF (nproc .EQ. 1) CALL write_particles()
 
   CASE DEFAULT
     CALL die('unknown mode')
@@ -294,6 +324,16 @@ CONTAINS
     CALL MPI_BCAST(calc_vecpot, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
     CALL MPI_BCAST(nrep, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
     CALL MPI_BCAST(interp_order, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(npart, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(stage, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(nsteps, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(dtime, 1, MPI_DOUBLE_PRECISION, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(min_vort, 1, MPI_DOUBLE_PRECISION, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(min_dvort, 1, MPI_DOUBLE_PRECISION, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(clear, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(incl_ghost, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(euler_vort, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(reload, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
     CALL MPI_BCAST(infile, 512, MPI_CHARACTER, 0, MPI_COMM_WORLD, e)
     CALL MPI_BCAST(outfile, 512, MPI_CHARACTER, 0, MPI_COMM_WORLD, e)
   END SUBROUTINE bcast_job
@@ -330,12 +370,48 @@ CONTAINS
 
   SUBROUTINE read_particles(with_rk)
     LOGICAL, INTENT(IN) :: with_rk
-    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
-    ALLOCATE(part%pos(3,npart), part%vel(3,npart), part%vort(3,npart), part%dvort(3,npart))
-    ALLOCATE(part%vel_rk(3,npart), part%dvort_rk(3,npart))
-    READ(10) part%pos
-    READ(10) part%vort
-    CLOSE(10)
+    REAL(MK), ALLOCATABLE :: gp(:,:), gv(:,:)
+    REAL(MK) :: sxmin(3), sxmax(3)
+    INTEGER  :: q, nloc
+    IF (nproc .EQ. 1) THEN
+      OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+      ALLOCATE(part%pos(3,npart), part%vel(3,npart), part%vort(3,npart), part%dvort(3,npart))
+      ALLOCATE(part%vel_rk(3,npart), part%dvort_rk(3,npart))
+      READ(10) part%pos
+      READ(10) part%vort
+      CLOSE(10)
+    ELSE
+      ! every rank reads the global set and keeps the particles inside its
+      ! subdomain [xmin_sub, xmax_sub) (the owner pmlib_particles_map uses)
+      ALLOCATE(gp(3,npart), gv(3,npart))
+      OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+      READ(10) gp
+      READ(10) gv
+      CLOSE(10)
+      sxmin = topo_all%cuboid(rank)%xmin
+      sxmax = topo_all%cuboid(rank)%xmax
+      nloc = 0
+      DO q = 1, npart
+        IF (ALL(gp(:,q) .GE. sxmin) .AND. ALL(gp(:,q) .LT. sxmax)) nloc = nloc + 1
+      END DO
+      ALLOCATE(part%pos(3,nloc), part%vel(3,nloc), part%vort(3,nloc), part%dvort(3,nloc))
+      ALLOCATE(part%vel_rk(3,nloc), part%dvort_rk(3,nloc))
+      nloc = 0
+      DO q = 1, npart
+        IF (ALL(gp(:,q) .GE. sxmin) .AND. ALL(gp(:,q) .LT. sxmax)) THEN
+          nloc = nloc + 1
+          part%pos(:,nloc) = gp(:,q)
+          part%vort(:,nloc) = gv(:,q)
+        END IF
+      END DO
+      DEALLOCATE(gp, gv)
+      topo_all%cuboid(rank)%npart = nloc
+      part%vel = 0.0_MK
+      part%dvort = 0.0_MK
+      part%vel_rk = 0.0_MK
+      part%dvort_rk = 0.0_MK
+      RETURN
+    END IF
     part%vel = 0.0_MK
     part%dvort = 0.0_MK
     part%vel_rk = 0.0_MK
@@ -343,6 +419,25 @@ CONTAINS
     topo_all%cuboid(rank)%npart = npart
     IF (.NOT. with_rk) RETURN
   END SUBROUTINE read_particles
+
+  ! the step's input particles again (reload mode): arrays sized as read
+  SUBROUTINE restore_particles()
+    IF (ASSOCIATED(part%pos)) DEALLOCATE(part%pos)
+    IF (ASSOCIATED(part%vel)) DEALLOCATE(part%vel)
+    IF (ASSOCIATED(part%vort)) DEALLOCATE(part%vort)
+    IF (ASSOCIATED(part%dvort)) DEALLOCATE(part%dvort)
+    IF (ASSOCIATED(part%vel_rk)) DEALLOCATE(part%vel_rk)
+    IF (ASSOCIATED(part%dvort_rk)) DEALLOCATE(part%dvort_rk)
+    ALLOCATE(part%pos(3,np0), part%vel(3,np0), part%vort(3,np0), part%dvort(3,np0))
+    ALLOCATE(part%vel_rk(3,np0), part%dvort_rk(3,np0))
+    part%pos = pos0
+    part%vort = vort0
+    part%vel = 0.0_MK
+    part%dvort = 0.0_MK
+    part%vel_rk = 0.0_MK
+    part%dvort_rk = 0.0_MK
+    topo_all%cuboid(rank)%npart = np0
+  END SUBROUTINE restore_particles
 
   SUBROUTINE write_particles()
     npart = topo_all%cuboid(rank)%npart

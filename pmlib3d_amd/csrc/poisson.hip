@@ -849,10 +849,16 @@ int pmgpu_poisson_batch_create(const pmgpu_poisson_t *handles, int32_t n, void *
   std::unique_ptr<pmgpu_poisson_batch_s> b(new pmgpu_poisson_batch_s);
   b->h.assign(handles, handles + n);
   b->stream = (hipStream_t)stream;
-  for (auto hh : b->h) {
-    PM_CHECK(hh, "pmgpu_poisson_batch_create: null handle");
-    if (hh->impl->tune_scalar()) return -1;  // tuning launches may not be captured
+  for (int i = 0; i < n; ++i) {
+    PM_CHECK(b->h[i], "pmgpu_poisson_batch_create: null handle");
+    // each handle owns its FFT workspaces: two branches of one graph on the
+    // same handle would write the same scratch concurrently
+    for (int j = 0; j < i; ++j)
+      PM_CHECK(b->h[j] != b->h[i], "pmgpu_poisson_batch_create: a solver handle appears twice in the batch "
+                                   "(each patch needs its own handle)");
   }
+  for (auto hh : b->h)
+    if (hh->impl->tune_scalar()) return -1;  // tuning launches may not be captured
   // side streams of the captured graph (PMGPU_BATCH_STREAMS, default 8)
   int nside = 8;
   if (const char *env = getenv("PMGPU_BATCH_STREAMS")) nside = atoi(env) > 0 ? atoi(env) : 8;

@@ -71,6 +71,32 @@ class VortexStep:
         a["vort"][:3 * n].copy_(self.torch.from_numpy(np.asfortranarray(vort).ravel(order="F")))
         self.np = n
 
+    def load_device(self, d_pos, d_vort, n):
+        """The same as load from device tensors: flat (3, n) Fortran-ordered
+        float64 positions and vorticities (device-to-device copies)."""
+        if n > self.cap:
+            raise _lib.PmgpuError("more particles than the step's capacity")
+        a = self.arrays
+        for k in _FIELDS:
+            a[k].zero_()
+        a["pos"][:3 * n].copy_(d_pos[:3 * n])
+        a["vort"][:3 * n].copy_(d_vort[:3 * n])
+        self.np = n
+
+    def seed_from_mesh(self, d_mesh_vort, min_vort):
+        """Particles from a vorticity mesh in the step's precision (3, E...):
+        one at each interior cell centre with |omega| > min_vort, carrying
+        the cell's value (pmlib_remesh, remesh/pmlib_remesh.f:84-226).
+        Returns the particle count."""
+        a = self.arrays
+        n = self.ops.remesh(d_mesh_vort, None, None, min_vort, 0.0,
+                            out=[a[k].data_ptr() for k in ("pos", "vort", "vel", "dvort", "vel_rk", "dvort_rk")],
+                            cap=self.cap, dvort=False, vel=False)
+        if n > self.cap:
+            raise _lib.PmgpuError("remesh made %d particles, capacity %d" % (n, self.cap))
+        self.np = n
+        return n
+
     def fetch(self):
         a = self.arrays
         n = self.np

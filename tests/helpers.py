@@ -102,3 +102,90 @@ def oracle_vortex_step(g, p, pos, vort, dt, nsteps, min_vort, Gh=None, visc=None
         pos, vort, vel, dvort = O.remesh(g, mesh, None, vel_m, min_vort, 0.0, False, dvort=False, vel=True)
         vel_rk, dvort_rk = np.zeros_like(pos), np.zeros_like(pos)
     return pos, vort, vel
+
+
+def plane_checksums(u, seed=12345):
+    """Size-independent fingerprint of a (3, E0, E1, E2) mesh field: per
+    component c and z plane k the weighted sum S[c, k] = sum_ij u[c,i,j,k] r[i,j]
+    with seeded normal weights r, plus the bound B = max|u| * sum|r| that a
+    relative L-inf difference of 1 between two fields would reach in any S."""
+    u = np.asarray(u)
+    r = np.random.default_rng(seed).normal(size=u.shape[1:3])
+    S = np.einsum("cijk,ij->ck", u, r)
+    return {"S": S.tolist(), "B": float(np.abs(u).max() * np.abs(r).sum()), "seed": seed,
+            "l2": [float(np.sqrt((u[c] ** 2).sum())) for c in range(u.shape[0])]}
+
+
+def particle_checksums(pos, vort, vel, seed=54321):
+    """Fingerprint of a (3, np) particle set: SHA-256 of the positions'
+    bytes (remeshed particles sit exactly on cell centres), and per component
+    weighted sums of vort and vel with seeded normal weights, with their
+    relative-L-inf bounds (max|a| * sum|r|)."""
+    import hashlib
+    r = np.random.default_rng(seed).normal(size=pos.shape[1])
+    out = {"np": int(pos.shape[1]), "seed": seed,
+           "pos_sha256": hashlib.sha256(np.ascontiguousarray(np.asfortranarray(pos).ravel(order="F"))
+                                        .tobytes()).hexdigest()}
+    for name, a in (("vort", vort), ("vel", vel)):
+        out[name] = {"S": (np.asarray(a) @ r).tolist(), "B": float(np.abs(a).max() * np.abs(r).sum())}
+    return out
+
+
+def turbulence_modes(kmax=32, seed=5):
+    """Seeded random Fourier coefficients of a solenoidal vector field on the
+    periodic box [-1, 1]^3: c[q, nx, ny, nz] for integer wave indices
+    nx, ny in [-kmax, kmax], nz in [0, kmax] (wavenumber pi |n|), complex
+    normal with variance E(k) / (4 pi k^2), E(k) = k^4 exp(-2 (k/8)^2),
+    k = |n|, zero outside |n| <= kmax, then projected onto the plane
+    perpendicular to n (divergence-free)."""
+    rng = np.random.default_rng(seed)
+    m = 2 * kmax + 1
+    shape = (3, m, m, kmax + 1)
+    c = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    n1 = np.arange(-kmax, kmax + 1, dtype=np.float64)
+    NX, NY, NZ = np.meshgrid(n1, n1, np.arange(kmax + 1, dtype=np.float64), indexing="ij")
+    k = np.sqrt(NX ** 2 + NY ** 2 + NZ ** 2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amp = np.where((k > 0) & (k <= kmax), np.sqrt(k ** 4 * np.exp(-2.0 * (k / 8.0) ** 2)
+                                                      / (4 * np.pi * k ** 2)), 0.0)
+        kk = np.where(k > 0, k, 1.0)
+    c = c * amp
+    nvec = np.stack([NX / kk, NY / kk, NZ / kk])
+    c = c - nvec * np.sum(nvec * c, axis=0)
+    return c
+
+
+def turbulence_vorticity(n, kmax=32, seed=5, zwidth=0.3, torch_device=None):
+    """Vorticity of the mixed-BC configuration on n^3 cell centres of
+    [-1, 1]^3: the real field sum_n c_n exp(i pi n . x) of turbulence_modes
+    (all modes with |n| <= kmax, Hermitian completion implied), multiplied
+    by exp(-z^2 / zwidth^2) so it decays toward the free z boundaries.
+    Returns (3, n, n, n) float64 in Fortran order (numpy), or a flat
+    Fortran-ordered torch tensor on torch_device."""
+    assert 2 * kmax < n, "grid too coarse for the modes"
+    c = turbulence_modes(kmax, seed)
+    full = np.zeros((3, n, n, n // 2 + 1), dtype=np.complex128)
+    idx = np.arange(-kmax, kmax + 1) % n
+    # sample at cell centres x_j = -1 + (j + 1/2) 2/n: exp(i pi n x_j) =
+    # exp(2 pi i n j / n_grid) * exp(i pi n (1/n_grid - 1))
+    n1 = np.arange(-kmax, kmax + 1, dtype=np.float64)
+    ph1 = np.exp(1j * np.pi * n1 * (1.0 / n - 1.0))
+    phz = np.exp(1j * np.pi * np.arange(kmax + 1) * (1.0 / n - 1.0))
+    c = c * ph1[None, :, None, None] * ph1[None, None, :, None] * phz[None, None, None, :]
+    full[:, idx[:, None], idx[None, :], :kmax + 1] = c
+    x = -1.0 + (np.arange(n) + 0.5) * (2.0 / n)
+    env = np.exp(-x * x / (zwidth * zwidth))
+    if torch_device is None:
+        w = np.fft.irfftn(full, s=(n, n, n), axes=(1, 2, 3), norm="forward")
+        w *= env[None, None, None, :]
+        return np.asfortranarray(w)
+    import torch
+    out = []
+    envt = torch.from_numpy(env).to(torch_device)
+    for q in range(3):
+        f = torch.from_numpy(full[q]).to(torch_device)
+        w = torch.fft.irfftn(f, s=(n, n, n), norm="forward") * envt[None, None, :]
+        out.append(w.permute(2, 1, 0).reshape(-1))  # Fortran order of (i, j, k)
+        del f
+    # interleave the components fastest: (3, n, n, n) Fortran order
+    return torch.stack(out, dim=1).reshape(-1)

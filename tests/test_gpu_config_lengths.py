@@ -1,0 +1,183 @@
+"""GPU parity at the transform lengths the benchmark configurations run.
+
+The benchmark grids reach FFT lengths that the small geometries of
+test_gpu_poisson.py do not: the 24-patch batch's edges 96 and 160 give
+y / z lengths 192 and 320 (x half-lengths 96 and 160), the mixed-BC 384^3
+grid gives periodic lengths 384 (x half-length 192) and the doubled free z
+axis 768.  Those lengths run length-specialised register-resident kernels
+with their own radix order (odd radix first), so each is checked here
+against the oracle (oracle/pmoracle.c, pinned against the compiled
+reference) on cheap shapes that put the long length on one axis:
+FP64 within 1e-12 relative L-inf, FP32 within 1e-5 relative L2.
+Reference: poisson/pmlib_poisson_solve.f:119-233 (forward passes),
+:697-954 (fused z pass), fourier/pmlib_fourier_ifft.f:5-301."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import blob_field, rel_l2, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(g, p, precision):
+    import pmlib3d_amd as pm
+    patch = pm.Patch(xmin=tuple(g.xmin), xmax=tuple(g.xmax), ncell=tuple(g.ncell), dx=tuple(g.dx),
+                     bound_cond=tuple(g.bound_cond), nghost=g.nghost)
+    s = pm.PoissonSolver(patch, precision, order=p.poisson_order, kernel=p.poisson_kernel,
+                         eps_radius=p.regularisation_radius, interp_order=g.interp_order)
+    s.setup()
+    return s
+
+
+def _field(g, seed):
+    # blobs wide enough to reach across a long axis of a thin box
+    return blob_field(g, seed=seed, nblob=4, width=0.02)
+
+
+# ncell, bound_cond: the long length on one axis (x: half-length NH, y: Ly, z: Lz)
+LONG = [
+    ((96, 8, 8), (0, 0, 0)),      # NH = 96   (batch edge 96, x pass)
+    ((160, 8, 8), (0, 0, 0)),     # NH = 160  (batch edge 160, x pass)
+    ((192, 8, 8), (0, 0, 0)),     # NH = 192
+    ((8, 96, 8), (0, 0, 0)),      # Ly = 192  (batch edge 96, y passes)
+    ((8, 160, 8), (0, 0, 0)),     # Ly = 320  (batch edge 160, y passes)
+    ((8, 8, 96), (0, 0, 0)),      # Lz = 192  (z pass)
+    ((8, 8, 160), (0, 0, 0)),     # Lz = 320
+    ((8, 8, 384), (0, 0, 0)),     # Lz = 768  (mixed-BC grid, free z)
+    ((8, 8, 192), (0, 0, 0)),     # Lz = 384
+    ((384, 16, 12), (1, 1, 0)),   # periodic x of 384: NH = 192
+    ((16, 384, 12), (1, 1, 0)),   # periodic y of 384: Ly = 384
+    ((16, 16, 384), (1, 1, 0)),   # the mixed-BC grid's form: periodic x, y, free z of Lz = 768
+    ((8, 320, 8), (0, 1, 0)),     # periodic y of 320
+]
+
+
+@pytest.mark.parametrize("zkernel", [0, 1], ids=["zreg", "zlds"])
+@pytest.mark.parametrize("case", LONG, ids=["x".join(map(str, c[0])) + "_bc" + "".join(map(str, c[1]))
+                                            for c in LONG])
+def test_three_component_solve_at_long_lengths(case, zkernel, monkeypatch):
+    """Velocity-only solve (register-resident fused z pass, or the forced
+    LDS-resident one) and the all-outputs solve (psi, reprojected vorticity),
+    FP64 against the oracle and FP32 against FP64."""
+    ncell, bc = case
+    monkeypatch.setenv("PMGPU_Z_KERNEL", str(zkernel))
+    monkeypatch.setenv("PMGPU_STRIDED_KERNEL", str(zkernel))
+    print("case", ncell, bc, "z kernel", zkernel, flush=True)
+    g = O.make_geom(ncell, 0, [-0.5, -0.6, -0.7], 1.0 / 64, bc)
+    p = O.make_params(4, 1, 2.0)
+    w = _field(g, seed=sum(ncell))
+    Gh = O.poisson_setup(g, p)
+    ref_v = O.poisson_solve(g, p, Gh, w, False, False, False)[0]
+    ref_all = O.poisson_solve(g, p, Gh, w, True, True, True)
+    s64 = _solver(g, p, 0)
+    v64 = s64.solve(w)[0]
+    e = rel_linf(v64, ref_v)
+    print("fp64 velocity-only rel Linf %.3e" % e, flush=True)
+    assert e <= 1e-12
+    got = s64.solve(w, True, True, True)
+    for name, a, b in zip(("vel", "vort", "psi"), got, ref_all):
+        e = rel_linf(a, b)
+        print("fp64 all-outputs %s rel Linf %.3e" % (name, e), flush=True)
+        assert e <= 1e-12, (name, e)
+    s32 = _solver(g, p, 1)
+    v32 = s32.solve(w)[0]
+    e32 = rel_l2(v32, v64)
+    print("fp32 velocity rel L2 vs fp64 %.3e" % e32, flush=True)
+    assert e32 <= 1e-5
+    got32 = s32.solve(w, True, True, True)
+    for name, a, b in zip(("vel", "vort", "psi"), got32, got):
+        assert rel_l2(a, b) <= 1e-5, name
+
+
+# scalar z kernels: 1..16 LDS-resident with that many columns per block,
+# 100 + TC register-resident
+Z1_KERNELS = [1, 4, 16, 104, 108, 116]
+SCALAR_LONG = [
+    ((16, 8, 96), (0, 0, 0)),     # Lz = 192
+    ((16, 8, 160), (0, 0, 0)),    # Lz = 320
+    ((16, 8, 384), (0, 0, 0)),    # Lz = 768
+    ((32, 16, 384), (1, 1, 0)),   # Lz = 768 under periodic x, y
+    ((96, 160, 8), (0, 0, 0)),    # NH = 96, Ly = 320
+    ((160, 96, 8), (0, 0, 0)),    # NH = 160, Ly = 192
+]
+# (the scalar solve needs a length-specialised x half-length: at least 16)
+
+
+@pytest.mark.parametrize("kern", Z1_KERNELS)
+@pytest.mark.parametrize("case", SCALAR_LONG, ids=["x".join(map(str, c[0])) + "_bc" + "".join(map(str, c[1]))
+                                                   for c in SCALAR_LONG])
+def test_scalar_solve_at_long_lengths(case, kern, monkeypatch):
+    """Single-component psi = G * w with every scalar z kernel forced,
+    against the oracle's psi of a vorticity with that one component."""
+    import pmlib3d_amd as pm
+    ncell, bc = case
+    monkeypatch.setenv("PMGPU_Z1_KERNEL", str(kern))
+    print("case", ncell, bc, "z1 kernel", kern, flush=True)
+    g = O.make_geom(ncell, 0, [-0.5, -0.6, -0.7], 1.0 / 64, bc)
+    p = O.make_params(4, 1, 2.0)
+    w3 = _field(g, seed=3 + sum(ncell))
+    w3[0] = 0.0
+    w3[1] = 0.0
+    psi = O.poisson_solve(g, p, O.poisson_setup(g, p), w3, False, False, True)[2]
+    got = _solver(g, p, pm.FP64).solve_scalar(w3[2])
+    e = rel_linf(got, psi[2])
+    print("fp64 rel Linf %.3e" % e, flush=True)
+    assert e <= 1e-12
+    got32 = _solver(g, p, pm.FP32).solve_scalar(w3[2])
+    assert rel_l2(got32, psi[2]) <= 1e-5
+
+
+@pytest.mark.parametrize("grouped", [1, 0], ids=["grouped", "per_patch"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_batch_with_edges_96_and_160_matches_oracle(precision, grouped, monkeypatch):
+    """A batch whose patches carry the 24-patch benchmark's long edges (96
+    and 160 cells, h = 1/128) beside short ones, in both batch forms: each
+    patch's psi against the oracle on every replay."""
+    import torch
+    import pmlib3d_amd as pm
+    monkeypatch.setenv("PMGPU_BATCH_GROUPED", str(grouped))
+    print("precision", precision, "grouped", grouped, flush=True)
+    dev = torch.device("cuda:0")
+    dt = torch.float64 if precision == 0 else torch.float32
+    sizes = [(96, 16, 32), (32, 160, 16), (16, 32, 160), (160, 96, 16), (48, 96, 32), (64, 32, 96),
+             (16, 160, 48), (32, 32, 32)]
+    h = 1.0 / 128
+    p = O.make_params(4, 1, 2.0)
+    solvers, ins, outs, refs = [], [], [], []
+    for i, n in enumerate(sizes):
+        print("patch", i, n, flush=True)
+        g = O.make_geom(n, 0, [-v * h / 2 for v in n], h, (0, 0, 0))
+        w3 = blob_field(g, seed=21 + i, nblob=3, width=0.01)
+        w3[0] = 0.0
+        w3[1] = 0.0
+        psi = O.poisson_solve(g, p, O.poisson_setup(g, p), w3, False, False, True)[2]
+        solvers.append(_solver(g, p, precision))
+        w = torch.from_numpy(np.asfortranarray(w3[2]).ravel(order="F")).to(dev).to(dt)
+        ins.append(w)
+        outs.append(torch.full_like(w, float("nan")))
+        refs.append(psi[2])
+    batch = pm.PoissonBatch(solvers)
+    for rep in range(2):
+        batch.solve_scalar([t.data_ptr() for t in ins], [t.data_ptr() for t in outs])
+        torch.cuda.synchronize()
+        for i, (o, r) in enumerate(zip(outs, refs)):
+            got = o.cpu().numpy().astype(np.float64).reshape(r.shape, order="F")
+            if precision == 0:
+                assert rel_linf(got, r) <= 1e-12, (rep, i, sizes[i])
+            else:
+                assert rel_l2(got, r) <= 1e-5, (rep, i, sizes[i])
+        for o in outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
+
+
+def test_batch_rejects_a_repeated_handle():
+    """Each patch needs its own solver handle (a handle owns its FFT
+    workspaces, which two branches of one graph would share)."""
+    import pmlib3d_amd as pm
+    from pmlib3d_amd import _lib
+    g = O.make_geom((16, 16, 16), 0, [-0.5] * 3, 1.0 / 32, (0, 0, 0))
+    s = _solver(g, O.make_params(4, 1, 2.0), 1)
+    with pytest.raises(_lib.PmgpuError, match="appears twice"):
+        pm.PoissonBatch([s, s])
