@@ -25,6 +25,8 @@ template <typename R> int fx_z(int L, ZArgs a, int Ly, const cplx_t<R> *tw, hipS
 template <typename R> int fx_xfwd(int NH, int NC, XFwdArgs a, const cplx_t<R> *tw, hipStream_t st);
 template <typename R> int fx_xinv(int NH, XInvArgs a, const cplx_t<R> *tw, hipStream_t st);
 template <typename R> int fx_z1(int L, int T, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st);
+// 1 when fx_z1 has kernel variant T for length L (checked without launching)
+template <typename R> int fx_z1_avail(int L, int T);
 // group launches over several patches (same return convention) and the
 // tile sizes their tables are built with (0: length not specialised)
 template <typename R> int fx_tile_x1(int NH);
@@ -564,8 +566,11 @@ template <typename R> struct PoissonGPU : PoissonBase {
   int tune_scalar() override {
     if (tz1) return 0;
     if (const char *env = getenv("PMGPU_Z1_KERNEL")) {  // fixed choice, no timing
-      tz1 = atoi(env);
-      PM_CHECK(tz1 > 0, "pmgpu: bad PMGPU_Z1_KERNEL");
+      const int t = atoi(env);
+      PM_CHECK(t > 0, "pmgpu: bad PMGPU_Z1_KERNEL");
+      PM_CHECK(fx_z1_avail<R>(L[2], t) == 1,
+               "pmgpu: PMGPU_Z1_KERNEL=" + std::to_string(t) + " is not available for z length " + std::to_string(L[2]));
+      tz1 = t;
       return 0;
     }
     const std::string key = z1_key();

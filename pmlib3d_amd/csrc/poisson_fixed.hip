@@ -186,6 +186,27 @@ template <typename R, int L, int TC> int z1r_T(ZArgs a, int Ly, const cplx_t<R> 
     return hipGetLastError() == hipSuccess ? 1 : -1;
   }
 }
+// whether z1_T / z1r_T have that variant for length L (without launching)
+template <typename R, int L, int T> constexpr bool z1_fits() {
+  return T * Geo<L>::ST * (int)sizeof(cplx_t<R>) <= kLdsBudget && Geo<L>::P * T <= 1024;
+}
+template <typename R, int L, int TC> constexpr bool z1r_fits() {
+  return regs_ok<L>() && TC * reg_line_stride<L, sizeof(cplx_t<R>), TC>() * (int)sizeof(cplx_t<R>) <= kLdsBudget &&
+         Geo<L>::P * TC <= 1024;
+}
+template <typename R, int L> int z1_avail(int T) {
+  switch (T) {
+    case 1: return z1_fits<R, L, 1>();
+    case 2: return z1_fits<R, L, 2>();
+    case 4: return z1_fits<R, L, 4>();
+    case 8: return z1_fits<R, L, 8>();
+    case 16: return z1_fits<R, L, 16>();
+    case kZ1Reg + 4: return z1r_fits<R, L, 4>();
+    case kZ1Reg + 8: return z1r_fits<R, L, 8>();
+    case kZ1Reg + 16: return z1r_fits<R, L, 16>();
+    default: return 0;
+  }
+}
 // T selects the kernel: 1..16 the LDS-resident one with T columns per
 // block, kZ1Reg + TC the register-resident one with TC columns per block
 template <typename R, int L> int z1_L(int T, ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
@@ -320,6 +341,15 @@ template <typename R> int fx_z1(int L, int T, ZArgs a, int Ly, const cplx_t<R> *
     default: return 0;
   }
 }
+template <typename R> int fx_z1_avail(int L, int T) {
+  switch (L) {
+#define X(n) \
+  case n: return fx::z1_avail<R, n>(T);
+    PM_FX_LENGTHS(X)
+#undef X
+    default: return 0;
+  }
+}
 template <typename R> int fx_xinv(int NH, XInvArgs a, const cplx_t<R> *tw, hipStream_t st) {
   switch (NH) {
 #define X(n) \
@@ -421,6 +451,8 @@ template int fx_xfwd<double>(int, int, XFwdArgs, const double2 *, hipStream_t);
 template int fx_xinv<float>(int, XInvArgs, const float2 *, hipStream_t);
 template int fx_xinv<double>(int, XInvArgs, const double2 *, hipStream_t);
 template int fx_z1<float>(int, int, ZArgs, int, const float2 *, hipStream_t);
+template int fx_z1_avail<float>(int, int);
+template int fx_z1_avail<double>(int, int);
 template int fx_z1<double>(int, int, ZArgs, int, const double2 *, hipStream_t);
 
 }  // namespace pmgpu

@@ -111,6 +111,7 @@ def test_scalar_solve_at_long_lengths(case, kern, monkeypatch):
     """Single-component psi = G * w with every scalar z kernel forced,
     against the oracle's psi of a vorticity with that one component."""
     import pmlib3d_amd as pm
+    from pmlib3d_amd import _lib
     ncell, bc = case
     monkeypatch.setenv("PMGPU_Z1_KERNEL", str(kern))
     print("case", ncell, bc, "z1 kernel", kern, flush=True)
@@ -120,12 +121,25 @@ def test_scalar_solve_at_long_lengths(case, kern, monkeypatch):
     w3[0] = 0.0
     w3[1] = 0.0
     psi = O.poisson_solve(g, p, O.poisson_setup(g, p), w3, False, False, True)[2]
-    got = _solver(g, p, pm.FP64).solve_scalar(w3[2])
-    e = rel_linf(got, psi[2])
-    print("fp64 rel Linf %.3e" % e, flush=True)
-    assert e <= 1e-12
-    got32 = _solver(g, p, pm.FP32).solve_scalar(w3[2])
-    assert rel_l2(got32, psi[2]) <= 1e-5
+    ran = 0
+    for prec in (pm.FP64, pm.FP32):
+        # a variant whose block exceeds the LDS budget or 1024 threads at this
+        # length is refused up front (the tuner never picks it)
+        try:
+            got = _solver(g, p, prec).solve_scalar(w3[2])
+        except _lib.PmgpuError as exc:
+            assert "is not available for z length" in str(exc), exc
+            print("precision", prec, "variant not available:", exc, flush=True)
+            continue
+        ran += 1
+        if prec == pm.FP64:
+            e = rel_linf(got, psi[2])
+            print("fp64 rel Linf %.3e" % e, flush=True)
+            assert e <= 1e-12
+        else:
+            assert rel_l2(got, psi[2]) <= 1e-5
+    if not ran:
+        pytest.skip("z1 kernel %d has no variant for this length in either precision" % kern)
 
 
 @pytest.mark.parametrize("grouped", [1, 0], ids=["grouped", "per_patch"])
