@@ -205,6 +205,20 @@ int pmgpu_penalise(const pmgpu_geom *g, const double *vel, const double *chi, co
 int pmgpu_penalise_device(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_chi,
                           const void *d_solid_vel, const double us_const[3], double relax,
                           int32_t fd_order, void *d_vort, double *diag, void *stream);
+/* Allocation-free variant of pmgpu_penalise_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc/hipFree and never synchronises: the
+ * sums go to d_diag, a DEVICE array of 5 doubles written on the stream (NULL:
+ * no sums), and the per-block partial sums live in a fixed scratch array in
+ * the library, one per GPU (10 MiB, held until the process ends), enough for
+ * ceil(N1/64) * ceil(N2/4) * N3 <= 262144 blocks (e.g. 384^3); larger meshes
+ * are refused with an error.  Calls that overlap on one GPU share that
+ * scratch and race: order them on one stream. */
+int pmgpu_penalise_device_noalloc(const pmgpu_geom *g, int32_t precision, const void *d_vel,
+                                  const void *d_chi, const void *d_solid_vel,
+                                  const double us_const[3], double relax, int32_t fd_order,
+                                  void *d_vort, double *d_diag, void *stream);
 
 /* ------------------------------------------------------------------------
  * Vorticity right-hand side (stretching and diffusion)

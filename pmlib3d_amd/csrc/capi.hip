@@ -37,7 +37,7 @@ int pmgpu_init(int32_t device) {
   PM_CHECK(device >= 0 && device < n, "pmgpu_init: no such HIP device");
   PM_HIP(hipSetDevice(device));
   device_lds_limit();
-  return 0;
+  return pmgpu::penalise_noalloc_init();
 }
 
 int pmgpu_finalize(void) {

@@ -10,6 +10,8 @@
 // Every floating-point operation is an explicitly rounded one, in the order
 // the reference writes it, so the FP64 result is bit-identical to it.
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "../../include/pmgpu.h"
 #include "pm_common.h"
@@ -125,6 +127,12 @@ __global__ void __launch_bounds__(256) k_penalise_sums(const double *partial, in
   if (threadIdx.x < 5) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
+constexpr int kMaxDevices = 64;
+
+static dim3 penalise_grid(const pmgpu_geom *geom) {
+  return dim3((geom->ncell[0] + kTx - 1) / kTx, (geom->ncell[1] + kTy - 1) / kTy, geom->ncell[2]);
+}
+
 struct PenWS {
   double *partial = nullptr;
   int64_t cap = 0;
@@ -141,10 +149,14 @@ void penalise_release() {
   }
 }
 
+// Builds the arguments and launches the penalisation kernel and, when
+// partial is non-null, the reduction of its per-block partial sums (5
+// doubles per block, at most cap blocks) into sums[0..4].  Stream-ordered,
+// no allocation, no host synchronisation.
 template <typename T>
-static int penalise_launch(const pmgpu_geom *geom, const void *vel, const void *chi, const void *us_field,
-                           const double *us_const, double relax, int fd_order, void *vort, double *diag,
-                           hipStream_t st) {
+static int penalise_run(const pmgpu_geom *geom, const void *vel, const void *chi, const void *us_field,
+                        const double *us_const, double relax, int fd_order, void *vort, double *partial,
+                        int64_t cap, double *sums, hipStream_t st) {
   PM_CHECK(geom && vel && chi && vort, "pmgpu_penalise: null argument");
   PM_CHECK(fd_order == 2 || fd_order == 4, "pmgpu_penalise: fd_order must be 2 or 4");
   PenArgs<T> a;
@@ -171,10 +183,34 @@ static int penalise_launch(const pmgpu_geom *geom, const void *vel, const void *
   a.cz = T(1.0) / (T(2.0) * T(geom->dx[2]));
   a.vort = (T *)vort;
   PM_CHECK(geom->ncell[2] < 65536 && geom->ncell[1] < (1 << 20), "pmgpu_penalise: mesh too large");
-  dim3 grid((geom->ncell[0] + kTx - 1) / kTx, (geom->ncell[1] + kTy - 1) / kTy, geom->ncell[2]);
+  const dim3 grid = penalise_grid(geom);
   const int64_t nb = (int64_t)grid.x * grid.y * grid.z;
-  a.partial = nullptr;
+  PM_CHECK(!partial || nb <= cap, "pmgpu_penalise: more blocks than the partial-sum buffer holds");
+  a.partial = partial;
+  if (fd_order == 4)
+    hipLaunchKernelGGL((k_penalise<T, 4>), grid, dim3(kTx, kTy), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_penalise<T, 2>), grid, dim3(kTx, kTy), 0, st, a);
+  PM_HIP(hipGetLastError());
+  if (partial) {
+    hipLaunchKernelGGL(k_penalise_sums, dim3(1), dim3(256), 0, st, (const double *)partial, nb, sums);
+    PM_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+// The allocating form: the partial sums live in a library buffer grown on
+// demand, and the totals are copied to the host (one stream synchronisation).
+template <typename T>
+static int penalise_launch(const pmgpu_geom *geom, const void *vel, const void *chi, const void *us_field,
+                           const double *us_const, double relax, int fd_order, void *vort, double *diag,
+                           hipStream_t st) {
+  PM_CHECK(geom, "pmgpu_penalise: null argument");
+  double *partial = nullptr;
+  int64_t cap = 0;
   if (diag) {
+    const dim3 grid = penalise_grid(geom);
+    const int64_t nb = (int64_t)grid.x * grid.y * grid.z;
     if (!g_pen) g_pen = new PenWS;
     if (g_pen->cap < nb) {
       if (g_pen->partial) (void)hipFree(g_pen->partial);
@@ -184,16 +220,13 @@ static int penalise_launch(const pmgpu_geom *geom, const void *vel, const void *
       g_pen->cap = nb;
     }
     if (!g_pen->sums) PM_HIP(hipMalloc(&g_pen->sums, sizeof(double) * 8));
-    a.partial = g_pen->partial;
+    partial = g_pen->partial;
+    cap = g_pen->cap;
   }
-  if (fd_order == 4)
-    hipLaunchKernelGGL((k_penalise<T, 4>), grid, dim3(kTx, kTy), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_penalise<T, 2>), grid, dim3(kTx, kTy), 0, st, a);
-  PM_HIP(hipGetLastError());
+  if (penalise_run<T>(geom, vel, chi, us_field, us_const, relax, fd_order, vort, partial, cap,
+                      diag ? g_pen->sums : nullptr, st))
+    return -1;
   if (diag) {
-    hipLaunchKernelGGL(k_penalise_sums, dim3(1), dim3(256), 0, st, (const double *)a.partial, nb, g_pen->sums);
-    PM_HIP(hipGetLastError());
     double h[5];
     PM_HIP(hipMemcpyAsync(h, g_pen->sums, sizeof(h), hipMemcpyDeviceToHost, st));
     PM_HIP(hipStreamSynchronize(st));
@@ -201,6 +234,37 @@ static int penalise_launch(const pmgpu_geom *geom, const void *vel, const void *
   }
   return 0;
 }
+
+// Scratch of the allocation-free form: the per-block partial sums of at
+// most kNoAllocBlocks blocks (10 MiB), one copy per GPU, held until the
+// process ends.  Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol): it uses the device address
+// that hipGetSymbolAddress returns, fetched once per device.
+constexpr int64_t kNoAllocBlocks = int64_t(1) << 18;
+static __device__ __align__(256) double scratch[5 * kNoAllocBlocks];
+
+static double *scratch_address() {
+  static std::mutex mu;
+  static double *addr[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("pmgpu_penalise_device_noalloc: no current HIP device");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("pmgpu_penalise_device_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    addr[dev] = static_cast<double *>(p);
+  }
+  return addr[dev];
+}
+
+int penalise_noalloc_init() { return scratch_address() ? 0 : -1; }
 
 }  // namespace pmgpu
 
@@ -243,6 +307,29 @@ int pmgpu_penalise_device(const pmgpu_geom *g, int32_t precision, const void *d_
   PM_CHECK(precision == PMGPU_FP64, "pmgpu_penalise_device: precision must be PMGPU_FP64 or PMGPU_FP32");
   return penalise_launch<double>(g, d_vel, d_chi, d_solid_vel, us_const, relax, fd_order, d_vort, diag,
                                  (hipStream_t)stream);
+}
+
+int pmgpu_penalise_device_noalloc(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_chi,
+                                  const void *d_solid_vel, const double us_const[3], double relax, int32_t fd_order,
+                                  void *d_vort, double *d_diag, void *stream) {
+  PM_CHECK(precision == PMGPU_FP64 || precision == PMGPU_FP32,
+           "pmgpu_penalise_device_noalloc: precision must be PMGPU_FP64 or PMGPU_FP32");
+  PM_CHECK(g, "pmgpu_penalise_device_noalloc: null argument");
+  double *partial = nullptr;
+  if (d_diag) {
+    const dim3 grid = penalise_grid(g);
+    const int64_t nb = (int64_t)grid.x * grid.y * grid.z;
+    PM_CHECK(nb <= kNoAllocBlocks, "pmgpu_penalise_device_noalloc: the mesh needs " + std::to_string(nb) +
+                                       " partial-sum blocks, more than the " + std::to_string(kNoAllocBlocks) +
+                                       " its fixed scratch holds (use pmgpu_penalise_device)");
+    partial = scratch_address();
+    if (!partial) return -1;
+  }
+  if (precision == PMGPU_FP32)
+    return penalise_run<float>(g, d_vel, d_chi, d_solid_vel, us_const, relax, fd_order, d_vort, partial,
+                               kNoAllocBlocks, d_diag, (hipStream_t)stream);
+  return penalise_run<double>(g, d_vel, d_chi, d_solid_vel, us_const, relax, fd_order, d_vort, partial,
+                              kNoAllocBlocks, d_diag, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -72,3 +72,18 @@ def brinkman_factor(chi_s, lam, dt):
     the penalisation factor that replaces relax * chi."""
     a = lam * dt * chi_s
     return a / (1.0 + a)
+
+
+def penalise_device_noalloc(patch, precision, d_vel, d_chi, d_vort, d_diag=None, d_solid_vel=None, us_const=None,
+                            relax=1.0, fd_order=4, stream=None, interp_order=None):
+    """Allocation-free device form: no device allocation and no host
+    synchronisation; the FP64 sums go to the device array d_diag (5
+    doubles), if given.  Partial sums use the library's fixed per-GPU
+    scratch (at most 262144 blocks of 64 x 4 cells); calls that overlap on
+    one GPU race."""
+    _lib.init()
+    g = patch.geom(interp_order)
+    uc = None if us_const is None else (C.c_double * 3)(*[float(v) for v in us_const])
+    _lib.check(_lib.load().pmgpu_penalise_device_noalloc(C.byref(g), int(precision), d_vel, d_chi, d_solid_vel,
+                                                         uc, float(relax), int(fd_order), d_vort, d_diag, stream),
+               "pmgpu_penalise_device_noalloc")
