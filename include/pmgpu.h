@@ -240,6 +240,32 @@ int pmgpu_vorticity_rhs(const pmgpu_geom *g, const double *vort, const double *v
 int pmgpu_vorticity_rhs_device(const pmgpu_geom *g, int32_t precision, const void *d_vort, const void *d_vel,
                                double visc, int32_t fd_order, void *d_dvort, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Global diagnostics (FP64)
+ * ------------------------------------------------------------------------ */
+
+/* diagnostics_ring(topo, part, ierr)
+ * (vortex_codes/vortex_structures/source/diagnostics/diagnostics_ring.f:5,
+ * sums :72-158, normalisations :166-226) on one rank, ring axis x: out =
+ * impulse(3), angular impulse(3), kinetic energy, enstrophy, helicity,
+ * circulation, vortex centroid(3), ring radius, core radius, max |vort|
+ * (16 doubles), each particle weighted by dx(1) dx(2) dx(3).  d_pos, d_vort,
+ * d_vel are (3, np) FP64 device arrays.  The totals go to d_out (device, 16
+ * doubles, stream-ordered) and/or h_out (host; the call then synchronises);
+ * either may be NULL.  Sums are reduced in a fixed order (run-to-run
+ * identical), not in the reference's serial order. */
+int pmgpu_diagnostics_particles(const double dx[3], const double *d_pos, const double *d_vort,
+                                const double *d_vel, int64_t np, double *d_out, double *h_out,
+                                void *stream);
+/* diagnostics_energy_mesh(topo, mesh, ierr)
+ * (vortex_codes/vortex_structures/source/diagnostics/diagnostics_energy_mesh.f:5,
+ * :72-134) on one rank: kinetic energy, enstrophy, helicity energy, Lamb
+ * energy and max |vort| over the interior cells (5 doubles), accumulated in
+ * FP64 from FP64 or FP32 (precision) meshes (3, E...).  As in the reference,
+ * the helicity and Lamb terms divide by the cell's enstrophy. */
+int pmgpu_diagnostics_mesh(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_vort,
+                           double *d_out, double *h_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

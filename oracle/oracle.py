@@ -90,6 +90,9 @@ def lib():
         L.or_penalise.restype = C.c_int
         L.or_vort_rhs.argtypes = [G, P, P, C.c_double, C.c_int, P]
         L.or_vort_rhs.restype = C.c_int
+        L.or_diag_particles.argtypes = [P, P, P, P, C.c_int64, P]
+        L.or_diag_mesh.argtypes = [G, P, P, P]
+        L.or_diag_mesh.restype = C.c_int
         _lib = L
     return _lib
 
@@ -218,6 +221,34 @@ def vort_rhs(g, vort, vel, visc, fd_order=4, dvort=None):
     out = np.zeros_like(vort, order="F") if dvort is None else np.array(dvort, dtype=np.float64, order="F", copy=True)
     if lib().or_vort_rhs(g, _p(vort), _p(vel), float(visc), int(fd_order), _p(out)):
         raise ValueError("vort_rhs needs fd_order 2 or 4 and fd_order/2 ghost layers")
+    return out
+
+
+DIAG_PARTICLE_NAMES = ("impulse_x", "impulse_y", "impulse_z", "ang_impulse_x", "ang_impulse_y", "ang_impulse_z",
+                       "kin_energy", "enstrophy", "helicity", "circulation", "centroid_x", "centroid_y",
+                       "centroid_z", "vort_radius", "vort_core", "vort_max")
+DIAG_MESH_NAMES = ("kin_energy", "enstrophy", "helicity", "lamb", "vort_max")
+
+
+def diag_particles(dx, pos, vort, vel):
+    """Ring diagnostics of a particle set (diagnostics_ring.f), 16 values in
+    the order of DIAG_PARTICLE_NAMES."""
+    pos = np.asfortranarray(pos, dtype=np.float64)
+    vort = np.asfortranarray(vort, dtype=np.float64)
+    vel = np.asfortranarray(vel, dtype=np.float64)
+    out = np.zeros(16)
+    d = np.asarray(dx, dtype=np.float64)
+    lib().or_diag_particles(_p(d), _p(pos), _p(vort), _p(vel), pos.shape[1], _p(out))
+    return out
+
+
+def diag_mesh(g, vel, vort):
+    """Mesh energy diagnostics (diagnostics_energy_mesh.f), 5 values in the
+    order of DIAG_MESH_NAMES."""
+    vel = np.asfortranarray(vel, dtype=np.float64)
+    vort = np.asfortranarray(vort, dtype=np.float64)
+    out = np.zeros(5)
+    lib().or_diag_mesh(g, _p(vel), _p(vort), _p(out))
     return out
 
 

@@ -27,6 +27,11 @@
  *   or_vort_rhs        vortex_codes/vortex_penalisation/source/
  *                      particle_derivatives/part_deriv_vorticity.f:57-208
  *                      (stretching + diffusion, 2nd/4th-order FD, ghost map)
+ *   or_diag_particles  vortex_codes/vortex_structures/source/diagnostics/
+ *                      diagnostics_ring.f:72-226 (impulse, angular impulse,
+ *                      energy, enstrophy, helicity, circulation, centroid,
+ *                      ring and core radius)
+ *   or_diag_mesh       .../diagnostics/diagnostics_energy_mesh.f:72-134
  *
  * Array layout is the reference's: meshes are (3, i, j, k) with the component
  * fastest and ghost layers included, particles are (3, np).  Fixed-size fp64
@@ -927,5 +932,107 @@ int or_vort_rhs(const or_geom *g, const double *vort, const double *vel, double 
         }
       }
   ghost_fill_periodic(g, E, dvort);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Global diagnostics                                                        */
+
+/* Particle diagnostics of a vortex ring (ring axis x), in FP64
+ * (vortex_codes/vortex_structures/source/diagnostics/diagnostics_ring.f:
+ * first loop :72-158, centroid :166-168, radius and circulation :174-207,
+ * core radius :211-226), one rank.  dx is the mesh spacing whose cell volume
+ * weights every particle.  out[0..15] = impulse(3), angular impulse(3),
+ * kinetic energy, enstrophy, helicity, circulation, vortex centroid(3),
+ * vortex radius, core radius, max |vort|.  Each term is formed in the
+ * reference's operation order; the sums run over particles in index order. */
+void or_diag_particles(const double *dx, const double *pos, const double *vort, const double *vel, int64_t np,
+                       double *out) {
+  const double PI = acos(-1.0), c_1_3 = 1.0 / 3.0;
+  double imp[3] = {0, 0, 0}, ang[3] = {0, 0, 0}, kin = 0, enst = 0, hel = 0, cen[3] = {0, 0, 0}, vmax = 0;
+  for (int64_t i = 0; i < np; ++i) {
+    const double *x = pos + 3 * i, *w = vort + 3 * i, *u = vel + 3 * i;
+    double m1[3], m2[3];
+    m1[0] = x[1] * w[2] - x[2] * w[1];
+    m1[1] = x[2] * w[0] - x[0] * w[2];
+    m1[2] = x[0] * w[1] - x[1] * w[0];
+    m2[0] = x[1] * m1[2] - x[2] * m1[1];
+    m2[1] = x[2] * m1[0] - x[0] * m1[2];
+    m2[2] = x[0] * m1[1] - x[1] * m1[0];
+    for (int d = 0; d < 3; ++d) imp[d] = imp[d] + 0.5 * m1[d] * dx[0] * dx[1] * dx[2];
+    for (int d = 0; d < 3; ++d) ang[d] = ang[d] + c_1_3 * m2[d] * dx[0] * dx[1] * dx[2];
+    kin = kin + (u[0] * m1[0] + u[1] * m1[1] + u[2] * m1[2]) * dx[0] * dx[1] * dx[2];
+    const double e = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    enst = enst + e * dx[0] * dx[1] * dx[2];
+    hel = hel + (u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) * dx[0] * dx[1] * dx[2];
+    for (int d = 0; d < 3; ++d) cen[d] = cen[d] + e * x[d] * dx[0] * dx[1] * dx[2];
+    const double a = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (a > vmax) vmax = a;
+  }
+  for (int d = 0; d < 3; ++d) cen[d] = cen[d] / enst;
+  double rad = 0, circ = 0;
+  for (int64_t i = 0; i < np; ++i) {
+    const double *x = pos + 3 * i, *w = vort + 3 * i;
+    const double e = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double dy = x[1] - cen[1], dz = x[2] - cen[2];
+    rad = rad + e * sqrt(dy * dy + dz * dz) * dx[0] * dx[1] * dx[2];
+    circ = circ - dz / (dy * dy + dz * dz) * w[1] + dy / (dy * dy + dz * dz) * w[2];
+  }
+  rad = rad / enst;
+  circ = circ * dx[0] * dx[1] * dx[2] / (2.0 * PI);
+  double core = 0;
+  for (int64_t i = 0; i < np; ++i) {
+    const double *x = pos + 3 * i, *w = vort + 3 * i;
+    const double e = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double dy = x[1] - cen[1], dz = x[2] - cen[2], dxx = x[0] - cen[0];
+    const double s = sqrt(dy * dy + dz * dz) - rad;
+    core = core + e * sqrt(s * s + dxx * dxx) * dx[0] * dx[1] * dx[2];
+  }
+  core = core / enst;
+  for (int d = 0; d < 3; ++d) {
+    out[d] = imp[d];
+    out[3 + d] = ang[d];
+    out[10 + d] = cen[d];
+  }
+  out[6] = kin;
+  out[7] = enst;
+  out[8] = hel;
+  out[9] = circ;
+  out[13] = rad;
+  out[14] = core;
+  out[15] = vmax;
+}
+
+/* Mesh energy diagnostics over the interior cells, in FP64
+ * (vortex_codes/vortex_structures/source/diagnostics/diagnostics_energy_mesh.f:
+ * :72-134), one rank.  out[0..4] = kinetic energy, enstrophy, helicity
+ * energy, Lamb energy, max |vort|.  The helicity and Lamb terms divide by the
+ * local enstrophy as the reference does (a cell with zero vorticity makes
+ * them NaN there too).  Sums over k, j, i in that loop order. */
+int or_diag_mesh(const or_geom *g, const double *vel, const double *vort, double *out) {
+  int E[3], ngl[3];
+  for (int d = 0; d < 3; ++d) { E[d] = mesh_ext(g, d); ngl[d] = ng_side(g, d); }
+  const double *dx = g->dx;
+  double kin = 0, enst = 0, hel = 0, lamb = 0, vmax = 0;
+  for (int k = 0; k < g->ncell[2]; ++k)
+    for (int j = 0; j < g->ncell[1]; ++j)
+      for (int i = 0; i < g->ncell[0]; ++i) {
+        size_t o = 3 * ((size_t)(i + ngl[0]) + (size_t)E[0] * ((j + ngl[1]) + (size_t)E[1] * (k + ngl[2])));
+        const double *w = vort + o, *u = vel + o;
+        const double e = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        kin = kin + 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) * dx[0] * dx[1] * dx[2];
+        enst = enst + e * dx[0] * dx[1] * dx[2];
+        const double h = w[0] * u[0] + w[1] * u[1] + w[2] * u[2];
+        hel = hel + h * h / e * dx[0] * dx[1] * dx[2];
+        const double l0 = w[1] * u[2] - w[2] * u[1], l1 = w[2] * u[0] - w[0] * u[2], l2 = w[0] * u[1] - w[1] * u[0];
+        lamb = lamb + (l0 * l0 + l1 * l1 + l2 * l2) / e * dx[0] * dx[1] * dx[2];
+        const double a = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        if (a > vmax) vmax = a;
+      }
+  out[0] = kin;
+  out[1] = enst;
+  out[2] = hel;
+  out[3] = lamb;
+  out[4] = vmax;
   return 0;
 }

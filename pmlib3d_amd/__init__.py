@@ -15,6 +15,8 @@ from .particles import (cell_index, pmlib_interp_particle_mesh, pmlib_interp_mes
 from .penalisation import penalise, penalise_device, penalise_device_noalloc, sphere_mask, brinkman_factor
 from .derivatives import vorticity_rhs, vorticity_rhs_device, part_deriv_vorticity
 from .step import VortexStep
+from .diagnostics import (diagnostics_ring, diagnostics_energy_mesh, diagnostics_particles_device,
+                          diagnostics_mesh_device)
 from ._lib import PmgpuError, FP32, FP64
 
 __all__ = ["Patch", "Mesh", "Particles", "Parameters", "params", "pmlib_write", "mesh_extent",
@@ -22,4 +24,5 @@ __all__ = ["Patch", "Mesh", "Particles", "Parameters", "params", "pmlib_write", 
            "FP32", "FP64", "cell_index", "pmlib_interp_particle_mesh", "pmlib_interp_mesh_particle",
            "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps", "penalise",
            "penalise_device", "penalise_device_noalloc", "sphere_mask", "brinkman_factor", "vorticity_rhs", "vorticity_rhs_device",
-           "part_deriv_vorticity", "VortexStep"]
+           "part_deriv_vorticity", "VortexStep", "diagnostics_ring", "diagnostics_energy_mesh",
+           "diagnostics_particles_device", "diagnostics_mesh_device"]

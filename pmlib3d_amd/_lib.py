@@ -66,6 +66,8 @@ SIGNATURES = {
     "pmgpu_cell_index": (C.c_int, [C.POINTER(Geom), _P, C.c_int64, _P]),
     "pmgpu_penalise": (C.c_int, [C.POINTER(Geom), _P, _P, _P, _P, C.c_double, _I, _P, _P]),
     "pmgpu_penalise_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P, _P]),
+    "pmgpu_diagnostics_particles": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P]),
+    "pmgpu_diagnostics_mesh": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, _P]),
     "pmgpu_penalise_device_noalloc": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P,
                                                 _P]),
     "pmgpu_vorticity_rhs": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_double, _I, _P]),

@@ -44,6 +44,7 @@ int pmgpu_finalize(void) {
   PM_HIP(hipDeviceSynchronize());
   pmgpu::particles_release();
   pmgpu::penalise_release();
+  pmgpu::diagnostics_release();
   return 0;
 }
 

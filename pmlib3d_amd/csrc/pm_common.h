@@ -101,6 +101,7 @@ __global__ void k_ghost_fill(R *m, int E0, int E1, int E2, int d, int N, int ng,
 // free the workspaces of particles.hip and penalise.hip
 void particles_release();
 void penalise_release();
+void diagnostics_release();
 // fetches the device address of the allocation-free penalisation scratch on
 // the current device (once per device)
 int penalise_noalloc_init();

@@ -453,3 +453,82 @@ def test_vort_rhs_periodic_modified_wavenumbers(fd_order):
         hi[d + 1] = slice(ng + N, ng + N + ng)
         src2[d + 1] = slice(ng, 2 * ng)
         assert np.array_equal(got[tuple(hi)], got[tuple(src2)])
+
+
+# ---------------------------------------------------------------- diagnostics
+def _ring_particles(n, R, x0, gamma, h, U):
+    """n particles on a circle of radius R in the plane x = x0 about the x
+    axis, vorticity gamma ds / h^3 along e_theta (ds = 2 pi R / n), velocity U."""
+    th = 2 * np.pi * (np.arange(n) + 0.25) / n
+    pos = np.stack([np.full(n, x0), R * np.cos(th), R * np.sin(th)])
+    s = gamma * (2 * np.pi * R / n) / h ** 3
+    vort = np.stack([np.zeros(n), -s * np.sin(th), s * np.cos(th)])
+    vel = np.tile(np.asarray(U, dtype=np.float64)[:, None], (1, n))
+    return np.asfortranarray(pos), np.asfortranarray(vort), np.asfortranarray(vel), s
+
+
+def test_diag_particles_discrete_ring_closed_forms():
+    """A discrete circular ring (axis x, radius R, circulation gamma):
+    impulse (pi R^2 gamma, 0, 0), zero angular impulse and helicity,
+    kinetic energy 2 U . impulse for a uniform U, enstrophy n s^2 h^3,
+    centroid (x0, 0, 0), ring radius R, core radius 0, circulation gamma,
+    max |vort| = s.  A transposed cross product, a sign or a swapped y/z
+    fails one of these."""
+    n, R, x0, gamma, h = 720, 0.37, 0.11, 1.3, 0.02
+    U = (0.7, 0.2, -0.3)
+    pos, vort, vel, s = _ring_particles(n, R, x0, gamma, h, U)
+    d = dict(zip(O.DIAG_PARTICLE_NAMES, O.diag_particles((h, h, h), pos, vort, vel)))
+    imp = np.pi * R * R * gamma
+    assert abs(d["impulse_x"] - imp) <= 1e-13 * imp
+    scale = n * s * h ** 3  # sum of |terms| of the vanishing sums
+    for k in ("impulse_y", "impulse_z", "ang_impulse_x", "ang_impulse_y", "ang_impulse_z", "helicity"):
+        assert abs(d[k]) <= 1e-13 * scale, (k, d[k])
+    assert abs(d["vort_core"]) <= 1e-13 * R
+    assert abs(d["kin_energy"] - 2 * U[0] * imp) <= 1e-13 * imp
+    assert abs(d["enstrophy"] - n * s * s * h ** 3) <= 1e-13 * n * s * s * h ** 3
+    assert abs(d["centroid_x"] - x0) <= 1e-14 and abs(d["centroid_y"]) <= 1e-14 and abs(d["centroid_z"]) <= 1e-14
+    assert abs(d["vort_radius"] - R) <= 1e-14
+    assert abs(d["circulation"] - gamma) <= 1e-13 * gamma
+    assert abs(d["vort_max"] - s) <= 1e-14 * s
+
+
+def test_diag_particles_shifted_ring_and_scaling():
+    """Shifting the ring in y, z moves the centroid and leaves radius and
+    circulation; reversing the vorticity flips impulse and circulation and
+    keeps enstrophy.  (Impulse depends on the origin; it is taken about 0.)"""
+    n, R, h = 400, 0.25, 0.03
+    pos, vort, vel, s = _ring_particles(n, R, 0.0, 1.0, h, (0.0, 0.0, 0.0))
+    pos2 = pos.copy()
+    pos2[1] += 0.2
+    pos2[2] -= 0.15
+    d = dict(zip(O.DIAG_PARTICLE_NAMES, O.diag_particles((h, h, h), pos2, vort, vel)))
+    assert abs(d["centroid_y"] - 0.2) <= 1e-14 and abs(d["centroid_z"] + 0.15) <= 1e-14
+    assert abs(d["vort_radius"] - R) <= 1e-13 and abs(d["circulation"] - 1.0) <= 1e-13
+    r = dict(zip(O.DIAG_PARTICLE_NAMES, O.diag_particles((h, h, h), pos, -vort, vel)))
+    assert abs(r["circulation"] + 1.0) <= 1e-13
+    assert abs(r["impulse_x"] + np.pi * R * R) <= 1e-13
+    assert abs(r["enstrophy"] - n * s * s * h ** 3) <= 1e-13 * n * s * s * h ** 3
+
+
+def test_diag_mesh_uniform_fields_closed_forms():
+    """Uniform u and w on the interior (ghosts filled with junk that must be
+    ignored): kinetic energy |u|^2 V / 2, enstrophy |w|^2 V, helicity
+    (w.u)^2 / |w|^2 V, Lamb |w x u|^2 / |w|^2 V (so helicity + Lamb = 2 kin),
+    max |w|."""
+    g = O.make_geom((7, 5, 6), 2, [-0.3, 0.1, -0.2], 0.05, (0, 0, 0))
+    E = O.mesh_extent(g)
+    u = np.array([0.3, -1.1, 0.7])
+    w = np.array([2.0, 0.5, -1.5])
+    vel = np.full((3,) + E, 1e3, order="F")
+    vort = np.full((3,) + E, -1e3, order="F")
+    vel[:, 2:-2, 2:-2, 2:-2] = u[:, None, None, None]
+    vort[:, 2:-2, 2:-2, 2:-2] = w[:, None, None, None]
+    V = 7 * 5 * 6 * 0.05 ** 3
+    d = dict(zip(O.DIAG_MESH_NAMES, O.diag_mesh(g, vel, vort)))
+    ww, uu = w @ w, u @ u
+    assert abs(d["kin_energy"] - 0.5 * uu * V) <= 1e-13 * uu * V
+    assert abs(d["enstrophy"] - ww * V) <= 1e-13 * ww * V
+    assert abs(d["helicity"] - (w @ u) ** 2 / ww * V) <= 1e-13 * uu * V
+    assert abs(d["lamb"] - np.cross(w, u) @ np.cross(w, u) / ww * V) <= 1e-13 * uu * V
+    assert abs(d["helicity"] + d["lamb"] - 2 * d["kin_energy"]) <= 1e-13 * uu * V
+    assert abs(d["vort_max"] - np.sqrt(ww)) <= 1e-15 * np.sqrt(ww)
