@@ -39,7 +39,7 @@ typedef struct pmgpu_geom {
 typedef struct pmgpu_params {
   double regularisation_radius; /* eps / max(dx)                          */
   int32_t poisson_order;        /* 0, 2, 4, 6, 8, 10                      */
-  int32_t poisson_kernel;       /* 1 (G)                                  */
+  int32_t poisson_kernel;       /* 1 (G) or 2 (grad G)                    */
 } pmgpu_params;
 
 enum { PMGPU_FP64 = 0, PMGPU_FP32 = 1 };
@@ -64,14 +64,21 @@ int pmgpu_poisson_create(const pmgpu_geom *g, const pmgpu_params *p, int32_t pre
 int pmgpu_poisson_destroy(pmgpu_poisson_t h);
 
 /* Replaces pmlib_poisson_setup(patch, topo_all, mesh, ierr)
- * (poisson/pmlib_poisson_setup.f:5-19): builds G-hat on the device. */
+ * (poisson/pmlib_poisson_setup.f:5-19): builds G-hat on the device (kernel
+ * 1: the real octant of G-hat; kernel 2: the complex half-spectrum of grad G,
+ * poisson/pmlib_poisson_setup.f:580-635 and the other orders' kernel-2
+ * blocks; nothing on an all-periodic patch, where the z pass forms the
+ * spectral kernel). */
 int pmgpu_poisson_setup(pmgpu_poisson_t h);
 
 /* Replaces pmlib_poisson_solve(patch, topo_all, mesh, ierr, reg_vort, reproj,
  * calc_vecpot) (poisson/pmlib_poisson_solve.f:15-34).  Host FP64 meshes:
  * vort (in), vel (out), vort_out (out when reg_vort or reproj, else may be
  * NULL; it may alias vort, as mesh%vort is overwritten in the reference),
- * vecpot (out when calc_vecpot, else may be NULL). */
+ * vecpot (out when calc_vecpot with kernel 1, else may be NULL; with kernel
+ * 2 the reference computes no stream function and vecpot is left as it is).
+ * Kernel 2 forms u-hat = K x w-hat and the reprojection K (2 pi i k . w-hat)
+ * (poisson/pmlib_poisson_solve.f:873-913). */
 int pmgpu_poisson_solve(pmgpu_poisson_t h, const double *vort, double *vel, double *vort_out,
                         double *vecpot, int32_t reg_vort, int32_t reproj, int32_t calc_vecpot);
 

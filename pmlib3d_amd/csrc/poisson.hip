@@ -196,6 +196,8 @@ template <typename R> struct PoissonGPU : PoissonBase {
   V *d_tw = nullptr;
   V *A = nullptr, *B = nullptr, *Bpsi = nullptr, *Bvort = nullptr;
   R *Ghat = nullptr;
+  V *Khat = nullptr;  // kernel 2: complex half-spectrum of grad G, 3 components
+  bool ker2 = false;  // kernel 2 on a patch with a free axis (all-periodic: spectral)
   R *mesh_in = nullptr, *mesh_out = nullptr;  // staging for host calls
   size_t nA = 0, nB = 0, nmesh = 0;
   int Tx = 1, Ts = 1, Tz = 1;
@@ -206,7 +208,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
   int lds_limit = 65536;
 
   ~PoissonGPU() override {
-    void *bufs[] = {d_tw, A, B, Bpsi, Bvort, Ghat, mesh_in, mesh_out};
+    void *bufs[] = {d_tw, A, B, Bpsi, Bvort, Ghat, Khat, mesh_in, mesh_out};
     for (void *b : bufs)
       if (b) (void)hipFree(b);
   }
@@ -215,13 +217,16 @@ template <typename R> struct PoissonGPU : PoissonBase {
     g = gg;
     prm = pp;
     stream = s;
-    PM_CHECK(prm.poisson_kernel == 1, "pmgpu: only poisson_kernel = 1 is implemented on the GPU path");
+    PM_CHECK(prm.poisson_kernel == 1 || prm.poisson_kernel == 2, "pmgpu: poisson_kernel must be 1 or 2");
     int o = prm.poisson_order;
     PM_CHECK(o == 0 || o == 2 || o == 4 || o == 6 || o == 8 || o == 10,
              "pmgpu: poisson_order must be 0, 2, 4, 6, 8 or 10");
     PM_CHECK(prm.poisson_order == 0 || prm.regularisation_radius >= 1.0,
              "pmgpu: regularisation_radius must be >= 1 (parameters/pmlib_parameters_check.f:67)");
     allper = g.bound_cond[0] == 1 && g.bound_cond[1] == 1 && g.bound_cond[2] == 1;
+    // all-periodic kernel 2 is K = zeta i k / (2 pi |k|^2) = G 2 pi i k with
+    // the kernel-1 spectral G: the same z pass serves both
+    ker2 = prm.poisson_kernel == 2 && !allper;
     for (int d = 0; d < 3; ++d) {
       PM_CHECK(g.ncell[d] > 0 && g.dx[d] > 0, "pmgpu: bad geometry");
       int ng = g.bound_cond[d] == 0 ? g.nghost : (g.interp_order == 4 ? 3 : 2);
@@ -313,6 +318,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
   // ---- pmlib_poisson_setup ---------------------------------------------
   int setup() override {
     if (allper) return 0;  // G = zeta / (4 pi^2 k^2) is formed in the z pass
+    if (ker2) return setup_kernel2();
     // FP64 kernel grid and its transform (x R2C, y and z C2C), then the octant
     double *Gr = nullptr;
     double2 *Gc = nullptr;
@@ -373,7 +379,74 @@ template <typename R> struct PoissonGPU : PoissonBase {
     return 0;
   }
 
+  // Kernel 2: grad G on the kernel grid (three interleaved components), its
+  // shifted forward transform (x R2C, y and z C2C) in FP64, kept as the
+  // complex half-spectrum [((kz * Ly + ky) * Hx + kx) * 3 + c] in the
+  // working precision (poisson/pmlib_poisson_setup.f:580-635, :1533-1535).
+  int setup_kernel2() {
+    double *Gr = nullptr;
+    double2 *Gc = nullptr;
+    const size_t nG = (size_t)L[0] * L[1] * L[2];
+    const size_t nGc = (size_t)Hx * L[1] * L[2];
+    PM_HIP(hipMalloc(&Gr, 3 * nG * sizeof(double)));
+    PM_HIP(hipMalloc(&Gc, 3 * nGc * sizeof(double2)));
+    GreenArgs ga;
+    ga.out = Gr;
+    double dmax = g.dx[0] > g.dx[1] ? g.dx[0] : g.dx[1];
+    dmax = dmax > g.dx[2] ? dmax : g.dx[2];
+    ga.eps = prm.regularisation_radius * dmax;
+    ga.order = prm.poisson_order;
+    for (int d = 0; d < 3; ++d) {
+      ga.L[d] = L[d];
+      ga.bc[d] = g.bound_cond[d];
+      ga.dx[d] = g.dx[d];
+      ga.xk[d] = -(0.5 * (double)L[d] + 0.5) * g.dx[d];
+      ga.len[d] = g.dx[d] * (double)L[d];
+    }
+    hipLaunchKernelGGL(k_green2, dim3(4096), dim3(256), 0, stream, ga);
+    PM_HIP(hipGetLastError());
+    TwTable<double> t64;
+    FftPlan q0, q1, q2;
+    make_plan(Nh, q0);
+    make_plan(L[1], q1);
+    make_plan(L[2], q2);
+    q0.tw_off = t64.add(Nh);
+    q1.tw_off = t64.add(L[1]);
+    q2.tw_off = t64.add(L[2]);
+    int tl = t64.add(L[0]);
+    double2 *dt = nullptr;
+    PM_HIP(hipMalloc(&dt, t64.host.size() * sizeof(double2)));
+    PM_HIP(hipMemcpyAsync(dt, t64.host.data(), t64.host.size() * sizeof(double2), hipMemcpyHostToDevice, stream));
+    auto pick64 = [&](int len, int lines_per_unit, int tcap) {
+      int t = tcap;
+      while (t > 1 && ((size_t)t * lines_per_unit * lds_stride(len) * sizeof(double2) > (size_t)lds_limit ||
+                       t * lines_per_unit * ((len + 7) / 8) > 1024))
+        t /= 2;
+      return t;
+    };
+    const int tx = pick64(Nh, 3, 8), ty = pick64(L[1], 1, 8), tz = pick64(L[2], 1, 8);
+    const int64_t H3 = 3 * (int64_t)Hx;
+    int e = 0;
+    e |= launch_xfwd<double>(Gr, Gc, 1, 3, 3 * (int64_t)L[0], 3 * (int64_t)L[0] * L[1], 1, 3, H3, H3 * L[1], 3, L[0],
+                             L[1], L[2], tx, q0, tl, dt);
+    e |= launch_strided<double, -1>(Gc, Gc, H3, H3, H3 * L[1], H3 * L[1], (int)H3, L[1], L[1], L[2], 1, 1.0, ty, q1,
+                                    dt);
+    e |= launch_strided<double, -1>(Gc, Gc, H3 * L[1], H3 * L[1], H3, H3, (int)H3, L[2], L[2], L[1], 1, 1.0, tz, q2,
+                                    dt);
+    if (e) return -1;
+    if (!Khat) PM_HIP(hipMalloc(&Khat, 3 * nGc * sizeof(V)));
+    hipLaunchKernelGGL(k_spec_convert<R>, dim3(2048), dim3(256), 0, stream, (const double2 *)Gc, Khat,
+                       (int64_t)(3 * nGc));
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipStreamSynchronize(stream));
+    (void)hipFree(Gr);
+    (void)hipFree(Gc);
+    (void)hipFree(dt);
+    return 0;
+  }
+
   int octant(void *host, int64_t cap, int32_t *dims) override {
+    PM_CHECK(!ker2, "pmgpu: kernel 2 keeps a complex spectrum, not a real octant");
     dims[0] = Hx;
     dims[1] = Hy;
     dims[2] = Hz;
@@ -396,7 +469,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
                    int psi) override {
     const bool want_vort = reg || reproj;
     const bool want_psi = psi && prm.poisson_kernel == 1;
-    PM_CHECK(allper || Ghat != nullptr, "pmgpu: pmgpu_poisson_setup was not called");
+    PM_CHECK(allper || (ker2 ? Khat != nullptr : Ghat != nullptr), "pmgpu: pmgpu_poisson_setup was not called");
     PM_CHECK(vort && vel, "pmgpu: null mesh pointer");
     PM_CHECK(!want_vort || vort_out, "pmgpu: vort_out required with reg_vort/reproj");
     PM_CHECK(!want_psi || vecpot, "pmgpu: vecpot required with calc_vecpot");
@@ -419,7 +492,8 @@ template <typename R> struct PoissonGPU : PoissonBase {
     za.B = B;
     za.Bpsi = Bpsi;
     za.Bvort = Bvort;
-    za.Ghat = Ghat;
+    za.Ghat = ker2 ? (const void *)Khat : (const void *)Ghat;
+    za.ker2 = ker2 ? 1 : 0;
     za.rs = H3 * L[1];
     za.cs = 1;
     za.kxs = 3;
@@ -578,6 +652,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
       tz1 = v;
       return 0;
     }
+    PM_CHECK(prm.poisson_kernel == 1, "pmgpu: the single-component (stream function) solve needs poisson_kernel = 1");
     PM_CHECK(allper || Ghat != nullptr, "pmgpu: pmgpu_poisson_setup was not called");
     PM_HIP(hipMemsetAsync(B, 0, nB * sizeof(V), stream));
     hipEvent_t e0, e1;
@@ -614,6 +689,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
   // Stream-ordered; no host synchronisation and no memset, so it can be
   // captured into a graph once tune_scalar has run.
   int solve_scalar_device(const void *w, void *psi) override {
+    PM_CHECK(prm.poisson_kernel == 1, "pmgpu: the single-component (stream function) solve needs poisson_kernel = 1");
     PM_CHECK(allper || Ghat != nullptr, "pmgpu: pmgpu_poisson_setup was not called");
     PM_CHECK(w && psi, "pmgpu: null mesh pointer");
     PM_CHECK(tz1 > 0, "pmgpu: scalar solve not tuned (pmgpu_poisson_tune_scalar)");
@@ -649,6 +725,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
   // The passes of solve_scalar_device as group entries: the same arguments,
   // the same kernels (the tile sizes of the length-specialised launchers).
   int scalar_passes(const void *w, void *psi, ScalarPasses &sp) override {
+    PM_CHECK(prm.poisson_kernel == 1, "pmgpu: the single-component (stream function) solve needs poisson_kernel = 1");
     PM_CHECK(allper || Ghat != nullptr, "pmgpu: pmgpu_poisson_setup was not called");
     PM_CHECK(w && psi, "pmgpu: null mesh pointer");
     PM_CHECK(tz1 > 0, "pmgpu: scalar solve not tuned (pmgpu_poisson_tune_scalar)");

@@ -407,11 +407,20 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
       if (!(FULL || n < L)) continue;
       const R kk = wavenumber<R>(n, L, a.dk[2], a.ksmp[2]);
       const bool nz = (2 * n == L);
-      R g;
+      R g = (R)0;
       R zeta = (R)1;
+      V K0 = cmake<V>(0, 0), K1 = K0, K2 = K0;
       if (kind == 2 || a.allper)
         if (a.want_vort) zeta = zeta_filter<R>(a.order, (R)a.eps2pi * sqrt(ki * ki + kj * kj + kk * kk));
-      if (a.allper) {
+      if (a.ker2) {
+        // kernel 2: K = FFT(grad G) h^3, a complex half-spectrum
+        if (col_ok) {
+          const V *Kp = reinterpret_cast<const V *>(a.Ghat) + (((int64_t)n * a.Ly + ky) * a.Hx + kx) * 3;
+          K0 = cscale(Kp[0], (R)a.h3);
+          K1 = cscale(Kp[1], (R)a.h3);
+          K2 = cscale(Kp[2], (R)a.h3);
+        }
+      } else if (a.allper) {
         const R kmag = ki * ki + kj * kj + kk * kk;
         g = kmag > (R)0 ? zeta / ((R)4 * (pi * pi) * kmag) : (R)0;
       } else {
@@ -425,7 +434,27 @@ __global__ void __launch_bounds__(Geo<L>::P * 3 * TC) k_z3_f(ZArgs a, const cplx
       const V w0 = cl3[pad(n)], w1 = cl3[ST + pad(n)], w2 = cl3[2 * ST + pad(n)];
       const V wc = comp == 0 ? w0 : (comp == 1 ? w1 : w2);
       if (EXTRA && kd == 0) own[EXTRA ? e : 0] = wc;
-      if (!EXTRA || kind == 0) {
+      if (a.ker2) {
+        if (!EXTRA || kind == 0) {
+          // u-hat = K x w-hat (poisson/pmlib_poisson_solve.f:898-900)
+          if (comp == 0) r[e] = csub(cmul(K1, w2), cmul(K2, w1));
+          else if (comp == 1) r[e] = csub(cmul(K2, w0), cmul(K0, w2));
+          else r[e] = csub(cmul(K0, w1), cmul(K1, w0));
+        } else {
+          V q = cscale(wc, zeta);
+          if (a.reproj) {
+            // + K_c 2 pi i (k . w-hat) (:903-910); a Nyquist wavenumber drops
+            // out of the Hermitian part
+            V acc = cmake<V>(0, 0);
+            if (!nx) acc = cadd(acc, cscale(w0, ki));
+            if (!ny) acc = cadd(acc, cscale(w1, kj));
+            if (!nz) acc = cadd(acc, cscale(w2, kk));
+            const V Kc = comp == 0 ? K0 : (comp == 1 ? K1 : K2);
+            q = cadd(q, cmul(Kc, cmake<V>(-twopi * acc.y, twopi * acc.x)));
+          }
+          r[e] = q;
+        }
+      } else if (!EXTRA || kind == 0) {
         // u-hat = G 2 pi i (k x w-hat); Nyquist wavenumbers drop out of the
         // real part of the inverse (Hermitian projection)
         const R kx_ = nx ? (R)0 : ki, ky_ = ny ? (R)0 : kj, kz_ = nz ? (R)0 : kk;

@@ -104,7 +104,7 @@ template <typename R, int L> int zr_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipS
 }
 
 template <typename R, int L> int z_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
-  if (!(a.want_psi || a.want_vort) && a.zkernel != 1) {
+  if (!(a.want_psi || a.want_vort) && a.zkernel != 1 && !a.ker2) {
     const int r = zr_L<R, L>(a, Ly, tw, st);
     if (r != 0) return r;
   }

@@ -207,6 +207,8 @@ struct ZArgs {
   int want_psi, want_vort, reproj;
   int tune_tc;        // z-pass columns per block (0 = default)
   int zkernel;        // velocity-only z pass: 0 register-resident if available, 1 LDS-resident
+  int ker2;           // kernel 2 (grad G): Ghat is the complex half-spectrum
+                      // [((kz * Ly + ky) * Hx + kx) * 3 + c] of the three components
   double dk[3], ksmp[3], h3, eps2pi;
   FftPlan plan;       // length Lz
 };
@@ -292,15 +294,37 @@ __global__ void __launch_bounds__(1024) k_z_convolve(ZArgs a, const cplx_t<R> *t
           zeta = zeta_filter<R>(a.order, s);
         }
       }
-      R g;
-      if (a.allper) {
+      R g = (R)0;
+      V K[3] = {cmake<V>(0, 0), cmake<V>(0, 0), cmake<V>(0, 0)};
+      if (a.ker2) {
+        // kernel 2: K = FFT(grad G) h^3, a complex half-spectrum
+        if (col_ok) {
+          const V *Kp = reinterpret_cast<const V *>(a.Ghat) + (((int64_t)n * a.Ly + ky) * a.Hx + kx) * 3;
+          for (int c = 0; c < 3; ++c) K[c] = cscale(Kp[c], (R)a.h3);
+        }
+      } else if (a.allper) {
         R kmag = ki * ki + kj * kj + kk * kk;
         g = kmag > (R)0 ? zeta / ((R)4 * (pi * pi) * kmag) : (R)0;
       } else {
         g = col_ok ? G[((int64_t)kzf * a.Hy + kyf) * a.Hx + kx] * (R)a.h3 : (R)0;
       }
       V r;
-      if (kind == 0) {
+      if (a.ker2 && kind == 0) {
+        // u-hat = K x w-hat (poisson/pmlib_poisson_solve.f:898-900)
+        const int c1 = comp == 2 ? 0 : comp + 1, c2 = comp == 0 ? 2 : comp - 1;
+        r = csub(cmul(K[c1], w[c2][e]), cmul(K[c2], w[c1][e]));
+      } else if (a.ker2) {
+        // filtered vorticity + K_c 2 pi i (k . w-hat) (:903-910); a Nyquist
+        // wavenumber drops out of the Hermitian part
+        r = cscale(w[comp][e], zeta);
+        if (a.reproj) {
+          V acc = cmake<V>(0, 0);
+          if (!nx) acc = cadd(acc, cscale(w[0][e], ki));
+          if (!ny) acc = cadd(acc, cscale(w[1][e], kj));
+          if (!nz) acc = cadd(acc, cscale(w[2][e], kk));
+          r = cadd(r, cmul(K[comp], cmake<V>(-twopi * acc.y, twopi * acc.x)));
+        }
+      } else if (kind == 0) {
         // u-hat = G 2 pi i (k x w-hat); Nyquist wavenumbers drop out of the
         // real part of the inverse (Hermitian projection)
         R kx_ = nx ? (R)0 : ki, ky_ = ny ? (R)0 : kj, kz_ = nz ? (R)0 : kk;
@@ -415,6 +439,69 @@ static __global__ void k_green(GreenArgs a) {
         }
     a.out[idx] = acc;
   }
+}
+
+// Regularised kernel 2 (grad G, the three components interleaved
+// [idx * 3 + c]) on the same grid and in the same FFT order:
+// -x_c / (4 pi r^3) (poly(rho) E + F), 0 at r = 0
+// (poisson/pmlib_poisson_setup.f:580-635 (order 4), :287-320 (0), :389-430
+// (2), :783-839 (6), :1001-1060 (8), :1231-1293 (10)).
+static __global__ void k_green2(GreenArgs a) {
+  const int64_t n = (int64_t)a.L[0] * a.L[1] * a.L[2];
+  const double pi = 3.14159265358979323846;
+  const double c_1_4pi = 0.25 / pi;
+  const double c_1_sqrt2 = 0.70710678118654752440;
+  const double c_1_sqrt2pi = 0.39894228040143267794;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int ix = idx % a.L[0];
+    int iy = (idx / a.L[0]) % a.L[1];
+    int iz = idx / ((int64_t)a.L[0] * a.L[1]);
+    int nn[3] = {ix, iy, iz};
+    double base[3];
+    for (int d = 0; d < 3; ++d) {
+      int m = nn[d] < a.L[d] / 2 ? nn[d] : nn[d] - a.L[d];
+      int i1 = m + a.L[d] / 2 + 1;
+      base[d] = a.xk[d] + ((double)(i1 - 1) + 0.5) * a.dx[d];
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int kk = -a.bc[2]; kk <= a.bc[2]; ++kk)
+      for (int jj = -a.bc[1]; jj <= a.bc[1]; ++jj)
+        for (int ii = -a.bc[0]; ii <= a.bc[0]; ++ii) {
+          const double p[3] = {base[0] + a.len[0] * (double)ii, base[1] + a.len[1] * (double)jj,
+                               base[2] + a.len[2] * (double)kk};
+          const double r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+          if (r == 0.0) continue;
+          const double rho = r / a.eps;
+          double f;
+          if (a.order == 0) {
+            f = 1.0;
+          } else {
+            const double r2 = rho * rho, r3 = r2 * rho, r5 = r3 * r2, r7 = r5 * r2, r9 = r7 * r2;
+            double poly;
+            switch (a.order) {
+              case 2: poly = -2.0 * rho; break;
+              case 4: poly = -2.0 * rho + r3; break;
+              case 6: poly = -2.0 * rho + 2.25 * r3 - 0.25 * r5; break;
+              case 8: poly = -2.0 * rho + (89.0 / 24.0) * r3 + (-20.0 / 24.0) * r5 + (1.0 / 24.0) * r7; break;
+              default:
+                poly = -2.0 * rho + (1027.0 / 192.0) * r3 + (-349.0 / 192.0) * r5 + (35.0 / 192.0) * r7 +
+                       (-1.0 / 192.0) * r9;
+                break;
+            }
+            f = poly * exp(-0.5 * (rho * rho)) * c_1_sqrt2pi + erf(c_1_sqrt2 * rho);
+          }
+          const double s = c_1_4pi / (r * r * r) * f;
+          for (int c = 0; c < 3; ++c) acc[c] -= p[c] * s;
+        }
+    for (int c = 0; c < 3; ++c) a.out[idx * 3 + c] = acc[c];
+  }
+}
+
+// Working-precision copy of the kernel-2 half-spectrum
+template <typename R> __global__ void k_spec_convert(const double2 *in, cplx_t<R> *out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = cmake<cplx_t<R>>((R)in[i].x, (R)in[i].y);
 }
 
 // Octant extraction of the real, even kernel spectrum:

@@ -32,13 +32,26 @@ SOLVE_CASES = {
     "mixed_o4": ((8, 8, 12), 2, (1, 1, 0), 4, (True, False, True)),
     "periodic_o6": ((12, 8, 8), 0, (1, 1, 1), 6, (True, True, True)),
     "free_o2_r35": ((20, 12, 18), 0, (0, 0, 0), 2, (False, False, True)),
+    # kernel 2 (the gradient of G, poisson/pmlib_poisson_setup.f:580-635 and
+    # the kernel-2 branch of poisson/pmlib_poisson_solve.f:873-913)
+    "free_k2_o4_all": ((16, 12, 10), 0, (0, 0, 0), 4, (True, True, False), 2),
+    "free_ng2_k2_o6": ((10, 12, 8), 2, (0, 0, 0), 6, (False, False, False), 2),
+    "mixed_k2_o8": ((8, 12, 10), 2, (1, 0, 1), 8, (True, True, False), 2),
+    "periodic_k2_o2": ((12, 8, 10), 0, (1, 1, 1), 2, (True, True, False), 2),
+    "free_k2_o10": ((12, 10, 14), 1, (0, 0, 0), 10, (False, True, False), 2),
 }
 
 
+def solve_spec(spec):
+    """(ncell, nghost, bound_cond, order, (reg_vort, reproj, calc_vecpot),
+    kernel); kernel 1 when the spec has five fields."""
+    return tuple(spec) if len(spec) == 6 else tuple(spec) + (1,)
+
+
 def solve_case(name, spec):
-    ncell, ng, bc, order, (rv, rp, cv) = spec
+    ncell, ng, bc, order, (rv, rp, cv), kernel = solve_spec(spec)
     g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
-    p = O.make_params(order, 1, 2.0)
+    p = O.make_params(order, kernel, 2.0)
     w = blob_field(g, seed=sum(ncell) + order)
     job = dict(O.geom_job(g, p), mode="solve", reg_vort=rv, reproj=rp, calc_vecpot=cv)
     raw, _ = O.ref_run(job, [w])
@@ -49,7 +62,7 @@ def solve_case(name, spec):
     if rv or rp:
         outs["vort"] = ref[q * n:(q + 1) * n]
         q += 1
-    if cv:
+    if cv and kernel == 1:
         outs["psi"] = ref[q * n:(q + 1) * n]
     np.savez_compressed(os.path.join(HERE, "ref_solve_%s.npz" % name), vort_in=w,
                         spec=np.array(json.dumps(spec)),
@@ -143,6 +156,10 @@ def kat():
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref is not built: make -f oracle/Makefile.ref")
+    if len(sys.argv) > 1:   # only the named solve cases
+        for name in sys.argv[1:]:
+            solve_case(name, SOLVE_CASES[name])
+        sys.exit(0)
     for name, spec in SOLVE_CASES.items():
         solve_case(name, spec)
     for name, spec in STEP_CASES.items():

@@ -116,15 +116,65 @@ def test_free_space_solve_converges_to_closed_form_ring():
                          ids=lambda p: os.path.basename(p)[10:-4])
 def test_oracle_solve_matches_reference_golden(path):
     d = np.load(path, allow_pickle=False)
-    ncell, ng, bc, order, (rv, rp, cv) = json.loads(str(d["spec"]))
+    ncell, ng, bc, order, (rv, rp, cv), kernel = solve_spec(json.loads(str(d["spec"])))
     g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
-    p = O.make_params(order, 1, 2.0)
+    p = O.make_params(order, kernel, 2.0)
     vel, vo, ps = O.poisson_solve(g, p, O.poisson_setup(g, p), d["vort_in"], rv, rp, cv)
     assert rel_linf(vel, d["vel"]) <= 2e-15
     if rv or rp:
         assert rel_linf(vo, d["vort"]) <= 2e-15
-    if cv:
+    if cv and kernel == 1:
         assert rel_linf(ps, d["psi"]) <= 2e-15
+
+
+def solve_spec(spec):
+    """Golden solve spec with the kernel (1 when absent)."""
+    return tuple(spec) if len(spec) == 6 else tuple(spec) + (1,)
+
+
+def _green_closed_form(order, r, eps):
+    """Kernel-1 G(r) for r > 0 from the order table (poly(rho) E + F) / (4 pi r);
+    complex r allowed (scipy's erf is analytic)."""
+    from scipy.special import erf
+    rho = r / eps
+    E = np.exp(-0.5 * rho * rho) / np.sqrt(2 * np.pi)
+    F = erf(rho / np.sqrt(2))
+    poly = {0: None, 2: 0 * rho, 4: rho, 6: 1.75 * rho - 0.25 * rho ** 3,
+            8: 2.375 * rho - (2 / 3) * rho ** 3 + rho ** 5 / 24,
+            10: 2.921875 * rho - (233 / 192) * rho ** 3 + (29 / 192) * rho ** 5 - rho ** 7 / 192}[order]
+    if poly is None:
+        return 1 / (4 * np.pi * r)
+    return (poly * E + F) / (4 * np.pi * r)
+
+
+@pytest.mark.parametrize("order", [0, 2, 4, 6, 8, 10])
+def test_green_kernel2_is_the_gradient_of_kernel1(order):
+    """Kernel 2 (grad G) against the complex-step derivative of the kernel-1
+    closed form (itself checked against the oracle's kernel-1 grid first):
+    K = (x / r) dG/dr at every grid point, 0 at r = 0, and -x / (4 pi r^3)
+    in the far field.  A wrong polynomial coefficient fails here."""
+    g = O.make_geom((6, 5, 4), 0, [-0.3, -0.2, -0.25], 0.05, (0, 0, 0))
+    L = [2 * n for n in g.ncell]
+    x = [(np.arange(L[d]) - L[d] // 2) * g.dx[d] for d in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    eps = 2.0 * max(g.dx)
+    G1 = O.green_real(g, O.make_params(order, 1, 2.0)).reshape(L, order="F")
+    K = O.green_real(g, O.make_params(order, 2, 2.0)).reshape([3] + L, order="F")
+    nz = r > 0
+    g_np = _green_closed_form(order, r[nz], eps)
+    assert np.abs(g_np - G1[nz]).max() <= 1e-14 * np.abs(G1[nz]).max()
+    h = 1e-30
+    dG = (_green_closed_form(order, r[nz] + 1j * h, eps)).imag / h
+    for c, P in enumerate((X, Y, Z)):
+        want = P[nz] / r[nz] * dG
+        assert np.abs(K[c][nz] - want).max() <= 1e-12 * np.abs(want).max(), (order, c)
+        assert K[c][~nz].tolist() == [0.0]
+    far = r > 8 * eps
+    if far.any():
+        for c, P in enumerate((X, Y, Z)):
+            want = -P[far] / (4 * np.pi * r[far] ** 3)
+            assert np.abs(K[c][far] - want).max() <= 1e-6 * np.abs(want).max()
 
 
 # -------------------------------------------------------- interpolation
