@@ -212,6 +212,42 @@ int pmgpu_penalise(const pmgpu_geom *g, const double *vel, const double *chi, co
 int pmgpu_penalise_device(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_chi,
                           const void *d_solid_vel, const double us_const[3], double relax,
                           int32_t fd_order, void *d_vort, double *diag, void *stream);
+/* The penalisation iteration's module globals (vortex_codes/
+ * vortex_penalisation/source/mod_parameters.f:43-74, pmlib_fd_order). */
+typedef struct pmgpu_penal_params {
+  double relaxation;       /* penal_relaxation                                */
+  double tolerance;        /* penal_tolerance (relative force increment)      */
+  int32_t max_iter;        /* penal_max                                       */
+  int32_t fd_order;        /* pmlib_fd_order: 2 or 4                          */
+  double dtime;            /* dtime                                           */
+  double vel_inf[3];       /* vel_inf, added to the velocity at the end       */
+  int32_t reproject;       /* reproject: the final solve reprojects vort      */
+  int32_t fix_free_stream; /* 0: as the reference (vel(2) += vel_inf(3), and
+                              vel(3) gets nothing); 1: vel(3) += vel_inf(3)   */
+} pmgpu_penal_params;
+
+/* penalisation_solve(patch, topo_all, mesh, ierr)
+ * (vortex_codes/vortex_penalisation/source/penalisation/penalisation_solve.f:6,
+ * set-up :61-111, iteration :133-343, map back :348-365, final solve
+ * :372-398, free stream :405-414) with the penalisation mesh on the solver's
+ * own patch: vel_pen = us - vel; then, until the relative force increment
+ * falls to the tolerance (at least once, at most max_iter times),
+ * vort_pen += relax curl(chi (vel_pen - u_pen)) and u_pen = the velocity of
+ * vort_pen (pmgpu_poisson_solve_device on h); then vort += vort_pen (periodic
+ * ghosts mapped), vel = the velocity of vort (reprojected vorticity into vort
+ * when reproject), and vel += vel_inf.  chi, solid_vel (NULL: us_const, NULL
+ * for rest) and the meshes are device arrays in h's precision; h must be set
+ * up.  force, npen and residual (each may be NULL) receive the accumulated
+ * force, the iterations and the last relative force increment.  The solve
+ * of the penalisation vorticity uses the library's free-space solver where
+ * the reference calls an external one (poisson_solver_solve3d, not
+ * vendored; the reference's commented-out alternative is
+ * pmlib_poisson_solve, :295-296).  Synchronises the stream once per
+ * iteration (the stopping test). */
+int pmgpu_penalisation_solve_device(pmgpu_poisson_t h, const pmgpu_penal_params *pp, const void *d_chi,
+                                    const void *d_solid_vel, const double us_const[3], void *d_vel,
+                                    void *d_vort, double force[3], int32_t *npen, double *residual);
+
 /* Allocation-free variant of pmgpu_penalise_device.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -224,6 +224,66 @@ def vort_rhs(g, vort, vel, visc, fd_order=4, dvort=None):
     return out
 
 
+def _ghost_fill_periodic(g, m):
+    """Periodic forward ghost map of a (3, E...) mesh, x then y then z
+    (mesh/pmlib_mesh_map_ghost.f, incl_edges = .FALSE., op = replace)."""
+    E = mesh_extent(g)
+    for d in range(3):
+        if g.bound_cond[d] != 1:
+            continue
+        ng = (E[d] - g.ncell[d]) // 2
+        N = g.ncell[d]
+        for t in range(ng):
+            lo = [slice(None)] * 4
+            src = [slice(None)] * 4
+            lo[d + 1], src[d + 1] = t, N + t
+            m[tuple(lo)] = m[tuple(src)]
+            lo[d + 1], src[d + 1] = ng + N + t, ng + t
+            m[tuple(lo)] = m[tuple(src)]
+    return m
+
+
+def penalisation_solve(g, p, Ghat, vel, vort, chi, solid_vel=None, relaxation=1.0, tolerance=1e-3, max_iter=10,
+                       fd_order=4, dtime=1.0, vel_inf=(0.0, 0.0, 0.0), reproject=False, fix_free_stream=False):
+    """penalisation_solve(patch, topo_all, mesh, ierr)
+    (vortex_codes/vortex_penalisation/source/penalisation/penalisation_solve.f:
+    set-up :61-111, iteration :133-343, map back :348-365, final solve
+    :372-398, free stream :405-414) on one rank with the penalisation mesh on
+    the patch itself, composed of or_penalise and or_poisson_solve.  solid_vel
+    None: the solid at rest.  Returns (vel, vort, force, iterations,
+    residual_force)."""
+    vel = np.asfortranarray(vel, dtype=np.float64)
+    us = np.zeros_like(vel) if solid_vel is None else np.asfortranarray(solid_vel, dtype=np.float64)
+    vel_pen = np.asfortranarray(us - vel)
+    vort_pen = np.zeros_like(vel, order="F")
+    u_pen = np.zeros_like(vel, order="F")
+    c_1_dt = 1.0 / dtime
+    force = np.zeros(3)
+    residual = 0.0
+    npen = 0
+    while (residual > tolerance and npen < max_iter) or npen == 0:
+        npen += 1
+        # chi (vel_pen - u_pen): in the first iteration u_pen is zero
+        vort_pen, d = penalise(g, u_pen, chi, vort_pen, solid_vel=vel_pen, relax=relaxation, fd_order=fd_order)
+        u_pen = poisson_solve(g, p, Ghat, vort_pen)[0]
+        dforce = np.array([-d[c] * g.dx[0] * g.dx[1] * g.dx[2] * c_1_dt for c in range(3)])
+        force = force + dforce
+        residual = float(np.sqrt(dforce[0] ** 2 + dforce[1] ** 2 + dforce[2] ** 2) /
+                         np.sqrt(force[0] ** 2 + force[1] ** 2 + force[2] ** 2))
+    vort = _ghost_fill_periodic(g, np.asfortranarray(np.asarray(vort, dtype=np.float64) + vort_pen))
+    out_v, out_w, _ = poisson_solve(g, p, Ghat, vort, False, bool(reproject), False)
+    if reproject:
+        vort = out_w
+    out_v = np.asfortranarray(out_v)
+    out_v[0] = out_v[0] + vel_inf[0]
+    out_v[1] = out_v[1] + vel_inf[1]
+    if fix_free_stream:
+        out_v[2] = out_v[2] + vel_inf[2]
+    else:
+        out_v[1] = out_v[1] + vel_inf[2]
+    return out_v, vort, force, npen, residual
+
+
 DIAG_PARTICLE_NAMES = ("impulse_x", "impulse_y", "impulse_z", "ang_impulse_x", "ang_impulse_y", "ang_impulse_z",
                        "kin_energy", "enstrophy", "helicity", "circulation", "centroid_x", "centroid_y",
                        "centroid_z", "vort_radius", "vort_core", "vort_max")

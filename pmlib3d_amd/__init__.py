@@ -12,7 +12,8 @@ from .core import (Patch, Mesh, Particles, Parameters, params, pmlib_write, mesh
                    pmlib_poisson_setup, pmlib_poisson_solve, PoissonSolver, PoissonBatch)
 from .particles import (cell_index, pmlib_interp_particle_mesh, pmlib_interp_mesh_particle,
                         interp_mesh_field, pmlib_remesh, pmlib_particles_advance, ParticleOps)
-from .penalisation import penalise, penalise_device, penalise_device_noalloc, sphere_mask, brinkman_factor
+from .penalisation import (penalise, penalise_device, penalise_device_noalloc, penalisation_solve_device,
+                           sphere_mask, brinkman_factor)
 from .derivatives import vorticity_rhs, vorticity_rhs_device, part_deriv_vorticity
 from .step import VortexStep
 from .diagnostics import (diagnostics_ring, diagnostics_energy_mesh, diagnostics_particles_device,
@@ -23,6 +24,6 @@ __all__ = ["Patch", "Mesh", "Particles", "Parameters", "params", "pmlib_write", 
            "pmlib_poisson_setup", "pmlib_poisson_solve", "PoissonSolver", "PoissonBatch", "PmgpuError",
            "FP32", "FP64", "cell_index", "pmlib_interp_particle_mesh", "pmlib_interp_mesh_particle",
            "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps", "penalise",
-           "penalise_device", "penalise_device_noalloc", "sphere_mask", "brinkman_factor", "vorticity_rhs", "vorticity_rhs_device",
+           "penalise_device", "penalise_device_noalloc", "penalisation_solve_device", "sphere_mask", "brinkman_factor", "vorticity_rhs", "vorticity_rhs_device",
            "part_deriv_vorticity", "VortexStep", "diagnostics_ring", "diagnostics_energy_mesh",
            "diagnostics_particles_device", "diagnostics_mesh_device"]

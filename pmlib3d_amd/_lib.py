@@ -29,6 +29,12 @@ class Params(C.Structure):
                 ("poisson_kernel", C.c_int32)]
 
 
+class PenalParams(C.Structure):
+    _fields_ = [("relaxation", C.c_double), ("tolerance", C.c_double), ("max_iter", C.c_int32),
+                ("fd_order", C.c_int32), ("dtime", C.c_double), ("vel_inf", C.c_double * 3),
+                ("reproject", C.c_int32), ("fix_free_stream", C.c_int32)]
+
+
 # name -> (restype, argtypes); every exported symbol of include/pmgpu.h
 _P = C.c_void_p
 _I = C.c_int32
@@ -68,6 +74,8 @@ SIGNATURES = {
     "pmgpu_penalise_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P, _P]),
     "pmgpu_diagnostics_particles": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P]),
     "pmgpu_diagnostics_mesh": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, _P]),
+    "pmgpu_penalisation_solve_device": (C.c_int, [_P, C.POINTER(PenalParams), _P, _P, _P, _P, _P, _P,
+                                                  C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pmgpu_penalise_device_noalloc": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P,
                                                 _P]),
     "pmgpu_vorticity_rhs": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_double, _I, _P]),

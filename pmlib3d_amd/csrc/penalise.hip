@@ -9,6 +9,7 @@
 // (the neighbours along i are the adjacent lanes, so the re-reads hit L1/L2).
 // Every floating-point operation is an explicitly rounded one, in the order
 // the reference writes it, so the FP64 result is bit-identical to it.
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -266,6 +267,125 @@ static double *scratch_address() {
 
 int penalise_noalloc_init() { return scratch_address() ? 0 : -1; }
 
+// ---- iterative penalisation (penalisation_solve) -------------------------
+// vp = us - u on every cell, us the solid-velocity field or the constants
+// (penalisation_solve.f:88-95)
+template <typename T>
+__global__ void k_pen_velpen(const T *__restrict__ us, T us0, T us1, T us2, const T *__restrict__ u,
+                             T *__restrict__ vp, int64_t ncell) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * ncell;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % 3);
+    const T s = us ? us[i] : (c == 0 ? us0 : (c == 1 ? us1 : us2));
+    vp[i] = s - u[i];
+  }
+}
+template <typename T> __global__ void k_pen_add(T *__restrict__ a, const T *__restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = a[i] + b[i];
+}
+// free-stream velocity on every cell (penalisation_solve.f:405-414); quirk:
+// the reference adds vel_inf(3) to the second component, as written there
+template <typename T>
+__global__ void k_pen_free_stream(T *__restrict__ vel, T v0, T v1, T v2, int64_t ncell, int quirk) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncell; i += (int64_t)gridDim.x * blockDim.x) {
+    T *u = vel + 3 * i;
+    u[0] = u[0] + v0;
+    u[1] = u[1] + v1;
+    if (quirk)
+      u[1] = u[1] + v2;
+    else
+      u[2] = u[2] + v2;
+  }
+}
+
+struct PenSolveWS {
+  void *velpen = nullptr, *upen = nullptr, *vpen = nullptr;
+  size_t bytes = 0;
+};
+static PenSolveWS *g_psolve = nullptr;
+
+void penalisation_solve_release() {
+  if (g_psolve) {
+    void *b[] = {g_psolve->velpen, g_psolve->upen, g_psolve->vpen};
+    for (void *p : b)
+      if (p) (void)hipFree(p);
+    delete g_psolve;
+    g_psolve = nullptr;
+  }
+}
+
+template <typename T>
+static int penalisation_solve_impl(pmgpu_poisson_t h, const pmgpu_geom &g, hipStream_t st,
+                                   const pmgpu_penal_params &pp, const void *chi, const void *solid_vel,
+                                   const double *us_const, void *vel, void *vort, double *force_out,
+                                   int32_t *npen_out, double *residual_out) {
+  int32_t E[3];
+  if (pmgpu_mesh_extent(&g, E)) return -1;
+  const int64_t ncell = (int64_t)E[0] * E[1] * E[2];
+  const size_t bytes = sizeof(T) * 3 * (size_t)ncell;
+  if (!g_psolve) g_psolve = new PenSolveWS;
+  if (g_psolve->bytes < bytes) {
+    penalisation_solve_release();
+    g_psolve = new PenSolveWS;
+    PM_HIP(hipMalloc(&g_psolve->velpen, bytes));
+    PM_HIP(hipMalloc(&g_psolve->upen, bytes));
+    PM_HIP(hipMalloc(&g_psolve->vpen, bytes));
+    g_psolve->bytes = bytes;
+  }
+  T *velpen = (T *)g_psolve->velpen, *upen = (T *)g_psolve->upen, *vpen = (T *)g_psolve->vpen;
+  const dim3 eg(2048), eb(256);
+  // vel_pen = solid_vel - vel (the velocity mapped to the penalisation mesh)
+  hipLaunchKernelGGL(k_pen_velpen<T>, eg, eb, 0, st, (const T *)solid_vel, T(us_const ? us_const[0] : 0.0),
+                     T(us_const ? us_const[1] : 0.0), T(us_const ? us_const[2] : 0.0), (const T *)vel, velpen,
+                     ncell);
+  PM_HIP(hipGetLastError());
+  // the penalisation vorticity starts at zero (cleared after every call in
+  // the reference), and so does the velocity it induces
+  PM_HIP(hipMemsetAsync(vpen, 0, bytes, st));
+  PM_HIP(hipMemsetAsync(upen, 0, bytes, st));
+  const double c_1_dt = 1.0 / pp.dtime;
+  double force[3] = {0.0, 0.0, 0.0}, residual = 0.0;
+  int npen = 0;
+  while ((residual > pp.tolerance && npen < pp.max_iter) || npen == 0) {
+    ++npen;
+    // vort_pen += relax curl(chi (vel_pen - u_pen)) and the force sums
+    double d[5];
+    if (penalise_launch<T>(&g, upen, chi, velpen, nullptr, pp.relaxation, pp.fd_order, vpen, d, st)) return -1;
+    // the velocity of the penalisation vorticity (ghost cells included)
+    if (pmgpu_poisson_solve_device(h, vpen, upen, nullptr, nullptr, 0, 0, 0)) return -1;
+    double dforce[3];
+    for (int c = 0; c < 3; ++c) {
+      dforce[c] = -d[c] * g.dx[0] * g.dx[1] * g.dx[2] * c_1_dt;
+      force[c] = force[c] + dforce[c];
+    }
+    residual = std::sqrt(dforce[0] * dforce[0] + dforce[1] * dforce[1] + dforce[2] * dforce[2]) /
+               std::sqrt(force[0] * force[0] + force[1] * force[1] + force[2] * force[2]);
+  }
+  // add the penalisation vorticity to the mesh vorticity (its ghost cells
+  // are zero), then the periodic ghost map of the vorticity
+  hipLaunchKernelGGL(k_pen_add<T>, eg, eb, 0, st, (T *)vort, (const T *)vpen, 3 * ncell);
+  PM_HIP(hipGetLastError());
+  for (int d = 0; d < 3; ++d)
+    if (g.bound_cond[d] == 1) {
+      hipLaunchKernelGGL(k_ghost_fill<T>, dim3(1024), dim3(256), 0, st, (T *)vort, E[0], E[1], E[2], d, g.ncell[d],
+                         (E[d] - g.ncell[d]) / 2, 3);
+      PM_HIP(hipGetLastError());
+    }
+  // the velocity on the whole mesh, with the reprojected vorticity if asked
+  if (pmgpu_poisson_solve_device(h, vort, vel, pp.reproject ? vort : nullptr, nullptr, 0, pp.reproject ? 1 : 0, 0))
+    return -1;
+  hipLaunchKernelGGL(k_pen_free_stream<T>, eg, eb, 0, st, (T *)vel, T(pp.vel_inf[0]), T(pp.vel_inf[1]),
+                     T(pp.vel_inf[2]), ncell, pp.fix_free_stream ? 0 : 1);
+  PM_HIP(hipGetLastError());
+  PM_HIP(hipStreamSynchronize(st));
+  if (force_out)
+    for (int c = 0; c < 3; ++c) force_out[c] = force[c];
+  if (npen_out) *npen_out = npen;
+  if (residual_out) *residual_out = residual;
+  return 0;
+}
+
 }  // namespace pmgpu
 
 using namespace pmgpu;
@@ -307,6 +427,23 @@ int pmgpu_penalise_device(const pmgpu_geom *g, int32_t precision, const void *d_
   PM_CHECK(precision == PMGPU_FP64, "pmgpu_penalise_device: precision must be PMGPU_FP64 or PMGPU_FP32");
   return penalise_launch<double>(g, d_vel, d_chi, d_solid_vel, us_const, relax, fd_order, d_vort, diag,
                                  (hipStream_t)stream);
+}
+
+int pmgpu_penalisation_solve_device(pmgpu_poisson_t h, const pmgpu_penal_params *pp, const void *d_chi,
+                                    const void *d_solid_vel, const double us_const[3], void *d_vel, void *d_vort,
+                                    double force[3], int32_t *npen, double *residual) {
+  PM_CHECK(h && pp && d_chi && d_vel && d_vort, "pmgpu_penalisation_solve_device: null argument");
+  PM_CHECK(pp->dtime > 0.0, "pmgpu_penalisation_solve_device: dtime must be positive");
+  PM_CHECK(pp->max_iter >= 1, "pmgpu_penalisation_solve_device: penal_max must be at least 1");
+  pmgpu_geom g;
+  int precision = 0;
+  hipStream_t st = nullptr;
+  if (poisson_handle_info(h, &g, &precision, &st)) return -1;
+  if (precision == PMGPU_FP32)
+    return penalisation_solve_impl<float>(h, g, st, *pp, d_chi, d_solid_vel, us_const, d_vel, d_vort, force, npen,
+                                          residual);
+  return penalisation_solve_impl<double>(h, g, st, *pp, d_chi, d_solid_vel, us_const, d_vel, d_vort, force, npen,
+                                         residual);
 }
 
 int pmgpu_penalise_device_noalloc(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_chi,

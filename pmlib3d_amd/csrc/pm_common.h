@@ -102,8 +102,16 @@ __global__ void k_ghost_fill(R *m, int E0, int E1, int E2, int d, int N, int ng,
 void particles_release();
 void penalise_release();
 void diagnostics_release();
+void penalisation_solve_release();
 // fetches the device address of the allocation-free penalisation scratch on
 // the current device (once per device)
 int penalise_noalloc_init();
+
+}  // namespace pmgpu
+typedef struct pmgpu_poisson_s *pmgpu_poisson_t;
+typedef struct pmgpu_geom pmgpu_geom;
+namespace pmgpu {
+// geometry, precision and stream of a solver handle (poisson.hip)
+int poisson_handle_info(pmgpu_poisson_t h, pmgpu_geom *g, int *precision, hipStream_t *stream);
 
 }  // namespace pmgpu

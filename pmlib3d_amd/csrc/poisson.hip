@@ -141,6 +141,7 @@ struct PoissonBase {
   virtual int solve_scalar_device(const void *w, void *psi) = 0;
   virtual int solve_scalar_host(const double *w, double *psi) = 0;
   virtual int tune_scalar() = 0;
+  virtual int precision() const = 0;
   pmgpu_geom g;
   pmgpu_params prm;
   hipStream_t stream = nullptr;
@@ -206,6 +207,8 @@ template <typename R> struct PoissonGPU : PoissonBase {
   int zkernel = 0;   // 0: register-resident velocity z pass where available, 1: LDS-resident
   int skernel = 0;   // the same choice for the strided (y and z) passes
   int lds_limit = 65536;
+
+  int precision() const override { return sizeof(R) == 8 ? PMGPU_FP64 : PMGPU_FP32; }
 
   ~PoissonGPU() override {
     void *bufs[] = {d_tw, A, B, Bpsi, Bvort, Ghat, Khat, mesh_in, mesh_out};
@@ -887,6 +890,20 @@ int pmgpu_poisson_solve_device(pmgpu_poisson_t h, const void *d_vort, void *d_ve
   PM_CHECK(h, "pmgpu_poisson_solve_device: null handle");
   return h->impl->solve_device(d_vort, d_vel, d_vort_out, d_vecpot, reg_vort, reproj, calc_vecpot);
 }
+
+}  // extern "C"
+
+namespace pmgpu {
+int poisson_handle_info(pmgpu_poisson_t h, pmgpu_geom *g, int *precision, hipStream_t *stream) {
+  PM_CHECK(h, "pmgpu: null solver handle");
+  *g = h->impl->g;
+  *precision = h->impl->precision();
+  *stream = h->impl->stream;
+  return 0;
+}
+}  // namespace pmgpu
+
+extern "C" {
 
 int pmgpu_poisson_kernel_octant(pmgpu_poisson_t h, void *host, int64_t cap_bytes, int32_t dims[3]) {
   PM_CHECK(h && host && dims, "pmgpu_poisson_kernel_octant: null argument");

@@ -87,3 +87,29 @@ def penalise_device_noalloc(patch, precision, d_vel, d_chi, d_vort, d_diag=None,
     _lib.check(_lib.load().pmgpu_penalise_device_noalloc(C.byref(g), int(precision), d_vel, d_chi, d_solid_vel,
                                                          uc, float(relax), int(fd_order), d_vort, d_diag, stream),
                "pmgpu_penalise_device_noalloc")
+
+
+def penalisation_solve_device(solver, d_chi, d_vel, d_vort, relaxation=1.0, tolerance=1e-3, max_iter=10,
+                              fd_order=4, dtime=1.0, vel_inf=(0.0, 0.0, 0.0), reproject=False,
+                              d_solid_vel=None, us_const=None, fix_free_stream=False):
+    """penalisation_solve(patch, topo_all, mesh, ierr) on device meshes of the
+    solver's patch and precision (vortex_codes/vortex_penalisation/source/
+    penalisation/penalisation_solve.f:6): iterates penalisation and solve
+    until the relative force increment is at most `tolerance` (at least once,
+    at most `max_iter` times), adds the penalisation vorticity to d_vort,
+    solves for d_vel and adds vel_inf (as the reference does, vel_inf[2] goes
+    to the second component unless fix_free_stream).  The module globals
+    penal_relaxation, penal_tolerance, penal_max, pmlib_fd_order, dtime,
+    vel_inf and reproject are arguments.  Returns (force[3], iterations,
+    last relative force increment)."""
+    pp = _lib.PenalParams(float(relaxation), float(tolerance), int(max_iter), int(fd_order), float(dtime),
+                          (C.c_double * 3)(*[float(v) for v in vel_inf]), int(bool(reproject)),
+                          int(bool(fix_free_stream)))
+    uc = None if us_const is None else (C.c_double * 3)(*[float(v) for v in us_const])
+    force = np.zeros(3)
+    npen = C.c_int32(0)
+    res = C.c_double(0.0)
+    _lib.check(_lib.load().pmgpu_penalisation_solve_device(solver.h, C.byref(pp), d_chi, d_solid_vel, uc, d_vel,
+                                                           d_vort, _ptr(force), C.byref(npen), C.byref(res)),
+               "pmgpu_penalisation_solve_device")
+    return force, npen.value, res.value

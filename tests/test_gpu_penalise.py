@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.helpers import blob_field
+from tests.helpers import blob_field, rel_l2, rel_linf
 
 pytestmark = pytest.mark.gpu
 
@@ -95,3 +95,69 @@ def test_penalise_rejects_bad_input():
         pm.penalise(_patch(g), z, np.zeros(E), z, fd_order=4)   # needs 2 ghost layers
     with pytest.raises(pm.PmgpuError):
         pm.penalise(_patch(g), z, np.zeros(E), z, fd_order=3)
+
+
+# ------------------------------------------------------ penalisation_solve
+SOLVE_CASES = [
+    # ncell, nghost, bound_cond, fd_order, reproject, solid: "rest" / "field" / "const", max_iter, tolerance
+    ((16, 16, 16), 2, (0, 0, 0), 4, False, "rest", 3, 0.0),
+    ((16, 12, 20), 2, (0, 0, 0), 2, True, "const", 6, 0.05),
+    ((12, 12, 12), 0, (1, 1, 1), 4, True, "field", 2, 0.0),
+    ((12, 16, 10), 2, (1, 1, 0), 4, False, "rest", 4, 0.0),
+]
+
+
+@pytest.mark.parametrize("case", SOLVE_CASES, ids=[str(i) for i in range(len(SOLVE_CASES))])
+def test_penalisation_solve_matches_oracle(case):
+    """The whole iteration (penalise, solve, force residual test, map back,
+    final solve, free stream with the reference's component quirk) against
+    the oracle composition, FP64 within 1e-12; the same iteration count."""
+    import torch
+    import pmlib3d_amd as pm
+    ncell, ng, bc, fd, reproj, solid, nmax, tol = case
+    print("penalisation_solve case", case, flush=True)
+    g = O.make_geom(ncell, ng, [-0.6, -0.5, -0.7], 1.2 / 16, bc)
+    E = O.mesh_extent(g)
+    p = O.make_params(4, 1, 2.0)
+    patch = _patch(g)
+    vel = blob_field(g, seed=31, width=0.15)
+    vel[0] += 1.0
+    vort = blob_field(g, seed=32, width=0.15)
+    chi = pm.brinkman_factor(pm.sphere_mask(patch, 0.3, (0.05, -0.02, 0.0)), 1e4, 0.02)
+    rng = np.random.default_rng(7)
+    sv, uc = None, None
+    if solid == "field":
+        sv = np.asfortranarray(0.1 * rng.normal(size=(3,) + E))
+    elif solid == "const":
+        uc = (0.3, -0.1, 0.2)
+        sv = np.zeros((3,) + E, order="F")
+        sv[0], sv[1], sv[2] = uc
+    kw = dict(relaxation=0.9, tolerance=tol, max_iter=nmax, fd_order=fd, dtime=0.02, vel_inf=(1.0, 0.5, 0.25),
+              reproject=reproj)
+    rv, rw, rf, rn, rres = O.penalisation_solve(g, p, O.poisson_setup(g, p), vel, vort, chi, solid_vel=sv, **kw)
+    dev = torch.device("cuda:0")
+    for prec, dt in ((pm.FP64, torch.float64), (pm.FP32, torch.float32)):
+        s = pm.PoissonSolver(patch, prec, order=4, kernel=1, eps_radius=2.0, interp_order=3)
+        s.setup()
+        t = {k: torch.from_numpy(np.ascontiguousarray(a.ravel(order="F"))).to(dev).to(dt)
+             for k, a in (("vel", vel), ("vort", vort), ("chi", chi))}
+        d_sv = None
+        if solid == "field":
+            tsv = torch.from_numpy(np.ascontiguousarray(sv.ravel(order="F"))).to(dev).to(dt)
+            d_sv = tsv.data_ptr()
+        f, n, res = pm.penalisation_solve_device(s, t["chi"].data_ptr(), t["vel"].data_ptr(), t["vort"].data_ptr(),
+                                                 d_solid_vel=d_sv, us_const=uc if solid == "const" else None, **kw)
+        gv = t["vel"].double().cpu().numpy().reshape(vel.shape, order="F")
+        gw = t["vort"].double().cpu().numpy().reshape(vort.shape, order="F")
+        if prec == pm.FP64:
+            print("iterations gpu %d oracle %d; force %s / %s" % (n, rn, f, rf), flush=True)
+            assert n == rn
+            assert np.abs(f - rf).max() <= 1e-12 * np.abs(rf).max()
+            ev, ew = rel_linf(gv, rv), rel_linf(gw, rw)
+            print("fp64 vel %.3e vort %.3e" % (ev, ew), flush=True)
+            assert ev <= 1e-12 and ew <= 1e-12
+            v64, w64 = gv, gw
+        else:
+            ev, ew = rel_l2(gv, v64), rel_l2(gw, w64)
+            print("fp32 vs fp64 vel %.3e vort %.3e, iterations %d" % (ev, ew, n), flush=True)
+            assert ev <= 1e-5 and ew <= 1e-5

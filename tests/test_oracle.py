@@ -582,3 +582,67 @@ def test_diag_mesh_uniform_fields_closed_forms():
     assert abs(d["lamb"] - np.cross(w, u) @ np.cross(w, u) / ww * V) <= 1e-13 * uu * V
     assert abs(d["helicity"] + d["lamb"] - 2 * d["kin_energy"]) <= 1e-13 * uu * V
     assert abs(d["vort_max"] - np.sqrt(ww)) <= 1e-15 * np.sqrt(ww)
+
+
+# ------------------------------------------------------ penalisation_solve
+def _penal_case(n=16, ng=2):
+    g = O.make_geom((n, n, n), ng, [-1.0, -1.0, -1.0], 2.0 / n, (0, 0, 0))
+    E = O.mesh_extent(g)
+    axes = [g.xmin[d] + (np.arange(E[d]) - ng + 0.5) * g.dx[d] for d in range(3)]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    chi = np.asfortranarray((r < 0.45).astype(np.float64))
+    vel = np.zeros((3,) + E, order="F")
+    vel[0] = 1.0
+    return g, chi, vel, r
+
+
+def test_penalisation_first_force_is_twice_the_penalised_momentum():
+    """One iteration, uniform U = e_x through a body at rest: dvort = curl(f)
+    with f = -chi U, and sum x x curl(f) h^3 = 2 sum f h^3 for compactly
+    supported f (summation by parts of the central differences), so
+    force = -(sum x x dvort) h^3 / dt = 2 U V_chi / dt along the flow, zero
+    across it.  A wrong sign or a missing h^3 or 1/dt fails."""
+    g, chi, vel, _ = _penal_case()
+    p = O.make_params(4, 1, 2.0)
+    Gh = O.poisson_setup(g, p)
+    dt = 0.05
+    vort = np.zeros_like(vel, order="F")
+    _, _, force, npen, _ = O.penalisation_solve(g, p, Gh, vel, vort, chi, max_iter=1, dtime=dt, fd_order=4)
+    V = chi[2:-2, 2:-2, 2:-2].sum() * np.prod(g.dx)
+    assert npen == 1
+    assert abs(force[0] - 2 * V / dt) <= 1e-10 * (2 * V / dt), (force, 2 * V / dt)
+    assert abs(force[1]) <= 1e-10 * force[0] and abs(force[2]) <= 1e-10 * force[0]
+
+
+def test_penalisation_iterations_reduce_the_slip_and_stop_at_the_tolerance():
+    """Each iteration lowers the slip |chi (us - u - u_pen)| inside the body,
+    the loop stops at the first iteration whose relative force increment is
+    <= tolerance (or at penal_max), and the free stream is added as the
+    reference writes it (vel_inf(3) onto the second component)."""
+    g, chi, vel, r = _penal_case()
+    p = O.make_params(4, 1, 2.0)
+    Gh = O.poisson_setup(g, p)
+    vort = np.zeros_like(vel, order="F")
+    slips, incs = [], []
+    for it in (1, 2, 4, 8):
+        v, w, f, n, res = O.penalisation_solve(g, p, Gh, vel, vort, chi, tolerance=0.0, max_iter=it, dtime=0.1)
+        assert n == it
+        # with no mesh vorticity the output is the velocity induced by the
+        # penalisation vorticity, which cancels more of the unit stream
+        # inside the body with every iteration
+        slips.append(np.sqrt(((vel + v) ** 2).sum(0))[r < 0.45].mean())
+        incs.append(res)
+    assert all(a > b for a, b in zip(slips, slips[1:])), slips
+    assert slips[-1] < 0.3 * slips[0]
+    assert all(a > b for a, b in zip(incs[1:], incs[2:])), incs
+    tol = incs[2] * 1.0001
+    v, w, f, n, res = O.penalisation_solve(g, p, Gh, vel, vort, chi, tolerance=tol, max_iter=50, dtime=0.1)
+    assert n <= 4 and res <= tol
+    vq, _, _, _, _ = O.penalisation_solve(g, p, Gh, vel, vort, chi, max_iter=1, vel_inf=(0.0, 0.0, 2.5))
+    v0, _, _, _, _ = O.penalisation_solve(g, p, Gh, vel, vort, chi, max_iter=1)
+    assert np.array_equal(vq[0], v0[0]) and np.array_equal(vq[2], v0[2])
+    assert np.allclose(vq[1] - v0[1], 2.5, rtol=0, atol=1e-13)
+    vf, _, _, _, _ = O.penalisation_solve(g, p, Gh, vel, vort, chi, max_iter=1, vel_inf=(0.0, 0.0, 2.5),
+                                          fix_free_stream=True)
+    assert np.allclose(vf[2] - v0[2], 2.5, rtol=0, atol=1e-13) and np.array_equal(vf[1], v0[1])
