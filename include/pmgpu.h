@@ -247,6 +247,10 @@ typedef struct pmgpu_penal_params {
 int pmgpu_penalisation_solve_device(pmgpu_poisson_t h, const pmgpu_penal_params *pp, const void *d_chi,
                                     const void *d_solid_vel, const double us_const[3], void *d_vel,
                                     void *d_vort, double force[3], int32_t *npen, double *residual);
+/* Same on host FP64 meshes (an FP64 handle); copies over PCIe. */
+int pmgpu_penalisation_solve(pmgpu_poisson_t h, const pmgpu_penal_params *pp, const double *chi,
+                             const double *solid_vel, const double us_const[3], double *vel, double *vort,
+                             double force[3], int32_t *npen, double *residual);
 
 /* Allocation-free variant of pmgpu_penalise_device.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code

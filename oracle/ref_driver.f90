@@ -36,6 +36,10 @@ PROGRAM ref_driver
   ! the same client code on the GPU subroutines (pmlib3d_amd/fortran)
   USE pmlib_mod_particles, ONLY: class_particles
   USE pmgpu_pmlib
+  ! the client's penalisation globals and the drop-in mod_penalisation
+  USE mod_parameters, ONLY: penal_relaxation, penal_tolerance, penal_max, vel_inf, reproject, &
+       & client_dtime => dtime
+  USE mod_penalisation, ONLY: penalisation_solve, topo_penal, mesh_penal, penal_force, penal_iterations
 #else
   USE pmlib_mod_particles
   USE pmlib_mod_poisson
@@ -52,13 +56,17 @@ PROGRAM ref_driver
   LOGICAL            :: reg_vort = .FALSE., reproj = .FALSE., calc_vecpot = .FALSE.
   LOGICAL            :: clear = .TRUE., incl_ghost = .FALSE., euler_vort = .FALSE.
   LOGICAL            :: reload = .FALSE.
+  ! penal (shim build only): penalisation_solve with these globals
+  REAL(MK)           :: penal_relax = 1.0_MK, penal_tol = 0.0_MK, vinf(3) = 0.0_MK
+  INTEGER            :: penal_nmax = 1, fd_order = 4
+  LOGICAL            :: with_solid_vel = .FALSE.
   INTEGER            :: npart = 0, stage = 1, nrep = 1, nsteps = 1
   REAL(MK)           :: dtime = 0.0_MK, min_vort = 0.0_MK, min_dvort = 0.0_MK
   CHARACTER(LEN=512) :: infile = 'in.bin', outfile = 'out.bin'
   NAMELIST /job/ mode, ncell, nghost, bound_cond, fft_seq, xmin, xmax, dx, order, kernel, &
        & interp_order, rk_order, eps_radius, reg_vort, reproj, calc_vecpot, clear, &
        & incl_ghost, euler_vort, npart, stage, nrep, nsteps, dtime, min_vort, min_dvort, &
-       & infile, outfile, reload
+       & infile, outfile, reload, penal_relax, penal_tol, vinf, penal_nmax, fd_order, with_solid_vel
 
   TYPE(class_patch)        :: patch
   TYPE(class_topology_all) :: topo_all
@@ -284,6 +292,38 @@ PROGRAM ref_driver
       WRITE(*,'(A,I12,A,I4)') 'PARTICLES', ntot, ' ranks=', nproc
     END IF
     IF (nproc .EQ. 1) CALL write_particles()
+
+#ifdef PMGPU_SHIM
+  CASE ('penal')
+    ! in: vel, vort (3, E...), chi (E...), [solid_vel (3, E...)];
+    ! out: vel, vort, force(3), iterations
+    pmlib_fd_order = fd_order
+    penal_relaxation = penal_relax
+    penal_tolerance = penal_tol
+    penal_max = penal_nmax
+    vel_inf = vinf
+    reproject = reproj
+    client_dtime = dtime
+    topo_penal = topo_all
+    ALLOCATE(mesh_penal%solid(1, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)))
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) mesh%vel
+    READ(10) mesh%vort
+    READ(10) mesh_penal%solid
+    IF (with_solid_vel) THEN
+      ALLOCATE(mesh_penal%solid_vel(3, lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)))
+      READ(10) mesh_penal%solid_vel
+    END IF
+    CLOSE(10)
+    CALL penalisation_solve(patch, topo_all, mesh, ierr)
+    IF (ierr .NE. 0) CALL die('penalisation_solve')
+    OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(20) mesh%vel
+    WRITE(20) mesh%vort
+    WRITE(20) penal_force
+    WRITE(20) REAL(penal_iterations, MK)
+    CLOSE(20)
+#endif
 
   CASE DEFAULT
     CALL die('unknown mode')

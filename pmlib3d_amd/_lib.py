@@ -74,6 +74,8 @@ SIGNATURES = {
     "pmgpu_penalise_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P, _P]),
     "pmgpu_diagnostics_particles": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P, _P]),
     "pmgpu_diagnostics_mesh": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, _P]),
+    "pmgpu_penalisation_solve": (C.c_int, [_P, C.POINTER(PenalParams), _P, _P, _P, _P, _P, _P,
+                                           C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pmgpu_penalisation_solve_device": (C.c_int, [_P, C.POINTER(PenalParams), _P, _P, _P, _P, _P, _P,
                                                   C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pmgpu_penalise_device_noalloc": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P,

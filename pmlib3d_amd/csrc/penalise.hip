@@ -446,6 +446,41 @@ int pmgpu_penalisation_solve_device(pmgpu_poisson_t h, const pmgpu_penal_params 
                                          residual);
 }
 
+int pmgpu_penalisation_solve(pmgpu_poisson_t h, const pmgpu_penal_params *pp, const double *chi,
+                             const double *solid_vel, const double us_const[3], double *vel, double *vort,
+                             double force[3], int32_t *npen, double *residual) {
+  PM_CHECK(h && pp && chi && vel && vort, "pmgpu_penalisation_solve: null argument");
+  pmgpu_geom g;
+  int precision = 0;
+  hipStream_t st = nullptr;
+  if (poisson_handle_info(h, &g, &precision, &st)) return -1;
+  PM_CHECK(precision == PMGPU_FP64, "pmgpu_penalisation_solve: host meshes need an FP64 solver handle");
+  int32_t E[3];
+  if (pmgpu_mesh_extent(&g, E)) return -1;
+  const size_t n = (size_t)E[0] * E[1] * E[2];
+  double *dc = nullptr, *ds = nullptr, *dv = nullptr, *dw = nullptr;
+  int rc = -1;
+  if (hipMalloc(&dc, 8 * n) == hipSuccess && hipMalloc(&dv, 24 * n) == hipSuccess &&
+      hipMalloc(&dw, 24 * n) == hipSuccess && (!solid_vel || hipMalloc(&ds, 24 * n) == hipSuccess) &&
+      hipMemcpy(dc, chi, 8 * n, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(dv, vel, 24 * n, hipMemcpyHostToDevice) == hipSuccess &&
+      hipMemcpy(dw, vort, 24 * n, hipMemcpyHostToDevice) == hipSuccess &&
+      (!solid_vel || hipMemcpy(ds, solid_vel, 24 * n, hipMemcpyHostToDevice) == hipSuccess)) {
+    rc = pmgpu_penalisation_solve_device(h, pp, dc, ds, us_const, dv, dw, force, npen, residual);
+    if (rc == 0 && (hipMemcpy(vel, dv, 24 * n, hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(vort, dw, 24 * n, hipMemcpyDeviceToHost) != hipSuccess)) {
+      set_error("pmgpu_penalisation_solve: copy back failed");
+      rc = -1;
+    }
+  } else {
+    set_error("pmgpu_penalisation_solve: device allocation or copy failed");
+  }
+  void *bufs[] = {dc, ds, dv, dw};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  return rc;
+}
+
 int pmgpu_penalise_device_noalloc(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_chi,
                                   const void *d_solid_vel, const double us_const[3], double relax, int32_t fd_order,
                                   void *d_vort, double *d_diag, void *stream) {

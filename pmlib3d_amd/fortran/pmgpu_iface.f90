@@ -28,6 +28,17 @@ MODULE pmgpu_iface
     INTEGER(C_INT32_T) :: poisson_kernel
   END TYPE pmgpu_params
 
+  TYPE, BIND(C) :: pmgpu_penal_params
+    REAL(C_DOUBLE)     :: relaxation
+    REAL(C_DOUBLE)     :: tolerance
+    INTEGER(C_INT32_T) :: max_iter
+    INTEGER(C_INT32_T) :: fd_order
+    REAL(C_DOUBLE)     :: dtime
+    REAL(C_DOUBLE)     :: vel_inf(3)
+    INTEGER(C_INT32_T) :: reproject
+    INTEGER(C_INT32_T) :: fix_free_stream
+  END TYPE pmgpu_penal_params
+
   INTERFACE
     INTEGER(C_INT) FUNCTION pmgpu_init(device) BIND(C, NAME='pmgpu_init')
       IMPORT :: C_INT, C_INT32_T
@@ -132,6 +143,15 @@ MODULE pmgpu_iface
       REAL(C_DOUBLE), VALUE :: relax
       INTEGER(C_INT32_T), VALUE :: fd_order
     END FUNCTION pmgpu_penalise
+    ! penalisation_solve (vortex_codes/vortex_penalisation/source/
+    ! penalisation/penalisation_solve.f:6) on host FP64 meshes
+    INTEGER(C_INT) FUNCTION pmgpu_penalisation_solve(h, pp, chi, solid_vel, us_const, vel, vort, &
+        & force, npen, residual) BIND(C, NAME='pmgpu_penalisation_solve')
+      IMPORT :: C_INT, C_PTR, pmgpu_penal_params
+      TYPE(C_PTR), VALUE :: h
+      TYPE(pmgpu_penal_params), INTENT(IN) :: pp
+      TYPE(C_PTR), VALUE :: chi, solid_vel, us_const, vel, vort, force, npen, residual
+    END FUNCTION pmgpu_penalisation_solve
     ! part_deriv_vorticity (vortex_codes/*/source/particle_derivatives/
     ! part_deriv_vorticity.f:57-208): dvort from vort and vel, (3, E...) meshes
     INTEGER(C_INT) FUNCTION pmgpu_vorticity_rhs(g, vort, vel, visc, fd_order, dvort) &

@@ -43,7 +43,7 @@ MODULE pmgpu_pmlib
 
   PUBLIC :: pmlib_poisson_setup, pmlib_poisson_solve, pmlib_interp_particle_mesh, &
           & pmlib_interp_mesh_particle, pmlib_remesh, pmlib_particles_advance, pmlib_penalise, &
-          & pmgpu_release
+          & pmgpu_release, pmgpu_patch_solver, mesh_ptr, geom_of
 
   ! solver handles, one per (geometry, order, kernel, radius)
   TYPE solver_entry
@@ -211,10 +211,24 @@ CONTAINS
     TYPE(class_mesh), INTENT(INOUT)      :: mesh
     INTEGER, INTENT(OUT)                 :: ierr
     CHARACTER(LEN=19) :: caller = 'pmlib_poisson_setup'
+    INTEGER :: i
+    i = solver_index(patch, topo_all, .TRUE., ierr, caller)
+  END SUBROUTINE pmlib_poisson_setup
+
+  ! the solver entry of the patch for the current pmlib parameters: created
+  ! on first use; set up (G-hat built) when new or when always_setup
+  INTEGER FUNCTION solver_index(patch, topo_all, always_setup, ierr, caller)
+    TYPE(class_patch), INTENT(IN)        :: patch
+    TYPE(class_topology_all), INTENT(IN) :: topo_all
+    LOGICAL, INTENT(IN)                  :: always_setup
+    INTEGER, INTENT(OUT)                 :: ierr
+    CHARACTER(LEN=*), INTENT(IN)         :: caller
     TYPE(pmgpu_geom)   :: g
     TYPE(pmgpu_params) :: p
     INTEGER :: i
+    LOGICAL :: fresh
     ierr = 0
+    solver_index = 0
     IF (.NOT. ensure_init(caller)) THEN
       ierr = -1
       RETURN
@@ -224,7 +238,8 @@ CONTAINS
     g%nghost = patch%nghost
     CALL current_params(p)
     i = find_solver(g, p)
-    IF (i .EQ. 0) THEN
+    fresh = i .EQ. 0
+    IF (fresh) THEN
       IF (nsolvers .EQ. SIZE(solvers)) THEN
         ierr = -1
         CALL pmlib_write(mpi_rank, caller, 'too many patch geometries')
@@ -240,8 +255,27 @@ CONTAINS
         RETURN
       END IF
     END IF
-    IF (pmgpu_poisson_setup(solvers(i)%h) .NE. 0) CALL fail(caller, ierr)
-  END SUBROUTINE pmlib_poisson_setup
+    IF (fresh .OR. always_setup) THEN
+      IF (pmgpu_poisson_setup(solvers(i)%h) .NE. 0) THEN
+        CALL fail(caller, ierr)
+        RETURN
+      END IF
+    END IF
+    solver_index = i
+  END FUNCTION solver_index
+
+  ! The FP64 solver handle of a patch (created and set up on first use), for
+  ! the drop-in modules that call the library directly.
+  TYPE(C_PTR) FUNCTION pmgpu_patch_solver(patch, topo_all, ierr, caller)
+    TYPE(class_patch), INTENT(IN)        :: patch
+    TYPE(class_topology_all), INTENT(IN) :: topo_all
+    INTEGER, INTENT(OUT)                 :: ierr
+    CHARACTER(LEN=*), INTENT(IN)         :: caller
+    INTEGER :: i
+    pmgpu_patch_solver = C_NULL_PTR
+    i = solver_index(patch, topo_all, .FALSE., ierr, caller)
+    IF (ierr .EQ. 0) pmgpu_patch_solver = solvers(i)%h
+  END FUNCTION pmgpu_patch_solver
 
   SUBROUTINE pmlib_poisson_solve(patch, topo_all, mesh, ierr, reg_vort, reproj, calc_vecpot)
     TYPE(class_patch), INTENT(IN)        :: patch

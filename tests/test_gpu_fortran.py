@@ -97,3 +97,41 @@ def test_shim_vortex_step_matches_reference(name):
     assert rel_linf(res[:, npart:2 * npart], d["vort"]) <= 1e-12
     assert rel_linf(res[:, 2 * npart:], d["vel"]) <= 1e-12
     print(out.strip().splitlines()[-1], flush=True)
+
+
+@pytest.mark.parametrize("case", [((16, 16, 16), 2, (0, 0, 0), False, 3, 0.0, False),
+                                  ((12, 12, 12), 0, (1, 1, 1), True, 8, 0.05, True)],
+                         ids=["free", "periodic_reproj"])
+def test_shim_penalisation_solve_matches_oracle(case):
+    """penalisation_solve(patch, topo_all, mesh, ierr) of the drop-in
+    mod_penalisation (pmlib3d_amd/fortran/pmgpu_penalisation.f90), on the
+    client's own module globals (mod_parameters.f), against the oracle
+    composition: velocity and vorticity within 1e-12, the same iteration
+    count and force."""
+    import pmlib3d_amd as pm
+    ncell, ng, bc, reproj, nmax, tol, with_sv = case
+    g = O.make_geom(ncell, ng, [-0.6, -0.5, -0.7], 1.2 / 16, bc)
+    E = O.mesh_extent(g)
+    p = O.make_params(4, 1, 2.0)
+    patch = pm.Patch(xmin=tuple(g.xmin), xmax=tuple(g.xmax), ncell=tuple(g.ncell), dx=tuple(g.dx),
+                     bound_cond=tuple(g.bound_cond), nghost=g.nghost)
+    from tests.helpers import blob_field
+    vel = blob_field(g, seed=41, width=0.15)
+    vel[0] += 1.0
+    vort = blob_field(g, seed=42, width=0.15)
+    chi = pm.brinkman_factor(pm.sphere_mask(patch, 0.3), 1e4, 0.02)
+    sv = np.asfortranarray(0.1 * np.random.default_rng(3).normal(size=(3,) + E)) if with_sv else None
+    kw = dict(relaxation=0.9, tolerance=tol, max_iter=nmax, fd_order=4, dtime=0.02, vel_inf=(1.0, 0.5, 0.25),
+              reproject=reproj)
+    rv, rw, rf, rn, _ = O.penalisation_solve(g, p, O.poisson_setup(g, p), vel, vort, chi, solid_vel=sv, **kw)
+    job = dict(O.geom_job(g, p), mode="penal", reproj=reproj, penal_relax=0.9, penal_tol=tol, penal_nmax=nmax,
+               vinf=[1.0, 0.5, 0.25], fd_order=4, dtime=0.02, with_solid_vel=with_sv)
+    raw, _ = run(job, [vel, vort, chi] + ([sv] if with_sv else []))
+    out = np.frombuffer(raw, dtype=np.float64)
+    n = vel.size
+    gv = out[:n].reshape(vel.shape, order="F")
+    gw = out[n:2 * n].reshape(vel.shape, order="F")
+    force, iters = out[2 * n:2 * n + 3], int(out[2 * n + 3])
+    assert iters == rn
+    assert np.abs(force - rf).max() <= 1e-12 * np.abs(rf).max()
+    assert rel_linf(gv, rv) <= 1e-12 and rel_linf(gw, rw) <= 1e-12
