@@ -17,6 +17,9 @@ Beside it, on the same GPU (each with its FP32-vs-FP64 error):
   mixed384   384^2 x 384 periodic-x,y / free-z solve of a turbulence-like
              field, then remeshed FP32 vortex steps on that grid
   rhs256     the vorticity right-hand side at 256^3
+  penalised256  penalisation_solve (iterated Brinkman penalisation and
+             solves) on the 256^3 mesh around a sphere
+  diagnostics   FP64 ring diagnostics of 2.1M particles, mesh energies
 and the compiled reference on the host's cores (cpu_baseline).
 
 Algorithmic bytes follow the 5-pass implicit-padding schedule (X R2C, Y,
@@ -478,6 +481,137 @@ def rhs_benchmark(reps, n=256):
     return res
 
 
+def penal256_benchmark(reps, n=256, ng=2):
+    """The penalised solve of the 256^3 configuration: a 252^3 interior with
+    the 2 ghost layers the 4th-order curl needs (a 256^3 mesh, FFT 512^3),
+    free space, order 4, eps 2h; the 32-blob vorticity reprojected, its
+    velocity plus the free stream U = (1, 0, 0); a sphere of radius 0.25 at
+    the origin with a tanh mask smoothed over 2h and the finite-lambda factor
+    lambda dt chi / (1 + lambda dt chi), lambda = 1e4, dt = h / 2; then
+    penalisation_solve with the reference sphere case's settings
+    (vortex_codes/vortex_penalisation/setup_sphere.in: tolerance 0.01,
+    relaxation 1, at most 1000 iterations, reprojection on).  Each timed call
+    starts from the same inputs (restored outside the events)."""
+    import numpy as np
+    import torch
+    import pmlib3d_amd as pm
+    N_int = n - 2 * ng
+    h = 2.0 / n
+    patch = pm.Patch(xmin=(-1.0 + ng * h,) * 3, xmax=(1.0 - ng * h,) * 3, ncell=(N_int,) * 3, dx=(h,) * 3,
+                     nghost=ng)
+    dev = torch.device("cuda:0")
+    w = torch.from_numpy(blob_vorticity(n).ravel(order="F")).to(dev)
+    chi = torch.from_numpy(np.ascontiguousarray(
+        pm.brinkman_factor(pm.sphere_mask(patch, 0.25), 1e4, 0.5 * h).ravel(order="F"))).to(dev)
+    res = {"workload": "penalised solve: 252^3 interior + 2 ghost layers (256^3 mesh, FFT 512^3), free, "
+                       "order 4, eps 2h, 32 reprojected blobs + U=(1,0,0), sphere r=0.25 (tanh mask over 2h, "
+                       "lambda=1e4, dt=h/2), penalisation_solve with tolerance 0.01, relaxation 1, "
+                       "reprojection (the reference sphere case's settings)"}
+    kw = dict(relaxation=1.0, tolerance=0.01, max_iter=1000, fd_order=4, dtime=0.5 * h, vel_inf=(1.0, 0.0, 0.0),
+              reproject=True)
+    outs = {}
+    for name, prec, dt, sz in (("fp64", pm.FP64, torch.float64, 8), ("fp32", pm.FP32, torch.float32, 4)):
+        s = pm.PoissonSolver(patch, prec, order=4, kernel=1, eps_radius=2.0, interp_order=3)
+        s.setup()
+        w0 = w.to(dt)
+        v0 = torch.empty_like(w0)
+        wp = torch.empty_like(w0)
+        s.solve_device(w0.data_ptr(), v0.data_ptr(), wp.data_ptr(), None, reg_vort=False, reproj=True)
+        w0 = wp.clone()
+        v0 = v0.view(-1, 3)
+        v0[:, 0] += 1.0
+        v0 = v0.view(-1)
+        c = chi.to(dt)
+        vel, vort = torch.empty_like(v0), torch.empty_like(w0)
+        times, iters = [], []
+        for r in range(reps + 1):
+            vel.copy_(v0)
+            vort.copy_(w0)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            f, it, rr = pm.penalisation_solve_device(s, c.data_ptr(), vel.data_ptr(), vort.data_ptr(), **kw)
+            e1.record()
+            torch.cuda.synchronize()
+            if r > 0:
+                times.append(e0.elapsed_time(e1))
+                iters.append(it)
+        ms = float(np.median(times))
+        E3 = n ** 3
+        solve_b = free_bytes((n,) * 3, 3, 3, sz, sz)[0]
+        it = iters[-1]
+        # per iteration: penalise (chi, vel_pen, u_pen read; vort_pen read and
+        # written: 13 s per cell) + one solve; once: vel_pen (6 s), the add (9 s),
+        # the final solve, the free stream (6 s)
+        nbytes = it * (13 * sz * E3 + solve_b) + 21 * sz * E3 + solve_b
+        res[name] = {"ms_per_penalised_solve": ms, "penalised_solves_per_s": 1e3 / ms, "iterations": it,
+                     "ms_per_iteration": (ms / (it + 1)), "force": [float(x) for x in f],
+                     "residual_force": rr, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / ms / 1e6,
+                     "frac_hbm": nbytes / ms / 1e6 / PEAK_HBM_GBS}
+        log("penalised 256^3 %s: %.3f ms, %d iterations, force %s" % (name, ms, it, f))
+        if name == "fp64":
+            outs.update(vel=vel, vort=vort)
+        else:
+            res[name]["rel_l2_vs_fp64"] = {"vel": rel_l2_t(vel, outs["vel"]), "vort": rel_l2_t(vort, outs["vort"])}
+        del s
+        torch.cuda.empty_cache()
+    return res
+
+
+def diagnostics_benchmark(reps, n=128):
+    """FP64 global diagnostics: the ring diagnostics of the 128^3 two-ring
+    particle set (2.1M particles; three sweeps reading 72 + 48 + 48 bytes
+    per particle) and the mesh energy integrals of a 256^3 FP32 mesh pair
+    (24 bytes per cell), timed with HIP events."""
+    import numpy as np
+    import torch
+    import pmlib3d_amd as pm
+    from tests.helpers import ring_particles
+    dev = torch.device("cuda:0")
+    pos, vort = ring_particles(n)
+    t = [torch.from_numpy(np.ascontiguousarray(a.T)).to(dev) for a in (pos, vort)]
+    vel = torch.randn_like(t[0])
+    d_out = torch.zeros(16, dtype=torch.float64, device=dev)
+    h = 2.0 / n
+    run = lambda: pm.diagnostics_particles_device((h,) * 3, t[0].data_ptr(), t[1].data_ptr(), vel.data_ptr(),
+                                                  pos.shape[1], d_out=d_out.data_ptr(), host=False)
+    res = {"workload": "ring diagnostics of 2097152 particles (128^3 two-ring set); mesh energy of a 256^3 "
+                       "FP32 mesh pair"}
+    for _ in range(3):
+        run()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nb = 168 * pos.shape[1]
+    res["particles"] = {"ms": ms, "algorithmic_bytes": nb, "achieved_GBs": nb / ms / 1e6,
+                        "frac_hbm": nb / ms / 1e6 / PEAK_HBM_GBS}
+    m = 256
+    mp = pm.Patch(xmin=(-1.0,) * 3, xmax=(1.0,) * 3, ncell=(m,) * 3, dx=(2.0 / m,) * 3)
+    u = torch.randn(3 * m ** 3, device=dev)
+    wv = torch.randn(3 * m ** 3, device=dev)
+    run = lambda: pm.diagnostics_mesh_device(mp, pm.FP32, u.data_ptr(), wv.data_ptr(), d_out=d_out.data_ptr(),
+                                             host=False)
+    for _ in range(3):
+        run()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nb = 24 * m ** 3
+    res["mesh"] = {"ms": ms, "algorithmic_bytes": nb, "achieved_GBs": nb / ms / 1e6,
+                   "frac_hbm": nb / ms / 1e6 / PEAK_HBM_GBS}
+    log("diagnostics: particles %.3f ms, mesh %.3f ms" % (res["particles"]["ms"], ms))
+    return res
+
+
 Z_KERNEL = "k_z3_r<float, 512, 4, false, true>"
 
 
@@ -513,6 +647,9 @@ def main():
                     help="remeshed FP32 vortex steps on the mixed-BC grid")
     ap.add_argument("--rhs-steps", type=int, default=20,
                     help="timed vorticity right-hand-side launches at 256^3 (0 skips)")
+    ap.add_argument("--penal-steps", type=int, default=3,
+                    help="timed penalised 256^3 solves (penalisation_solve; 0 skips)")
+    ap.add_argument("--diag-steps", type=int, default=20, help="timed FP64 diagnostics (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -614,6 +751,9 @@ def main():
         log("vorticity rhs 256^3: fp64 %.3f ms (%.0f GB/s), fp32 %.3f ms (%.0f GB/s)" %
             (rhs["fp64"]["ms"], rhs["fp64"]["achieved_GBs"], rhs["fp32"]["ms"], rhs["fp32"]["achieved_GBs"]))
 
+    penal = penal256_benchmark(args.penal_steps) if args.penal_steps > 0 else None
+    diag = diagnostics_benchmark(args.diag_steps) if args.diag_steps > 0 else None
+
     cpu = cpu_ring = cpu_step = None
     if not args.no_cpu_baseline:
         cpu = cpu_reference_solve(w_cpu, N, ORDER)
@@ -655,6 +795,8 @@ def main():
         "batched_multipatch": batch,
         "mixed384": mixed,
         "vorticity_rhs": rhs,
+        "penalised256": penal,
+        "diagnostics": diag,
         "roofline": {"bound": "hbm", "kernel": Z_KERNEL + " (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": z_gbs / PEAK_HBM_GBS, "traffic": measured_traffic(),
