@@ -313,6 +313,38 @@ int pmgpu_diagnostics_particles(const double dx[3], const double *d_pos, const d
 int pmgpu_diagnostics_mesh(const pmgpu_geom *g, int32_t precision, const void *d_vel, const void *d_vort,
                            double *d_out, double *h_out, void *stream);
 
+/* ------------------------------------------------------------------------
+ * On-disk formats (host I/O from device-resident or host arrays)
+ * ------------------------------------------------------------------------ */
+
+/* pmlib_output_mesh(outdir, tag, patch, topo, mesh, ierr)
+ * (source/output/pmlib_output_mesh.f:5-188) on one rank: writes
+ * outdir//tag//'.pvti' and outdir//tag//'_P00000.vti' (VTK image data, cell
+ * data vort and vel of the interior cells as Float32, appended raw),
+ * byte for byte as the reference writes them.  vort, vel: (3, E...) meshes
+ * in precision, device pointers when on_device, else host. */
+int pmgpu_output_mesh(const char *outdir, const char *tag, const pmgpu_geom *g, int32_t precision,
+                      const void *vort, const void *vel, int32_t on_device);
+/* output_particles(tag, topo, part, ierr)
+ * (vortex_codes/vortex_structures/source/output/output_particles.f:50-69) on
+ * one rank: the unformatted sequential file of records itime + 1,
+ * time + dtime, npart, then one record pos(1:3), vort(1:3) per particle
+ * (FP64, 4-byte record markers).  pos, vort: (3, np) FP64, device pointers
+ * when on_device. */
+int pmgpu_output_particles(const char *path, int32_t itime, double time, double dtime, const double *pos,
+                           const double *vort, int64_t np, int32_t on_device);
+/* The particle reading of input_particles(infile, patch, topo_all, mesh,
+ * part, trunc, ierr) (vortex_codes/vortex_structures/source/input/
+ * input_particles.f:81-135, 200-230) on one rank: reads the header (itime,
+ * time) and keeps, in file order, the particles with |vort| >= trunc; np
+ * receives their count and vort_max the largest kept |vort|.  With pos and
+ * vort NULL only the header and count are returned; otherwise they receive
+ * (3, np) FP64 arrays (device pointers when on_device, capacity cap
+ * particles).  No particle above trunc is an error, as in the reference.
+ * The reference then re-patches and distributes (out of this path). */
+int pmgpu_input_particles(const char *path, double trunc, int32_t *itime, double *time, int64_t *np,
+                          double *vort_max, double *pos, double *vort, int64_t cap, int32_t on_device);
+
 #ifdef __cplusplus
 }
 #endif

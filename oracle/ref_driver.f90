@@ -45,6 +45,11 @@ PROGRAM ref_driver
   USE pmlib_mod_poisson
   USE pmlib_mod_interpolation
   USE pmlib_mod_remesh
+  USE pmlib_mod_output
+  ! the vortex client's particle checkpoint writer and its globals
+  USE mod_output, ONLY: output_particles
+  USE mod_parameters, ONLY: client_itime => itime, client_time => time, client_dtime => dtime, &
+       & client_rank => rank, client_nproc => nproc, client_comm => comm
 #endif
   IMPLICIT NONE
 
@@ -60,13 +65,18 @@ PROGRAM ref_driver
   REAL(MK)           :: penal_relax = 1.0_MK, penal_tol = 0.0_MK, vinf(3) = 0.0_MK
   INTEGER            :: penal_nmax = 1, fd_order = 4
   LOGICAL            :: with_solid_vel = .FALSE.
+  ! vtk / partout: output directory and tag, checkpoint time stamp
+  CHARACTER(LEN=256) :: outdir = './', tag = 'mesh'
+  INTEGER            :: itime = 0
+  REAL(MK)           :: time = 0.0_MK
   INTEGER            :: npart = 0, stage = 1, nrep = 1, nsteps = 1
   REAL(MK)           :: dtime = 0.0_MK, min_vort = 0.0_MK, min_dvort = 0.0_MK
   CHARACTER(LEN=512) :: infile = 'in.bin', outfile = 'out.bin'
   NAMELIST /job/ mode, ncell, nghost, bound_cond, fft_seq, xmin, xmax, dx, order, kernel, &
        & interp_order, rk_order, eps_radius, reg_vort, reproj, calc_vecpot, clear, &
        & incl_ghost, euler_vort, npart, stage, nrep, nsteps, dtime, min_vort, min_dvort, &
-       & infile, outfile, reload, penal_relax, penal_tol, vinf, penal_nmax, fd_order, with_solid_vel
+       & infile, outfile, reload, penal_relax, penal_tol, vinf, penal_nmax, fd_order, with_solid_vel, &
+       & outdir, tag, itime, time
 
   TYPE(class_patch)        :: patch
   TYPE(class_topology_all) :: topo_all
@@ -292,6 +302,34 @@ PROGRAM ref_driver
       WRITE(*,'(A,I12,A,I4)') 'PARTICLES', ntot, ' ranks=', nproc
     END IF
     IF (nproc .EQ. 1) CALL write_particles()
+
+  CASE ('vtk')
+    ! in: vort, vel (3, E...); out: the .pvti / .vti pair in outdir
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) mesh%vort
+    READ(10) mesh%vel
+    CLOSE(10)
+    CALL pmlib_output_mesh(TRIM(outdir), TRIM(tag), patch, topo_all%cuboid, mesh, ierr)
+    IF (ierr .NE. 0) CALL die('pmlib_output_mesh')
+
+#ifndef PMGPU_SHIM
+  CASE ('partout')
+    ! in: pos, vort (3, npart); out: the checkpoint outdir//tag//'.dat'
+    ALLOCATE(part%pos(3, npart), part%vort(3, npart))
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) part%pos
+    READ(10) part%vort
+    CLOSE(10)
+    topo_all%cuboid(rank)%npart = npart
+    client_itime = itime
+    client_time = time
+    client_dtime = dtime
+    client_rank = rank
+    client_nproc = nproc
+    client_comm = MPI_COMM_WORLD
+    CALL output_particles(TRIM(outdir)//TRIM(tag), topo_all%cuboid, part, ierr)
+    IF (ierr .NE. 0) CALL die('output_particles')
+#endif
 
 #ifdef PMGPU_SHIM
   CASE ('penal')

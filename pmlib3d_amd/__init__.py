@@ -16,6 +16,8 @@ from .penalisation import (penalise, penalise_device, penalise_device_noalloc, p
                            sphere_mask, brinkman_factor)
 from .derivatives import vorticity_rhs, vorticity_rhs_device, part_deriv_vorticity
 from .step import VortexStep
+from .output import (pmlib_output_mesh, output_mesh_device, output_particles, output_particles_device,
+                     input_particles)
 from .diagnostics import (diagnostics_ring, diagnostics_energy_mesh, diagnostics_particles_device,
                           diagnostics_mesh_device)
 from ._lib import PmgpuError, FP32, FP64
@@ -26,4 +28,5 @@ __all__ = ["Patch", "Mesh", "Particles", "Parameters", "params", "pmlib_write", 
            "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps", "penalise",
            "penalise_device", "penalise_device_noalloc", "penalisation_solve_device", "sphere_mask", "brinkman_factor", "vorticity_rhs", "vorticity_rhs_device",
            "part_deriv_vorticity", "VortexStep", "diagnostics_ring", "diagnostics_energy_mesh",
-           "diagnostics_particles_device", "diagnostics_mesh_device"]
+           "diagnostics_particles_device", "diagnostics_mesh_device", "pmlib_output_mesh", "output_mesh_device",
+           "output_particles", "output_particles_device", "input_particles"]

@@ -78,6 +78,10 @@ SIGNATURES = {
                                            C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pmgpu_penalisation_solve_device": (C.c_int, [_P, C.POINTER(PenalParams), _P, _P, _P, _P, _P, _P,
                                                   C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "pmgpu_output_mesh": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(Geom), _I, _P, _P, _I]),
+    "pmgpu_output_particles": (C.c_int, [C.c_char_p, _I, C.c_double, C.c_double, _P, _P, C.c_int64, _I]),
+    "pmgpu_input_particles": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_double), _P, _P, C.c_int64, _I]),
     "pmgpu_penalise_device_noalloc": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P,
                                                 _P]),
     "pmgpu_vorticity_rhs": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_double, _I, _P]),

@@ -152,6 +152,16 @@ MODULE pmgpu_iface
       TYPE(pmgpu_penal_params), INTENT(IN) :: pp
       TYPE(C_PTR), VALUE :: chi, solid_vel, us_const, vel, vort, force, npen, residual
     END FUNCTION pmgpu_penalisation_solve
+    ! pmlib_output_mesh (source/output/pmlib_output_mesh.f:5): the .pvti /
+    ! .vti pair; outdir and tag NUL-terminated
+    INTEGER(C_INT) FUNCTION pmgpu_output_mesh(outdir, tag, g, precision, vort, vel, on_device) &
+        & BIND(C, NAME='pmgpu_output_mesh')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, C_CHAR, pmgpu_geom
+      CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: outdir, tag
+      TYPE(pmgpu_geom), INTENT(IN) :: g
+      INTEGER(C_INT32_T), VALUE :: precision, on_device
+      TYPE(C_PTR), VALUE :: vort, vel
+    END FUNCTION pmgpu_output_mesh
     ! part_deriv_vorticity (vortex_codes/*/source/particle_derivatives/
     ! part_deriv_vorticity.f:57-208): dvort from vort and vel, (3, E...) meshes
     INTEGER(C_INT) FUNCTION pmgpu_vorticity_rhs(g, vort, vel, visc, fd_order, dvort) &

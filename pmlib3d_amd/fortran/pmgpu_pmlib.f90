@@ -43,7 +43,7 @@ MODULE pmgpu_pmlib
 
   PUBLIC :: pmlib_poisson_setup, pmlib_poisson_solve, pmlib_interp_particle_mesh, &
           & pmlib_interp_mesh_particle, pmlib_remesh, pmlib_particles_advance, pmlib_penalise, &
-          & pmgpu_release, pmgpu_patch_solver, mesh_ptr, geom_of
+          & pmgpu_release, pmgpu_patch_solver, mesh_ptr, geom_of, pmlib_output_mesh
 
   ! solver handles, one per (geometry, order, kernel, radius)
   TYPE solver_entry
@@ -543,6 +543,37 @@ CONTAINS
     END IF
     IF (PRESENT(diag)) diag = d
   END SUBROUTINE pmlib_penalise
+
+!------------------------------------------------------------------ output
+  ! The VTK image-data pair of the mesh's interior vorticity and velocity
+  ! (Float32 cell data, appended raw), as pmlib_output_mesh writes it; one
+  ! rank.
+  SUBROUTINE pmlib_output_mesh(outdir, tag, patch, topo, mesh, ierr)
+    CHARACTER(LEN=*), INTENT(IN)                            :: outdir
+    CHARACTER(LEN=*), INTENT(IN)                            :: tag
+    TYPE(class_patch), INTENT(IN)                           :: patch
+    TYPE(class_topology), DIMENSION(:), POINTER             :: topo
+    TYPE(class_mesh), INTENT(IN)                            :: mesh
+    INTEGER, INTENT(OUT)                                    :: ierr
+    CHARACTER(LEN=17) :: caller = 'pmlib_output_mesh'
+    TYPE(pmgpu_geom) :: g
+    REAL(MK), DIMENSION(:,:,:,:), POINTER :: w, u
+    ierr = 0
+    CALL geom_of(topo(mpi_rank), g, ierr, caller)
+    IF (ierr .NE. 0) RETURN
+    ! origin and spacing are the patch's, as in the reference
+    g%xmin = patch%xmin
+    g%dx = patch%dx
+    w => mesh%vort
+    u => mesh%vel
+    IF (.NOT. (ASSOCIATED(w) .AND. ASSOCIATED(u))) THEN
+      ierr = -1
+      CALL pmlib_write(mpi_rank, caller, 'mesh%vort and mesh%vel must be allocated')
+      RETURN
+    END IF
+    IF (pmgpu_output_mesh(TRIM(outdir)//C_NULL_CHAR, TRIM(tag)//C_NULL_CHAR, g, PMGPU_FP64, mesh_ptr(w), &
+      & mesh_ptr(u), 0_C_INT32_T) .NE. 0) CALL fail(caller, ierr)
+  END SUBROUTINE pmlib_output_mesh
 
 !----------------------------------------------------------------- release
   SUBROUTINE pmgpu_release(ierr)

@@ -10,6 +10,9 @@ Outputs (small, committed):
   ref_solve_*.npz        inputs + reference outputs of small solves
   ref_particles_*.npz    inputs + reference outputs of P2M/M2P/remesh/advance
   ref_step_*.npz         inputs + reference outputs of whole two-ring vortex steps
+  ref_vtk_*.npz          inputs + the .pvti / .vti bytes of pmlib_output_mesh
+  ref_checkpoint_*.npz   inputs + the file bytes of output_particles
+(`python tests/golden/make_golden.py formats` rewrites only the last two.)
 """
 import json
 import os
@@ -153,9 +156,52 @@ def kat():
                "cases": cases}, open(os.path.join(HERE, "kat_ring_ref.json"), "w"), indent=1)
 
 
+def vtk_case(name, ncell, ng, bc, xmin, dx, seed):
+    """pmlib_output_mesh of a small mesh: the .pvti / .vti pair, byte for
+    byte, with the input meshes."""
+    import tempfile
+    g = O.make_geom(ncell, ng, xmin, dx, bc)
+    rng = np.random.default_rng(seed)
+    ext = O.mesh_extent(g)
+    w = np.asfortranarray(rng.normal(size=(3,) + ext))
+    u = np.asfortranarray(rng.normal(size=(3,) + ext))
+    with tempfile.TemporaryDirectory() as td:
+        job = dict(O.geom_job(g, O.make_params(4, 1, 2.0)), mode="vtk", outdir=td + "/", tag="out")
+        O.ref_run(job, [w, u])
+        pvti = open(os.path.join(td, "out.pvti"), "rb").read()
+        vti = open(os.path.join(td, "out_P00000.vti"), "rb").read()
+    np.savez_compressed(os.path.join(HERE, "ref_vtk_%s.npz" % name), vort=w, vel=u,
+                        spec=np.array(json.dumps([ncell, ng, bc, xmin, dx])),
+                        pvti=np.frombuffer(pvti, dtype=np.uint8), vti=np.frombuffer(vti, dtype=np.uint8))
+
+
+def checkpoint_case(name, npart, itime, time, dtime, seed):
+    """output_particles of a particle set: the checkpoint file, byte for
+    byte, with its inputs (some particles below a truncation of 0.5)."""
+    import tempfile
+    rng = np.random.default_rng(seed)
+    pos = np.asfortranarray(rng.uniform(-1, 1, (3, npart)))
+    vort = np.asfortranarray(rng.normal(size=(3, npart)))
+    vort[:, ::5] *= 0.1
+    g = O.make_geom((8, 8, 8), 0, [-1.0] * 3, 0.25, (0, 0, 0))
+    with tempfile.TemporaryDirectory() as td:
+        job = dict(O.geom_job(g), mode="partout", npart=npart, outdir=td + "/", tag="part", itime=itime,
+                   time=time, dtime=dtime)
+        O.ref_run(job, [pos, vort])
+        dat = open(os.path.join(td, "part.dat"), "rb").read()
+    np.savez_compressed(os.path.join(HERE, "ref_checkpoint_%s.npz" % name), pos=pos, vort=vort,
+                        spec=np.array(json.dumps([npart, itime, time, dtime])),
+                        dat=np.frombuffer(dat, dtype=np.uint8))
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref is not built: make -f oracle/Makefile.ref")
+    if len(sys.argv) > 1 and sys.argv[1] == "formats":
+        vtk_case("free_ng2", (5, 4, 3), 2, (0, 0, 0), [-0.5, -0.37, 0.1234], 1.0 / 7.0, 51)
+        vtk_case("periodic", (6, 3, 4), 0, (1, 1, 1), [0.0, -1.0, -2.5e-3], 0.03125, 52)
+        checkpoint_case("mixed", 37, 41, 0.37, 0.005, 53)
+        sys.exit(0)
     if len(sys.argv) > 1:   # only the named solve cases
         for name in sys.argv[1:]:
             solve_case(name, SOLVE_CASES[name])
