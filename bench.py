@@ -280,6 +280,24 @@ def step_benchmark(steps, n=128):
         res[name] = {"Mparticle_steps_per_s": npart / sec / 1e6, "ms_per_step": 1e3 * sec, "particles": npart,
                      "particles_after_remesh": kept}
         del st
+        # the same step with device-side counts, recorded once as a HIP graph
+        gs = pm.GraphVortexStep(patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3,
+                                capacity=max(npart, n ** 3), precision=prec)
+        gs.load_device(d_pos, d_vort, npart)
+        gs.step(0.5 * h, STEP_MIN_VORT)  # records the graph
+        gkept, status = gs.sync()
+        total = 0.0
+        for _ in range(steps):
+            gs.load_device(d_pos, d_vort, npart)
+            t0 = time.perf_counter()
+            gs.step(0.5 * h, STEP_MIN_VORT)
+            gs.sync()
+            total += time.perf_counter() - t0
+        gsec = total / steps
+        res[name]["graph"] = {"Mparticle_steps_per_s": npart / gsec / 1e6, "ms_per_step": 1e3 * gsec,
+                              "particles_after_remesh": gkept, "status": status,
+                              "same_count_as_eager": gkept == kept}
+        del gs
     (p64, w64, v64), (p32, w32, v32) = fetched["fp64"], fetched["fp32"]
     key = {tuple(c): i for i, c in enumerate(p64.T)}
     pairs = [(key[tuple(c)], j) for j, c in enumerate(p32.T) if tuple(c) in key]
@@ -290,6 +308,9 @@ def step_benchmark(steps, n=128):
     log("step128: fp64 %.1f, fp32 %.1f Mparticle-steps/s (%.3f / %.3f ms, %d particles); fp32 err vel %.2e" %
         (res["fp64"]["Mparticle_steps_per_s"], res["fp32"]["Mparticle_steps_per_s"], res["fp64"]["ms_per_step"],
          res["fp32"]["ms_per_step"], npart, res["fp32"]["rel_l2_vs_fp64"]["vel"]))
+    log("step128 graph: fp64 %.1f, fp32 %.1f Mparticle-steps/s (%.3f / %.3f ms)" %
+        (res["fp64"]["graph"]["Mparticle_steps_per_s"], res["fp32"]["graph"]["Mparticle_steps_per_s"],
+         res["fp64"]["graph"]["ms_per_step"], res["fp32"]["graph"]["ms_per_step"]))
     return res
 
 

@@ -314,6 +314,44 @@ int pmgpu_diagnostics_mesh(const pmgpu_geom *g, int32_t precision, const void *d
                            double *d_out, double *h_out, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Device-resident vortex time step, recorded once as a HIP graph
+ * ------------------------------------------------------------------------ */
+
+/* One time step of the vortex client's loop
+ * (vortex_codes/vortex_structures/source/main.f:453-505: per RK stage
+ * pmlib_interp_particle_mesh, pmlib_poisson_solve, pmlib_interp_mesh_particle,
+ * pmlib_particles_advance; then pmlib_interp_particle_mesh and pmlib_remesh
+ * with the stage's velocity mesh, no vorticity right-hand side) on a
+ * particle set of fixed capacity held by the handle.  The particle count
+ * lives in device memory: every launch covers the capacity and reads the
+ * live count, so a step needs no host synchronisation; with use_graph the
+ * step is recorded once as a HIP graph (again when dt or min_vort change)
+ * and replayed.  The arithmetic and the particle order are those of the
+ * eager entry points (pmgpu_interp_*_device, pmgpu_particles_advance_device,
+ * pmgpu_remesh_device).  Migration makes one pass per axis: a particle still
+ * outside a periodic axis afterwards (it moved more than a period in one
+ * stage) sets status bit 2; a remesh above the capacity keeps the first cap
+ * particles and sets status bit 1 (read with pmgpu_vstep_sync).  No
+ * reference counterpart: it is the graph-replayed time step of the build. */
+typedef struct pmgpu_vstep_s *pmgpu_vstep_t;
+int pmgpu_vstep_create(const pmgpu_geom *g, const pmgpu_params *p, int32_t precision, int32_t rk_order,
+                       int64_t capacity, pmgpu_vstep_t *out);
+int pmgpu_vstep_destroy(pmgpu_vstep_t h);
+/* (re)starts from device (3, np) FP64 positions and vorticities; the other
+ * particle fields start at zero (synchronises). */
+int pmgpu_vstep_load(pmgpu_vstep_t h, const void *d_pos, const void *d_vort, int64_t np);
+/* enqueues one step on the handle's stream (no synchronisation) */
+int pmgpu_vstep_step(pmgpu_vstep_t h, double dt, double min_vort, int32_t use_graph);
+/* waits for the stream; the particle count and the status bits */
+int pmgpu_vstep_sync(pmgpu_vstep_t h, int64_t *np, int32_t *status);
+/* device pointers of pos, vel, vort, dvort, vel_rk, dvort_rk ((3, cap)
+ * FP64 each) and the handle's stream */
+int pmgpu_vstep_arrays(pmgpu_vstep_t h, void *d_arrays[6], void **stream);
+/* copies the live particles' pos, vort, vel (each may be NULL) to host
+ * (3, np) arrays of capacity cap (synchronises) */
+int pmgpu_vstep_fetch(pmgpu_vstep_t h, double *pos, double *vort, double *vel, int64_t cap, int64_t *np);
+
+/* ------------------------------------------------------------------------
  * On-disk formats (host I/O from device-resident or host arrays)
  * ------------------------------------------------------------------------ */
 

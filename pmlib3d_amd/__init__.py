@@ -15,7 +15,7 @@ from .particles import (cell_index, pmlib_interp_particle_mesh, pmlib_interp_mes
 from .penalisation import (penalise, penalise_device, penalise_device_noalloc, penalisation_solve_device,
                            sphere_mask, brinkman_factor)
 from .derivatives import vorticity_rhs, vorticity_rhs_device, part_deriv_vorticity
-from .step import VortexStep
+from .step import VortexStep, GraphVortexStep
 from .output import (pmlib_output_mesh, output_mesh_device, output_particles, output_particles_device,
                      input_particles)
 from .diagnostics import (diagnostics_ring, diagnostics_energy_mesh, diagnostics_particles_device,
@@ -27,6 +27,6 @@ __all__ = ["Patch", "Mesh", "Particles", "Parameters", "params", "pmlib_write", 
            "FP32", "FP64", "cell_index", "pmlib_interp_particle_mesh", "pmlib_interp_mesh_particle",
            "interp_mesh_field", "pmlib_remesh", "pmlib_particles_advance", "ParticleOps", "penalise",
            "penalise_device", "penalise_device_noalloc", "penalisation_solve_device", "sphere_mask", "brinkman_factor", "vorticity_rhs", "vorticity_rhs_device",
-           "part_deriv_vorticity", "VortexStep", "diagnostics_ring", "diagnostics_energy_mesh",
+           "part_deriv_vorticity", "VortexStep", "GraphVortexStep", "diagnostics_ring", "diagnostics_energy_mesh",
            "diagnostics_particles_device", "diagnostics_mesh_device", "pmlib_output_mesh", "output_mesh_device",
            "output_particles", "output_particles_device", "input_particles"]

@@ -82,6 +82,13 @@ SIGNATURES = {
     "pmgpu_output_particles": (C.c_int, [C.c_char_p, _I, C.c_double, C.c_double, _P, _P, C.c_int64, _I]),
     "pmgpu_input_particles": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_double), _P, _P, C.c_int64, _I]),
+    "pmgpu_vstep_create": (C.c_int, [C.POINTER(Geom), C.POINTER(Params), _I, _I, C.c_int64, C.POINTER(_P)]),
+    "pmgpu_vstep_destroy": (C.c_int, [_P]),
+    "pmgpu_vstep_load": (C.c_int, [_P, _P, _P, C.c_int64]),
+    "pmgpu_vstep_step": (C.c_int, [_P, C.c_double, C.c_double, _I]),
+    "pmgpu_vstep_sync": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "pmgpu_vstep_arrays": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "pmgpu_vstep_fetch": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     "pmgpu_penalise_device_noalloc": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P,
                                                 _P]),
     "pmgpu_vorticity_rhs": (C.c_int, [C.POINTER(Geom), _P, _P, C.c_double, _I, _P]),

@@ -235,7 +235,7 @@ static void remesh_create_launch(const PGeom &g, int64_t nn, const int *flag, co
                                  const void *dvort_m, const void *vel_m, double *pos, double *vort, double *vel,
                                  double *dvort, double *vel_rk, double *dvort_rk, hipStream_t st) {
   hipLaunchKernelGGL(k_remesh_create<T>, grid_for(nn), dim3(256), 0, st, g, flag, offs, (const T *)vort_m,
-                     (const T *)dvort_m, (const T *)vel_m, pos, vort, vel, dvort, vel_rk, dvort_rk);
+                     (const T *)dvort_m, (const T *)vel_m, pos, vort, vel, dvort, vel_rk, dvort_rk, INT64_MAX);
 }
 
 int remesh_device(const pmgpu_geom *geom, int prec, const void *vort_m, const void *dvort_m, const void *vel_m,
@@ -559,6 +559,408 @@ int pmgpu_cell_index(const pmgpu_geom *g, const double *pos, int64_t np, int32_t
   if (to_dev(dp, pos, sizeof(double) * 3 * np) || to_dev(di, nullptr, sizeof(int32_t) * 3 * np)) return -1;
   if (cell_index_device(g, (double *)dp.p, np, (int32_t *)di.p, nullptr)) return -1;
   if (np) PM_HIP(hipMemcpy(idx, di.p, sizeof(int32_t) * 3 * np, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"
+
+// ===================================================== device-resident step
+// One vortex time step (per RK stage: P2M, solve, M2P, RK update and
+// migration; then P2M and remesh) with every count kept in device memory:
+// the grids cover the capacity and the kernels read the live particle count
+// (NCount), so the step has no host synchronisation and is recorded once as
+// a HIP graph and replayed.  The same kernels and the same per-particle
+// arithmetic as the eager entry points, in the client's order
+// (vortex_codes/vortex_structures/source/main.f:453-505).
+namespace pmgpu {
+
+struct Arr6 {
+  double *a[6];
+};
+
+struct Mig3 {
+  int w, e, d;
+};
+struct Mig3Plus {
+  __host__ __device__ Mig3 operator()(const Mig3 &x, const Mig3 &y) const { return Mig3{x.w + y.w, x.e + y.e, x.d + y.d}; }
+};
+
+__global__ void k_fill_int(int *p, int64_t n, int v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+// class indicators of the live particles (zero past the count)
+__global__ void k_mig_keys(NCount npc, int64_t cap, const int *cls, Mig3 *key) {
+  const int64_t np = npc.get();
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < cap; q += (int64_t)gridDim.x * blockDim.x) {
+    const int c = q < np ? cls[q] : 0;
+    key[q] = Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0};
+  }
+}
+// exclusive offsets: off[0] = 0, off[q + 1] = inclusive scan
+__global__ void k_zero_first3(Mig3 *off) { off[0] = Mig3{0, 0, 0}; }
+
+// west movers, east movers, then stayers, each in their original order; a
+// stayer's rank is q minus the movers and drops before it
+__global__ void k_migrate_dev(NCount npc, const int *cls, const Mig3 *off, const int *counts, int d, double len,
+                              Arr6 src, Arr6 dst) {
+  if (counts[1] + counts[2] + counts[3] == 0) return;
+  const int64_t np = npc.get();
+  const int n1 = counts[1], n2 = counts[2];
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    const int c = cls[q];
+    if (c == 3) continue;
+    const Mig3 o = off[q];
+    const int64_t to = c == 1 ? o.w : (c == 2 ? n1 + o.e : n1 + n2 + (q - o.w - o.e - o.d));
+    for (int a = 0; a < 6; ++a) {
+      double v0 = src.a[a][3 * q], v1 = src.a[a][3 * q + 1], v2 = src.a[a][3 * q + 2];
+      if (a == 0) {
+        double &x = d == 0 ? v0 : (d == 1 ? v1 : v2);
+        if (c == 1) x = x + len;
+        if (c == 2) x = x - len;
+      }
+      dst.a[a][3 * to] = v0;
+      dst.a[a][3 * to + 1] = v1;
+      dst.a[a][3 * to + 2] = v2;
+    }
+  }
+}
+__global__ void k_copy_back(NCount npc, const int *counts, Arr6 src, Arr6 dst) {
+  if (counts[1] + counts[2] + counts[3] == 0) return;
+  const int64_t n = 3 * (npc.get() - counts[3]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    for (int a = 0; a < 6; ++a) dst.a[a][i] = src.a[a][i];
+}
+__global__ void k_np_drop(int *np, const int *counts) { *np -= counts[3]; }
+// a particle still outside a periodic axis after one wrap moved more than a
+// period in one stage: flagged (bit 2 of the status word)
+__global__ void k_check_inside(NCount npc, const double *pos, int d, double lo, double hi, int *err) {
+  const int64_t np = npc.get();
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    const double x = pos[3 * q + d];
+    if (x < lo || x >= hi) atomicOr(err, 2);
+  }
+}
+// the remeshed count, capped (bit 1 of the status word: over capacity)
+__global__ void k_np_from_offs(int *np, const int *offs, int64_t nn, int64_t cap, int *err) {
+  const int64_t t = offs[nn];
+  if (t > cap) atomicOr(err, 1);
+  *np = (int)(t > cap ? cap : t);
+}
+
+struct VStep {
+  pmgpu_geom geom{};
+  PGeom g{};
+  int prec = PMGPU_FP64, rk = 2;
+  int64_t cap = 0;
+  hipStream_t st = nullptr;
+  pmgpu_poisson_t solver = nullptr;
+  PartWS w;         // this step's own P2M / remesh workspace (the graph keeps its pointers)
+  Scratch parts;    // the particle set: 6 arrays of 3 * cap doubles
+  Scratch tmp;      // migration target: 6 arrays
+  Scratch mesh;     // vorticity and velocity meshes
+  Scratch mig;      // cls, counts, keys, offsets
+  Scratch scan3;    // rocPRIM temporary of the offset scan
+  Scratch state;    // np and the status word
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double g_dt = 0.0, g_min = 0.0;
+
+  size_t mesh_elems() const { return 3 * (size_t)g.E[0] * g.E[1] * g.E[2]; }
+  Arr6 arrays(Scratch &s) {
+    Arr6 r;
+    for (int a = 0; a < 6; ++a) r.a[a] = s.as<double>() + (size_t)a * 3 * cap;
+    return r;
+  }
+  int *d_np() { return state.as<int>(); }
+  int *d_err() { return state.as<int>() + 1; }
+  void *mvort() { return mesh.p; }
+  void *mvel() { return (char *)mesh.p + mesh_elems() * (prec == PMGPU_FP64 ? 8 : 4); }
+
+  ~VStep() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (solver) (void)pmgpu_poisson_destroy(solver);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  // every buffer at its final size before anything is recorded
+  int prepare() {
+    const int nbins = g.nb[0] * g.nb[1] * g.nb[2];
+    const int64_t nn = mesh_cells(g);
+    PM_CHECK(cap < ((int64_t)1 << 31) && nbins < (1 << 30) && nn < ((int64_t)1 << 31),
+             "pmgpu_vstep: capacity or mesh too large");
+    const size_t rec_bytes = (prec == PMGPU_FP64 ? sizeof(double) : sizeof(float)) * PRec<double, 6>::kSlots;
+    if (w.key.ensure(sizeof(int) * ((size_t)cap + 1)) || w.ids.ensure(sizeof(int) * 2 * ((size_t)cap + 1)) ||
+        w.count.ensure(sizeof(int) * (nbins + 1)) || w.start.ensure(sizeof(int) * (nbins + 1)) ||
+        w.rec.ensure(rec_bytes * ((size_t)cap + 1)) || w.flag.ensure(sizeof(int) * (nn + 1)) ||
+        w.offs.ensure(sizeof(int) * (nn + 1)))
+      return -1;
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    PM_HIP(rocprim::inclusive_scan(nullptr, b1, (int *)nullptr, (int *)nullptr, (size_t)nbins, rocprim::plus<int>(),
+                                   st));
+    PM_HIP(rocprim::inclusive_scan(nullptr, b2, (int *)nullptr, (int *)nullptr, (size_t)nn, rocprim::plus<int>(), st));
+    PM_HIP(rocprim::inclusive_scan(nullptr, b3, (Mig3 *)nullptr, (Mig3 *)nullptr, (size_t)cap, Mig3Plus(), st));
+    if (w.scan.ensure((b1 > b2 ? b1 : b2) + 16) || scan3.ensure(b3 + 16)) return -1;
+    const size_t pb = sizeof(double) * 3 * (size_t)cap * 6;
+    if (parts.ensure(pb) || tmp.ensure(pb) || mesh.ensure(2 * mesh_elems() * (prec == PMGPU_FP64 ? 8 : 4)) ||
+        mig.ensure(sizeof(int) * ((size_t)cap + 8) + sizeof(Mig3) * (2 * (size_t)cap + 2)) ||
+        state.ensure(sizeof(int) * 8))
+      return -1;
+    return 0;
+  }
+  int *cls() { return mig.as<int>(); }
+  int *counts() { return mig.as<int>() + cap; }
+  Mig3 *key3() { return reinterpret_cast<Mig3 *>(mig.as<int>() + cap + 8); }
+  Mig3 *off3() { return key3() + cap; }
+
+  // deposition of the particle vorticity onto the vorticity mesh (clear)
+  int p2m() {
+    const int nbins = g.nb[0] * g.nb[1] * g.nb[2];
+    const NCount n(cap, d_np());
+    Arr6 A = arrays(parts);
+    int *key = w.key.as<int>(), *rank = w.ids.as<int>(), *id_out = rank + cap + 1;
+    hipLaunchKernelGGL(k_fill_int, grid_for(nbins + 1), dim3(256), 0, st, w.count.as<int>(), (int64_t)nbins + 1, 0);
+    hipLaunchKernelGGL(k_p2m_keys, grid_for(cap), dim3(256), 0, st, g, (const double *)A.a[0], n, nbins, key, rank,
+                       w.count.as<int>());
+    hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, st, w.start.as<int>());
+    size_t bytes = w.scan.cap - 16;
+    PM_HIP(rocprim::inclusive_scan(w.scan.p, bytes, w.count.as<int>(), w.start.as<int>() + 1, (size_t)nbins,
+                                   rocprim::plus<int>(), st));
+    hipLaunchKernelGGL(k_p2m_scatter, grid_for(cap), dim3(256), 0, st, n, nbins, key, rank, w.start.as<int>(),
+                       id_out);
+    if (prec == PMGPU_FP64) {
+      if (g.ns == 4) p2m_stage_gather<double, 4>(g, A.a[0], A.a[2], cap, id_out, w, nbins, (double *)mvort(), 0, st);
+      else p2m_stage_gather<double, 6>(g, A.a[0], A.a[2], cap, id_out, w, nbins, (double *)mvort(), 0, st);
+    } else {
+      if (g.ns == 4) p2m_stage_gather<float, 4>(g, A.a[0], A.a[2], cap, id_out, w, nbins, (float *)mvort(), 0, st);
+      else p2m_stage_gather<float, 6>(g, A.a[0], A.a[2], cap, id_out, w, nbins, (float *)mvort(), 0, st);
+    }
+    for (int d = 0; d < 3; ++d)
+      if (g.bc[d] != 0) {
+        const dim3 grid = grid_for(2LL * g.ngl[d] * 3 * g.E[(d + 1) % 3] * g.E[(d + 2) % 3]);
+        if (prec == PMGPU_FP64)
+          hipLaunchKernelGGL(k_halo_add<double>, grid, dim3(256), 0, st, (double *)mvort(), g, d);
+        else
+          hipLaunchKernelGGL(k_halo_add<float>, grid, dim3(256), 0, st, (float *)mvort(), g, d);
+      }
+    PM_HIP(hipGetLastError());
+    return 0;
+  }
+
+  int migrate(int d) {
+    const NCount n(cap, d_np());
+    Arr6 A = arrays(parts), T = arrays(tmp);
+    const double lo = geom.xmin[d], hi = geom.xmax[d], len = geom.xmax[d] - geom.xmin[d];
+    const int per = geom.bound_cond[d] != 0;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, counts(), (int64_t)4, 0);
+    hipLaunchKernelGGL(k_classify, grid_for(cap), dim3(256), 0, st, n, (const double *)A.a[0], d, lo, hi, per, cls(),
+                       counts());
+    hipLaunchKernelGGL(k_mig_keys, grid_for(cap), dim3(256), 0, st, n, cap, (const int *)cls(), key3());
+    hipLaunchKernelGGL(k_zero_first3, dim3(1), dim3(1), 0, st, off3());
+    size_t bytes = scan3.cap - 16;
+    PM_HIP(rocprim::inclusive_scan(scan3.p, bytes, key3(), off3() + 1, (size_t)cap, Mig3Plus(), st));
+    hipLaunchKernelGGL(k_migrate_dev, grid_for(cap), dim3(256), 0, st, n, (const int *)cls(), (const Mig3 *)off3(),
+                       (const int *)counts(), d, len, A, T);
+    hipLaunchKernelGGL(k_copy_back, grid_for(3 * cap), dim3(256), 0, st, n, (const int *)counts(), T, A);
+    hipLaunchKernelGGL(k_np_drop, dim3(1), dim3(1), 0, st, d_np(), (const int *)counts());
+    if (per)
+      hipLaunchKernelGGL(k_check_inside, grid_for(cap), dim3(256), 0, st, NCount(cap, d_np()), (const double *)A.a[0],
+                         d, lo, hi, d_err());
+    PM_HIP(hipGetLastError());
+    return 0;
+  }
+
+  // the whole step, stream-ordered (recorded into the graph)
+  int enqueue(double dt, double min_vort) {
+    Arr6 A = arrays(parts);
+    const NCount n(cap, d_np());
+    for (int s = 1; s <= rk; ++s) {
+      if (p2m()) return -1;
+      if (pmgpu_poisson_solve_device(solver, mvort(), mvel(), nullptr, nullptr, 0, 0, 0)) return -1;
+      if (prec == PMGPU_FP64) {
+        if (g.ns == 4)
+          hipLaunchKernelGGL((k_m2p<double, 4>), grid_for(cap), dim3(256), 0, st, g, (const double *)mvel(),
+                             (const double *)A.a[0], n, A.a[1]);
+        else
+          hipLaunchKernelGGL((k_m2p<double, 6>), grid_for(cap), dim3(256), 0, st, g, (const double *)mvel(),
+                             (const double *)A.a[0], n, A.a[1]);
+      } else {
+        if (g.ns == 4)
+          hipLaunchKernelGGL((k_m2p<float, 4>), grid_for(cap), dim3(256), 0, st, g, (const float *)mvel(),
+                             (const double *)A.a[0], n, A.a[1]);
+        else
+          hipLaunchKernelGGL((k_m2p<float, 6>), grid_for(cap), dim3(256), 0, st, g, (const float *)mvel(),
+                             (const double *)A.a[0], n, A.a[1]);
+      }
+      const RkCoef k = rk_coef(rk, s, 0);
+      hipLaunchKernelGGL(k_rk_update, grid_for(3 * cap), dim3(256), 0, st, k, dt, n, A.a[0], (const double *)A.a[1],
+                         A.a[2], (const double *)A.a[3], A.a[4], A.a[5]);
+      PM_HIP(hipGetLastError());
+      for (int d = 0; d < 3; ++d)
+        if (migrate(d)) return -1;
+    }
+    // remesh: deposit, flag, scan, create (velocity from the last stage's mesh)
+    if (p2m()) return -1;
+    const int64_t nn = mesh_cells(g);
+    if (prec == PMGPU_FP64)
+      remesh_flags_launch<double>(g, nn, mvort(), nullptr, nullptr, min_vort, 0.0, 0, w.flag.as<int>(), st);
+    else
+      remesh_flags_launch<float>(g, nn, mvort(), nullptr, nullptr, min_vort, 0.0, 0, w.flag.as<int>(), st);
+    hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, st, w.offs.as<int>());
+    size_t bytes = w.scan.cap - 16;
+    PM_HIP(rocprim::inclusive_scan(w.scan.p, bytes, w.flag.as<int>(), w.offs.as<int>() + 1, (size_t)nn,
+                                   rocprim::plus<int>(), st));
+    if (prec == PMGPU_FP64)
+      hipLaunchKernelGGL(k_remesh_create<double>, grid_for(nn), dim3(256), 0, st, g, (const int *)w.flag.as<int>(),
+                         (const int *)w.offs.as<int>(), (const double *)mvort(), (const double *)nullptr,
+                         (const double *)mvel(), A.a[0], A.a[2], A.a[1], A.a[3], A.a[4], A.a[5], cap);
+    else
+      hipLaunchKernelGGL(k_remesh_create<float>, grid_for(nn), dim3(256), 0, st, g, (const int *)w.flag.as<int>(),
+                         (const int *)w.offs.as<int>(), (const float *)mvort(), (const float *)nullptr,
+                         (const float *)mvel(), A.a[0], A.a[2], A.a[1], A.a[3], A.a[4], A.a[5], cap);
+    hipLaunchKernelGGL(k_np_from_offs, dim3(1), dim3(1), 0, st, d_np(), (const int *)w.offs.as<int>(), nn, cap,
+                       d_err());
+    PM_HIP(hipGetLastError());
+    return 0;
+  }
+
+  int record(double dt, double min_vort) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    PM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int e = enqueue(dt, min_vort);
+    hipGraph_t gr = nullptr;
+    const hipError_t ce = hipStreamEndCapture(st, &gr);
+    if (e) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return -1;
+    }
+    PM_HIP(ce);
+    graph = gr;
+    PM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    g_dt = dt;
+    g_min = min_vort;
+    return 0;
+  }
+};
+
+}  // namespace pmgpu
+
+struct pmgpu_vstep_s {
+  pmgpu::VStep s;
+};
+
+extern "C" {
+
+int pmgpu_vstep_create(const pmgpu_geom *g, const pmgpu_params *p, int32_t precision, int32_t rk_order,
+                       int64_t capacity, pmgpu_vstep_t *out) {
+  PM_CHECK(g && p && out, "pmgpu_vstep_create: null argument");
+  PM_CHECK(precision == PMGPU_FP64 || precision == PMGPU_FP32, "pmgpu_vstep_create: bad precision");
+  PM_CHECK(rk_order >= 1 && rk_order <= 3, "pmgpu_vstep_create: time integration order must be 1, 2 or 3");
+  PM_CHECK(capacity > 0, "pmgpu_vstep_create: capacity must be positive");
+  auto h = new pmgpu_vstep_s;
+  VStep &s = h->s;
+  s.geom = *g;
+  s.prec = precision;
+  s.rk = rk_order;
+  s.cap = capacity;
+  int e = make_pgeom(g, s.g);
+  if (!e) {
+    for (int d = 0; d < 3; ++d)
+      if (s.g.bc[d] != 0 && s.g.ncell[d] < 2 * s.g.ngl[d]) {
+        set_error("pmgpu: periodic axis shorter than two ghost layers");
+        e = -1;
+      }
+  }
+  if (!e && hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess) {
+    set_error("pmgpu_vstep_create: stream creation failed");
+    e = -1;
+  }
+  if (!e) e = pmgpu_poisson_create(g, p, precision, s.st, &s.solver);
+  if (!e) e = pmgpu_poisson_setup(s.solver);
+  if (!e) e = s.prepare();
+  if (!e && (hipMemsetAsync(s.state.p, 0, sizeof(int) * 8, s.st) != hipSuccess ||
+             hipMemsetAsync(s.parts.p, 0, s.parts.cap, s.st) != hipSuccess ||
+             hipStreamSynchronize(s.st) != hipSuccess)) {
+    set_error("pmgpu_vstep_create: initialisation failed");
+    e = -1;
+  }
+  if (e) {
+    delete h;
+    return -1;
+  }
+  *out = h;
+  return 0;
+}
+
+int pmgpu_vstep_destroy(pmgpu_vstep_t h) {
+  delete h;
+  return 0;
+}
+
+int pmgpu_vstep_load(pmgpu_vstep_t h, const void *d_pos, const void *d_vort, int64_t np) {
+  PM_CHECK(h && (np == 0 || (d_pos && d_vort)), "pmgpu_vstep_load: null argument");
+  VStep &s = h->s;
+  PM_CHECK(np >= 0 && np <= s.cap, "pmgpu_vstep_load: more particles than the capacity");
+  // the particle set starts from (pos, vort) with every other field zero
+  PM_HIP(hipMemsetAsync(s.parts.p, 0, s.parts.cap, s.st));
+  Arr6 A = s.arrays(s.parts);
+  if (np) {
+    PM_HIP(hipMemcpyAsync(A.a[0], d_pos, 24 * (size_t)np, hipMemcpyDeviceToDevice, s.st));
+    PM_HIP(hipMemcpyAsync(A.a[2], d_vort, 24 * (size_t)np, hipMemcpyDeviceToDevice, s.st));
+  }
+  const int st[2] = {(int)np, 0};
+  PM_HIP(hipMemcpyAsync(s.state.p, st, sizeof(st), hipMemcpyHostToDevice, s.st));
+  PM_HIP(hipStreamSynchronize(s.st));
+  return 0;
+}
+
+int pmgpu_vstep_step(pmgpu_vstep_t h, double dt, double min_vort, int32_t use_graph) {
+  PM_CHECK(h, "pmgpu_vstep_step: null handle");
+  VStep &s = h->s;
+  if (!use_graph) return s.enqueue(dt, min_vort);
+  if (!s.exec || s.g_dt != dt || s.g_min != min_vort)
+    if (s.record(dt, min_vort)) return -1;
+  PM_HIP(hipGraphLaunch(s.exec, s.st));
+  return 0;
+}
+
+int pmgpu_vstep_sync(pmgpu_vstep_t h, int64_t *np, int32_t *status) {
+  PM_CHECK(h, "pmgpu_vstep_sync: null handle");
+  VStep &s = h->s;
+  int st[2] = {0, 0};
+  PM_HIP(hipMemcpyAsync(st, s.state.p, sizeof(st), hipMemcpyDeviceToHost, s.st));
+  PM_HIP(hipStreamSynchronize(s.st));
+  if (np) *np = st[0];
+  if (status) *status = st[1];
+  return 0;
+}
+
+int pmgpu_vstep_fetch(pmgpu_vstep_t h, double *pos, double *vort, double *vel, int64_t cap, int64_t *np) {
+  PM_CHECK(h && np, "pmgpu_vstep_fetch: null argument");
+  VStep &s = h->s;
+  int st[2] = {0, 0};
+  PM_HIP(hipMemcpyAsync(st, s.state.p, sizeof(st), hipMemcpyDeviceToHost, s.st));
+  PM_HIP(hipStreamSynchronize(s.st));
+  *np = st[0];
+  PM_CHECK(st[0] <= cap, "pmgpu_vstep_fetch: host arrays too small");
+  Arr6 A = s.arrays(s.parts);
+  double *dst[3] = {pos, vort, vel};
+  const int src[3] = {0, 2, 1};
+  for (int q = 0; q < 3; ++q)
+    if (dst[q] && st[0]) PM_HIP(hipMemcpy(dst[q], A.a[src[q]], 24 * (size_t)st[0], hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmgpu_vstep_arrays(pmgpu_vstep_t h, void *d_arrays[6], void **stream) {
+  PM_CHECK(h && d_arrays, "pmgpu_vstep_arrays: null argument");
+  VStep &s = h->s;
+  Arr6 A = s.arrays(s.parts);
+  for (int a = 0; a < 6; ++a) d_arrays[a] = A.a[a];
+  if (stream) *stream = s.st;
   return 0;
 }
 

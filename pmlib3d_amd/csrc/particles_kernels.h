@@ -104,6 +104,18 @@ __device__ __forceinline__ float weight_f(int m6, int s, float d) {
   return c3 * (x - 1.0f) * (25.0f * x4 - 38.0f * x3 - 3.0f * x2 + 12.0f * x + 12.0f);
 }
 
+// Particle count of a launch: a host value, or, in a step recorded once
+// and replayed (HIP graph), the count in device memory that earlier kernels
+// of the same step wrote; the grid then covers the capacity and threads past
+// the count do nothing.
+struct NCount {
+  int64_t n;
+  const int *dev;
+  __host__ __device__ NCount(int64_t v = 0) : n(v), dev(nullptr) {}
+  __host__ __device__ NCount(int64_t cap, const int *d) : n(cap), dev(d) {}
+  __device__ __forceinline__ int64_t get() const { return dev ? (int64_t)*dev : n; }
+};
+
 // compensated (Kahan) accumulation: sum + c carries the low-order bits the
 // float sum drops (this file is compiled without contraction, so the
 // compensation survives)
@@ -121,7 +133,8 @@ struct Kahan {
 // particle lies outside a periodic axis, where the reference zeroes every
 // weight) and its arrival rank in the bin (an atomic count: the order within a
 // bin is restored by particle id when the bin is staged)
-__global__ void k_p2m_keys(PGeom g, const double *pos, int64_t np, int nbins, int *key, int *rank, int *count) {
+__global__ void k_p2m_keys(PGeom g, const double *pos, NCount npc, int nbins, int *key, int *rank, int *count) {
+  const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     int b[3];
     bool ok = true;
@@ -143,7 +156,8 @@ __global__ void k_p2m_keys(PGeom g, const double *pos, int64_t np, int nbins, in
 }
 
 // counting-sort scatter: particle q to slot start[key] + rank of its bin
-__global__ void k_p2m_scatter(int64_t np, int nbins, const int *key, const int *rank, const int *start, int *ids) {
+__global__ void k_p2m_scatter(NCount npc, int nbins, const int *key, const int *rank, const int *start, int *ids) {
+  const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     const int k = key[q];
     if (k < nbins) ids[start[k] + rank[q]] = (int)q;
@@ -417,7 +431,8 @@ template <typename T> __global__ void k_halo_add(T *m, PGeom g, int d) {
 // (FP32 mesh) takes FP32 weights and Kahan sums.  Stencil weights and indices
 // live in registers (the stencil width is a template parameter).
 template <typename T, int NS>
-__global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const double *pos, int64_t np, double *out) {
+__global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const double *pos, NCount npc, double *out) {
+  const int64_t np = npc.get();
   constexpr bool F64 = sizeof(T) == sizeof(double);
   constexpr bool M6 = NS == 6;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
@@ -544,12 +559,13 @@ __global__ void k_remesh_flags(PGeom g, const T *vort, const T *dvort, const uin
 template <typename T>
 __global__ void k_remesh_create(PGeom g, const int *flag, const int *offs, const T *vort, const T *dvort,
                                 const T *vel, double *pos, double *pvort, double *pvel, double *pdvort,
-                                double *vel_rk, double *dvort_rk) {
+                                double *vel_rk, double *dvort_rk, int64_t cap) {
   const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
        idx += (int64_t)gridDim.x * blockDim.x) {
     if (!flag[idx]) continue;
     int64_t p = offs[idx];
+    if (p >= cap) continue;  // over capacity: the caller reports it
     int ii[3];
     ii[0] = (int)(idx % g.E[0]) + g.lo[0];
     ii[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
@@ -571,8 +587,9 @@ struct RkCoef {
   double a1, b1, c1, a2, b2, c2;
 };
 
-__global__ void k_rk_update(RkCoef k, double dt, int64_t np, double *pos, const double *vel, double *vort,
+__global__ void k_rk_update(RkCoef k, double dt, NCount npc, double *pos, const double *vel, double *vort,
                             const double *dvort, double *vel_rk, double *dvort_rk) {
+  const int64_t np = npc.get();
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * np; i += (int64_t)gridDim.x * blockDim.x) {
     double vr = __dadd_rn(__dmul_rn(k.a1, vel_rk[i]), __dmul_rn(__dmul_rn(k.b1, dt), vel[i]));
     double dr = __dadd_rn(__dmul_rn(k.a2, dvort_rk[i]), __dmul_rn(__dmul_rn(k.b2, dt), dvort[i]));
@@ -587,7 +604,8 @@ __global__ void k_rk_update(RkCoef k, double dt, int64_t np, double *pos, const 
 struct PBox {
   double lo[3], hi[3];
 };
-__global__ void k_any_outside(int64_t np, const double *pos, PBox b, int *flag) {
+__global__ void k_any_outside(NCount npc, const double *pos, PBox b, int *flag) {
+  const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     bool out = false;
 #pragma unroll
@@ -600,8 +618,9 @@ __global__ void k_any_outside(int64_t np, const double *pos, PBox b, int *flag) 
 }
 
 // migration class along axis d: 0 stay, 1 west (wrap), 2 east (wrap), 3 drop
-__global__ void k_classify(int64_t np, const double *pos, int d, double lo, double hi, int per, int *cls,
+__global__ void k_classify(NCount npc, const double *pos, int d, double lo, double hi, int per, int *cls,
                            int *counts) {
+  const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     double x = pos[3 * q + d];
     int c = x < lo ? 1 : (x >= hi ? 2 : 0);

@@ -134,3 +134,75 @@ class VortexStep:
         for s in range(1, self.rk_order + 1):
             self.stage(s, dt)
         self.remesh(min_vort, min_dvort)
+
+
+class GraphVortexStep:
+    """The same vortex step with the particle count kept in device memory and
+    the whole step (every stage's deposition, solve, gather, RK update and
+    migration, then deposition and remesh) recorded once as a HIP graph and
+    replayed: no host synchronisation inside a step (pmgpu_vstep_*; the
+    client loop of vortex_codes/vortex_structures/source/main.f:453-505
+    without the vorticity right-hand side)."""
+
+    def __init__(self, patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3, capacity=None,
+                 precision=_lib.FP64, device="cuda:0"):
+        import ctypes as C
+        import torch
+        self.torch, self.C = torch, C
+        self.dev = torch.device(device)
+        _lib.init()
+        self.lib = _lib.load()
+        g = patch.geom(interp_order)
+        p = _lib.Params(float(eps_radius), int(order), 1)
+        self.cap = int(capacity if capacity is not None else np.prod(mesh_extent(patch, interp_order)))
+        h = C.c_void_p()
+        _lib.check(self.lib.pmgpu_vstep_create(C.byref(g), C.byref(p), int(precision), int(rk_order), self.cap,
+                                               C.byref(h)), "pmgpu_vstep_create")
+        self.h = h
+        arr = (C.c_void_p * 6)()
+        st = C.c_void_p()
+        _lib.check(self.lib.pmgpu_vstep_arrays(self.h, arr, C.byref(st)), "pmgpu_vstep_arrays")
+        self.ptrs = [arr[i] for i in range(6)]
+        self.stream = st.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pmgpu_vstep_destroy(self.h)
+            self.h = None
+
+    def load(self, pos, vort):
+        pos = np.asfortranarray(pos, dtype=np.float64)
+        n = pos.shape[1]
+        t = [self.torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)).to(self.dev)
+             for a in (pos, vort)]
+        self.torch.cuda.synchronize()
+        _lib.check(self.lib.pmgpu_vstep_load(self.h, t[0].data_ptr(), t[1].data_ptr(), n), "pmgpu_vstep_load")
+
+    def load_device(self, d_pos, d_vort, n):
+        """The same as load from device tensors: flat (3, n) Fortran-ordered
+        float64 positions and vorticities."""
+        self.torch.cuda.synchronize()
+        _lib.check(self.lib.pmgpu_vstep_load(self.h, d_pos.data_ptr(), d_vort.data_ptr(), int(n)),
+                   "pmgpu_vstep_load")
+
+    def step(self, dt, min_vort, graph=True):
+        _lib.check(self.lib.pmgpu_vstep_step(self.h, float(dt), float(min_vort), 1 if graph else 0),
+                   "pmgpu_vstep_step")
+
+    def sync(self):
+        n = self.C.c_int64(0)
+        st = self.C.c_int32(0)
+        _lib.check(self.lib.pmgpu_vstep_sync(self.h, self.C.byref(n), self.C.byref(st)), "pmgpu_vstep_sync")
+        return n.value, st.value
+
+    def fetch(self):
+        """(pos, vort, vel) of the live particles as host (3, np) arrays."""
+        n, st = self.sync()
+        if st:
+            raise _lib.PmgpuError("vortex step status %d (1: remesh above capacity, 2: a particle moved more "
+                                  "than a period)" % st)
+        out = [np.empty((3, n), order="F") for _ in range(3)]
+        got = self.C.c_int64(0)
+        _lib.check(self.lib.pmgpu_vstep_fetch(self.h, *[o.ctypes.data_as(self.C.c_void_p) for o in out], n,
+                                              self.C.byref(got)), "pmgpu_vstep_fetch")
+        return tuple(out)
