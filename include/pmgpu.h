@@ -71,6 +71,18 @@ int pmgpu_poisson_destroy(pmgpu_poisson_t h);
  * spectral kernel). */
 int pmgpu_poisson_setup(pmgpu_poisson_t h);
 
+/* The refinement level of the patch a handle solves on (class_patch level and
+ * parent(1), pmlib_patch.f:15-27), read by the next pmgpu_poisson_setup.
+ * Level 1 is a base patch (the default); on any other level the kernel is
+ * the refinement-patch kernel of poisson/pmlib_poisson_setup.f:446-528 (order
+ * 2; :640-731 order 4, and the other orders' refinement blocks): the kernel
+ * regularised at eps minus the one regularised at a eps, a = 2^(level -
+ * parent_level), without periodic images (the reference adds image offsets
+ * that no loop sets there; they are taken as 0).  Order 0 then fails in
+ * setup, as in the reference (:335-339).  An all-periodic patch keeps its
+ * spectral kernel, which the reference's solve uses on every level. */
+int pmgpu_poisson_set_level(pmgpu_poisson_t h, int32_t level, int32_t parent_level);
+
 /* Replaces pmlib_poisson_solve(patch, topo_all, mesh, ierr, reg_vort, reproj,
  * calc_vecpot) (poisson/pmlib_poisson_solve.f:15-34).  Host FP64 meshes:
  * vort (in), vel (out), vort_out (out when reg_vort or reproj, else may be
@@ -140,6 +152,32 @@ int pmgpu_interp_particle_mesh(const pmgpu_geom *g, const double *xpart, const d
 int pmgpu_interp_particle_mesh_device(const pmgpu_geom *g, int32_t precision, const void *d_xpart,
                                       const void *d_part, int64_t np, void *d_mesh, int32_t clear,
                                       int32_t clear_ghost, void *stream);
+
+/* Replaces pmlib_topology_parent(topo, topo_parent, ierr)
+ * (topology/pmlib_topology_parent.f:5-100) on one rank: the parent cuboid of
+ * a patch on `level` whose parent is on `parent_level`: the same box and
+ * boundary conditions, cells reldx = 2^(level - parent_level) times larger
+ * (ncell must be a multiple of reldx), 2 (M'4) or 3 (M6*) ghost layers. */
+int pmgpu_topology_parent(const pmgpu_geom *g, int32_t level, int32_t parent_level, pmgpu_geom *out);
+
+/* Replaces pmlib_interp_mesh_mesh(topo_from, topo_to, mesh_from, mesh_to,
+ * ierr, clear) (interpolation/pmlib_interp_mesh_mesh.f:5-2078) for (3, E...)
+ * meshes on one rank, with the M'4 or M6* kernel of the geometries'
+ * interp_order.  reldx = dx_from(1) / dx_to(1).  reldx <= 0.9999 (patch to
+ * parent, :93-1392): every source cell, ghosts included, deposits reldx W
+ * per axis onto the target stencil (indices beyond a free side clamped with
+ * zero weight; on a periodic side a cell outside the box deposits nothing),
+ * then the periodic ghost layers are added onto their images.  Otherwise
+ * (parent to patch, :1395-2062): every target cell (interior cells only on
+ * periodic axes) adds the interpolated source value, then the periodic ghost
+ * layers are copied from their images.  clear zeroes the target first.
+ * Bit-identical to the reference's sequential loops; deterministic (the
+ * deposit is done as a gather in the reference's source-cell order, no
+ * atomics).  Cell ratios up to 8 are supported. */
+int pmgpu_interp_mesh_mesh(const pmgpu_geom *g_from, const pmgpu_geom *g_to, const double *from, double *to,
+                           int32_t clear);
+int pmgpu_interp_mesh_mesh_device(const pmgpu_geom *g_from, const pmgpu_geom *g_to, int32_t precision,
+                                  const void *d_from, void *d_to, int32_t clear, void *stream);
 
 /* Replaces one field of pmlib_interp_mesh_particle(topo, mesh, part, ierr,
  * vort, dvort, vel, clear) (interpolation/pmlib_interp_mesh_particle.f:5-6):

@@ -29,7 +29,7 @@ class Geom(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("regularisation_radius", C.c_double), ("poisson_order", C.c_int32),
-                ("poisson_kernel", C.c_int32)]
+                ("poisson_kernel", C.c_int32), ("refine", C.c_double)]
 
 
 def make_geom(ncell, nghost, xmin, dx, bound_cond=(0, 0, 0), interp_order=3):
@@ -45,11 +45,13 @@ def make_geom(ncell, nghost, xmin, dx, bound_cond=(0, 0, 0), interp_order=3):
     return g
 
 
-def make_params(order=4, kernel=1, eps_radius=2.0):
+def make_params(order=4, kernel=1, eps_radius=2.0, refine=0.0):
+    """refine: a = 2^(level - parent level) of a refinement patch (0: base level)."""
     p = Params()
     p.regularisation_radius = float(eps_radius)
     p.poisson_order = int(order)
     p.poisson_kernel = int(kernel)
+    p.refine = float(refine)
     return p
 
 
@@ -90,6 +92,10 @@ def lib():
         L.or_penalise.restype = C.c_int
         L.or_vort_rhs.argtypes = [G, P, P, C.c_double, C.c_int, P]
         L.or_vort_rhs.restype = C.c_int
+        L.or_topology_parent.argtypes = [G, C.c_int, C.c_int, G]
+        L.or_topology_parent.restype = C.c_int
+        L.or_interp_mesh_mesh.argtypes = [G, G, P, P, C.c_int]
+        L.or_interp_mesh_mesh.restype = C.c_int
         L.or_diag_particles.argtypes = [P, P, P, P, C.c_int64, P]
         L.or_diag_mesh.argtypes = [G, P, P, P]
         L.or_diag_mesh.restype = C.c_int
@@ -101,15 +107,16 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def ng_side(g, d):
+    """Ghost layers on each side of axis d (free: nghost; periodic: 2 for
+    M'4, 3 for M6*)."""
+    if g.bound_cond[d] == 0:
+        return g.nghost
+    return 3 if g.interp_order == 4 else 2
+
+
 def mesh_extent(g):
-    ext = []
-    for d in range(3):
-        if g.bound_cond[d] == 0:
-            ng = g.nghost
-        else:
-            ng = 3 if g.interp_order == 4 else 2
-        ext.append(g.ncell[d] + 2 * ng)
-    return tuple(ext)
+    return tuple(g.ncell[d] + 2 * ng_side(g, d) for d in range(3))
 
 
 def new_mesh(g):
@@ -282,6 +289,22 @@ def penalisation_solve(g, p, Ghat, vel, vort, chi, solid_vel=None, relaxation=1.
     else:
         out_v[1] = out_v[1] + vel_inf[2]
     return out_v, vort, force, npen, residual
+
+
+def topology_parent(g, level=2, parent_level=1):
+    """pmlib_topology_parent of a one-rank cuboid (a new geometry)."""
+    out = Geom()
+    if lib().or_topology_parent(g, int(level), int(parent_level), C.byref(out)):
+        raise ValueError("Number mesh cells does not match (or parent level 0)")
+    return out
+
+
+def interp_mesh_mesh(g_from, g_to, mesh_from, mesh_to=None, clear=True):
+    """pmlib_interp_mesh_mesh of a (3, E...) mesh onto another geometry."""
+    mesh_from = np.asfortranarray(mesh_from, dtype=np.float64)
+    out = new_mesh(g_to) if mesh_to is None else np.array(mesh_to, dtype=np.float64, order="F", copy=True)
+    lib().or_interp_mesh_mesh(g_from, g_to, _p(mesh_from), _p(out), int(bool(clear)))
+    return out
 
 
 DIAG_PARTICLE_NAMES = ("impulse_x", "impulse_y", "impulse_z", "ang_impulse_x", "ang_impulse_y", "ang_impulse_z",

@@ -60,6 +60,7 @@ typedef struct {
 typedef struct {
   double regularisation_radius;
   int32_t poisson_order, poisson_kernel;
+  double refine;        /* a = 2^(level - parent level) of a refinement patch; 0: base level */
 } or_params;
 
 /* ghost layers on the low/high side of axis d on a single rank:
@@ -157,6 +158,72 @@ int or_green_real(const or_geom *g, const or_params *p, double *G) {
   if (ord != 0 && ord != 2 && ord != 4 && ord != 6 && ord != 8 && ord != 10) return -1;
   const size_t n = (size_t)L[0] * L[1] * L[2];
   memset(G, 0, sizeof(double) * n * nvar);
+  if (p->refine > 0.0) {
+    /* refinement patch (level > 1, poisson/pmlib_poisson_setup.f:446-528,
+     * :640-731 and the other orders' refinement blocks): the difference of
+     * the kernels regularised at eps and at a eps, G(rho) - G(rho / a),
+     * without periodic images (the reference's image indices are unset
+     * there; they are taken as 0).  Order 0 has no refinement kernel. */
+    if (ord == 0) return -1;
+    const double ca = 1.0 / p->refine;
+    for (int k = 1; k <= L[2]; ++k) {
+      double pz = xk[2] + ((double)(k - 1) + 0.5) * g->dx[2];
+      for (int j = 1; j <= L[1]; ++j) {
+        double py = xk[1] + ((double)(j - 1) + 0.5) * g->dx[1];
+        for (int i = 1; i <= L[0]; ++i) {
+          double px = xk[0] + ((double)(i - 1) + 0.5) * g->dx[0];
+          double r = sqrt(px * px + py * py + pz * pz);
+          size_t o = ((size_t)(k - 1) * L[1] + (j - 1)) * L[0] + (i - 1);
+          if (r == 0.0) {
+            if (ker == 1) {
+              double c = ord == 2 ? 1.0 : ord == 4 ? 1.5 : ord == 6 ? 1.875 : ord == 8 ? 2.1875 : 2.4609375;
+              G[o] = c * c_sq2_4pieps * (1.0 - ca);
+            }
+            continue;
+          }
+          double part[2];
+          for (int q = 0; q < 2; ++q) {
+            /* q = 0: regularised at eps; q = 1: at a eps (rho / a) */
+            double rho = r / eps * (q ? ca : 1.0);
+            double E = exp(-0.5 * (rho * rho)) * c_1_sqrt2pi, F = erf(c_1_sqrt2 * rho);
+            double r2 = rho * rho, r3 = r2 * rho, r5 = r3 * r2, r7 = r5 * r2, r9 = r7 * r2, poly;
+            if (ker == 1) {
+              switch (ord) {
+                case 2: poly = 0.0; break;
+                case 4: poly = rho; break;
+                case 6: poly = 1.75 * rho - 0.25 * r3; break;
+                case 8: poly = 2.375 * rho + (-2.0 / 3.0) * r3 + (1.0 / 24.0) * r5; break;
+                default:
+                  poly = 2.921875 * rho + (-233.0 / 192.0) * r3 + (29.0 / 192.0) * r5 + (-1.0 / 192.0) * r7;
+                  break;
+              }
+            } else {
+              switch (ord) {
+                case 2: poly = -2.0 * rho; break;
+                case 4: poly = -2.0 * rho + r3; break;
+                case 6: poly = -2.0 * rho + 2.25 * r3 - 0.25 * r5; break;
+                case 8: poly = -2.0 * rho + (89.0 / 24.0) * r3 + (-20.0 / 24.0) * r5 + (1.0 / 24.0) * r7; break;
+                default:
+                  poly = -2.0 * rho + (1027.0 / 192.0) * r3 + (-349.0 / 192.0) * r5 + (35.0 / 192.0) * r7 +
+                         (-1.0 / 192.0) * r9;
+                  break;
+              }
+            }
+            part[q] = poly * E + F;
+          }
+          if (ker == 1) {
+            G[o] = c_1_4pi / r * (part[0] - part[1]);
+          } else {
+            double s = -c_1_4pi / (r * r * r) * (part[0] - part[1]);
+            G[o * 3 + 0] = px * s;
+            G[o * 3 + 1] = py * s;
+            G[o * 3 + 2] = pz * s;
+          }
+        }
+      }
+    }
+    return 0;
+  }
   const int b0 = g->bound_cond[0], b1 = g->bound_cond[1], b2 = g->bound_cond[2];
   for (int kk = -b2; kk <= b2; ++kk)
     for (int jj = -b1; jj <= b1; ++jj)
@@ -1034,5 +1101,144 @@ int or_diag_mesh(const or_geom *g, const double *vel, const double *vort, double
   out[2] = hel;
   out[3] = lamb;
   out[4] = vmax;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multiresolution: parent topology and mesh-to-mesh interpolation           */
+
+/* pmlib_topology_parent (topology/pmlib_topology_parent.f:5-100) on one
+ * rank: the parent cuboid over the same box with cells a = 2^(level -
+ * parent level) times larger, 2 (M'4) or 3 (M6*) ghost layers, the same
+ * boundary conditions.  Returns -1 if ncell is not a multiple of a. */
+int or_topology_parent(const or_geom *g, int level, int parent_level, or_geom *out) {
+  if (parent_level == 0) return -1;
+  const double reldx = pow(2.0, (double)(level - parent_level));
+  const int r = (int)lround(reldx);
+  *out = *g;
+  for (int d = 0; d < 3; ++d) {
+    if (g->ncell[d] % r != 0) return -1;
+    out->ncell[d] = (int)lround((double)g->ncell[d] / reldx);
+    out->dx[d] = reldx * g->dx[d];
+  }
+  out->nghost = g->interp_order == 4 ? 3 : 2;
+  return 0;
+}
+
+/* pmlib_interp_mesh_mesh (interpolation/pmlib_interp_mesh_mesh.f:5-2078) on
+ * one rank for (3, E...) meshes, M'4 or M6* (interp_order of the
+ * geometries): reldx = dx_from / dx_to.  Below 1 (patch to parent, :93-1392)
+ * every cell of the source mesh, ghosts included, is deposited onto the
+ * target with weights reldx W per axis (stencils cancelled as in P2M), then
+ * the periodic ghost layers are added to their images; above 1 (parent to
+ * patch, :1395-2062) every target cell (interior cells only on periodic
+ * sides) adds the interpolated source value, stencil indices clamped into
+ * the source mesh with zero weight, then the periodic ghosts are copied.
+ * clear zeroes the target first.  Loop orders and term orders are the
+ * reference's. */
+int or_interp_mesh_mesh(const or_geom *gf, const or_geom *gt, const double *from, double *to, int clear) {
+  const int m6 = gt->interp_order == 4, ns = m6 ? 6 : 4, off = m6 ? 2 : 1;
+  int Ef[3], Et[3], nf[3], nt[3];
+  for (int d = 0; d < 3; ++d) {
+    Ef[d] = mesh_ext(gf, d); Et[d] = mesh_ext(gt, d);
+    nf[d] = ng_side(gf, d); nt[d] = ng_side(gt, d);
+  }
+  const size_t Nt = (size_t)Et[0] * Et[1] * Et[2];
+  if (clear) memset(to, 0, sizeof(double) * 3 * Nt);
+  const double reldx = gf->dx[0] / gt->dx[0];
+#define TO(c, i, j, k) to[(c) + 3 * ((size_t)((i) - 1 + nt[0]) + (size_t)Et[0] * (((j) - 1 + nt[1]) + (size_t)Et[1] * ((k) - 1 + nt[2])))]
+#define FR(c, i, j, k) from[(c) + 3 * ((size_t)((i) - 1 + nf[0]) + (size_t)Ef[0] * (((j) - 1 + nf[1]) + (size_t)Ef[1] * ((k) - 1 + nf[2])))]
+  if (reldx <= 0.9999) {
+    double c1d[3];
+    for (int d = 0; d < 3; ++d) c1d[d] = 1.0 / gt->dx[d];
+    for (int k = 1 - nf[2]; k <= gf->ncell[2] + nf[2]; ++k)
+      for (int j = 1 - nf[1]; j <= gf->ncell[1] + nf[1]; ++j)
+        for (int i = 1 - nf[0]; i <= gf->ncell[0] + nf[0]; ++i) {
+          const int ii[3] = {i, j, k};
+          double p[3], w[3][6];
+          int idx[3][6], b[3];
+          for (int d = 0; d < 3; ++d) {
+            p[d] = gf->xmin[d] + ((double)(ii[d] - 1) + 0.5) * gf->dx[d];
+            b[d] = or_cell_index(p[d], gt->xmin[d], c1d[d]);
+          }
+          int inside = 1;
+          for (int d = 0; d < 3; ++d)
+            if (b[d] - off + ns - 1 < 1 - nt[d] || b[d] - off > gt->ncell[d] + nt[d]) inside = 0;
+          if (!inside) continue;
+          for (int d = 0; d < 3; ++d) {
+            double dd = (p[d] - gt->xmin[d] - gt->dx[d] * ((double)b[d] - 0.5)) * c1d[d], a[6];
+            if (m6) w_m6(dd, a);
+            else w_m4(dd, a);
+            for (int s = 0; s < ns; ++s) {
+              idx[d][s] = b[d] - off + s;
+              w[d][s] = reldx * a[s];
+            }
+          }
+          /* cancellation (:159-290): free sides clamp each index, periodic
+           * sides drop the whole cell when it lies outside the box */
+          for (int d = 0; d < 3; ++d) {
+            if (gt->bound_cond[d] == 0) {
+              for (int s = 0; s < ns; ++s) {
+                if (idx[d][s] < 1 - nt[d]) { w[d][s] = 0.0; idx[d][s] = 1 - nt[d]; }
+                if (idx[d][s] > gt->ncell[d] + nt[d]) { w[d][s] = 0.0; idx[d][s] = gt->ncell[d] + nt[d]; }
+              }
+            } else if (p[d] < gt->xmin[d] || p[d] > gt->xmax[d]) {
+              for (int e = 0; e < 3; ++e)
+                for (int s = 0; s < ns; ++s) w[e][s] = 0.0;
+              for (int s = 0; s < ns; ++s) idx[d][s] = p[d] < gt->xmin[d] ? 1 - nt[d] : gt->ncell[d] + nt[d];
+            }
+          }
+          for (int sz = 0; sz < ns; ++sz)
+            for (int sy = 0; sy < ns; ++sy)
+              for (int sx = 0; sx < ns; ++sx) {
+                const double ww = w[0][sx] * w[1][sy] * w[2][sz];
+                for (int c = 0; c < 3; ++c)
+                  TO(c, idx[0][sx], idx[1][sy], idx[2][sz]) = TO(c, idx[0][sx], idx[1][sy], idx[2][sz]) + ww * FR(c, i, j, k);
+              }
+        }
+    /* ghost layers added to their periodic images (incl_edges, op add) */
+    for (int d = 0; d < 3; ++d)
+      if (gt->bound_cond[d] == 1) halo_periodic(gt, to, d, 1);
+  } else if (reldx > 0.9999) {
+    double c1d[3];
+    for (int d = 0; d < 3; ++d) c1d[d] = 1.0 / gf->dx[d];
+    for (int k = 1 - nt[2]; k <= gt->ncell[2] + nt[2]; ++k)
+      for (int j = 1 - nt[1]; j <= gt->ncell[1] + nt[1]; ++j)
+        for (int i = 1 - nt[0]; i <= gt->ncell[0] + nt[0]; ++i) {
+          const int ii[3] = {i, j, k};
+          int skip = 0;
+          for (int d = 0; d < 3; ++d)
+            if (gt->bound_cond[d] != 0 && (ii[d] < 1 || ii[d] > gt->ncell[d])) skip = 1;
+          if (skip) continue;
+          double w[3][6];
+          int idx[3][6];
+          for (int d = 0; d < 3; ++d) {
+            double p = gt->xmin[d] + ((double)(ii[d] - 1) + 0.5) * gt->dx[d];
+            int b = or_cell_index(p, gf->xmin[d], c1d[d]);
+            double dd = (p - gf->xmin[d] - gf->dx[d] * ((double)b - 0.5)) * c1d[d], a[6];
+            if (m6) w_m6(dd, a);
+            else w_m4(dd, a);
+            for (int s = 0; s < ns; ++s) {
+              idx[d][s] = b - off + s;
+              w[d][s] = a[s];
+              if (idx[d][s] < 1 - nf[d]) { w[d][s] = 0.0; idx[d][s] = 1 - nf[d]; }
+              if (idx[d][s] > gf->ncell[d] + nf[d]) { w[d][s] = 0.0; idx[d][s] = gf->ncell[d] + nf[d]; }
+            }
+          }
+          for (int c = 0; c < 3; ++c) {
+            double acc = TO(c, i, j, k);
+            for (int sz = 0; sz < ns; ++sz)
+              for (int sy = 0; sy < ns; ++sy)
+                for (int sx = 0; sx < ns; ++sx)
+                  acc = acc + w[0][sx] * w[1][sy] * w[2][sz] * FR(c, idx[0][sx], idx[1][sy], idx[2][sz]);
+            TO(c, i, j, k) = acc;
+          }
+        }
+    /* periodic ghosts copied from their images (op replace) */
+    int E[3] = {Et[0], Et[1], Et[2]};
+    ghost_fill_periodic(gt, E, to);
+  }
+#undef TO
+#undef FR
   return 0;
 }

@@ -18,6 +18,7 @@
 !   m2p      : pmlib_interp_mesh_particle (vel)             (interpolation/)
 !   remesh   : pmlib_remesh                                 (remesh/)
 !   advance  : pmlib_particles_advance                      (particles/)
+!   m2m      : pmlib_topology_parent + pmlib_interp_mesh_mesh, both ways
 !   step     : 2 x (P2M, solve, M2P, advance) + (P2M, remesh), timed; with
 !              reload=.true. every step restarts from the input particles
 !              (timing a fixed particle count); on several ranks each rank
@@ -69,6 +70,8 @@ PROGRAM ref_driver
   CHARACTER(LEN=256) :: outdir = './', tag = 'mesh'
   INTEGER            :: itime = 0
   REAL(MK)           :: time = 0.0_MK
+  ! refinement patch: class_patch level and parent(1) (solve, m2m)
+  INTEGER            :: level = 1, parent_level = 0
   INTEGER            :: npart = 0, stage = 1, nrep = 1, nsteps = 1
   REAL(MK)           :: dtime = 0.0_MK, min_vort = 0.0_MK, min_dvort = 0.0_MK
   CHARACTER(LEN=512) :: infile = 'in.bin', outfile = 'out.bin'
@@ -76,12 +79,14 @@ PROGRAM ref_driver
        & interp_order, rk_order, eps_radius, reg_vort, reproj, calc_vecpot, clear, &
        & incl_ghost, euler_vort, npart, stage, nrep, nsteps, dtime, min_vort, min_dvort, &
        & infile, outfile, reload, penal_relax, penal_tol, vinf, penal_nmax, fd_order, with_solid_vel, &
-       & outdir, tag, itime, time
+       & outdir, tag, itime, time, level, parent_level
 
   TYPE(class_patch)        :: patch
   TYPE(class_topology_all) :: topo_all
   TYPE(class_mesh), SAVE   :: mesh
   TYPE(class_particles)    :: part
+  TYPE(class_topology), DIMENSION(:), POINTER :: topo_parent => NULL()
+  TYPE(class_mesh), SAVE   :: mesh_parent
   INTEGER                  :: ierr, rank, nproc, irep, istep, s, nstage
   INTEGER                  :: lo(3), hi(3), ng6(6)
   REAL(MK), ALLOCATABLE    :: vort_in(:,:,:,:)
@@ -108,9 +113,9 @@ PROGRAM ref_driver
   pmlib_fd_order               = 4
 
   patch%ptype = 1
-  patch%level = 1
+  patch%level = level
   patch%nsubs = nproc
-  patch%parent = (/ 0, 0 /)
+  patch%parent = (/ parent_level, 0 /)
   patch%xmin = xmin
   patch%xmax = xmax
   patch%dx = dx
@@ -303,6 +308,32 @@ PROGRAM ref_driver
     END IF
     IF (nproc .EQ. 1) CALL write_particles()
 
+  CASE ('m2m')
+    ! in: a patch field (3, E...), a parent field (3, E_parent...) and a patch
+    ! field to add onto; out: the patch field deposited on the parent mesh
+    ! (clear), then the parent field interpolated onto the patch field
+    ! (clear as given)
+    CALL pmlib_topology_parent(topo_all%cuboid, topo_parent, ierr)
+    IF (ierr .NE. 0) CALL die('topology parent')
+    CALL pmlib_mesh_allocate(topo_parent, mesh_parent, ierr, vort=.TRUE., dvort=.FALSE., vel=.TRUE., &
+         & mask=.FALSE.)
+    IF (ierr .NE. 0) CALL die('parent mesh allocate')
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) mesh%vort
+    READ(10) mesh_parent%vel
+    READ(10) mesh%vel
+    CLOSE(10)
+    mesh_parent%vort = -1.0_MK
+    CALL pmlib_interp_mesh_mesh(topo_all%cuboid, topo_parent, mesh%vort, mesh_parent%vort, ierr, &
+         & clear=.TRUE.)
+    IF (ierr .NE. 0) CALL die('patch to parent')
+    CALL pmlib_interp_mesh_mesh(topo_parent, topo_all%cuboid, mesh_parent%vel, mesh%vel, ierr, clear=clear)
+    IF (ierr .NE. 0) CALL die('parent to patch')
+    OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(20) mesh_parent%vort
+    WRITE(20) mesh%vel
+    CLOSE(20)
+
   CASE ('vtk')
     ! in: vort, vel (3, E...); out: the .pvti / .vti pair in outdir
     OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
@@ -414,6 +445,8 @@ CONTAINS
     CALL MPI_BCAST(reload, 1, MPI_LOGICAL, 0, MPI_COMM_WORLD, e)
     CALL MPI_BCAST(infile, 512, MPI_CHARACTER, 0, MPI_COMM_WORLD, e)
     CALL MPI_BCAST(outfile, 512, MPI_CHARACTER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(level, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
+    CALL MPI_BCAST(parent_level, 1, MPI_INTEGER, 0, MPI_COMM_WORLD, e)
   END SUBROUTINE bcast_job
 
   ! Reads this rank's block of a global (3, 1-ng:N+ng, ...) field.  The file

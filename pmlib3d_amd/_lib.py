@@ -46,6 +46,10 @@ SIGNATURES = {
     "pmgpu_poisson_create": (C.c_int, [C.POINTER(Geom), C.POINTER(Params), _I, _P, C.POINTER(_P)]),
     "pmgpu_poisson_destroy": (C.c_int, [_P]),
     "pmgpu_poisson_setup": (C.c_int, [_P]),
+    "pmgpu_poisson_set_level": (C.c_int, [_P, _I, _I]),
+    "pmgpu_topology_parent": (C.c_int, [C.POINTER(Geom), _I, _I, C.POINTER(Geom)]),
+    "pmgpu_interp_mesh_mesh": (C.c_int, [C.POINTER(Geom), C.POINTER(Geom), _P, _P, _I]),
+    "pmgpu_interp_mesh_mesh_device": (C.c_int, [C.POINTER(Geom), C.POINTER(Geom), _I, _P, _P, _I, _P]),
     "pmgpu_poisson_solve": (C.c_int, [_P, _P, _P, _P, _P, _I, _I, _I]),
     "pmgpu_poisson_solve_device": (C.c_int, [_P, _P, _P, _P, _P, _I, _I, _I]),
     "pmgpu_poisson_kernel_octant": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int32)]),

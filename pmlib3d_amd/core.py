@@ -140,6 +140,9 @@ class PoissonSolver:
                                                  C.byref(h)), "pmgpu_poisson_create")
         self.h = h
         self.ext = mesh_extent(patch, interp_order)
+        if int(patch.level) != 1:
+            _lib.check(self.lib.pmgpu_poisson_set_level(h, int(patch.level), int(patch.parent[0])),
+                       "pmgpu_poisson_set_level")
 
     def setup(self):
         _lib.check(self.lib.pmgpu_poisson_setup(self.h), "pmgpu_poisson_setup")

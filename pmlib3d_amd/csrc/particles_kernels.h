@@ -133,7 +133,7 @@ struct Kahan {
 // particle lies outside a periodic axis, where the reference zeroes every
 // weight) and its arrival rank in the bin (an atomic count: the order within a
 // bin is restored by particle id when the bin is staged)
-__global__ void k_p2m_keys(PGeom g, const double *pos, NCount npc, int nbins, int *key, int *rank, int *count) {
+static __global__ void k_p2m_keys(PGeom g, const double *pos, NCount npc, int nbins, int *key, int *rank, int *count) {
   const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     int b[3];
@@ -156,7 +156,7 @@ __global__ void k_p2m_keys(PGeom g, const double *pos, NCount npc, int nbins, in
 }
 
 // counting-sort scatter: particle q to slot start[key] + rank of its bin
-__global__ void k_p2m_scatter(NCount npc, int nbins, const int *key, const int *rank, const int *start, int *ids) {
+static __global__ void k_p2m_scatter(NCount npc, int nbins, const int *key, const int *rank, const int *start, int *ids) {
   const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     const int k = key[q];
@@ -210,7 +210,7 @@ template <typename T, int NS> struct PRec {
 
 // One thread per bin: order the bin's particles by id (the order a stable
 // sort by bin would give); bins of one particle, the usual case, do nothing.
-__global__ void __launch_bounds__(256) k_p2m_binsort(const int *start, int nbins, int *ids) {
+static __global__ void __launch_bounds__(256) k_p2m_binsort(const int *start, int nbins, int *ids) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nbins; k += gridDim.x * blockDim.x) {
     const int t0 = start[k], n = start[k + 1] - t0;
     if (n > 1) sort_ids(ids + t0, n);
@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const doubl
   }
 }
 
-__global__ void k_cell_index(PGeom g, const double *pos, int64_t np, int *out) {
+static __global__ void k_cell_index(PGeom g, const double *pos, int64_t np, int *out) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < 3 * np; q += (int64_t)gridDim.x * blockDim.x) {
     int d = q % 3;
     out[q] = cell_index(pos[q], g.xmin[d], g.c1dx[d]);
@@ -587,7 +587,7 @@ struct RkCoef {
   double a1, b1, c1, a2, b2, c2;
 };
 
-__global__ void k_rk_update(RkCoef k, double dt, NCount npc, double *pos, const double *vel, double *vort,
+static __global__ void k_rk_update(RkCoef k, double dt, NCount npc, double *pos, const double *vel, double *vort,
                             const double *dvort, double *vel_rk, double *dvort_rk) {
   const int64_t np = npc.get();
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * np; i += (int64_t)gridDim.x * blockDim.x) {
@@ -604,7 +604,7 @@ __global__ void k_rk_update(RkCoef k, double dt, NCount npc, double *pos, const 
 struct PBox {
   double lo[3], hi[3];
 };
-__global__ void k_any_outside(NCount npc, const double *pos, PBox b, int *flag) {
+static __global__ void k_any_outside(NCount npc, const double *pos, PBox b, int *flag) {
   const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     bool out = false;
@@ -618,7 +618,7 @@ __global__ void k_any_outside(NCount npc, const double *pos, PBox b, int *flag) 
 }
 
 // migration class along axis d: 0 stay, 1 west (wrap), 2 east (wrap), 3 drop
-__global__ void k_classify(NCount npc, const double *pos, int d, double lo, double hi, int per, int *cls,
+static __global__ void k_classify(NCount npc, const double *pos, int d, double lo, double hi, int per, int *cls,
                            int *counts) {
   const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {

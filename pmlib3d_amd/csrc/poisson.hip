@@ -144,6 +144,9 @@ struct PoissonBase {
   virtual int precision() const = 0;
   pmgpu_geom g;
   pmgpu_params prm;
+  // a = 2^(level - parent level) of a refinement patch (0: base level); the
+  // kernel becomes G(eps) - G(a eps) at the next setup
+  double refine = 0.0;
   hipStream_t stream = nullptr;
   // HIP-event timing of the fused z pass and of whole solves, recorded on
   // the handle's stream without host synchronisation; read back on demand
@@ -320,6 +323,9 @@ template <typename R> struct PoissonGPU : PoissonBase {
 
   // ---- pmlib_poisson_setup ---------------------------------------------
   int setup() override {
+    PM_CHECK(refine == 0.0 || prm.poisson_order != 0,
+             "pmgpu_poisson_setup: Regularisation must be used for multi-patch simulations "
+             "(poisson/pmlib_poisson_setup.f:276-280)");
     if (allper) return 0;  // G = zeta / (4 pi^2 k^2) is formed in the z pass
     if (ker2) return setup_kernel2();
     // FP64 kernel grid and its transform (x R2C, y and z C2C), then the octant
@@ -335,6 +341,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     dmax = dmax > g.dx[2] ? dmax : g.dx[2];
     ga.eps = prm.regularisation_radius * dmax;
     ga.order = prm.poisson_order;
+    ga.ca = refine > 0.0 ? 1.0 / refine : 0.0;
     for (int d = 0; d < 3; ++d) {
       ga.L[d] = L[d];
       ga.bc[d] = g.bound_cond[d];
@@ -399,6 +406,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     dmax = dmax > g.dx[2] ? dmax : g.dx[2];
     ga.eps = prm.regularisation_radius * dmax;
     ga.order = prm.poisson_order;
+    ga.ca = refine > 0.0 ? 1.0 / refine : 0.0;
     for (int d = 0; d < 3; ++d) {
       ga.L[d] = L[d];
       ga.bc[d] = g.bound_cond[d];
@@ -877,6 +885,18 @@ int pmgpu_poisson_destroy(pmgpu_poisson_t h) {
 int pmgpu_poisson_setup(pmgpu_poisson_t h) {
   PM_CHECK(h, "pmgpu_poisson_setup: null handle");
   return h->impl->setup();
+}
+
+int pmgpu_poisson_set_level(pmgpu_poisson_t h, int32_t level, int32_t parent_level) {
+  PM_CHECK(h, "pmgpu_poisson_set_level: null handle");
+  if (level == 1) {
+    h->impl->refine = 0.0;
+    return 0;
+  }
+  PM_CHECK(level - parent_level >= 0 && level - parent_level <= 30,
+           "pmgpu_poisson_set_level: the patch level must be 0 to 30 levels above its parent");
+  h->impl->refine = std::ldexp(1.0, level - parent_level);
+  return 0;
 }
 
 int pmgpu_poisson_solve(pmgpu_poisson_t h, const double *vort, double *vel, double *vort_out, double *vecpot,

@@ -377,7 +377,76 @@ struct GreenArgs {
   double dx[3], xk[3], len[3];
   double eps;
   int order;
+  double ca;         // refinement patch: 1 / a, a = 2^(level - parent level); 0 on a base patch
 };
+
+// poly(rho) E(rho) + F(rho) of the regularised kernels (order > 0):
+// kernel 1 G = (poly E + F) / (4 pi r), kernel 2 grad G = -x (poly E + F) / (4 pi r^3)
+__device__ __forceinline__ double green_part(int kernel, int order, double rho) {
+  const double c_1_sqrt2 = 0.70710678118654752440;
+  const double c_1_sqrt2pi = 0.39894228040143267794;
+  const double r2 = rho * rho, r3 = r2 * rho, r5 = r3 * r2, r7 = r5 * r2, r9 = r7 * r2;
+  double poly;
+  if (kernel == 1) {
+    switch (order) {
+      case 2: poly = 0.0; break;
+      case 4: poly = rho; break;
+      case 6: poly = 1.75 * rho - 0.25 * r3; break;
+      case 8: poly = 2.375 * rho + (-2.0 / 3.0) * r3 + (1.0 / 24.0) * r5; break;
+      default: poly = 2.921875 * rho + (-233.0 / 192.0) * r3 + (29.0 / 192.0) * r5 + (-1.0 / 192.0) * r7; break;
+    }
+  } else {
+    switch (order) {
+      case 2: poly = -2.0 * rho; break;
+      case 4: poly = -2.0 * rho + r3; break;
+      case 6: poly = -2.0 * rho + 2.25 * r3 - 0.25 * r5; break;
+      case 8: poly = -2.0 * rho + (89.0 / 24.0) * r3 + (-20.0 / 24.0) * r5 + (1.0 / 24.0) * r7; break;
+      default:
+        poly = -2.0 * rho + (1027.0 / 192.0) * r3 + (-349.0 / 192.0) * r5 + (35.0 / 192.0) * r7 +
+               (-1.0 / 192.0) * r9;
+        break;
+    }
+  }
+  return poly * (exp(-0.5 * (rho * rho)) * c_1_sqrt2pi) + erf(c_1_sqrt2 * rho);
+}
+
+// G(0) / (sqrt(2) / (4 pi^1.5 eps)) of the kernel-1 orders 2..10
+__device__ __forceinline__ double green_origin(int order) {
+  switch (order) {
+    case 2: return 1.0;
+    case 4: return 1.5;
+    case 6: return 1.875;
+    case 8: return 2.1875;
+    default: return 2.4609375;
+  }
+}
+
+// Refinement-patch kernels (level > 1, poisson/pmlib_poisson_setup.f:446-528
+// (order 2), :640-731 (4), :840-950 (6), :1060-1180 (8), :1300-1420 (10)):
+// the kernel regularised at eps minus the one regularised at a eps, i.e.
+// part(rho) - part(rho / a), at the grid point alone.  The reference adds the
+// image offsets ii, jj, kk there outside the loops that set them; they are
+// taken as 0 (no images).  Kernel 1 at r = 0: c (1 - 1/a) sqrt(2) / (4 pi^1.5
+// eps); kernel 2 at r = 0: 0.
+__device__ __forceinline__ void green_refine(const GreenArgs &a, int kernel, const double base[3], double *out) {
+  const double pi = 3.14159265358979323846;
+  const double c_1_4pi = 0.25 / pi;
+  const double r = sqrt(base[0] * base[0] + base[1] * base[1] + base[2] * base[2]);
+  if (r == 0.0) {
+    if (kernel == 1) out[0] = green_origin(a.order) * (1.41421356237309504880 / (4.0 * pow(pi, 1.5) * a.eps)) *
+                              (1.0 - a.ca);
+    else out[0] = out[1] = out[2] = 0.0;
+    return;
+  }
+  const double rho = r / a.eps;
+  const double f = green_part(kernel, a.order, rho) - green_part(kernel, a.order, rho * a.ca);
+  if (kernel == 1) {
+    out[0] = c_1_4pi / r * f;
+  } else {
+    const double s = c_1_4pi / (r * r * r) * f;
+    for (int c = 0; c < 3; ++c) out[c] = -base[c] * s;
+  }
+}
 
 static __global__ void k_green(GreenArgs a) {
   const int64_t n = (int64_t)a.L[0] * a.L[1] * a.L[2];
@@ -397,6 +466,10 @@ static __global__ void k_green(GreenArgs a) {
       int m = nn[d] < a.L[d] / 2 ? nn[d] : nn[d] - a.L[d];
       int i1 = m + a.L[d] / 2 + 1;
       base[d] = a.xk[d] + ((double)(i1 - 1) + 0.5) * a.dx[d];
+    }
+    if (a.ca > 0.0) {
+      green_refine(a, 1, base, a.out + idx);
+      continue;
     }
     double acc = 0.0;
     for (int kk = -a.bc[2]; kk <= a.bc[2]; ++kk)
@@ -463,6 +536,10 @@ static __global__ void k_green2(GreenArgs a) {
       int m = nn[d] < a.L[d] / 2 ? nn[d] : nn[d] - a.L[d];
       int i1 = m + a.L[d] / 2 + 1;
       base[d] = a.xk[d] + ((double)(i1 - 1) + 0.5) * a.dx[d];
+    }
+    if (a.ca > 0.0) {
+      green_refine(a, 2, base, a.out + idx * 3);
+      continue;
     }
     double acc[3] = {0.0, 0.0, 0.0};
     for (int kk = -a.bc[2]; kk <= a.bc[2]; ++kk)

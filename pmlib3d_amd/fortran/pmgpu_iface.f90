@@ -81,6 +81,21 @@ MODULE pmgpu_iface
       TYPE(C_PTR), VALUE :: handle
     END FUNCTION pmgpu_poisson_setup
 
+    INTEGER(C_INT) FUNCTION pmgpu_poisson_set_level(handle, level, parent_level) &
+        & BIND(C, NAME='pmgpu_poisson_set_level')
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: handle
+      INTEGER(C_INT32_T), VALUE :: level, parent_level
+    END FUNCTION pmgpu_poisson_set_level
+
+    INTEGER(C_INT) FUNCTION pmgpu_interp_mesh_mesh(g_from, g_to, from, to, clear) &
+        & BIND(C, NAME='pmgpu_interp_mesh_mesh')
+      IMPORT :: C_INT, C_INT32_T, C_PTR, pmgpu_geom
+      TYPE(pmgpu_geom), INTENT(IN) :: g_from, g_to
+      TYPE(C_PTR), VALUE :: from, to
+      INTEGER(C_INT32_T), VALUE :: clear
+    END FUNCTION pmgpu_interp_mesh_mesh
+
     INTEGER(C_INT) FUNCTION pmgpu_poisson_solve(handle, vort, vel, vort_out, vecpot, reg_vort, &
         & reproj, calc_vecpot) BIND(C, NAME='pmgpu_poisson_solve')
       IMPORT :: C_INT, C_INT32_T, C_PTR

@@ -43,12 +43,13 @@ MODULE pmgpu_pmlib
 
   PUBLIC :: pmlib_poisson_setup, pmlib_poisson_solve, pmlib_interp_particle_mesh, &
           & pmlib_interp_mesh_particle, pmlib_remesh, pmlib_particles_advance, pmlib_penalise, &
-          & pmgpu_release, pmgpu_patch_solver, mesh_ptr, geom_of, pmlib_output_mesh
+          & pmgpu_release, pmgpu_patch_solver, mesh_ptr, geom_of, pmlib_output_mesh, pmlib_interp_mesh_mesh
 
-  ! solver handles, one per (geometry, order, kernel, radius)
+  ! solver handles, one per (geometry, order, kernel, radius, refinement level)
   TYPE solver_entry
     TYPE(pmgpu_geom)   :: g
     TYPE(pmgpu_params) :: p
+    INTEGER            :: level = 1, parent = 0
     TYPE(C_PTR)        :: h = C_NULL_PTR
   END TYPE solver_entry
   TYPE(solver_entry), DIMENSION(64), SAVE :: solvers
@@ -178,12 +179,14 @@ CONTAINS
     a => b
   END SUBROUTINE shrink_part
 
-  INTEGER FUNCTION find_solver(g, p)
+  INTEGER FUNCTION find_solver(g, p, level, parent)
     TYPE(pmgpu_geom), INTENT(IN)   :: g
     TYPE(pmgpu_params), INTENT(IN) :: p
+    INTEGER, INTENT(IN)            :: level, parent
     INTEGER :: i
     find_solver = 0
     DO i = 1, nsolvers
+      IF (solvers(i)%level .NE. level .OR. (level .NE. 1 .AND. solvers(i)%parent .NE. parent)) CYCLE
       IF (ALL(solvers(i)%g%ncell .EQ. g%ncell) .AND. solvers(i)%g%nghost .EQ. g%nghost .AND. &
         & ALL(solvers(i)%g%bound_cond .EQ. g%bound_cond) .AND. &
         & solvers(i)%g%interp_order .EQ. g%interp_order .AND. &
@@ -237,7 +240,7 @@ CONTAINS
     IF (ierr .NE. 0) RETURN
     g%nghost = patch%nghost
     CALL current_params(p)
-    i = find_solver(g, p)
+    i = find_solver(g, p, patch%level, patch%parent(1))
     fresh = i .EQ. 0
     IF (fresh) THEN
       IF (nsolvers .EQ. SIZE(solvers)) THEN
@@ -249,8 +252,17 @@ CONTAINS
       i = nsolvers
       solvers(i)%g = g
       solvers(i)%p = p
+      solvers(i)%level = patch%level
+      solvers(i)%parent = patch%parent(1)
       IF (pmgpu_poisson_create(g, p, PMGPU_FP64, C_NULL_PTR, solvers(i)%h) .NE. 0) THEN
         nsolvers = nsolvers - 1
+        CALL fail(caller, ierr)
+        RETURN
+      END IF
+      ! a refinement patch solves with the difference kernel
+      ! (poisson/pmlib_poisson_setup.f:446-528)
+      IF (pmgpu_poisson_set_level(solvers(i)%h, INT(patch%level, C_INT32_T), &
+        & INT(patch%parent(1), C_INT32_T)) .NE. 0) THEN
         CALL fail(caller, ierr)
         RETURN
       END IF
@@ -297,7 +309,7 @@ CONTAINS
     IF (ierr .NE. 0) RETURN
     g%nghost = patch%nghost
     CALL current_params(p)
-    i = find_solver(g, p)
+    i = find_solver(g, p, patch%level, patch%parent(1))
     IF (i .EQ. 0) THEN
       ierr = -1
       CALL pmlib_write(mpi_rank, caller, 'pmlib_poisson_setup has not been called for this patch')
@@ -401,6 +413,40 @@ CONTAINS
       END IF
     END IF
   END SUBROUTINE pmlib_interp_mesh_particle
+
+!------------------------------------------------------------------ multiresolution
+  ! pmlib_interp_mesh_mesh(topo_from, topo_to, mesh_from, mesh_to, ierr, clear)
+  ! (interpolation/pmlib_interp_mesh_mesh.f:5-2078) on one rank: between a
+  ! patch mesh and its parent's (topologies from pmlib_topology_parent)
+  SUBROUTINE pmlib_interp_mesh_mesh(topo_from, topo_to, mesh_from, mesh_to, ierr, clear)
+    TYPE(class_topology), DIMENSION(:), POINTER, INTENT(IN) :: topo_from, topo_to
+    REAL(MK), DIMENSION(:,:,:,:), POINTER                   :: mesh_from, mesh_to
+    INTEGER, INTENT(OUT)                                    :: ierr
+    LOGICAL, OPTIONAL, INTENT(IN)                           :: clear
+    CHARACTER(LEN=22) :: caller = 'pmlib_interp_mesh_mesh'
+    TYPE(pmgpu_geom) :: gf, gt
+    ierr = 0
+    IF (SIZE(mesh_from, 1) .NE. SIZE(mesh_to, 1)) THEN
+      ierr = -1
+      CALL pmlib_write(mpi_rank, caller, 'Dimension of interpolation variables does not correspond.')
+      RETURN
+    END IF
+    IF (SIZE(mesh_from, 1) .NE. 3) THEN
+      ierr = -1
+      CALL pmlib_write(mpi_rank, caller, 'libpmgpu interpolates (3, ...) meshes')
+      RETURN
+    END IF
+    IF (.NOT. ensure_init(caller)) THEN
+      ierr = -1
+      RETURN
+    END IF
+    CALL geom_of(topo_from(mpi_rank), gf, ierr, caller)
+    IF (ierr .NE. 0) RETURN
+    CALL geom_of(topo_to(mpi_rank), gt, ierr, caller)
+    IF (ierr .NE. 0) RETURN
+    IF (pmgpu_interp_mesh_mesh(gf, gt, mesh_ptr(mesh_from), mesh_ptr(mesh_to), ci(flag(clear))) .NE. 0) &
+      & CALL fail(caller, ierr)
+  END SUBROUTINE pmlib_interp_mesh_mesh
 
 !------------------------------------------------------------------ remesh
   SUBROUTINE pmlib_remesh(patch, topo, mesh, part, min_vort, min_dvort, ierr, dvort, vel, mask, &

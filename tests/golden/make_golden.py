@@ -12,7 +12,10 @@ Outputs (small, committed):
   ref_step_*.npz         inputs + reference outputs of whole two-ring vortex steps
   ref_vtk_*.npz          inputs + the .pvti / .vti bytes of pmlib_output_mesh
   ref_checkpoint_*.npz   inputs + the file bytes of output_particles
-(`python tests/golden/make_golden.py formats` rewrites only the last two.)
+  ref_refine_*.npz       inputs + velocities of refinement-patch solves
+  ref_m2m_*.npz          inputs + both directions of pmlib_interp_mesh_mesh
+(`python tests/golden/make_golden.py formats` rewrites only the vtk and
+checkpoint fixtures, `... multires` only the last two kinds.)
 """
 import json
 import os
@@ -194,6 +197,58 @@ def checkpoint_case(name, npart, itime, time, dtime, seed):
                         dat=np.frombuffer(dat, dtype=np.uint8))
 
 
+# refinement patches (level 2 on a level-1 parent): the difference kernels
+# of poisson/pmlib_poisson_setup.f:446-528 and the other orders' blocks
+REFINE_CASES = {
+    # name: (ncell, nghost, bound_cond, order, kernel, level, parent)
+    "free_o4_k1": ((12, 10, 8), 0, (0, 0, 0), 4, 1, 2, 1),
+    "free_ng2_o6_k2": ((10, 12, 8), 2, (0, 0, 0), 6, 2, 2, 1),
+    "mixed_o2_k1": ((8, 8, 12), 2, (1, 1, 0), 2, 1, 2, 1),
+    "free_ng1_o10_k1_l3": ((12, 8, 10), 1, (0, 0, 0), 10, 1, 3, 1),
+    "mixed_o8_k2": ((8, 10, 12), 0, (0, 1, 0), 8, 2, 2, 1),
+}
+
+# mesh-to-mesh between a patch and its parent (pmlib_topology_parent +
+# pmlib_interp_mesh_mesh both ways)
+M2M_CASES = {
+    # name: (ncell, nghost, bound_cond, interp_order, level, clear)
+    "m4_free": ((12, 8, 8), 2, (0, 0, 0), 3, 2, True),
+    "m6_mixed": ((12, 12, 8), 3, (1, 1, 0), 4, 2, False),
+    "m4_periodic": ((8, 8, 12), 0, (1, 1, 1), 3, 2, True),
+    "m6_ratio4": ((16, 12, 12), 3, (0, 0, 0), 4, 3, True),
+    "m4_ratio4_mixed": ((8, 16, 8), 1, (0, 1, 0), 3, 3, False),
+}
+
+
+def refine_case(name, spec):
+    ncell, ng, bc, order, kernel, level, parent = spec
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+    p = O.make_params(order, kernel, 2.0)
+    w = blob_field(g, seed=sum(ncell) + order + level)
+    job = dict(O.geom_job(g, p), mode="solve", level=level, parent_level=parent)
+    raw, _ = O.ref_run(job, [w])
+    vel = np.frombuffer(raw, dtype=np.float64).reshape(w.shape, order="F")
+    np.savez_compressed(os.path.join(HERE, "ref_refine_%s.npz" % name), vort_in=w, vel=vel,
+                        spec=np.array(json.dumps(spec)))
+
+
+def m2m_case(name, spec, seed):
+    ncell, ng, bc, io, level, clear = spec
+    rng = np.random.default_rng(seed)
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc, io)
+    gp = O.topology_parent(g, level, 1)
+    fine = np.asfortranarray(rng.normal(size=(3,) + O.mesh_extent(g)))
+    coarse = np.asfortranarray(rng.normal(size=(3,) + O.mesh_extent(gp)))
+    base = np.asfortranarray(rng.normal(size=(3,) + O.mesh_extent(g)))
+    job = dict(O.geom_job(g), mode="m2m", level=level, parent_level=1, interp_order=io, clear=clear)
+    raw, _ = O.ref_run(job, [fine, coarse, base])
+    r = np.frombuffer(raw, dtype=np.float64)
+    down = r[:coarse.size].reshape(coarse.shape, order="F")
+    up = r[coarse.size:].reshape(base.shape, order="F")
+    np.savez_compressed(os.path.join(HERE, "ref_m2m_%s.npz" % name), fine=fine, coarse=coarse, base=base,
+                        down=down, up=up, parent_ncell=np.array(gp.ncell), spec=np.array(json.dumps(spec)))
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref is not built: make -f oracle/Makefile.ref")
@@ -201,6 +256,12 @@ if __name__ == "__main__":
         vtk_case("free_ng2", (5, 4, 3), 2, (0, 0, 0), [-0.5, -0.37, 0.1234], 1.0 / 7.0, 51)
         vtk_case("periodic", (6, 3, 4), 0, (1, 1, 1), [0.0, -1.0, -2.5e-3], 0.03125, 52)
         checkpoint_case("mixed", 37, 41, 0.37, 0.005, 53)
+        sys.exit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "multires":
+        for name, spec in REFINE_CASES.items():
+            refine_case(name, spec)
+        for q, (name, spec) in enumerate(M2M_CASES.items()):
+            m2m_case(name, spec, 60 + q)
         sys.exit(0)
     if len(sys.argv) > 1:   # only the named solve cases
         for name in sys.argv[1:]:
@@ -210,6 +271,10 @@ if __name__ == "__main__":
         solve_case(name, spec)
     for name, spec in STEP_CASES.items():
         step_case(name, spec)
+    for name, spec in REFINE_CASES.items():
+        refine_case(name, spec)
+    for q, (name, spec) in enumerate(M2M_CASES.items()):
+        m2m_case(name, spec, 60 + q)
     particle_case("m4_free", (10, 8, 12), 0, (0, 0, 0), 3, 11)
     particle_case("m4_ng2", (8, 10, 8), 2, (0, 0, 0), 3, 12)
     particle_case("m4_mixed", (8, 8, 10), 0, (1, 1, 0), 3, 13)

@@ -135,3 +135,37 @@ def test_shim_penalisation_solve_matches_oracle(case):
     assert iters == rn
     assert np.abs(force - rf).max() <= 1e-12 * np.abs(rf).max()
     assert rel_linf(gv, rv) <= 1e-12 and rel_linf(gw, rw) <= 1e-12
+
+
+REFINES = sorted(f for f in os.listdir(GOLD) if f.startswith("ref_refine_"))
+
+
+@pytest.mark.parametrize("name", REFINES)
+def test_shim_refinement_solve_matches_reference(name):
+    """pmlib_poisson_setup / pmlib_poisson_solve of the drop-in on a patch with
+    level != 1 (the solver handle gets the patch's level and parent)."""
+    d = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    ncell, ng, bc, order, kernel, level, parent = json.loads(str(d["spec"]))
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+    p = O.make_params(order, kernel, 2.0)
+    raw, _ = run(dict(O.geom_job(g, p), mode="solve", level=level, parent_level=parent), [d["vort_in"]])
+    out = np.frombuffer(raw, dtype=np.float64)[:d["vort_in"].size].reshape(d["vort_in"].shape, order="F")
+    assert rel_linf(out, d["vel"]) <= 1e-12
+
+
+M2MS = sorted(f for f in os.listdir(GOLD) if f.startswith("ref_m2m_"))
+
+
+@pytest.mark.parametrize("name", M2MS)
+def test_shim_m2m_matches_reference_bitwise(name):
+    """pmlib_interp_mesh_mesh of the drop-in, between the harness's patch and
+    the reference's pmlib_topology_parent of it, both directions."""
+    d = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    ncell, ng, bc, io, level, clear = json.loads(str(d["spec"]))
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc, io)
+    raw, _ = run(dict(O.geom_job(g), mode="m2m", level=level, parent_level=1, interp_order=io, clear=clear),
+                 [d["fine"], d["coarse"], d["base"]])
+    r = np.frombuffer(raw, dtype=np.float64)
+    nb = d["coarse"].size
+    assert np.array_equal(r[:nb].reshape(d["coarse"].shape, order="F"), d["down"])
+    assert np.array_equal(r[nb:].reshape(d["base"].shape, order="F"), d["up"])

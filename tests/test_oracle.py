@@ -646,3 +646,132 @@ def test_penalisation_iterations_reduce_the_slip_and_stop_at_the_tolerance():
     vf, _, _, _, _ = O.penalisation_solve(g, p, Gh, vel, vort, chi, max_iter=1, vel_inf=(0.0, 0.0, 2.5),
                                           fix_free_stream=True)
     assert np.allclose(vf[2] - v0[2], 2.5, rtol=0, atol=1e-13) and np.array_equal(vf[1], v0[1])
+
+
+# ---------------------------------------------------------------------------
+# Multiresolution: refinement-patch kernels, parent topology, mesh-to-mesh
+# (poisson/pmlib_poisson_setup.f:446-528 and the other orders' refinement
+# blocks, topology/pmlib_topology_parent.f:5-100,
+# interpolation/pmlib_interp_mesh_mesh.f:5-2078)
+
+def _refine_setup(spec):
+    ncell, ng, bc, order, kernel, level, parent = spec
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+    return g, O.make_params(order, kernel, 2.0, refine=2.0 ** (level - parent))
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "ref_refine_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[11:-4])
+def test_oracle_refinement_solve_matches_reference_golden(path):
+    d = np.load(path, allow_pickle=False)
+    g, p = _refine_setup(json.loads(str(d["spec"])))
+    vel, _, _ = O.poisson_solve(g, p, O.poisson_setup(g, p), d["vort_in"])
+    assert rel_linf(vel, d["vel"]) <= 2e-15
+
+
+@pytest.mark.parametrize("order,kernel", [(2, 1), (4, 1), (6, 2), (8, 1), (10, 2)])
+def test_refinement_kernel_is_the_difference_of_two_regularisations(order, kernel):
+    """G_refine(eps, a) = G(eps) - G(a eps) on a free-space kernel grid: the
+    base kernel built with the regularisation radius scaled by a, including
+    the r = 0 values (1 - 1/a) G(0) of kernel 1."""
+    g = O.make_geom((8, 6, 10), 1, [-0.3, -0.2, -0.4], 0.05)
+    a = 4.0
+    ref = O.green_real(g, O.make_params(order, kernel, 2.0, refine=a))
+    g1 = O.green_real(g, O.make_params(order, kernel, 2.0))
+    ga = O.green_real(g, O.make_params(order, kernel, 2.0 * a))
+    diff = g1 - ga
+    assert np.abs(ref - diff).max() <= 1e-13 * np.abs(g1).max()
+    # a = 1: the two regularisations cancel
+    assert np.abs(O.green_real(g, O.make_params(order, kernel, 2.0, refine=1.0))).max() == 0.0
+
+
+def test_refinement_kernel_rejects_order_0():
+    g = O.make_geom((8, 8, 8), 0, [-0.5] * 3, 1.0 / 8)
+    with pytest.raises(ValueError):
+        O.green_real(g, O.make_params(0, 1, 2.0, refine=2.0))
+
+
+def test_topology_parent_geometry():
+    g = O.make_geom((16, 12, 8), 2, [-0.5, -0.6, -0.7], 1.0 / 8, (0, 1, 0), 4)
+    gp = O.topology_parent(g, 3, 1)
+    assert tuple(gp.ncell) == (4, 3, 2) and gp.dx[0] == 0.5 and gp.nghost == 3
+    assert tuple(gp.bound_cond) == (0, 1, 0) and tuple(gp.xmin) == tuple(g.xmin)
+    assert O.topology_parent(O.make_geom((16, 12, 8), 0, [0.0] * 3, 0.1), 2, 1).nghost == 2
+    with pytest.raises(ValueError):
+        O.topology_parent(O.make_geom((16, 10, 8), 0, [0.0] * 3, 0.1), 3, 1)   # 10 % 4 != 0
+    with pytest.raises(ValueError):
+        O.topology_parent(g, 2, 0)
+
+
+def _m2m_geoms(spec):
+    ncell, ng, bc, io, level, clear = spec
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc, io)
+    return g, O.topology_parent(g, level, 1), clear
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "ref_m2m_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[8:-4])
+def test_oracle_m2m_matches_reference_golden_bitwise(path):
+    d = np.load(path, allow_pickle=False)
+    g, gp, clear = _m2m_geoms(json.loads(str(d["spec"])))
+    assert tuple(gp.ncell) == tuple(d["parent_ncell"])
+    np.testing.assert_array_equal(O.interp_mesh_mesh(g, gp, d["fine"], clear=True), d["down"])
+    np.testing.assert_array_equal(O.interp_mesh_mesh(gp, g, d["coarse"], d["base"], clear=clear), d["up"])
+
+
+def _interior_blob(g, seed, margin):
+    """A random field that vanishes within `margin` cells of every mesh edge."""
+    rng = np.random.default_rng(seed)
+    E = O.mesh_extent(g)
+    f = np.zeros((3,) + E, order="F")
+    sl = tuple(slice(margin, e - margin) for e in E)
+    f[(slice(None),) + sl] = rng.normal(size=(3,) + tuple(e - 2 * margin for e in E))
+    return np.asfortranarray(f)
+
+
+@pytest.mark.parametrize("io,level", [(3, 2), (4, 2), (3, 3), (4, 3)])
+def test_m2m_deposit_is_the_scaled_adjoint_of_the_interpolation(io, level):
+    """<D f, c>_parent = reldx^3 <f, I c>_patch in free space when no stencil
+    is clamped (f vanishes near the patch edges, every target of I is
+    computed): the deposit weights reldx W(d) and the interpolation weights
+    W(d) are the same kernel of the same offset, so a transposed operand, a
+    wrong stencil slot or index fails it.  The deposit also keeps the
+    total: sum D f = reldx^3 sum f."""
+    g = O.make_geom((16, 16, 16), 3, [-0.5, -0.6, -0.7], 1.0 / 16, (0, 0, 0), io)
+    gp = O.topology_parent(g, level, 1)
+    reldx = g.dx[0] / gp.dx[0]
+    f = _interior_blob(g, 31 + io, 6)
+    c = np.asfortranarray(np.random.default_rng(41 + level).normal(size=(3,) + O.mesh_extent(gp)))
+    Df = O.interp_mesh_mesh(g, gp, f, clear=True)
+    Ic = O.interp_mesh_mesh(gp, g, c, clear=True)
+    lhs = (Df * c).sum()
+    rhs = reldx ** 3 * (f * Ic).sum()
+    assert abs(lhs - rhs) <= 1e-12 * (np.abs(Df * c).sum())
+    assert abs(Df.sum() - reldx ** 3 * f.sum()) <= 1e-12 * np.abs(f).sum()
+
+
+@pytest.mark.parametrize("io,deg", [(3, 2), (4, 3)])
+def test_m2m_interpolation_reproduces_polynomials(io, deg):
+    """Parent -> patch reproduces polynomials up to the kernel's degree
+    (M'4: quadratics, M6*: cubics) wherever the stencil lies inside the
+    parent mesh."""
+    g = O.make_geom((16, 16, 16), 2, [-0.5, -0.6, -0.7], 1.0 / 16, (0, 0, 0), io)
+    gp = O.topology_parent(g, 2, 1)
+
+    def field(gg):
+        E = O.mesh_extent(gg)
+        x = [gg.xmin[d] + (np.arange(E[d]) - O.ng_side(gg, d) + 0.5) * gg.dx[d] for d in range(3)]
+        X, Y, Z = np.meshgrid(*x, indexing="ij")
+        p = 0.3 + X - 2 * Y + 0.5 * Z + X * Y - Z * Z
+        if deg == 3:
+            p = p + 0.7 * X ** 3 - Y * Z * Z
+        return np.asfortranarray(np.stack([p, 2 * p, -p]))
+
+    got = O.interp_mesh_mesh(gp, g, field(gp), clear=True)
+    want = field(g)
+    ngp, ng = O.ng_side(gp, 0), O.ng_side(g, 0)
+    # patch cells whose parent stencil stays inside the parent mesh: those
+    # within the patch interior, two parent cells from the parent's edge
+    sl = slice(ng + 4, O.mesh_extent(g)[0] - ng - 4)
+    assert np.abs(got[:, sl, sl, sl] - want[:, sl, sl, sl]).max() <= 1e-12 * np.abs(want).max()
+    assert ngp >= 2
