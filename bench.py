@@ -636,6 +636,56 @@ def diagnostics_benchmark(reps, n=128):
 Z_KERNEL = "k_z3_r<float, 512, 4, false, true>"
 
 
+def multires_benchmark(reps, n=256):
+    """Multiresolution coupling on a 256^3 free refinement patch (level 2, two
+    ghost layers, M'4) and its 128^3 parent: patch -> parent deposit (clear)
+    and parent -> patch interpolation (added onto the patch field), FP64 and
+    FP32, HIP events over `reps` launches on the null stream.  Algorithmic
+    bytes: deposit 3 E_f^3 s read + 3 E_c^3 s written; interpolation 3 E_c^3 s
+    read + 3 E_f^3 s read and written."""
+    import numpy as np
+    import torch
+    import pmlib3d_amd as pm
+    h = 2.0 / n
+    patch = pm.Patch(xmin=(-1.0,) * 3, xmax=(1.0,) * 3, ncell=(n,) * 3, dx=(h,) * 3, nghost=2, level=2,
+                     parent=(1, 0))
+    parent = pm.pmlib_topology_parent(patch, interp_order=3)
+    ef, ec = pm.mesh_extent(patch, 3), pm.mesh_extent(parent, 3)
+    nf, nc = 3 * int(np.prod(ef)), 3 * int(np.prod(ec))
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(11)
+    res = {"workload": "256^3 free refinement patch (2 ghost layers) and its 128^3 parent, M'4, "
+                       "seeded normal fields"}
+    for name, prec, dt, sz in (("fp64", pm.FP64, torch.float64, 8), ("fp32", pm.FP32, torch.float32, 4)):
+        fine = torch.randn(nf, dtype=dt, device=dev, generator=g)
+        coarse = torch.randn(nc, dtype=dt, device=dev, generator=g)
+        legs = {"deposit": (lambda: pm.interp_mesh_mesh_device(patch, parent, prec, fine.data_ptr(),
+                                                               coarse.data_ptr(), True, interp_order=3),
+                            (nf + nc) * sz),
+                "interpolate": (lambda: pm.interp_mesh_mesh_device(parent, patch, prec, coarse.data_ptr(),
+                                                                   fine.data_ptr(), False, interp_order=3),
+                                (nc + 2 * nf) * sz)}
+        res[name] = {}
+        for leg, (run, nbytes) in legs.items():
+            for _ in range(2):
+                run()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                run()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            res[name][leg] = {"ms": ms, "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / ms / 1e6,
+                              "frac_hbm": nbytes / ms / 1e6 / PEAK_HBM_GBS}
+        del fine, coarse
+    log("multires 256^3 <-> 128^3: deposit fp64 %.3f / fp32 %.3f ms, interpolation fp64 %.3f / fp32 %.3f ms" %
+        (res["fp64"]["deposit"]["ms"], res["fp32"]["deposit"]["ms"], res["fp64"]["interpolate"]["ms"],
+         res["fp32"]["interpolate"]["ms"]))
+    return res
+
+
 def measured_traffic():
     """HBM bytes per launch of the z pass from the committed PMC summary
     (profiles/z_pass_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per
@@ -671,6 +721,8 @@ def main():
     ap.add_argument("--penal-steps", type=int, default=3,
                     help="timed penalised 256^3 solves (penalisation_solve; 0 skips)")
     ap.add_argument("--diag-steps", type=int, default=20, help="timed FP64 diagnostics (0 skips)")
+    ap.add_argument("--multires-steps", type=int, default=10,
+                    help="timed patch <-> parent mesh interpolations at 256^3 / 128^3 (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -774,6 +826,7 @@ def main():
 
     penal = penal256_benchmark(args.penal_steps) if args.penal_steps > 0 else None
     diag = diagnostics_benchmark(args.diag_steps) if args.diag_steps > 0 else None
+    multires = multires_benchmark(args.multires_steps) if args.multires_steps > 0 else None
 
     cpu = cpu_ring = cpu_step = None
     if not args.no_cpu_baseline:
@@ -818,6 +871,7 @@ def main():
         "vorticity_rhs": rhs,
         "penalised256": penal,
         "diagnostics": diag,
+        "multires": multires,
         "roofline": {"bound": "hbm", "kernel": Z_KERNEL + " (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": z_gbs / PEAK_HBM_GBS, "traffic": measured_traffic(),

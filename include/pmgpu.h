@@ -171,9 +171,13 @@ int pmgpu_topology_parent(const pmgpu_geom *g, int32_t level, int32_t parent_lev
  * (parent to patch, :1395-2062): every target cell (interior cells only on
  * periodic axes) adds the interpolated source value, then the periodic ghost
  * layers are copied from their images.  clear zeroes the target first.
- * Bit-identical to the reference's sequential loops; deterministic (the
- * deposit is done as a gather in the reference's source-cell order, no
- * atomics).  Cell ratios up to 8 are supported. */
+ * Both directions are tensor products of per-axis operators and run as three
+ * 1D passes (the reference's term-by-term sums regrouped: agreement to
+ * rounding, ~1e-15 relative); with PMGPU_M2M_EXACT=1 in the environment they
+ * run as gathers in the reference's term order instead, bit-identical to it.
+ * Deterministic either way (no atomics).  Cell ratios up to 8 are supported.
+ * The passes use a library workspace (kept until pmgpu_finalize): calls that
+ * overlap in time race. */
 int pmgpu_interp_mesh_mesh(const pmgpu_geom *g_from, const pmgpu_geom *g_to, const double *from, double *to,
                            int32_t clear);
 int pmgpu_interp_mesh_mesh_device(const pmgpu_geom *g_from, const pmgpu_geom *g_to, int32_t precision,

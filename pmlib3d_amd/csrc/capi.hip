@@ -45,6 +45,7 @@ int pmgpu_finalize(void) {
   pmgpu::particles_release();
   pmgpu::penalise_release();
   pmgpu::diagnostics_release();
+  pmgpu::m2m_release();
   pmgpu::penalisation_solve_release();
   return 0;
 }

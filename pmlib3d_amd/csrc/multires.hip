@@ -6,16 +6,21 @@
 //
 // Patch -> parent (the source is finer, reldx = dx_from / dx_to < 1) is a
 // deposit in the reference: every source cell, ghosts included, spreads
-// reldx W_x W_y W_z of its value over the target stencil.  Here it is a
-// gather: each target cell sums the terms of the source cells whose stencils
-// cover it, in the reference's source-cell order (k, j, i ascending), so the
-// result is bit-identical to the sequential deposit and no atomics touch the
-// mesh.  A block handles 64 consecutive target cells of one (y, z) row; the
-// per-axis base indices and weights of the source cells it needs are staged
-// once in LDS.  Parent -> patch is a gather per target cell in the
-// reference's term order.  Compiled without multiply-add contraction.
+// reldx W_x W_y W_z of its value over the target stencil; parent -> patch
+// gathers W_x W_y W_z-weighted source values per target cell.  Both are
+// tensor products of per-axis operators, so the default form runs each as
+// three 1D gather passes over per-axis tap tables (HBM-streaming, no
+// atomics).  The bit-identical forms (PMGPU_M2M_EXACT=1) keep the
+// reference's term order: the deposit as a gather in which each target cell
+// sums the terms of the source cells whose stencils cover it in the
+// reference's source-cell order (k, j, i ascending), a block of 64 target
+// cells of one (y, z) row staging the per-axis bases and weights it needs in
+// LDS; the interpolation per target cell in the reference's term order.
+// Compiled without multiply-add contraction.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <mutex>
 
 #include "../../include/pmgpu.h"
 // (the particle kernels of the header that this file does not launch)
@@ -230,8 +235,243 @@ __global__ void k_m2m_interp(const T *__restrict__ from, T *__restrict__ to, MGe
   }
 }
 
+// ---------------------------------------------------------------------------
+// Separable form (the default).  Both directions are tensor products of
+// per-axis operators: the deposit weight of a source cell onto a target cell
+// is (reldx W_x)(reldx W_y)(reldx W_z), a cell dropped on a periodic axis or
+// clamped on a free one has a zero factor on that axis, and the
+// interpolation weight is W_x W_y W_z with the same zero factors.  So each
+// direction runs as three 1D passes, each a gather along one axis over a
+// per-axis tap table, reducing (deposit) or expanding (interpolation) that
+// axis: deposit x, y, z (intermediates 1/2 and 1/4 of the patch mesh),
+// interpolation z, y, x (1/4 and 1/2).  Each pass streams its input once.
+// The sums are regrouped against the reference's sequential loop (which
+// adds ((W_x W_y) W_z) m term by term), so the result agrees to rounding
+// (~1e-16 relative) rather than bit for bit; PMGPU_M2M_EXACT=1 selects the
+// bit-identical gathers above.
+
+constexpr int kMaxTaps = 64;  // taps of one output index on one axis (deposit: NS * ratio + 2)
+
+struct AxisTab {
+  int *first;   // first input index (0-based, ghosts included) of each output index
+  int *count;   // number of consecutive input indices
+  double *w;    // weights, tap t of output o at w[t * stride + o] (coalesced over o)
+  int stride;
+};
+
+// Tap table of axis d.  Deposit (src finer): output index T (reference
+// numbering) takes the source indices whose base b satisfies b - off <= T <=
+// b - off + NS - 1, weight reldx W_{T - b + off}, zero for a cell outside
+// the box on a periodic axis.  Interpolation (src coarser): output index T
+// takes the stencil slots inside the source mesh, weight W_s.
+template <int NS>
+__global__ void k_m2m_table(MGeom src, MGeom dst, int d, int deposit, double reldx, AxisTab tab) {
+  constexpr int off = NS == 6 ? 2 : 1;
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= dst.E[d]) return;
+  const int T = o + 1 - dst.ng[d];
+  double *w = tab.w + o;
+  const int ws = tab.stride;
+  int first = 0, n = 0;
+  if (deposit) {
+    int ilo, ihi;
+    src_range(src, dst, d, T + off - NS + 1, T + off, ilo, ihi);
+    for (int i = ilo; i <= ihi; ++i) {
+      const double p = __dadd_rn(src.xmin[d], __dmul_rn(__dadd_rn((double)(i - 1), 0.5), src.dx[d]));
+      const int b = cell_index(p, dst.xmin[d], dst.c1dx[d]);
+      const int s = T - b + off;
+      if (s < 0 || s >= NS) continue;
+      if (n == 0) first = i - 1 + src.ng[d];
+      if (n >= kMaxTaps) break;
+      const bool live = !(dst.bc[d] != 0 && (p < dst.xmin[d] || p > dst.xmax[d]));
+      const double dd = stencil_offset(p, dst.xmin[d], dst.dx[d], b, dst.c1dx[d]);
+      w[ws * n++] = live ? __dmul_rn(reldx, weight(NS == 6, s, dd)) : 0.0;
+    }
+  } else {
+    const double p = __dadd_rn(dst.xmin[d], __dmul_rn(__dadd_rn((double)(T - 1), 0.5), dst.dx[d]));
+    const int b = cell_index(p, src.xmin[d], src.c1dx[d]);
+    const double dd = stencil_offset(p, src.xmin[d], src.dx[d], b, src.c1dx[d]);
+    for (int s = 0; s < NS; ++s) {
+      const int q = b - off + s;
+      if (q < 1 - src.ng[d] || q > src.ncell[d] + src.ng[d]) continue;
+      if (n == 0) first = q - 1 + src.ng[d];
+      w[ws * n++] = weight(NS == 6, s, dd);
+    }
+  }
+  tab.first[o] = first;
+  tab.count[o] = n;
+}
+
+struct PassArgs {
+  int d;             // axis of the pass
+  int Ein[3], Eout[3];
+  AxisTab tab;
+  int final_pass;    // writes the caller's mesh: clear / accumulate / skip rules apply
+  int clear;
+  int skip_bc[3];    // final interpolation pass: periodic axes whose ghost cells are not written
+  int nc[3], ng[3];  // (of the output mesh, for the skip rule)
+};
+
+// One 1D pass: grid (ceil(Eout_x Eout_y / 256), Eout_z), a thread per
+// output cell (all three components).
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(256) k_m2m_pass(const Tin *__restrict__ in, Tout *__restrict__ out, PassArgs a) {
+  const int xy = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (xy >= a.Eout[0] * a.Eout[1]) return;
+  int x[3] = {xy % a.Eout[0], xy / a.Eout[0], (int)blockIdx.y};
+  const int64_t c = (int64_t)x[0] + (int64_t)a.Eout[0] * (x[1] + (int64_t)a.Eout[1] * x[2]);
+  const int64_t sin[3] = {3, 3 * (int64_t)a.Ein[0], 3 * (int64_t)a.Ein[0] * a.Ein[1]};
+  {
+    if (a.final_pass) {
+      bool skip = false;
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (a.skip_bc[q] && (x[q] < a.ng[q] || x[q] >= a.ng[q] + a.nc[q])) skip = true;
+      if (skip) {
+        if (a.clear) out[3 * c] = out[3 * c + 1] = out[3 * c + 2] = (Tout)0;
+        return;
+      }
+    }
+    const int o = x[a.d];
+    const int first = a.tab.first[o], cnt = a.tab.count[o];
+    const double *w = a.tab.w + o;
+    const int ws = a.tab.stride;
+    x[a.d] = first;
+    const Tin *p = in + sin[0] * x[0] + sin[1] * x[1] + sin[2] * x[2];
+    const int64_t st = sin[a.d];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int t = 0; t < cnt; ++t) {
+      const double wt = w[t * ws];
+      s0 += wt * (double)p[0];
+      s1 += wt * (double)p[1];
+      s2 += wt * (double)p[2];
+      p += st;
+    }
+    if (a.final_pass && !a.clear) {
+      s0 += (double)out[3 * c];
+      s1 += (double)out[3 * c + 1];
+      s2 += (double)out[3 * c + 2];
+    }
+    out[3 * c] = (Tout)s0;
+    out[3 * c + 1] = (Tout)s1;
+    out[3 * c + 2] = (Tout)s2;
+  }
+}
+
+// Intermediate meshes and tap tables, grown on demand, kept until
+// pmgpu_finalize (one set per process: calls that overlap race).
+struct M2MWS {
+  void *a = nullptr, *b = nullptr;
+  size_t na = 0, nb = 0;
+  int *first = nullptr, *count = nullptr;
+  double *w = nullptr;
+  int ntab = 0;
+};
+M2MWS g_m2m;
+std::mutex g_m2m_mu;
+
+int grow(void *&p, size_t &have, size_t want) {
+  if (have >= want) return 0;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  have = 0;
+  PM_HIP(hipMalloc(&p, want));
+  have = want;
+  return 0;
+}
+
 template <typename T>
-int m2m_launch(const MGeom &f, const MGeom &t, int ns, const void *from, void *to, int clear, hipStream_t st) {
+int m2m_separable(const MGeom &f, const MGeom &t, int ns, const void *from, void *to, int clear, hipStream_t st) {
+  const bool deposit = f.dx[0] / t.dx[0] <= 0.9999;
+  const double reldx = f.dx[0] / t.dx[0];
+  for (int d = 0; d < 3; ++d)
+    PM_CHECK(!deposit || t.dx[d] / f.dx[d] <= (double)kMaxRatio + 1e-9,
+             "pmgpu_interp_mesh_mesh: cell size ratios above 8 are not supported");
+  // the three shapes: deposit reduces x, y, z in turn; interpolation expands z, y, x
+  int E1[3], E2[3];
+  const int order[3] = {deposit ? 0 : 2, 1, deposit ? 2 : 0};
+  for (int q = 0; q < 3; ++q) E1[q] = f.E[q];
+  E1[order[0]] = t.E[order[0]];
+  for (int q = 0; q < 3; ++q) E2[q] = E1[q];
+  E2[order[1]] = t.E[order[1]];
+  const size_t bytes1 = sizeof(T) * 3 * (size_t)E1[0] * E1[1] * E1[2];
+  const size_t bytes2 = sizeof(T) * 3 * (size_t)E2[0] * E2[1] * E2[2];
+  const int emax = std::max(t.E[0], std::max(t.E[1], t.E[2]));
+  std::lock_guard<std::mutex> lock(g_m2m_mu);
+  if (grow(g_m2m.a, g_m2m.na, bytes1) || grow(g_m2m.b, g_m2m.nb, bytes2)) return -1;
+  if (g_m2m.ntab < emax) {
+    if (g_m2m.first) (void)hipFree(g_m2m.first);
+    if (g_m2m.count) (void)hipFree(g_m2m.count);
+    if (g_m2m.w) (void)hipFree(g_m2m.w);
+    g_m2m.first = g_m2m.count = nullptr;
+    g_m2m.w = nullptr;
+    g_m2m.ntab = 0;
+    PM_HIP(hipMalloc(&g_m2m.first, sizeof(int) * 3 * emax));
+    PM_HIP(hipMalloc(&g_m2m.count, sizeof(int) * 3 * emax));
+    PM_HIP(hipMalloc(&g_m2m.w, sizeof(double) * 3 * (size_t)emax * kMaxTaps));
+    g_m2m.ntab = emax;
+  }
+  AxisTab tabs[3];
+  for (int d = 0; d < 3; ++d) {
+    tabs[d] = AxisTab{g_m2m.first + d * emax, g_m2m.count + d * emax, g_m2m.w + (size_t)d * emax * kMaxTaps,
+                      emax};
+    const int blocks = (t.E[d] + 127) / 128;
+    if (ns == 6)
+      hipLaunchKernelGGL(k_m2m_table<6>, dim3(blocks), dim3(128), 0, st, f, t, d, (int)deposit, reldx, tabs[d]);
+    else
+      hipLaunchKernelGGL(k_m2m_table<4>, dim3(blocks), dim3(128), 0, st, f, t, d, (int)deposit, reldx, tabs[d]);
+    PM_HIP(hipGetLastError());
+  }
+  const void *ins[3] = {from, g_m2m.a, g_m2m.b};
+  void *outs[3] = {g_m2m.a, g_m2m.b, to};
+  const int *Eins[3] = {f.E, E1, E2};
+  const int *Eouts[3] = {E1, E2, t.E};
+  for (int q = 0; q < 3; ++q) {
+    PassArgs a{};
+    a.d = order[q];
+    for (int k = 0; k < 3; ++k) {
+      a.Ein[k] = Eins[q][k];
+      a.Eout[k] = Eouts[q][k];
+      a.skip_bc[k] = (!deposit && t.bc[k] != 0) ? 1 : 0;
+      a.nc[k] = t.ncell[k];
+      a.ng[k] = t.ng[k];
+    }
+    a.tab = tabs[a.d];
+    a.final_pass = q == 2;
+    a.clear = clear;
+    const dim3 grid((a.Eout[0] * a.Eout[1] + 255) / 256, a.Eout[2]);
+    hipLaunchKernelGGL((k_m2m_pass<T, T>), grid, dim3(256), 0, st, static_cast<const T *>(ins[q]),
+                       static_cast<T *>(outs[q]), a);
+    PM_HIP(hipGetLastError());
+  }
+  T *dst = static_cast<T *>(to);
+  if (deposit) {
+    PGeom pg{};
+    for (int d = 0; d < 3; ++d) {
+      pg.ncell[d] = t.ncell[d];
+      pg.ngl[d] = t.ng[d];
+      pg.E[d] = t.E[d];
+    }
+    for (int d = 0; d < 3; ++d)
+      if (t.bc[d] != 0) {
+        hipLaunchKernelGGL(k_halo_add<T>, dim3(256), dim3(256), 0, st, dst, pg, d);
+        PM_HIP(hipGetLastError());
+      }
+  } else {
+    for (int d = 0; d < 3; ++d)
+      if (t.bc[d] != 0) {
+        hipLaunchKernelGGL(k_ghost_fill<T>, dim3(1024), dim3(256), 0, st, dst, t.E[0], t.E[1], t.E[2], d,
+                           t.ncell[d], t.ng[d], 3);
+        PM_HIP(hipGetLastError());
+      }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Bit-identical gathers (PMGPU_M2M_EXACT=1)
+template <typename T>
+int m2m_exact(const MGeom &f, const MGeom &t, int ns, const void *from, void *to, int clear, hipStream_t st) {
   const double reldx = f.dx[0] / t.dx[0];
   const T *src = static_cast<const T *>(from);
   T *dst = static_cast<T *>(to);
@@ -286,6 +526,14 @@ int m2m_check(const pmgpu_geom *gf, const pmgpu_geom *gt, MGeom &f, MGeom &t) {
 }
 
 }  // namespace
+
+void m2m_release() {
+  std::lock_guard<std::mutex> lock(g_m2m_mu);
+  void *bufs[] = {g_m2m.a, g_m2m.b, g_m2m.first, g_m2m.count, g_m2m.w};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  g_m2m = M2MWS{};
+}
 }  // namespace pmgpu
 
 using namespace pmgpu;
@@ -319,8 +567,12 @@ int pmgpu_interp_mesh_mesh_device(const pmgpu_geom *g_from, const pmgpu_geom *g_
   PM_CHECK(precision == PMGPU_FP64 || precision == PMGPU_FP32, "pmgpu_interp_mesh_mesh: bad precision");
   const int ns = g_to->interp_order == 4 ? 6 : 4;
   hipStream_t st = (hipStream_t)stream;
-  return precision == PMGPU_FP64 ? m2m_launch<double>(f, t, ns, d_from, d_to, clear, st)
-                                 : m2m_launch<float>(f, t, ns, d_from, d_to, clear, st);
+  const char *ex = getenv("PMGPU_M2M_EXACT");
+  if (ex && atoi(ex) != 0)
+    return precision == PMGPU_FP64 ? m2m_exact<double>(f, t, ns, d_from, d_to, clear, st)
+                                   : m2m_exact<float>(f, t, ns, d_from, d_to, clear, st);
+  return precision == PMGPU_FP64 ? m2m_separable<double>(f, t, ns, d_from, d_to, clear, st)
+                                 : m2m_separable<float>(f, t, ns, d_from, d_to, clear, st);
 }
 
 int pmgpu_interp_mesh_mesh(const pmgpu_geom *g_from, const pmgpu_geom *g_to, const double *from, double *to,

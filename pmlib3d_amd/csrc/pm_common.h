@@ -102,6 +102,7 @@ __global__ void k_ghost_fill(R *m, int E0, int E1, int E2, int d, int N, int ng,
 void particles_release();
 void penalise_release();
 void diagnostics_release();
+void m2m_release();
 void penalisation_solve_release();
 // fetches the device address of the allocation-free penalisation scratch on
 // the current device (once per device)
