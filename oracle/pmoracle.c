@@ -134,6 +134,57 @@ static void kernel_dims(const or_geom *g, int *L) {
 /* Fills G (nvar x Lx x Ly x Lz, Fortran order, nvar = 1 for kernel 1 and 3
  * for kernel 2) with the real-space kernel on the kernel grid, images summed
  * over +-1 patch length on periodic axes.  Returns nonzero on a bad order. */
+/* The sine integral Si(x) as poisson/pmlib_poisson_setup.f:1739-1841
+ * evaluates it: the published cisi method (Numerical Recipes, sec. 6.9):
+ * for |x| > 2 the continued fraction of E1(ix) by the modified Lentz method,
+ * Si = pi/2 + Im(e^{-it} h); otherwise the power series, odd and even terms
+ * accumulated alternately; both to a relative 1e-10, with the reference's
+ * constants, so that the truncation matches. */
+double or_si(double x) {
+  const double pi = acos(-1.0), eps = 1.0e-10, fpmin = 1.0e-30, fpmax = 1.0e30, tmin = 2.0;
+  const int maxit = 1000;
+  const double t = fabs(x);
+  if (t == 0.0) return 0.0;
+  double si;
+  if (t > tmin) {
+    cplx b = 1.0 + t * I, c = fpmax, d = 1.0 / b, h = d;
+    for (int i = 2; i <= maxit; ++i) {
+      const double a = -(double)((i - 1) * (i - 1));
+      b = b + 2.0;
+      d = 1.0 / (a * d + b);
+      c = b + a / c;
+      const cplx del = c * d;
+      h = h * del;
+      if (fabs(creal(del - 1.0)) + fabs(cimag(del - 1.0)) <= eps) break;
+    }
+    h = (cos(t) - sin(t) * I) * h;
+    si = 0.5 * pi + cimag(h);
+  } else if (t < sqrt(fpmin)) {
+    si = t;
+  } else {
+    double sum = 0.0, sums = 0.0, sumc = 0.0, sign = 1.0, fact = 1.0;
+    int odd = 1;
+    for (int k = 1; k <= maxit; ++k) {
+      fact = fact * t / (double)k;
+      const double term = fact / (double)k;
+      sum = sum + sign * term;
+      const double err = term / fabs(sum);
+      if (odd) {
+        sign = -sign;
+        sums = sum;
+        sum = sumc;
+      } else {
+        sumc = sum;
+        sum = sums;
+      }
+      if (err < eps) break;
+      odd = !odd;
+    }
+    si = sums;
+  }
+  return x < 0.0 ? -si : si;
+}
+
 int or_green_real(const or_geom *g, const or_params *p, double *G) {
   int L[3];
   kernel_dims(g, L);
@@ -155,7 +206,7 @@ int or_green_real(const or_geom *g, const or_params *p, double *G) {
   const int ord = p->poisson_order, ker = p->poisson_kernel;
   const int nvar = ker == 1 ? 1 : 3;
   if (ker != 1 && ker != 2) return -1;
-  if (ord != 0 && ord != 2 && ord != 4 && ord != 6 && ord != 8 && ord != 10) return -1;
+  if (ord != 0 && ord != 2 && ord != 4 && ord != 6 && ord != 8 && ord != 10 && ord != 100) return -1;
   const size_t n = (size_t)L[0] * L[1] * L[2];
   memset(G, 0, sizeof(double) * n * nvar);
   if (p->refine > 0.0) {
@@ -164,7 +215,7 @@ int or_green_real(const or_geom *g, const or_params *p, double *G) {
      * the kernels regularised at eps and at a eps, G(rho) - G(rho / a),
      * without periodic images (the reference's image indices are unset
      * there; they are taken as 0).  Order 0 has no refinement kernel. */
-    if (ord == 0) return -1;
+    if (ord == 0 || ord == 100) return -1;
     const double ca = 1.0 / p->refine;
     for (int k = 1; k <= L[2]; ++k) {
       double pz = xk[2] + ((double)(k - 1) + 0.5) * g->dx[2];
@@ -237,6 +288,21 @@ int or_green_real(const or_geom *g, const or_params *p, double *G) {
               double r = sqrt(px * px + py * py + pz * pz);
               double rho = r / eps;
               size_t o = ((size_t)(k - 1) * L[1] + (j - 1)) * L[0] + (i - 1);
+              if (ord == 100) {
+                /* the band-limited kernel (:1419-1505): rho = pi r / h,
+                 * G = Si(rho) / (2 pi^2 r), G(0) = 1 / (2 pi h); grad G =
+                 * x (sin rho - Si(rho)) / (2 pi^2 r^3), 0 at r = 0 */
+                const double rb = pi * r / dmax;
+                if (ker == 1) {
+                  G[o] += r == 0.0 ? 1.0 / (2.0 * pi * dmax) : or_si(rb) / (2.0 * (pi * pi) * r);
+                } else if (r != 0.0) {
+                  const double f = (sin(rb) - or_si(rb)) / (2.0 * (pi * pi) * (r * r * r));
+                  G[o * 3 + 0] += px * f;
+                  G[o * 3 + 1] += py * f;
+                  G[o * 3 + 2] += pz * f;
+                }
+                continue;
+              }
               double E = exp(-0.5 * (rho * rho)) * c_1_sqrt2pi;
               double F = erf(c_1_sqrt2 * rho);
               double r2 = rho * rho, r3 = r2 * rho, r5 = r3 * r2, r7 = r5 * r2, r9 = r7 * r2;
@@ -357,6 +423,9 @@ int or_poisson_solve(const or_geom *g, const or_params *p, const cplx *Ghat, con
     if (L[d] % 2) return -2;                           /* fourier/pmlib_fourier_fft.f:126 */
   }
   const int ifft_vort = reg_vort || reproj;
+  /* order 100 has no vorticity filter (poisson/pmlib_poisson_solve.f:311-329
+   * sets none for it) */
+  if (ifft_vort && p->poisson_order == 100) return -3;
   const int ifft_psi = calc_vecpot && p->poisson_kernel == 1;
   const double pi = acos(-1.0);
   /* z-pencil array (3, Lx, Ly, Mz); the x and y passes fill it progressively */

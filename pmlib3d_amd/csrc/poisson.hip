@@ -225,8 +225,8 @@ template <typename R> struct PoissonGPU : PoissonBase {
     stream = s;
     PM_CHECK(prm.poisson_kernel == 1 || prm.poisson_kernel == 2, "pmgpu: poisson_kernel must be 1 or 2");
     int o = prm.poisson_order;
-    PM_CHECK(o == 0 || o == 2 || o == 4 || o == 6 || o == 8 || o == 10,
-             "pmgpu: poisson_order must be 0, 2, 4, 6, 8 or 10");
+    PM_CHECK(o == 0 || o == 2 || o == 4 || o == 6 || o == 8 || o == 10 || o == 100,
+             "pmgpu: poisson_order must be 0, 2, 4, 6, 8, 10 or 100 (parameters/pmlib_parameters_check.f:52-63)");
     PM_CHECK(prm.poisson_order == 0 || prm.regularisation_radius >= 1.0,
              "pmgpu: regularisation_radius must be >= 1 (parameters/pmlib_parameters_check.f:67)");
     allper = g.bound_cond[0] == 1 && g.bound_cond[1] == 1 && g.bound_cond[2] == 1;
@@ -325,7 +325,10 @@ template <typename R> struct PoissonGPU : PoissonBase {
   int setup() override {
     PM_CHECK(refine == 0.0 || prm.poisson_order != 0,
              "pmgpu_poisson_setup: Regularisation must be used for multi-patch simulations "
-             "(poisson/pmlib_poisson_setup.f:276-280)");
+             "(poisson/pmlib_poisson_setup.f:335-339)");
+    PM_CHECK(refine == 0.0 || prm.poisson_order != 100,
+             "pmgpu_poisson_setup: Gaussian regularisation must be used for multi-patch simulations "
+             "(poisson/pmlib_poisson_setup.f:1514-1518)");
     if (allper) return 0;  // G = zeta / (4 pi^2 k^2) is formed in the z pass
     if (ker2) return setup_kernel2();
     // FP64 kernel grid and its transform (x R2C, y and z C2C), then the octant
@@ -342,6 +345,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     ga.eps = prm.regularisation_radius * dmax;
     ga.order = prm.poisson_order;
     ga.ca = refine > 0.0 ? 1.0 / refine : 0.0;
+    ga.hmax = dmax;
     for (int d = 0; d < 3; ++d) {
       ga.L[d] = L[d];
       ga.bc[d] = g.bound_cond[d];
@@ -407,6 +411,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     ga.eps = prm.regularisation_radius * dmax;
     ga.order = prm.poisson_order;
     ga.ca = refine > 0.0 ? 1.0 / refine : 0.0;
+    ga.hmax = dmax;
     for (int d = 0; d < 3; ++d) {
       ga.L[d] = L[d];
       ga.bc[d] = g.bound_cond[d];
@@ -483,6 +488,9 @@ template <typename R> struct PoissonGPU : PoissonBase {
     PM_CHECK(allper || (ker2 ? Khat != nullptr : Ghat != nullptr), "pmgpu: pmgpu_poisson_setup was not called");
     PM_CHECK(vort && vel, "pmgpu: null mesh pointer");
     PM_CHECK(!want_vort || vort_out, "pmgpu: vort_out required with reg_vort/reproj");
+    PM_CHECK(!want_vort || prm.poisson_order != 100,
+             "pmgpu: order 100 has no regularised or reprojected vorticity output "
+             "(poisson/pmlib_poisson_solve.f:311-329 defines no filter for it)");
     PM_CHECK(!want_psi || vecpot, "pmgpu: vecpot required with calc_vecpot");
     if (want_psi && !Bpsi) PM_HIP(hipMalloc(&Bpsi, nB * sizeof(V)));
     if (want_vort && !Bvort) PM_HIP(hipMalloc(&Bvort, nB * sizeof(V)));

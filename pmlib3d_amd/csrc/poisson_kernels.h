@@ -378,7 +378,65 @@ struct GreenArgs {
   double eps;
   int order;
   double ca;         // refinement patch: 1 / a, a = 2^(level - parent level); 0 on a base patch
+  double hmax;       // max(dx): order 100 takes rho = pi r / hmax
 };
+
+// The sine integral Si(x) by the published cisi method (Numerical Recipes,
+// sec. 6.9), with the truncation of poisson/pmlib_poisson_setup.f:1739-1841:
+// for |x| > 2 the continued fraction of E1(ix) by the modified Lentz method
+// to a relative 1e-10, Si = pi/2 + Im(e^{-it} h); otherwise the power series
+// to 1e-10, odd and even terms accumulated alternately.
+__device__ __forceinline__ double2 z_mul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 z_inv(double2 a) {
+  const double m = a.x * a.x + a.y * a.y;
+  return make_double2(a.x / m, -a.y / m);
+}
+__device__ inline double si_cisi(double x) {
+  const double pi = 3.14159265358979323846, eps = 1.0e-10, tmin = 2.0;
+  const double t = fabs(x);
+  if (t == 0.0) return 0.0;
+  double si;
+  if (t > tmin) {
+    double2 b = make_double2(1.0, t), c = make_double2(1.0e30, 0.0), d = z_inv(b), h = d;
+    for (int i = 2; i <= 1000; ++i) {
+      const double a = -(double)((i - 1) * (i - 1));
+      b.x += 2.0;
+      d = z_inv(make_double2(a * d.x + b.x, a * d.y + b.y));
+      const double2 ac = z_inv(c);
+      c = make_double2(b.x + a * ac.x, b.y + a * ac.y);
+      const double2 del = z_mul(c, d);
+      h = z_mul(h, del);
+      if (fabs(del.x - 1.0) + fabs(del.y) <= eps) break;
+    }
+    h = z_mul(make_double2(cos(t), -sin(t)), h);
+    si = 0.5 * pi + h.y;
+  } else if (t < 1.0e-15) {
+    si = t;
+  } else {
+    double sum = 0.0, sums = 0.0, sumc = 0.0, sign = 1.0, fact = 1.0;
+    bool odd = true;
+    for (int k = 1; k <= 1000; ++k) {
+      fact = fact * t / (double)k;
+      const double term = fact / (double)k;
+      sum = sum + sign * term;
+      const double err = term / fabs(sum);
+      if (odd) {
+        sign = -sign;
+        sums = sum;
+        sum = sumc;
+      } else {
+        sumc = sum;
+        sum = sums;
+      }
+      if (err < eps) break;
+      odd = !odd;
+    }
+    si = sums;
+  }
+  return x < 0.0 ? -si : si;
+}
 
 // poly(rho) E(rho) + F(rho) of the regularised kernels (order > 0):
 // kernel 1 G = (poly E + F) / (4 pi r), kernel 2 grad G = -x (poly E + F) / (4 pi r^3)
@@ -481,7 +539,10 @@ static __global__ void k_green(GreenArgs a) {
           double r = sqrt(px * px + py * py + pz * pz);
           double rho = r / a.eps;
           double v;
-          if (r == 0.0) {
+          if (a.order == 100) {
+            // band-limited kernel (poisson/pmlib_poisson_setup.f:1419-1470)
+            v = r == 0.0 ? 1.0 / (2.0 * pi * a.hmax) : si_cisi(pi * r / a.hmax) / (2.0 * (pi * pi) * r);
+          } else if (r == 0.0) {
             switch (a.order) {
               case 0: v = 1.0; break;
               case 2: v = c_sq2_4pieps; break;
@@ -549,6 +610,13 @@ static __global__ void k_green2(GreenArgs a) {
                                base[2] + a.len[2] * (double)kk};
           const double r = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
           if (r == 0.0) continue;
+          if (a.order == 100) {
+            // x (sin rho - Si(rho)) / (2 pi^2 r^3), rho = pi r / h (:1472-1505)
+            const double rb = pi * r / a.hmax;
+            const double f = (sin(rb) - si_cisi(rb)) / (2.0 * (pi * pi) * (r * r * r));
+            for (int c = 0; c < 3; ++c) acc[c] += p[c] * f;
+            continue;
+          }
           const double rho = r / a.eps;
           double f;
           if (a.order == 0) {

@@ -45,6 +45,11 @@ SOLVE_CASES = {
     "mixed_k2_o8": ((8, 12, 10), 2, (1, 0, 1), 8, (True, True, False), 2),
     "periodic_k2_o2": ((12, 8, 10), 0, (1, 1, 1), 2, (True, True, False), 2),
     "free_k2_o10": ((12, 10, 14), 1, (0, 0, 0), 10, (False, True, False), 2),
+    # order 100: the band-limited kernel Si(pi r / h) / (2 pi^2 r)
+    # (poisson/pmlib_poisson_setup.f:1419-1505)
+    "free_ng1_o100_psi": ((12, 10, 8), 1, (0, 0, 0), 100, (False, False, True)),
+    "mixed_k2_o100": ((8, 10, 12), 0, (0, 1, 0), 100, (False, False, False), 2),
+    "periodic_o100_psi": ((8, 12, 8), 0, (1, 1, 1), 100, (False, False, True)),
 }
 
 

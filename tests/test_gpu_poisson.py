@@ -41,6 +41,10 @@ CASES = [
     ((16, 20, 10), 0, (0, 1, 0), 6, (False, True, True)),
     ((7, 11, 13), 0, (0, 0, 0), 4, (True, True, True)),      # lengths 14, 22, 26: radix 7, 11, 13
     ((14, 6, 10), 2, (1, 0, 0), 6, (False, False, True)),    # periodic x of length 14
+    # order 100: the band-limited kernel Si(pi r / h) / (2 pi^2 r)
+    ((12, 10, 8), 1, (0, 0, 0), 100, (False, False, True)),
+    ((16, 20, 10), 0, (0, 1, 0), 100, (False, False, True)),
+    ((12, 8, 8), 0, (1, 1, 1), 100, (False, False, True)),
 ]
 
 
@@ -168,3 +172,30 @@ def test_z_pass_tile_widths_agree(n, precision, monkeypatch):
             e = float((out - ref).abs().max()) / scale
             print("n %d precision %d tc %d rep %d rel max diff %.3e" % (n, precision, tc, r, e), flush=True)
             assert e <= tol, (tc, r, e)
+
+
+@pytest.mark.parametrize("name", ["free_ng1_o100_psi", "periodic_o100_psi"])
+def test_order100_matches_reference_golden(name):
+    """Order 100 (poisson/pmlib_poisson_setup.f:1419-1505, the sine integral
+    of :1739-1841) against the compiled reference's velocity and stream
+    function."""
+    import json
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_solve_%s.npz" % name), allow_pickle=False)
+    ncell, ng, bc, order, (rv, rp, cv) = json.loads(str(d["spec"]))[:5]
+    print("golden", name, flush=True)
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc)
+    p = O.make_params(order, 1, 2.0)
+    vel, _, psi = _solver(g, p).solve(d["vort_in"], rv, rp, cv)
+    assert rel_linf(vel, d["vel"]) <= TOL64
+    assert rel_linf(psi, d["psi"]) <= TOL64
+    v32, _, _ = _solver(g, p, 1).solve(d["vort_in"], rv, rp, cv)
+    assert rel_l2(v32, vel) <= 1e-5
+
+
+def test_order100_refuses_vorticity_outputs():
+    """The reference's solve defines no vorticity filter for order 100."""
+    import pmlib3d_amd as pm
+    g = O.make_geom((8, 8, 8), 0, [-0.5] * 3, 1.0 / 8)
+    s = _solver(g, O.make_params(100, 1, 2.0))
+    with pytest.raises(pm.PmgpuError, match="order 100"):
+        s.solve(np.zeros((3, 8, 8, 8), order="F"), True, False, False)

@@ -775,3 +775,50 @@ def test_m2m_interpolation_reproduces_polynomials(io, deg):
     sl = slice(ng + 4, O.mesh_extent(g)[0] - ng - 4)
     assert np.abs(got[:, sl, sl, sl] - want[:, sl, sl, sl]).max() <= 1e-12 * np.abs(want).max()
     assert ngp >= 2
+
+
+
+# ---------------------------------------------------------------------------
+# Order 100: the band-limited kernel (poisson/pmlib_poisson_setup.f:1419-1505)
+
+def test_sine_integral_restates_cisi():
+    """or_si follows the cisi method to its 1e-10 truncation: within 1e-10 of
+    scipy's Si, odd, 0 at 0, pi/2 at infinity, and continuous across the
+    switch from series to continued fraction at |x| = 2."""
+    import ctypes as C
+    from scipy.special import sici
+    L = O.lib()
+    L.or_si.restype = C.c_double
+    L.or_si.argtypes = [C.c_double]
+    xs = np.concatenate([np.linspace(-40, 40, 801), [1e-20, 1e-8, 1.9999999, 2.0000001, 123.4, 1e4]])
+    got = np.array([L.or_si(x) for x in xs])
+    assert np.abs(got - sici(xs)[0]).max() <= 1e-10
+    assert L.or_si(0.0) == 0.0 and L.or_si(-1.3) == -L.or_si(1.3)
+    assert abs(L.or_si(1e8) - np.pi / 2) <= 1e-7
+    assert abs(L.or_si(2.0 - 1e-12) - L.or_si(2.0 + 1e-12)) <= 1e-10
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_order100_kernel_closed_form(kernel):
+    """G = Si(pi r / h) / (2 pi^2 r), G(0) = 1 / (2 pi h); grad G = x (sin rho -
+    Si(rho)) / (2 pi^2 r^3), 0 at r = 0 (free space, scipy's Si)."""
+    from scipy.special import sici
+    g = O.make_geom((6, 4, 8), 0, [-0.3, -0.2, -0.4], 0.05)
+    G = O.green_real(g, O.make_params(100, kernel, 2.0))
+    L = [2 * n for n in g.ncell]
+    h = max(g.dx)
+    x = [(-(0.5 * L[d] + 0.5) + np.arange(L[d]) + 0.5) * g.dx[d] for d in range(3)]
+    Z, Y, X = np.meshgrid(x[2], x[1], x[0], indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    rho = np.pi * r / h
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if kernel == 1:
+            want = np.where(r == 0, 1.0 / (2 * np.pi * h), sici(rho)[0] / (2 * np.pi ** 2 * r)).ravel()
+            got = G
+        else:
+            f = np.where(r == 0, 0.0, (np.sin(rho) - sici(rho)[0]) / (2 * np.pi ** 2 * r ** 3))
+            want = np.stack([X * f, Y * f, Z * f], axis=-1).ravel()
+            got = G
+    assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
+    with pytest.raises(ValueError):    # no refinement kernel for order 100
+        O.green_real(g, O.make_params(100, kernel, 2.0, refine=2.0))
