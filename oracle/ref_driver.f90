@@ -19,6 +19,7 @@
 !   remesh   : pmlib_remesh                                 (remesh/)
 !   advance  : pmlib_particles_advance                      (particles/)
 !   m2m      : pmlib_topology_parent + pmlib_interp_mesh_mesh, both ways
+!   rhs      : the vortex client's part_deriv_vorticity (reference build)
 !   step     : 2 x (P2M, solve, M2P, advance) + (P2M, remesh), timed; with
 !              reload=.true. every step restarts from the input particles
 !              (timing a fixed particle count); on several ranks each rank
@@ -49,8 +50,9 @@ PROGRAM ref_driver
   USE pmlib_mod_output
   ! the vortex client's particle checkpoint writer and its globals
   USE mod_output, ONLY: output_particles
+  USE mod_particle_derivatives, ONLY: part_deriv_vorticity
   USE mod_parameters, ONLY: client_itime => itime, client_time => time, client_dtime => dtime, &
-       & client_rank => rank, client_nproc => nproc, client_comm => comm
+       & client_rank => rank, client_nproc => nproc, client_comm => comm, client_visc => visc
 #endif
   IMPLICIT NONE
 
@@ -72,6 +74,8 @@ PROGRAM ref_driver
   REAL(MK)           :: time = 0.0_MK
   ! refinement patch: class_patch level and parent(1) (solve, m2m)
   INTEGER            :: level = 1, parent_level = 0
+  ! rhs: kinematic viscosity of the client
+  REAL(MK)           :: visc = 0.0_MK
   INTEGER            :: npart = 0, stage = 1, nrep = 1, nsteps = 1
   REAL(MK)           :: dtime = 0.0_MK, min_vort = 0.0_MK, min_dvort = 0.0_MK
   CHARACTER(LEN=512) :: infile = 'in.bin', outfile = 'out.bin'
@@ -79,7 +83,7 @@ PROGRAM ref_driver
        & interp_order, rk_order, eps_radius, reg_vort, reproj, calc_vecpot, clear, &
        & incl_ghost, euler_vort, npart, stage, nrep, nsteps, dtime, min_vort, min_dvort, &
        & infile, outfile, reload, penal_relax, penal_tol, vinf, penal_nmax, fd_order, with_solid_vel, &
-       & outdir, tag, itime, time, level, parent_level
+       & outdir, tag, itime, time, level, parent_level, visc
 
   TYPE(class_patch)        :: patch
   TYPE(class_topology_all) :: topo_all
@@ -360,6 +364,23 @@ PROGRAM ref_driver
     client_comm = MPI_COMM_WORLD
     CALL output_particles(TRIM(outdir)//TRIM(tag), topo_all%cuboid, part, ierr)
     IF (ierr .NE. 0) CALL die('output_particles')
+
+  CASE ('rhs')
+    ! in: vort, vel, dvort (3, E...); out: dvort after part_deriv_vorticity
+    ! (fd_order of the namelist, viscosity visc)
+    pmlib_fd_order = fd_order
+    client_visc = visc
+    client_rank = rank
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) mesh%vort
+    READ(10) mesh%vel
+    READ(10) mesh%dvort
+    CLOSE(10)
+    CALL part_deriv_vorticity(topo_all%cuboid, mesh, ierr)
+    IF (ierr .NE. 0) CALL die('part_deriv_vorticity')
+    OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(20) mesh%dvort
+    CLOSE(20)
 #endif
 
 #ifdef PMGPU_SHIM

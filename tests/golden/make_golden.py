@@ -14,8 +14,10 @@ Outputs (small, committed):
   ref_checkpoint_*.npz   inputs + the file bytes of output_particles
   ref_refine_*.npz       inputs + velocities of refinement-patch solves
   ref_m2m_*.npz          inputs + both directions of pmlib_interp_mesh_mesh
+  ref_rhs_*.npz          inputs + dvort of the vortex client's part_deriv_vorticity
 (`python tests/golden/make_golden.py formats` rewrites only the vtk and
-checkpoint fixtures, `... multires` only the last two kinds.)
+checkpoint fixtures, `... multires` only the refinement and m2m ones, `...
+rhs` only the right-hand-side ones.)
 """
 import json
 import os
@@ -254,6 +256,30 @@ def m2m_case(name, spec, seed):
                         down=down, up=up, parent_ncell=np.array(gp.ncell), spec=np.array(json.dumps(spec)))
 
 
+# the vortex client's vorticity right-hand side (part_deriv_vorticity,
+# vortex_codes/vortex_structures/source/particle_derivatives/part_deriv_vorticity.f)
+RHS_CASES = {
+    # name: (ncell, nghost, bound_cond, interp_order, fd_order)
+    "free_fd4": ((12, 10, 8), 2, (0, 0, 0), 3, 4),
+    "free_fd2": ((10, 8, 12), 1, (0, 0, 0), 3, 2),
+    "mixed_fd4": ((8, 8, 12), 2, (1, 1, 0), 3, 4),
+    "periodic_m6_fd4": ((8, 10, 8), 0, (1, 1, 1), 4, 4),
+    "mixed_m6_fd2": ((12, 8, 10), 2, (0, 1, 0), 4, 2),
+}
+
+
+def rhs_case(name, spec, seed):
+    ncell, ng, bc, io, fd = spec
+    rng = np.random.default_rng(seed)
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc, io)
+    E = (3,) + O.mesh_extent(g)
+    w, u, dw = (np.asfortranarray(rng.normal(size=E)) for _ in range(3))
+    raw, _ = O.ref_run(dict(O.geom_job(g), mode="rhs", fd_order=fd, visc=0.013, interp_order=io), [w, u, dw])
+    out = np.frombuffer(raw, dtype=np.float64).reshape(E, order="F")
+    np.savez_compressed(os.path.join(HERE, "ref_rhs_%s.npz" % name), vort=w, vel=u, dvort_in=dw, dvort=out,
+                        visc=np.array(0.013), spec=np.array(json.dumps(spec)))
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref is not built: make -f oracle/Makefile.ref")
@@ -261,6 +287,10 @@ if __name__ == "__main__":
         vtk_case("free_ng2", (5, 4, 3), 2, (0, 0, 0), [-0.5, -0.37, 0.1234], 1.0 / 7.0, 51)
         vtk_case("periodic", (6, 3, 4), 0, (1, 1, 1), [0.0, -1.0, -2.5e-3], 0.03125, 52)
         checkpoint_case("mixed", 37, 41, 0.37, 0.005, 53)
+        sys.exit(0)
+    if len(sys.argv) > 1 and sys.argv[1] == "rhs":
+        for q, (name, spec) in enumerate(RHS_CASES.items()):
+            rhs_case(name, spec, 70 + q)
         sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "multires":
         for name, spec in REFINE_CASES.items():
@@ -280,6 +310,8 @@ if __name__ == "__main__":
         refine_case(name, spec)
     for q, (name, spec) in enumerate(M2M_CASES.items()):
         m2m_case(name, spec, 60 + q)
+    for q, (name, spec) in enumerate(RHS_CASES.items()):
+        rhs_case(name, spec, 70 + q)
     particle_case("m4_free", (10, 8, 12), 0, (0, 0, 0), 3, 11)
     particle_case("m4_ng2", (8, 10, 8), 2, (0, 0, 0), 3, 12)
     particle_case("m4_mixed", (8, 8, 10), 0, (1, 1, 0), 3, 13)

@@ -822,3 +822,34 @@ def test_order100_kernel_closed_form(kernel):
     assert np.abs(got - want).max() <= 1e-9 * np.abs(want).max()
     with pytest.raises(ValueError):    # no refinement kernel for order 100
         O.green_real(g, O.make_params(100, kernel, 2.0, refine=2.0))
+
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "ref_rhs_*.npz"))),
+                         ids=lambda p: os.path.basename(p)[8:-4])
+def test_oracle_rhs_matches_client_routine_bitwise(path):
+    """or_vort_rhs against the vortex client's own part_deriv_vorticity
+    (vortex_codes/vortex_structures/source/particle_derivatives/
+    part_deriv_vorticity.f), compiled with the reference library."""
+    d = np.load(path, allow_pickle=False)
+    ncell, ng, bc, io, fd = json.loads(str(d["spec"]))
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc, io)
+    got = O.vort_rhs(g, d["vort"], d["vel"], float(d["visc"]), fd, dvort=d["dvort_in"])
+    np.testing.assert_array_equal(got, d["dvort"])
+
+
+def test_ring_kat_is_what_the_reference_validation_program_prints():
+    """tests/golden/validation_poisson_convergence.dat is the table the
+    reference's own validation/poisson program prints when built with the
+    reference library here (oracle/Makefile.ref validation_poisson): the
+    rms velocity errors 0.02293 / 7.59e-4 of the harness's ring fixture, not
+    the 1.619e-6 / 2.049e-10 of the shipped validation/poisson/convergence.dat."""
+    rows = [list(map(float, l.split())) for l in open(os.path.join(GOLD, "validation_poisson_convergence.dat"))
+            if l.strip()]
+    kat = json.load(open(os.path.join(GOLD, "kat_ring_ref.json")))["cases"]
+    assert len(rows) == len(kat) == 2
+    for (h, err), case in zip(rows, kat):
+        assert h == case["h"]
+        # the table prints 12 significant digits
+        assert abs(err - case["rms_error"]) <= 1e-11 * case["rms_error"]
+        assert abs(err / case["convergence_dat"]) > 1e3
