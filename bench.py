@@ -721,6 +721,8 @@ def main():
     ap.add_argument("--penal-steps", type=int, default=3,
                     help="timed penalised 256^3 solves (penalisation_solve; 0 skips)")
     ap.add_argument("--diag-steps", type=int, default=20, help="timed FP64 diagnostics (0 skips)")
+    ap.add_argument("--full-steps", type=int, default=5,
+                    help="timed full-output 256^3 solves (velocity, stream function, filtered vorticity; 0 skips)")
     ap.add_argument("--multires-steps", type=int, default=10,
                     help="timed patch <-> parent mesh interpolations at 256^3 / 128^3 (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -783,6 +785,27 @@ def main():
     t64 = time.perf_counter() - t0
     z64_ms, tot64_ms = s64.last_timing()
     s64.set_timing(False)
+
+    # the full-output form (velocity + stream function + filtered vorticity),
+    # z pass and whole solve by HIP events, FP32 and FP64
+    full = {}
+    if args.full_steps > 0:
+        for name, s_, wi, vo in (("fp32", s32, w32, v32), ("fp64", s64, w64, v64)):
+            ps_, wo_ = torch.empty_like(wi), torch.empty_like(wi)
+            kw = dict(d_vort_out=wo_.data_ptr(), d_vecpot=ps_.data_ptr(), reg_vort=True, calc_vecpot=True)
+            for _ in range(2):
+                s_.solve_device(wi.data_ptr(), vo.data_ptr(), **kw)
+            torch.cuda.synchronize()
+            s_.set_timing(True)
+            for _ in range(args.full_steps):
+                s_.solve_device(wi.data_ptr(), vo.data_ptr(), **kw)
+            torch.cuda.synchronize()
+            zf, tf = s_.last_timing()
+            s_.set_timing(False)
+            full[name] = {"zpass_ms": zf, "solve_ms": tf, "zpass_over_velocity_zpass": zf / (z32_ms if name == "fp32" else z64_ms)}
+            del ps_, wo_
+        log("full-output solve (u, psi, filtered w): fp32 %.3f ms (z %.3f), fp64 %.3f ms (z %.3f)" %
+            (full["fp32"]["solve_ms"], full["fp32"]["zpass_ms"], full["fp64"]["solve_ms"], full["fp64"]["zpass_ms"]))
 
     if args.copy_calibration:
         # an elementwise kernel of known size: reads 1 GiB, writes 1 GiB
@@ -872,6 +895,7 @@ def main():
         "penalised256": penal,
         "diagnostics": diag,
         "multires": multires,
+        "full_output_solve256": full or None,
         "roofline": {"bound": "hbm", "kernel": Z_KERNEL + " (fused z pass)",
                      "achieved": z_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": z_gbs / PEAK_HBM_GBS, "traffic": measured_traffic(),

@@ -616,6 +616,133 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   }
 }
 
+// Full-output form of k_z3_r: velocity plus the stream function and/or the
+// filtered (and optionally reprojected) vorticity, kernel 1.  The three
+// forward spectra stay in registers; each requested field is formed from
+// them in registers, transformed back (one component at a time through the
+// same LDS line) and stored, velocity in place, psi-hat and zeta-hat into
+// their own spectra (poisson/pmlib_poisson_solve.f:767-872: psi = G w-hat,
+// vort = zeta w-hat [+ reprojection]).  With every axis periodic the
+// spectral G carries the filter zeta as well when the vorticity is filtered
+// (:339-349).
+template <typename R, int L, int TC, bool ALLPER, bool HALF>
+__global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_rx(ZArgs a, const cplx_t<R> *tw_all) {
+  using V = cplx_t<R>;
+  constexpr int P = Geo<L>::P, E = L / P;
+  constexpr int ST = reg_line_stride<L, sizeof(V), TC>();
+  __shared__ V lds[TC * ST];
+  const V *tw = tw_all + a.plan.tw_off;
+  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
+  const int kx0 = (int)(lt % gridDim.x) * TC, ky = (int)(lt / gridDim.x);
+  const int tid = threadIdx.x;
+  const int col = tid % TC, p = tid / TC;
+  Twid<L, V> twd;
+  twid_load<L>(twd, p, tw);
+  const int kx = kx0 + col;
+  const bool col_ok = kx < a.Hx;
+  const int64_t toff = (int64_t)ky * a.kys + (int64_t)kx0 * 3;
+  V *base = reinterpret_cast<V *>(a.B) + toff;
+  const int64_t rsb = a.rs * (int64_t)sizeof(V);
+  const int ncols_here = a.Hx - kx0 < TC ? a.Hx - kx0 : TC;
+  const uint32_t extent = (uint32_t)((a.Mz - 1) * rsb + (int64_t)ncols_here * 3 * sizeof(V));
+  const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(base, extent);
+  const int voff = (int)(p * rsb) + col * 3 * (int)sizeof(V);
+  V w[3][E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const bool ok = col_ok && (HALF ? m < E / 2 : p + P * m < a.Mz);
+    const int soff = (int)((int64_t)P * m * rsb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      w[c][m] = ok ? buf_load(rsrc, voff + c * (int)sizeof(V), soff, V{}) : cmake<V>(0, 0);
+  }
+  V *line = lds + col * ST;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) fft_regs<L, -1, V, HALF, false, 1>(w[c], line, p, twd);
+  const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
+  const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
+  const bool nx = (2 * kx == a.Lx), ny = (2 * ky == a.Ly);
+  const R kx_ = nx ? (R)0 : ki, ky_ = ny ? (R)0 : kj;
+  const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
+  const R pi = (R)3.14159265358979323846;
+  const R twopi = (R)6.28318530717958647692;
+  // per row: G h^3 (with zeta when all axes are periodic and the vorticity
+  // is filtered) and zeta
+  R gz[E], zt[E];
+  {
+    const R *G = reinterpret_cast<const R *>(a.Ghat) + (int64_t)kyf * a.Hx + (col_ok ? kx : 0);
+    const int64_t gzs = (int64_t)a.Hy * a.Hx;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = p + P * m;
+      const R kk = wavenumber<R>(n, L, a.dk[2], a.ksmp[2]);
+      zt[m] = a.want_vort ? zeta_filter<R>(a.order, (R)a.eps2pi * sqrt(ki * ki + kj * kj + kk * kk)) : (R)1;
+      if constexpr (ALLPER) {
+        const R kmag = ki * ki + kj * kj + kk * kk;
+        gz[m] = kmag > (R)0 ? zt[m] / ((R)4 * (pi * pi) * kmag) : (R)0;
+      } else {
+        const int kzf = n <= L - n ? n : L - n;
+        gz[m] = col_ok ? G[kzf * gzs] * (R)a.h3 : (R)0;
+      }
+    }
+  }
+  const R scale = (R)1 / (R)L;
+  const int nkind = 1 + (a.want_psi ? 1 : 0) + (a.want_vort ? 1 : 0);
+  // one output component at a time: only its E words are live beside the
+  // three forward spectra
+#pragma unroll 1
+  for (int kd = 0; kd < nkind; ++kd) {
+    const int kind = kd == 0 ? 0 : ((kd == 1 && a.want_psi) ? 1 : 2);
+    V *out = (kind == 0 ? reinterpret_cast<V *>(a.B)
+                        : (kind == 1 ? reinterpret_cast<V *>(a.Bpsi) : reinterpret_cast<V *>(a.Bvort))) + toff;
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+      V r[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int n = p + P * m;
+        const R kk = wavenumber<R>(n, L, a.dk[2], a.ksmp[2]);
+        const bool nz = (2 * n == L);
+        const V w0 = w[0][m], w1 = w[1][m], w2 = w[2][m];
+        const V wc = c == 0 ? w0 : (c == 1 ? w1 : w2);
+        if (kind == 0) {
+          // u-hat = G 2 pi i (k x w-hat), Nyquist wavenumbers dropped
+          const R kz_ = nz ? (R)0 : kk;
+          V cr;
+          if (c == 0) cr = csub(cscale(w2, ky_), cscale(w1, kz_));
+          else if (c == 1) cr = csub(cscale(w0, kz_), cscale(w2, kx_));
+          else cr = csub(cscale(w1, kx_), cscale(w0, ky_));
+          const R f = gz[m] * twopi;
+          r[m] = cmake<V>(-cr.y * f, cr.x * f);
+        } else if (kind == 1) {
+          r[m] = cscale(wc, gz[m]);
+        } else {
+          V q = cscale(wc, zt[m]);
+          if (a.reproj) {
+            // -4 pi^2 G k_c sum_d k_d w_d over the pairs (c, d) of equal
+            // Nyquist parity (:861-872)
+            const bool nc = c == 0 ? nx : (c == 1 ? ny : nz);
+            const R kc = c == 0 ? ki : (c == 1 ? kj : kk);
+            V acc = cmake<V>(0, 0);
+            if (nx == nc) acc = cadd(acc, cscale(w0, ki));
+            if (ny == nc) acc = cadd(acc, cscale(w1, kj));
+            if (nz == nc) acc = cadd(acc, cscale(w2, kk));
+            q = cadd(q, cscale(acc, -(R)4 * (pi * pi) * gz[m] * kc));
+          }
+          r[m] = q;
+        }
+      }
+      fft_regs<L, 1, V, false, HALF, 1>(r, line, p, twd);
+      if (col_ok) {
+        V *dst = out + (int64_t)p * a.rs + col * 3 + c;
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+          if (HALF ? m < E / 2 : p + P * m < a.Mz) dst[(int64_t)P * m * a.rs] = cscale(r[m], scale);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------- scalar z pass
 // One component (psi-hat = G w-hat, Cin = Cout = 1): a block owns T
 // consecutive kx columns of one ky row, B[((z*Ly + ky)*rowp + kx)], and

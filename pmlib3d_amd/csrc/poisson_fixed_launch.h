@@ -93,6 +93,24 @@ template <typename R, int L, int TC> int zr_T(ZArgs a, int Ly, const cplx_t<R> *
     return hipGetLastError() == hipSuccess ? 1 : -1;
   }
 }
+// register-resident full-output z pass (psi and/or filtered vorticity)
+template <typename R, int L, int TC> int zrx_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  if constexpr (!regs_ok<L>() || TC * reg_line_stride<L, sizeof(cplx_t<R>), TC>() * (int)sizeof(cplx_t<R>) > kLdsBudget ||
+                Geo<L>::P * TC > 1024) {
+    return 0;
+  } else {
+    if ((double)a.Mz * (double)a.rs * sizeof(cplx_t<R>) >= 2147483647.0) return 0;
+    dim3 grid((a.Hx + TC - 1) / TC, Ly);
+    if (a.allper)
+      hipLaunchKernelGGL((k_z3_rx<R, L, TC, true, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    else if (2 * a.Mz == L && (L / Geo<L>::P) % 2 == 0)
+      hipLaunchKernelGGL((k_z3_rx<R, L, TC, false, true>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    else
+      hipLaunchKernelGGL((k_z3_rx<R, L, TC, false, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+    return hipGetLastError() == hipSuccess ? 1 : -1;
+  }
+}
+
 template <typename R, int L> int zr_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
   const int want = a.tune_tc > 0 ? a.tune_tc : 4;
   switch (want) {
@@ -106,6 +124,10 @@ template <typename R, int L> int zr_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipS
 template <typename R, int L> int z_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
   if (!(a.want_psi || a.want_vort) && a.zkernel != 1 && !a.ker2) {
     const int r = zr_L<R, L>(a, Ly, tw, st);
+    if (r != 0) return r;
+  }
+  if ((a.want_psi || a.want_vort) && a.zkernel != 1 && !a.ker2) {
+    const int r = zrx_T<R, L, 4>(a, Ly, tw, st);
     if (r != 0) return r;
   }
   // tuned tile width (columns per block) for the hot lengths, velocity only
