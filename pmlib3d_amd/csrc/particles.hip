@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "../../include/pmgpu.h"
+
 #include "particles_kernels.h"
 
 #include <rocprim/device/device_scan.hpp>
