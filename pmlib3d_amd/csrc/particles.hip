@@ -4,7 +4,6 @@
 #include <vector>
 
 #include "../../include/pmgpu.h"
-
 #include "particles_kernels.h"
 
 #include <rocprim/device/device_scan.hpp>
