@@ -625,12 +625,15 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
 // vort = zeta w-hat [+ reprojection]).  With every axis periodic the
 // spectral G carries the filter zeta as well when the vorticity is filtered
 // (:339-349).
-template <typename R, int L, int TC, bool ALLPER, bool HALF>
+// SL: the forward spectra wait in LDS (each thread reads back only the words
+// it wrote, so no barrier) instead of registers, for FP64's register budget.
+template <typename R, int L, int TC, bool ALLPER, bool HALF, bool SL>
 __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_rx(ZArgs a, const cplx_t<R> *tw_all) {
   using V = cplx_t<R>;
   constexpr int P = Geo<L>::P, E = L / P;
   constexpr int ST = reg_line_stride<L, sizeof(V), TC>();
   __shared__ V lds[TC * ST];
+  __shared__ V spec[SL ? TC * 3 * L : 1];
   const V *tw = tw_all + a.plan.tw_off;
   const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y);
   const int kx0 = (int)(lt % gridDim.x) * TC, ky = (int)(lt / gridDim.x);
@@ -659,6 +662,14 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_rx(ZArgs a, const cplx_t<R
   V *line = lds + col * ST;
 #pragma unroll
   for (int c = 0; c < 3; ++c) fft_regs<L, -1, V, HALF, false, 1>(w[c], line, p, twd);
+  V *sp = spec + (SL ? col * 3 * L + p : 0);   // word (c, m) at sp[(c E + m) P]
+  if constexpr (SL) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int m = 0; m < E; ++m) sp[(c * E + m) * P] = w[c][m];
+  }
+  auto spw = [&](int c, int m) -> V { return SL ? sp[(c * E + m) * P] : w[c][m]; };
   const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
   const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
   const bool nx = (2 * kx == a.Lx), ny = (2 * ky == a.Ly);
@@ -703,7 +714,7 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_rx(ZArgs a, const cplx_t<R
         const int n = p + P * m;
         const R kk = wavenumber<R>(n, L, a.dk[2], a.ksmp[2]);
         const bool nz = (2 * n == L);
-        const V w0 = w[0][m], w1 = w[1][m], w2 = w[2][m];
+        const V w0 = spw(0, m), w1 = spw(1, m), w2 = spw(2, m);
         const V wc = c == 0 ? w0 : (c == 1 ? w1 : w2);
         if (kind == 0) {
           // u-hat = G 2 pi i (k x w-hat), Nyquist wavenumbers dropped

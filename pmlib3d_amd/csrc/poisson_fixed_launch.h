@@ -5,6 +5,7 @@
 // launcher table (fx_ops) for a few transform lengths, so that the lengths
 // compile in parallel.
 #pragma once
+#include <cstdlib>
 #include "poisson_fixed.h"
 #include "poisson_fixed_ops.h"
 
@@ -93,22 +94,37 @@ template <typename R, int L, int TC> int zr_T(ZArgs a, int Ly, const cplx_t<R> *
     return hipGetLastError() == hipSuccess ? 1 : -1;
   }
 }
-// register-resident full-output z pass (psi and/or filtered vorticity)
-template <typename R, int L, int TC> int zrx_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
-  if constexpr (!regs_ok<L>() || TC * reg_line_stride<L, sizeof(cplx_t<R>), TC>() * (int)sizeof(cplx_t<R>) > kLdsBudget ||
-                Geo<L>::P * TC > 1024) {
+// register-resident full-output z pass (psi and/or filtered vorticity); SL
+// keeps the forward spectra in LDS
+template <typename R, int L, int TC, bool SL>
+int zrx_T(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  constexpr int lds_bytes = (TC * reg_line_stride<L, sizeof(cplx_t<R>), TC>() + (SL ? TC * 3 * L : 0)) *
+                            (int)sizeof(cplx_t<R>);
+  if constexpr (!regs_ok<L>() || lds_bytes > kLdsBudget || Geo<L>::P * TC > 1024) {
     return 0;
   } else {
     if ((double)a.Mz * (double)a.rs * sizeof(cplx_t<R>) >= 2147483647.0) return 0;
     dim3 grid((a.Hx + TC - 1) / TC, Ly);
     if (a.allper)
-      hipLaunchKernelGGL((k_z3_rx<R, L, TC, true, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+      hipLaunchKernelGGL((k_z3_rx<R, L, TC, true, false, SL>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
     else if (2 * a.Mz == L && (L / Geo<L>::P) % 2 == 0)
-      hipLaunchKernelGGL((k_z3_rx<R, L, TC, false, true>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+      hipLaunchKernelGGL((k_z3_rx<R, L, TC, false, true, SL>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
     else
-      hipLaunchKernelGGL((k_z3_rx<R, L, TC, false, false>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
+      hipLaunchKernelGGL((k_z3_rx<R, L, TC, false, false, SL>), grid, dim3(Geo<L>::P * TC), 0, st, a, tw);
     return hipGetLastError() == hipSuccess ? 1 : -1;
   }
+}
+template <typename R, int L> int zrx_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
+  // FP32: spectra in registers, 4 columns per block; FP64: spectra in LDS
+  // (the register form runs at one wave per SIMD), 2 or 1 columns as LDS allows
+  static const int form = [] {
+    const char *e = getenv("PMGPU_ZRX");
+    return e ? atoi(e) : -1;
+  }();
+  const bool sl = form >= 0 ? form == 1 : sizeof(R) == 8;
+  if (!sl) return zrx_T<R, L, 4, false>(a, Ly, tw, st);
+  const int r = zrx_T<R, L, 2, true>(a, Ly, tw, st);
+  return r != 0 ? r : zrx_T<R, L, 1, true>(a, Ly, tw, st);
 }
 
 template <typename R, int L> int zr_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipStream_t st) {
@@ -127,7 +143,7 @@ template <typename R, int L> int z_L(ZArgs a, int Ly, const cplx_t<R> *tw, hipSt
     if (r != 0) return r;
   }
   if ((a.want_psi || a.want_vort) && a.zkernel != 1 && !a.ker2) {
-    const int r = zrx_T<R, L, 4>(a, Ly, tw, st);
+    const int r = zrx_L<R, L>(a, Ly, tw, st);
     if (r != 0) return r;
   }
   // tuned tile width (columns per block) for the hot lengths, velocity only
