@@ -107,13 +107,16 @@ void particles_release() {
 template <typename T, int NS>
 static void p2m_stage_gather(const PGeom &g, const double *xpart, const double *part, int64_t np, int *ids,
                              PartWS &w, int nbins, T *mesh, int accumulate, hipStream_t st) {
+  // the records are planar with np + 1 records per field (the allocation
+  // holds kSlots fields of np + 1)
+  const int64_t rs = np + 1;
   if (np > 0) {
     hipLaunchKernelGGL(k_p2m_binsort, grid_for(nbins), dim3(256), 0, st, w.start.as<int>(), nbins, ids);
     hipLaunchKernelGGL((k_p2m_stage<T, NS>), grid_for(np), dim3(256), 0, st, g, xpart, part, ids,
-                       w.start.as<int>(), nbins, w.rec.as<T>());
+                       w.start.as<int>(), nbins, w.rec.as<T>(), rs);
   }
   const dim3 grid((g.E[0] + 63) / 64, g.E[1], g.E[2]);
-  hipLaunchKernelGGL((k_p2m_rows<T, NS>), grid, dim3(64), 0, st, g, w.start.as<int>(), w.rec.as<T>(), mesh,
+  hipLaunchKernelGGL((k_p2m_rows<T, NS>), grid, dim3(64), 0, st, g, w.start.as<int>(), w.rec.as<T>(), rs, mesh,
                      accumulate);
 }
 

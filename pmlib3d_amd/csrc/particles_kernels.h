@@ -222,7 +222,7 @@ static __global__ void __launch_bounds__(256) k_p2m_binsort(const int *start, in
 // weights in single precision from the FP64 offset).
 template <typename T, int NS>
 __global__ void __launch_bounds__(256) k_p2m_stage(PGeom g, const double *pos, const double *val, const int *ids,
-                                                   const int *start, int nbins, T *rec) {
+                                                   const int *start, int nbins, T *rec, int64_t rs) {
   const int nvalid = start[nbins];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nvalid; t += gridDim.x * blockDim.x) {
     const int q = ids[t];
@@ -242,13 +242,16 @@ __global__ void __launch_bounds__(256) k_p2m_stage(PGeom g, const double *pos, c
           wt[d * NS + s2] = weight_f(NS == 6, s2, (float)dd);
       }
     }
-    T *r = rec + (int64_t)t * PRec<T, NS>::kSlots;
+    // planar records: field i of record t at rec[i * rs + t], so that the
+    // stores of consecutive threads, and the gather's loads of one field of
+    // consecutive records, are contiguous
+    T *r = rec + t;
 #pragma unroll
-    for (int i = 0; i < 3 * NS; ++i) r[i] = wt[i];
-    r[PRec<T, NS>::kVal] = (T)val[3 * q];
-    r[PRec<T, NS>::kVal + 1] = (T)val[3 * q + 1];
-    r[PRec<T, NS>::kVal + 2] = (T)val[3 * q + 2];
-    r[PRec<T, NS>::kBx] = (T)bx;
+    for (int i = 0; i < 3 * NS; ++i) r[i * rs] = wt[i];
+    r[PRec<T, NS>::kVal * rs] = (T)val[3 * q];
+    r[(PRec<T, NS>::kVal + 1) * rs] = (T)val[3 * q + 1];
+    r[(PRec<T, NS>::kVal + 2) * rs] = (T)val[3 * q + 2];
+    r[PRec<T, NS>::kBx * rs] = (T)bx;
   }
 }
 
@@ -275,30 +278,31 @@ template <typename T, int NS> struct RowStage {
 };
 
 template <typename T, int NS>
-__device__ __forceinline__ void row_fetch(RowStage<T, NS> &st, const T *rec, int t0, int n, int lane, int sy,
-                                          int sz, int x0, int blo0) {
-  constexpr int K = PRec<T, NS>::kSlots, KV = PRec<T, NS>::kVal, KB = PRec<T, NS>::kBx;
+__device__ __forceinline__ void row_fetch(RowStage<T, NS> &st, const T *rec, int64_t rs, int t0, int n, int lane,
+                                          int sy, int sz, int x0, int blo0) {
+  constexpr int KV = PRec<T, NS>::kVal, KB = PRec<T, NS>::kBx;
 #pragma unroll
   for (int h = 0; h < kRowCap / 64; ++h) {
     const int i = lane + 64 * h;
     // unconditional loads (no branch for the wait counters to merge over):
     // an empty row reads record t0 <= np, inside the allocation of np + 1
     const int t = t0 + (i < n ? i : (n > 0 ? n - 1 : 0));
-    const T *r = static_cast<const T *>(__builtin_assume_aligned(rec + (int64_t)t * K, 16));
+    const T *r = rec + t;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) st.val[h][s] = r[s];
-    st.val[h][NS] = r[NS + sy];
-    st.val[h][NS + 1] = r[2 * NS + sz];
-    st.val[h][NS + 2] = r[KV];
-    st.val[h][NS + 3] = r[KV + 1];
-    st.val[h][NS + 4] = r[KV + 2];
+    for (int s = 0; s < NS; ++s) st.val[h][s] = r[s * rs];
+    st.val[h][NS] = r[(NS + sy) * rs];
+    st.val[h][NS + 1] = r[(2 * NS + sz) * rs];
+    st.val[h][NS + 2] = r[KV * rs];
+    st.val[h][NS + 3] = r[(KV + 1) * rs];
+    st.val[h][NS + 4] = r[(KV + 2) * rs];
     // x slot of lane 0: sx = lane + (x0 + ns - 1 - kx)
-    st.val[h][NS + 5] = (T)(x0 + NS - 1 - ((int)r[KB] - blo0));
+    st.val[h][NS + 5] = (T)(x0 + NS - 1 - ((int)r[KB * rs] - blo0));
   }
 }
 
 template <typename T, int NS>
-__global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, const T *rec, T *mesh, int accumulate) {
+__global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, const T *rec, int64_t rs, T *mesh,
+                                                 int accumulate) {
   constexpr int NV = RowStage<T, NS>::NV, NR = NS * NS;
   constexpr bool F64 = sizeof(T) == sizeof(double);
   __shared__ T lds[NV][kRowCap];
@@ -365,7 +369,7 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
     // rows in order (the loop is unrolled: every index is static); row r + 1's
     // first pass loads while row r is summed
     RowStage<T, NS> cur, nxt;
-    row_fetch<T, NS>(cur, rec, rt0[0], min(rt1[0] - rt0[0], kRowCap), lane, NS - 1, NS - 1, x0, blo0);
+    row_fetch<T, NS>(cur, rec, rs, rt0[0], min(rt1[0] - rt0[0], kRowCap), lane, NS - 1, NS - 1, x0, blo0);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const int T0 = rt0[r], n = rt1[r] - rt0[r], a0 = ra0[r], a1 = ra1[r];
@@ -375,14 +379,14 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
         __syncthreads();
       }
       if (r + 1 < NR)
-        row_fetch<T, NS>(nxt, rec, rt0[min(r + 1, NR - 1)], min(rt1[min(r + 1, NR - 1)] - rt0[min(r + 1, NR - 1)], kRowCap),
+        row_fetch<T, NS>(nxt, rec, rs, rt0[min(r + 1, NR - 1)], min(rt1[min(r + 1, NR - 1)] - rt0[min(r + 1, NR - 1)], kRowCap),
                          lane, NS - 1 - (r + 1) % NS, NS - 1 - (r + 1) / NS, x0, blo0);
       if (n > 0) {
         sum(a0, min(a1, T0 + kRowCap), T0);
         __syncthreads();
         for (int p = kRowCap; p < n; p += kRowCap) {   // rows of more than kRowCap records
           const int m = min(n - p, kRowCap);
-          row_fetch<T, NS>(cur, rec, T0 + p, m, lane, sy, sz, x0, blo0);
+          row_fetch<T, NS>(cur, rec, rs, T0 + p, m, lane, sy, sz, x0, blo0);
           to_lds(cur, m);
           __syncthreads();
           sum(max(a0, T0 + p), min(a1, T0 + p + m), T0 + p);
