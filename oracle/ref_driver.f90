@@ -38,6 +38,9 @@ PROGRAM ref_driver
   ! the same client code on the GPU subroutines (pmlib3d_amd/fortran)
   USE pmlib_mod_particles, ONLY: class_particles
   USE pmgpu_pmlib
+  ! the C entry point a client calls in place of its part_deriv_vorticity
+  USE pmgpu_iface, ONLY: pmgpu_geom, pmgpu_vorticity_rhs
+  USE, INTRINSIC :: ISO_C_BINDING, ONLY: C_INT32_T
   ! the client's penalisation globals and the drop-in mod_penalisation
   USE mod_parameters, ONLY: penal_relaxation, penal_tolerance, penal_max, vel_inf, reproject, &
        & client_dtime => dtime
@@ -98,6 +101,9 @@ PROGRAM ref_driver
   REAL(MK)                 :: tp2m, tsolve, tm2p, tadv, tremesh
   REAL(MK), ALLOCATABLE    :: treps(:), pos0(:,:), vort0(:,:)
   INTEGER                  :: np0, ntot
+#ifdef PMGPU_SHIM
+  TYPE(pmgpu_geom)         :: rhs_geom
+#endif
 
   CALL MPI_INIT(ierr)
   CALL MPI_COMM_RANK(MPI_COMM_WORLD, rank, ierr)
@@ -384,6 +390,24 @@ PROGRAM ref_driver
 #endif
 
 #ifdef PMGPU_SHIM
+  CASE ('rhs')
+    ! in: vort, vel, dvort (3, E...); out: dvort from pmgpu_vorticity_rhs
+    ! through its BIND(C) interface, called as a client would call it in
+    ! place of part_deriv_vorticity
+    pmlib_fd_order = fd_order
+    OPEN(10, FILE=TRIM(infile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+    READ(10) mesh%vort
+    READ(10) mesh%vel
+    READ(10) mesh%dvort
+    CLOSE(10)
+    CALL geom_of(topo_all%cuboid(rank), rhs_geom, ierr, 'rhs')
+    IF (ierr .NE. 0) CALL die('rhs geometry')
+    IF (pmgpu_vorticity_rhs(rhs_geom, mesh_ptr(mesh%vort), mesh_ptr(mesh%vel), visc, &
+         & INT(pmlib_fd_order, C_INT32_T), mesh_ptr(mesh%dvort)) .NE. 0) CALL die('pmgpu_vorticity_rhs')
+    OPEN(20, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(20) mesh%dvort
+    CLOSE(20)
+
   CASE ('penal')
     ! in: vel, vort (3, E...), chi (E...), [solid_vel (3, E...)];
     ! out: vel, vort, force(3), iterations

@@ -170,3 +170,20 @@ def test_shim_m2m_matches_reference(name):
     nb = d["coarse"].size
     assert rel_linf(r[:nb].reshape(d["coarse"].shape, order="F"), d["down"]) <= 1e-14
     assert rel_linf(r[nb:].reshape(d["base"].shape, order="F"), d["up"]) <= 1e-14
+
+
+RHSS = sorted(f for f in os.listdir(GOLD) if f.startswith("ref_rhs_"))
+
+
+@pytest.mark.parametrize("name", RHSS)
+def test_shim_vorticity_rhs_matches_client_routine_bitwise(name):
+    """pmgpu_vorticity_rhs called through its BIND(C) interface
+    (pmgpu_iface.f90: geometry by reference, meshes as C_PTR values, visc and
+    fd_order by value), as a client calls it in place of its
+    part_deriv_vorticity, against that routine's own output."""
+    d = np.load(os.path.join(GOLD, name), allow_pickle=False)
+    ncell, ng, bc, io, fd = json.loads(str(d["spec"]))
+    g = O.make_geom(ncell, ng, [-0.5, -0.6, -0.7], 1.0 / 8, bc, io)
+    raw, _ = run(dict(O.geom_job(g), mode="rhs", fd_order=fd, visc=float(d["visc"]), interp_order=io),
+                 [d["vort"], d["vel"], d["dvort_in"]])
+    assert np.array_equal(np.frombuffer(raw, dtype=np.float64).reshape(d["dvort"].shape, order="F"), d["dvort"])
