@@ -217,8 +217,11 @@ int pmgpu_remesh_device(const pmgpu_geom *g, int32_t precision, const void *d_vo
 /* Replaces pmlib_particles_advance(patch, topo, part, itimestage, dtime, ierr,
  * euler_vort) (particles/pmlib_particles_advance.f:5-6) on one rank: the
  * low-storage RK stage update of order rk_order (1-3) followed by the
- * periodic wrap / free-side removal of pmlib_particles_map.  The arrays are
- * compacted in place; the new count is returned in *np_out. */
+ * periodic wrap / free-side removal of pmlib_particles_map.  The arrays the
+ * map carries (particles/pmlib_particles_advance.f:121-166: pos and vort,
+ * and above first order vel_rk and dvort_rk) are reordered and compacted in
+ * place; vel and dvort are not carried, as in the reference, and keep the
+ * values the update left in them.  The new count is returned in *np_out. */
 int pmgpu_particles_advance(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
                             int32_t euler_vort, double *pos, double *vel, double *vort, double *dvort,
                             double *vel_rk, double *dvort_rk, int64_t np, int64_t *np_out);

@@ -874,8 +874,9 @@ static void rk_coeffs(int order, int stage, int euler_vort, double *a1, double *
 
 /* pmlib_particles_advance on one rank: the RK stage update, then per
  * direction the stable three-way partition of pmlib_particles_map
- * (moved west & wrapped, moved east & wrapped, staying; free sides drop).
- * Arrays are (3, np) in/out; returns the new particle count. */
+ * (moved west & wrapped, moved east & wrapped, staying; free sides drop)
+ * applied to the arrays the map carries.  Arrays are (3, np) in/out; the
+ * carried ones are compacted to the new count, which is returned. */
 int64_t or_advance(const or_geom *g, int rk_order, int stage, double dt, int euler_vort,
                    double *pos, double *vel, double *vort, double *dvort, double *vel_rk,
                    double *dvort_rk, int64_t np) {
@@ -889,7 +890,11 @@ int64_t or_advance(const or_geom *g, int rk_order, int stage, double dt, int eul
       pos[o] = pos[o] + c1 * vel_rk[o];
       vort[o] = vort[o] + c2 * dvort_rk[o];
     }
-  double *arrs[6] = {pos, vel, vort, dvort, vel_rk, dvort_rk};
+  /* the arrays the map carries (pmlib_particles_advance.f:121-166): pos and
+   * vort, and above first order vel_rk and dvort_rk; vel and dvort are not
+   * carried and keep what the update left in them */
+  double *arrs[4] = {pos, vort, vel_rk, dvort_rk};
+  const int narr = rk_order > 1 ? 4 : 2;
   int64_t *perm = (int64_t *)malloc(sizeof(int64_t) * (np > 0 ? np : 1));
   double *tmp = (double *)malloc(sizeof(double) * 3 * (np > 0 ? np : 1));
   int8_t *cls = (int8_t *)malloc((size_t)(np > 0 ? np : 1));
@@ -915,7 +920,7 @@ int64_t or_advance(const or_geom *g, int rk_order, int stage, double dt, int eul
           perm[n++] = q;
         }
       }
-      for (int a = 0; a < 6; ++a) {
+      for (int a = 0; a < narr; ++a) {
         for (int64_t q = 0; q < n; ++q)
           for (int c = 0; c < 3; ++c) tmp[3 * q + c] = arrs[a][3 * perm[q] + c];
         memcpy(arrs[a], tmp, sizeof(double) * 3 * (size_t)n);

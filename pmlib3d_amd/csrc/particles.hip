@@ -47,6 +47,17 @@ static int scan_exclusive(const int *in, int *out, int64_t n, Scratch &tmp, hipS
   return 0;
 }
 
+// Per-particle counts of west movers, east movers and drops (the migration
+// classes 1, 2, 3) summed by one scan: exclusive offsets off[0] = 0,
+// off[q + 1] = inclusive scan.
+struct Mig3 {
+  int w, e, d;
+};
+struct Mig3Plus {
+  __host__ __device__ Mig3 operator()(const Mig3 &x, const Mig3 &y) const { return Mig3{x.w + y.w, x.e + y.e, x.d + y.d}; }
+};
+__global__ void k_zero_first3(Mig3 *off) { off[0] = Mig3{0, 0, 0}; }
+
 // ------------------------------------------------------------ geometry
 static int make_pgeom(const pmgpu_geom *g, PGeom &pg) {
   PM_CHECK(g, "pmgpu: null geometry");
@@ -83,7 +94,7 @@ static dim3 grid_for(int64_t n, int threads = 256) {
 
 // Workspace shared by the particle entry points of one process.
 struct PartWS {
-  Scratch key, ids, sort, count, start, scan, flag, offs, cls, tmp6, rec;
+  Scratch key, ids, sort, count, start, scan, flag, offs, cls, migbuf, rec;
 };
 // Allocated on first use and released by pmgpu_finalize (never from a static
 // destructor, which may run after the HIP runtime has shut down).
@@ -278,30 +289,60 @@ int remesh_device(const pmgpu_geom *geom, int prec, const void *vort_m, const vo
 }
 
 // ------------------------------------------------------------ advance
-__global__ void k_class_keys(const int *cls, int64_t np, int want, int *key) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x)
-    key[q] = cls[q] == want ? 1 : 0;
-}
+// The map of the eager path works on an index list: each pass of each axis
+// partitions the list (west movers, then east movers, then stayers, each in
+// list order; drops removed) and wraps the axis coordinate it carries along,
+// so that the particle arrays are permuted once, after the last axis.  The
+// final order and every wrapped coordinate are those of permuting the arrays
+// at every pass, which is what the reference's map does.
 
-// destination of particle q: west movers, then east movers, then stayers,
-// each in their original order; dropped particles are not copied
-__global__ void k_migrate(int64_t np, const int *cls, const int *o1, const int *o2, const int *o0, int n1, int n2,
-                          int d, double len, double *const *src, double *const *dst, int narr) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    int c = cls[q];
+// coordinate d of the listed particles (the identity list when perm is null)
+__global__ void k_list_coord(int64_t n, const int *perm, const double *pos, int d, double *cd) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    cd[i] = pos[3 * (int64_t)(perm ? perm[i] : (int)i) + d];
+}
+// migration class of each listed particle (0 stay, 1 west, 2 east, 3 drop)
+// with its class indicators for the offset scan and the class counts
+__global__ void k_list_classify(int64_t n, const double *cd, double lo, double hi, int per, int *cls, Mig3 *key,
+                                int *counts) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double x = cd[i];
+    int c = x < lo ? 1 : (x >= hi ? 2 : 0);
+    if (c != 0 && !per) c = 3;
+    cls[i] = c;
+    key[i] = Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0};
+    if (c != 0) atomicAdd(&counts[c], 1);
+  }
+}
+// one partition pass of the list; movers' coordinates wrapped by one period
+__global__ void k_list_partition(int64_t n, const int *cls, const Mig3 *off, int n1, int n2, double len,
+                                 const int *perm, const double *cd, int *perm2, double *cd2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = cls[i];
     if (c == 3) continue;
-    int64_t to = c == 1 ? o1[q] : (c == 2 ? n1 + o2[q] : n1 + n2 + o0[q]);
-    for (int a = 0; a < narr; ++a) {
-      double v0 = src[a][3 * q], v1 = src[a][3 * q + 1], v2 = src[a][3 * q + 2];
-      if (a == 0) {
-        double &x = d == 0 ? v0 : (d == 1 ? v1 : v2);
-        if (c == 1) x = x + len;
-        if (c == 2) x = x - len;
-      }
-      dst[a][3 * to] = v0;
-      dst[a][3 * to + 1] = v1;
-      dst[a][3 * to + 2] = v2;
-    }
+    const Mig3 o = off[i];
+    const int64_t to = c == 1 ? o.w : (c == 2 ? n1 + o.e : n1 + n2 + (i - o.w - o.e - o.d));
+    double x = cd[i];
+    if (c == 1) x = x + len;
+    if (c == 2) x = x - len;
+    perm2[to] = perm ? perm[i] : (int)i;
+    cd2[to] = x;
+  }
+}
+// the wrapped coordinates back into their particles' rows
+__global__ void k_list_store_coord(int64_t n, const int *perm, const double *cd, int d, double *pos) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    pos[3 * (int64_t)perm[i] + d] = cd[i];
+}
+struct Arr4 {
+  double *a[4];
+};
+// row i of each carried array from row perm[i]
+__global__ void k_list_gather(int64_t n, const int *perm, Arr4 src, Arr4 dst, int narr) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = i / 3, c = i - 3 * q;
+    const int64_t from = 3 * (int64_t)perm[q] + c;
+    for (int a = 0; a < narr; ++a) dst.a[a][i] = src.a[a][from];
   }
 }
 
@@ -360,48 +401,68 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
       return 0;
     }
   }
+  // the arrays the reference's map carries (pmlib_particles_advance.f:121-166):
+  // pos and vort, and above first order vel_rk and dvort_rk; vel and dvort
+  // keep what the update left in them
+  const int carry[4] = {0, 2, 4, 5};
+  const int narr = rk_order > 1 ? 4 : 2;
+  const int64_t n0 = n;
+  // list workspace: cls, two index lists (ints), keys and offsets (Mig3),
+  // two coordinate columns and the gathered rows (doubles)
+  const size_t ni = (size_t)n0 + 1;
+  if (w.cls.ensure(sizeof(int) * 3 * ni + sizeof(Mig3) * 2 * ni + 16) || w.count.ensure(sizeof(int) * 8) ||
+      w.migbuf.ensure(sizeof(double) * (2 * ni + 3 * ni * narr)))
+    return -1;
+  int *cls = w.cls.as<int>();
+  int *plist[2] = {cls + ni, cls + 2 * ni};
+  Mig3 *key = reinterpret_cast<Mig3 *>(cls + 3 * ni), *off = key + ni;
+  double *cdl[2] = {w.migbuf.as<double>(), w.migbuf.as<double>() + ni};
+  int cur = 0;
+  bool listed = false;   // plist[cur] holds the list (else the identity)
   for (int d = 0; d < 3 && n > 0; ++d) {
     const double lo = geom->xmin[d], hi = geom->xmax[d], len = geom->xmax[d] - geom->xmin[d];
     const int per = geom->bound_cond[d] != 0;
+    hipLaunchKernelGGL(k_list_coord, grid_for(n), dim3(256), 0, st, n, listed ? (const int *)plist[cur] : nullptr,
+                       (const double *)arr[0], d, cdl[cur]);
+    bool wrapped = false;
     for (int pass = 0; pass < 64; ++pass) {
-      if (w.cls.ensure(sizeof(int) * (n + 1) * 5) || w.count.ensure(sizeof(int) * 8)) return -1;
-      int *cls = w.cls.as<int>();
-      int *o1 = cls + (n + 1), *o2 = o1 + (n + 1), *o0 = o2 + (n + 1), *kk = o0 + (n + 1);
       PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * 4, st));
-      hipLaunchKernelGGL(k_classify, grid_for(n), dim3(256), 0, st, n, arr[0], d, lo, hi, per, cls,
-                         w.count.as<int>());
+      hipLaunchKernelGGL(k_list_classify, grid_for(n), dim3(256), 0, st, n, (const double *)cdl[cur], lo, hi, per, cls,
+                         key, w.count.as<int>());
       int counts[4];
       PM_HIP(hipMemcpyAsync(counts, w.count.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
       PM_HIP(hipStreamSynchronize(st));
       if (counts[1] == 0 && counts[2] == 0 && counts[3] == 0) break;  // nobody moves
-      int *outs[3] = {o1, o2, o0};
-      int wants[3] = {1, 2, 0};
-      for (int c = 0; c < 3; ++c) {
-        hipLaunchKernelGGL(k_class_keys, grid_for(n), dim3(256), 0, st, cls, n, wants[c], kk);
-        if (scan_exclusive(kk, outs[c], n, w.scan, st)) return -1;
-      }
-      int64_t nnew = n - counts[3];
-      if (w.tmp6.ensure(sizeof(double) * 3 * (n + 1) * 6 + sizeof(double *) * 12)) return -1;
-      double *base = w.tmp6.as<double>();
-      double *dst[6];
-      for (int a = 0; a < 6; ++a) dst[a] = base + (size_t)a * 3 * (n + 1);
-      double **ptrs = reinterpret_cast<double **>(base + (size_t)6 * 3 * (n + 1));
-      double *hp[12];
-      for (int a = 0; a < 6; ++a) {
-        hp[a] = arr[a];
-        hp[6 + a] = dst[a];
-      }
-      PM_HIP(hipMemcpyAsync(ptrs, hp, sizeof(hp), hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_migrate, grid_for(n), dim3(256), 0, st, n, cls, o1, o2, o0, counts[1], counts[2], d, len,
-                         ptrs, ptrs + 6, 6);
+      hipLaunchKernelGGL(k_zero_first3, dim3(1), dim3(1), 0, st, off);
+      size_t bytes = 0;
+      PM_HIP(rocprim::inclusive_scan(nullptr, bytes, key, off + 1, (size_t)n, Mig3Plus(), st));
+      if (w.scan.ensure(bytes + 16)) return -1;
+      PM_HIP(rocprim::inclusive_scan(w.scan.p, bytes, key, off + 1, (size_t)n, Mig3Plus(), st));
+      hipLaunchKernelGGL(k_list_partition, grid_for(n), dim3(256), 0, st, n, (const int *)cls, (const Mig3 *)off,
+                         counts[1], counts[2], len, listed ? (const int *)plist[cur] : nullptr,
+                         (const double *)cdl[cur], plist[1 - cur], cdl[1 - cur]);
       PM_HIP(hipGetLastError());
-      for (int a = 0; a < 6; ++a)
-        PM_HIP(hipMemcpyAsync(arr[a], dst[a], sizeof(double) * 3 * nnew, hipMemcpyDeviceToDevice, st));
-      PM_HIP(hipStreamSynchronize(st));
-      n = nnew;
+      cur = 1 - cur;
+      listed = true;
+      wrapped = wrapped || counts[1] + counts[2] > 0;
+      n -= counts[3];
       if (!per || counts[1] + counts[2] == 0) break;
     }
+    if (wrapped && n > 0)
+      hipLaunchKernelGGL(k_list_store_coord, grid_for(n), dim3(256), 0, st, n, (const int *)plist[cur],
+                         (const double *)cdl[cur], d, arr[0]);
   }
+  if (listed && n > 0) {
+    Arr4 src{}, dst{};
+    for (int a = 0; a < narr; ++a) {
+      src.a[a] = arr[carry[a]];
+      dst.a[a] = w.migbuf.as<double>() + 2 * ni + (size_t)a * 3 * ni;
+    }
+    hipLaunchKernelGGL(k_list_gather, grid_for(3 * n), dim3(256), 0, st, n, (const int *)plist[cur], src, dst, narr);
+    for (int a = 0; a < narr; ++a)
+      PM_HIP(hipMemcpyAsync(src.a[a], dst.a[a], sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+  }
+  PM_HIP(hipGetLastError());
   *np_out = n;
   return 0;
 }
@@ -581,12 +642,6 @@ struct Arr6 {
   double *a[6];
 };
 
-struct Mig3 {
-  int w, e, d;
-};
-struct Mig3Plus {
-  __host__ __device__ Mig3 operator()(const Mig3 &x, const Mig3 &y) const { return Mig3{x.w + y.w, x.e + y.e, x.d + y.d}; }
-};
 
 __global__ void k_fill_int(int *p, int64_t n, int v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -600,8 +655,6 @@ __global__ void k_mig_keys(NCount npc, int64_t cap, const int *cls, Mig3 *key) {
     key[q] = Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0};
   }
 }
-// exclusive offsets: off[0] = 0, off[q + 1] = inclusive scan
-__global__ void k_zero_first3(Mig3 *off) { off[0] = Mig3{0, 0, 0}; }
 
 // west movers, east movers, then stayers, each in their original order; a
 // stayer's rank is q minus the movers and drops before it

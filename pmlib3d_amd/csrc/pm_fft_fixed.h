@@ -149,8 +149,23 @@ template <int L, typename V> __device__ __forceinline__ void twid_load(Twid<L, V
   TwidLoad<L, 0, V>::run(t, p, tw);
 }
 
-template <int L, int S, int R, int NS, bool WAVE, typename V, int PM = 0>
-__device__ __forceinline__ void stage(V *line, int p, const V *twrow) {
+// v[r] *= w^r (conjugated for S > 0), r = 1 .. R - 1, for butterfly m of
+// stage I, the powers of the base twiddle by a multiplication chain
+template <int L, int I, int S, int R, typename V>
+__device__ __forceinline__ void twiddle(V *v, const Twid<L, V> &t, int m) {
+  V w1 = t.w[I][m];
+  if (S > 0) w1.y = -w1.y;
+  V w = w1;
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    v[r] = cmul(v[r], w);
+    if (r + 1 < R) w = cmul(w, w1);
+  }
+}
+
+template <int L, int S, int I, bool WAVE, typename V, int PM>
+__device__ __forceinline__ void stage(V *line, int p, const Twid<L, V> &tws) {
+  constexpr int R = radix_at(L, I), NS = ns_before(L, I);
   constexpr int P = Geo<L>::P;
   constexpr int NBF = L / R;
   constexpr int MB = (NBF + P - 1) / P;
@@ -169,16 +184,7 @@ __device__ __forceinline__ void stage(V *line, int p, const V *twrow) {
     const int j = p + m * P;
     if (MB * P == NBF || j < NBF) {
       const int k = NS == 1 ? 0 : j % NS;
-      if (NS > 1) {
-        V w1 = twrow[m];
-        if (S > 0) w1.y = -w1.y;
-        V w = w1;
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          v[m][r] = cmul(v[m][r], w);
-          if (r + 1 < R) w = cmul(w, w1);
-        }
-      }
+      if (NS > 1) twiddle<L, I, S, R>(v[m], tws, m);
       Dft<R, S, V>::run(v[m]);
       const int base = (j - k) * R + k;
 #pragma unroll
@@ -194,7 +200,7 @@ template <int L, int S, int I, bool WAVE, typename V, int PM> struct StageLoop<L
 };
 template <int L, int S, int I, bool WAVE, typename V, int PM> struct StageLoop<L, S, I, WAVE, V, PM, false> {
   static __device__ __forceinline__ void run(V *line, int p, const Twid<L, V> &t) {
-    stage<L, S, radix_at(L, I), ns_before(L, I), WAVE, V, PM>(line, p, t.w[I]);
+    stage<L, S, I, WAVE, V, PM>(line, p, t);
     StageLoop<L, S, I + 1, WAVE, V, PM>::run(line, p, t);
   }
 };
@@ -226,7 +232,7 @@ template <int L, int S, int I, typename V, int PM> struct MidStages<L, S, I, V, 
 };
 template <int L, int S, int I, typename V, int PM> struct MidStages<L, S, I, V, PM, false> {
   static __device__ __forceinline__ void run(V *line, int p, const Twid<L, V> &t) {
-    stage<L, S, radix_at(L, I), ns_before(L, I), false, V, PM>(line, p, t.w[I]);
+    stage<L, S, I, false, V, PM>(line, p, t);
     MidStages<L, S, I + 1, V, PM>::run(line, p, t);
   }
 };
@@ -260,14 +266,7 @@ __device__ __forceinline__ void fft_regs(V (&x)[L / Geo<L>::P], V *line, int p, 
       V v[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) v[r] = line[padm<PM>(jj + r * NBF)];
-      V w1 = t.w[I][b];
-      if (S > 0) w1.y = -w1.y;
-      V w = w1;
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        v[r] = cmul(v[r], w);
-        if (r + 1 < R) w = cmul(w, w1);
-      }
+      twiddle<L, I, S, R>(v, t, b);
       Dft<R, S, V>::run(v);
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -275,110 +274,6 @@ __device__ __forceinline__ void fft_regs(V (&x)[L / Geo<L>::P], V *line, int p, 
     }
   }
   __syncthreads();  // the line buffer is free again
-}
-
-// NL lines transformed in lockstep (line l at line0 + l * lstride): the same
-// as NL calls of fft_regs, with one barrier per exchange for all of them.
-template <int L, int S, int R, int NS, int NL, typename V>
-__device__ __forceinline__ void stage_multi(V *line0, int lstride, int p, const V *twrow) {
-  constexpr int P = Geo<L>::P;
-  constexpr int NBF = L / R;
-  constexpr int MB = (NBF + P - 1) / P;
-  V v[NL][MB][R];
-#pragma unroll
-  for (int l = 0; l < NL; ++l)
-#pragma unroll
-    for (int m = 0; m < MB; ++m) {
-      const int j = p + m * P;
-      if (MB * P == NBF || j < NBF) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[l][m][r] = line0[l * lstride + pad(j + r * NBF)];
-      }
-    }
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < MB; ++m) {
-    const int j = p + m * P;
-    if (MB * P == NBF || j < NBF) {
-      const int k = NS == 1 ? 0 : j % NS;
-      V w1 = twrow[m];
-      if (S > 0) w1.y = -w1.y;
-#pragma unroll
-      for (int l = 0; l < NL; ++l) {
-        if (NS > 1) {
-          V w = w1;
-#pragma unroll
-          for (int r = 1; r < R; ++r) {
-            v[l][m][r] = cmul(v[l][m][r], w);
-            if (r + 1 < R) w = cmul(w, w1);
-          }
-        }
-        Dft<R, S, V>::run(v[l][m]);
-        const int base = (j - k) * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) line0[l * lstride + pad(base + r * NS)] = v[l][m][r];
-      }
-    }
-  }
-  __syncthreads();
-}
-
-template <int L, int S, int I, int NL, typename V, bool DONE = (I >= nstages(L) - 1)> struct MidStagesMulti;
-template <int L, int S, int I, int NL, typename V> struct MidStagesMulti<L, S, I, NL, V, true> {
-  static __device__ __forceinline__ void run(V *, int, int, const Twid<L, V> &) {}
-};
-template <int L, int S, int I, int NL, typename V> struct MidStagesMulti<L, S, I, NL, V, false> {
-  static __device__ __forceinline__ void run(V *line0, int lstride, int p, const Twid<L, V> &t) {
-    stage_multi<L, S, radix_at(L, I), ns_before(L, I), NL>(line0, lstride, p, t.w[I]);
-    MidStagesMulti<L, S, I + 1, NL, V>::run(line0, lstride, p, t);
-  }
-};
-
-template <int L, int S, int NL, typename V>
-__device__ __forceinline__ void fft_regs_multi(V (&x)[NL][L / Geo<L>::P], V *line0, int lstride, int p,
-                                               const Twid<L, V> &t) {
-  constexpr int P = Geo<L>::P;
-  {
-    constexpr int R = radix_at(L, 0), NBF = L / R, MB = NBF / P;
-#pragma unroll
-    for (int l = 0; l < NL; ++l)
-#pragma unroll
-      for (int b = 0; b < MB; ++b) {
-        V v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = x[l][b + MB * r];
-        Dft<R, S, V>::run(v);
-        const int jj = p + b * P;
-#pragma unroll
-        for (int r = 0; r < R; ++r) line0[l * lstride + pad(jj * R + r)] = v[r];
-      }
-  }
-  __syncthreads();
-  MidStagesMulti<L, S, 1, NL, V>::run(line0, lstride, p, t);
-  {
-    constexpr int I = nstages(L) - 1, R = radix_at(L, I), NBF = L / R, MB = NBF / P;
-#pragma unroll
-    for (int l = 0; l < NL; ++l)
-#pragma unroll
-      for (int b = 0; b < MB; ++b) {
-        const int jj = p + b * P;
-        V v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = line0[l * lstride + pad(jj + r * NBF)];
-        V w1 = t.w[I][b];
-        if (S > 0) w1.y = -w1.y;
-        V w = w1;
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          v[r] = cmul(v[r], w);
-          if (r + 1 < R) w = cmul(w, w1);
-        }
-        Dft<R, S, V>::run(v);
-#pragma unroll
-        for (int r = 0; r < R; ++r) x[l][b + MB * r] = v[r];
-      }
-  }
-  __syncthreads();
 }
 
 }  // namespace fx

@@ -557,12 +557,13 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
 #pragma unroll
   for (int c = 0; c < 3; ++c) fft_regs<L, -1, V, HALF, false, 1>(w[c], line, p, twd);
   // u-hat = G 2 pi i (k x w-hat); Nyquist wavenumbers drop out (Hermitian
-  // projection of the real inverse)
+  // projection of the real inverse); the 1 / L of the inverse transform
+  // rides on the same factor
   const R ki = wavenumber<R>(kx, a.Lx, a.dk[0], a.ksmp[0]);
   const R kj = wavenumber<R>(ky, a.Ly, a.dk[1], a.ksmp[1]);
   const R kx_ = (2 * kx == a.Lx) ? (R)0 : ki, ky_ = (2 * ky == a.Ly) ? (R)0 : kj;
   const int kyf = ky <= a.Ly - ky ? ky : a.Ly - ky;
-  const R twopi = (R)6.28318530717958647692;
+  const R twopi = (R)6.28318530717958647692 / (R)L;
   // G-hat of the thread's rows, all loads issued before any is used (a
   // column past Hx reads column 0 and is masked)
   const R *G = reinterpret_cast<const R *>(a.Ghat) + (int64_t)kyf * a.Hx + (col_ok ? kx : 0);
@@ -603,14 +604,13 @@ __global__ void __launch_bounds__(Geo<L>::P *TC) k_z3_r(ZArgs a, const cplx_t<R>
   // Stores are plain global stores: buffer-resource stores of these
   // 8-byte-aligned records, which the compiler merges into 16-byte stores
   // across components, have returned corrupted first components.
-  const R scale = (R)1 / (R)L;
   if (col_ok) {
     V *dst = base + (int64_t)p * a.rs + col * 3;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       if (HALF ? m < E / 2 : p + P * m < a.Mz) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) dst[(int64_t)P * m * a.rs + c] = cscale(w[c][m], scale);
+        for (int c = 0; c < 3; ++c) dst[(int64_t)P * m * a.rs + c] = w[c][m];
       }
     }
   }

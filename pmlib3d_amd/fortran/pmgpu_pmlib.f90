@@ -541,6 +541,7 @@ CONTAINS
       RETURN
     END IF
     ! particles that left a free side are gone: keep the compacted prefix
+    ! (vel and dvort are not carried by the map; their prefix is the update's)
     np = INT(n)
     CALL shrink_part(part%pos, np)
     CALL shrink_part(part%vel, np)

@@ -136,7 +136,10 @@ def pmlib_remesh(patch, mesh, part, min_vort, min_dvort, dvort=False, vel=False,
 def pmlib_particles_advance(patch, part, itimestage, dtime, euler_vort=False, rk_order=None,
                             interp_order=None):
     """One low-storage RK stage of order pmlib_time_integration_order, then
-    the periodic wrap / free-side removal; part's arrays are replaced."""
+    the periodic wrap / free-side removal; part's arrays are replaced.  The
+    map reorders the arrays the reference's map carries (pos and vort, and
+    above first order vel_rk and dvort_rk); vel and dvort keep the update's
+    order (particles/pmlib_particles_advance.f:121-166)."""
     try:
         g = _geom(patch, interp_order)
         order = params.pmlib_time_integration_order if rk_order is None else rk_order
