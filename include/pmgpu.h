@@ -138,7 +138,8 @@ int pmgpu_poisson_batch_destroy(pmgpu_poisson_batch_t b);
  * meshes; the *_device variants take device pointers, a hipStream_t (NULL:
  * the default stream) and the mesh precision: PMGPU_FP64 reproduces the
  * reference's arithmetic, PMGPU_FP32 uses an FP32 mesh with FP32 kernel
- * weights and Kahan-compensated sums.
+ * weights, Kahan-compensated FP32 sums in deposition and FP64 sums of the
+ * weighted FP32 node values in the gather.
  * ------------------------------------------------------------------------ */
 
 /* Replaces pmlib_interp_particle_mesh(topo, xpart, part, mesh, ierr, clear,

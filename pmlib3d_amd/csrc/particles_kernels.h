@@ -432,8 +432,12 @@ template <typename T> __global__ void k_halo_add(T *m, PGeom g, int d) {
 // (interp_mesh_particle.f:87-409: sz, sy, sx ascending; the M'4 branch scales
 // the y and z offsets by 1/dx(1), the M6* branch reproduces the k5/k6 quirk of
 // :587-590), FP64 rounded per operation as the reference does; the FP32 form
-// (FP32 mesh) takes FP32 weights and Kahan sums.  Stencil weights and indices
-// live in registers (the stencil width is a template parameter).
+// (FP32 mesh) takes FP32 weights, forms their product in FP64 and sums the
+// weighted FP32 node values in FP64 (the output is FP64; one fused
+// multiply-add per term and component, against four FP32 operations of a
+// compensated FP32 sum).
+// Stencil weights and indices live in registers (the stencil width is a
+// template parameter).
 template <typename T, int NS>
 __global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const double *pos, NCount npc, double *out) {
   const int64_t np = npc.get();
@@ -481,15 +485,22 @@ __global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const doubl
 #pragma unroll
     for (int s2 = 0; s2 < NS; ++s2) idx[1][s2] = min(max(idx[1][s2], g.lo[1]), g.hi[1]);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    Kahan k0, k1, k2;
+    // node (sx, sy, sz) at row base (sy, sz) + x offset sx: one 64-bit base
+    // per stencil row, 32-bit offsets along x
+    int xo[NS];
+#pragma unroll
+    for (int sx = 0; sx < NS; ++sx) xo[sx] = 3 * (idx[0][sx] - g.lo[0]);
 #pragma unroll
     for (int sz = 0; sz < NS; ++sz)
 #pragma unroll
       for (int sy = 0; sy < NS; ++sy) {
         const int64_t row = (int64_t)g.E[0] * ((idx[1][sy] - g.lo[1]) + (int64_t)g.E[1] * (idx[2][sz] - g.lo[2]));
+        const T *mrow = mesh + 3 * row;
+        // FP32 form: the FP32 weights' product in FP64, y and z once per row
+        const double wyz = F64 ? 0.0 : (double)w[1][sy] * (double)w[2][sz];
 #pragma unroll
         for (int sx = 0; sx < NS; ++sx) {
-          const T *m = mesh + 3 * ((int64_t)(idx[0][sx] - g.lo[0]) + row);
+          const T *m = mrow + xo[sx];
           if constexpr (F64) {
             const double wt = __dmul_rn(__dmul_rn(w[0][sx], w[1][sy]), w[2][sz]);
             const double t0 = __dmul_rn(wt, m[0]), t1 = __dmul_rn(wt, m[1]), t2 = __dmul_rn(wt, m[2]);
@@ -503,16 +514,17 @@ __global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const doubl
               a2 = __dadd_rn(a2, t2);
             }
           } else {
-            const float wt = w[0][sx] * w[1][sy] * w[2][sz];
-            k0.add(wt * m[0]);
-            k1.add(wt * m[1]);
-            k2.add(wt * m[2]);
+            // weight times FP32 node value fused into the FP64 sum
+            const double wt = (double)w[0][sx] * wyz;
+            a0 = __fma_rn(wt, (double)m[0], a0);
+            a1 = __fma_rn(wt, (double)m[1], a1);
+            a2 = __fma_rn(wt, (double)m[2], a2);
           }
         }
       }
-    out[3 * q] = F64 ? a0 : (double)k0.s;
-    out[3 * q + 1] = F64 ? a1 : (double)k1.s;
-    out[3 * q + 2] = F64 ? a2 : (double)k2.s;
+    out[3 * q] = a0;
+    out[3 * q + 1] = a1;
+    out[3 * q + 2] = a2;
   }
 }
 
