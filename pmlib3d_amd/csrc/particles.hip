@@ -291,10 +291,11 @@ int remesh_device(const pmgpu_geom *geom, int prec, const void *vort_m, const vo
 // ------------------------------------------------------------ advance
 // The map of the eager path works on an index list: each pass of each axis
 // partitions the list (west movers, then east movers, then stayers, each in
-// list order; drops removed) and wraps the axis coordinate it carries along,
-// so that the particle arrays are permuted once, after the last axis.  The
-// final order and every wrapped coordinate are those of permuting the arrays
-// at every pass, which is what the reference's map does.
+// list order; drops removed) and wraps the axis coordinate it carries along
+// (with the columns of earlier axes that wrapped), so that the particle
+// arrays are permuted once, after the last axis.  The final order and every
+// wrapped coordinate are those of permuting the arrays at every pass, which
+// is what the reference's map does.
 
 // coordinate d of the listed particles (the identity list when perm is null)
 __global__ void k_list_coord(int64_t n, const int *perm, const double *pos, int d, double *cd) {
@@ -314,35 +315,42 @@ __global__ void k_list_classify(int64_t n, const double *cd, double lo, double h
     if (c != 0) atomicAdd(&counts[c], 1);
   }
 }
+// coordinate columns moved with the list: the current axis' column (index 0)
+// and those of earlier axes whose particles wrapped
+struct ListCols {
+  const double *src[3];
+  double *dst[3];
+  int n;
+};
 // one partition pass of the list; movers' coordinates wrapped by one period
 __global__ void k_list_partition(int64_t n, const int *cls, const Mig3 *off, int n1, int n2, double len,
-                                 const int *perm, const double *cd, int *perm2, double *cd2) {
+                                 const int *perm, int *perm2, ListCols cols) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int c = cls[i];
     if (c == 3) continue;
     const Mig3 o = off[i];
     const int64_t to = c == 1 ? o.w : (c == 2 ? n1 + o.e : n1 + n2 + (i - o.w - o.e - o.d));
-    double x = cd[i];
+    double x = cols.src[0][i];
     if (c == 1) x = x + len;
     if (c == 2) x = x - len;
     perm2[to] = perm ? perm[i] : (int)i;
-    cd2[to] = x;
+    cols.dst[0][to] = x;
+    for (int k = 1; k < cols.n; ++k) cols.dst[k][to] = cols.src[k][i];
   }
-}
-// the wrapped coordinates back into their particles' rows
-__global__ void k_list_store_coord(int64_t n, const int *perm, const double *cd, int d, double *pos) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    pos[3 * (int64_t)perm[i] + d] = cd[i];
 }
 struct Arr4 {
   double *a[4];
 };
-// row i of each carried array from row perm[i]
-__global__ void k_list_gather(int64_t n, const int *perm, Arr4 src, Arr4 dst, int narr) {
+// row i of each carried array from row perm[i]; the position rows take the
+// wrapped coordinates from the columns of the axes that wrapped (col[d] not
+// null)
+__global__ void k_list_gather(int64_t n, const int *perm, Arr4 src, Arr4 dst, int narr, ListCols col) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t q = i / 3, c = i - 3 * q;
     const int64_t from = 3 * (int64_t)perm[q] + c;
-    for (int a = 0; a < narr; ++a) dst.a[a][i] = src.a[a][from];
+    const double *cc = col.src[c];
+    dst.a[0][i] = cc ? cc[q] : src.a[0][from];
+    for (int a = 1; a < narr; ++a) dst.a[a][i] = src.a[a][from];
   }
 }
 
@@ -408,27 +416,31 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
   const int narr = rk_order > 1 ? 4 : 2;
   const int64_t n0 = n;
   // list workspace: cls, two index lists (ints), keys and offsets (Mig3),
-  // two coordinate columns and the gathered rows (doubles)
+  // two buffers of each axis' coordinate column and the gathered rows
+  // (doubles)
   const size_t ni = (size_t)n0 + 1;
   if (w.cls.ensure(sizeof(int) * 3 * ni + sizeof(Mig3) * 2 * ni + 16) || w.count.ensure(sizeof(int) * 8) ||
-      w.migbuf.ensure(sizeof(double) * (2 * ni + 3 * ni * narr)))
+      w.migbuf.ensure(sizeof(double) * (6 * ni + 3 * ni * narr)))
     return -1;
   int *cls = w.cls.as<int>();
   int *plist[2] = {cls + ni, cls + 2 * ni};
   Mig3 *key = reinterpret_cast<Mig3 *>(cls + 3 * ni), *off = key + ni;
-  double *cdl[2] = {w.migbuf.as<double>(), w.migbuf.as<double>() + ni};
+  double *colbuf[3][2];
+  for (int d = 0; d < 3; ++d)
+    for (int b = 0; b < 2; ++b) colbuf[d][b] = w.migbuf.as<double>() + (size_t)(2 * d + b) * ni;
+  bool carried[3] = {false, false, false};   // axis d wrapped: its column travels with the list
   int cur = 0;
   bool listed = false;   // plist[cur] holds the list (else the identity)
   for (int d = 0; d < 3 && n > 0; ++d) {
     const double lo = geom->xmin[d], hi = geom->xmax[d], len = geom->xmax[d] - geom->xmin[d];
     const int per = geom->bound_cond[d] != 0;
     hipLaunchKernelGGL(k_list_coord, grid_for(n), dim3(256), 0, st, n, listed ? (const int *)plist[cur] : nullptr,
-                       (const double *)arr[0], d, cdl[cur]);
+                       (const double *)arr[0], d, colbuf[d][cur]);
     bool wrapped = false;
     for (int pass = 0; pass < 64; ++pass) {
       PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * 4, st));
-      hipLaunchKernelGGL(k_list_classify, grid_for(n), dim3(256), 0, st, n, (const double *)cdl[cur], lo, hi, per, cls,
-                         key, w.count.as<int>());
+      hipLaunchKernelGGL(k_list_classify, grid_for(n), dim3(256), 0, st, n, (const double *)colbuf[d][cur], lo, hi,
+                         per, cls, key, w.count.as<int>());
       int counts[4];
       PM_HIP(hipMemcpyAsync(counts, w.count.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
       PM_HIP(hipStreamSynchronize(st));
@@ -438,9 +450,17 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
       PM_HIP(rocprim::inclusive_scan(nullptr, bytes, key, off + 1, (size_t)n, Mig3Plus(), st));
       if (w.scan.ensure(bytes + 16)) return -1;
       PM_HIP(rocprim::inclusive_scan(w.scan.p, bytes, key, off + 1, (size_t)n, Mig3Plus(), st));
+      ListCols cols{};
+      cols.src[0] = colbuf[d][cur];
+      cols.dst[0] = colbuf[d][1 - cur];
+      cols.n = 1;
+      for (int e = 0; e < d; ++e)
+        if (carried[e]) {
+          cols.src[cols.n] = colbuf[e][cur];
+          cols.dst[cols.n++] = colbuf[e][1 - cur];
+        }
       hipLaunchKernelGGL(k_list_partition, grid_for(n), dim3(256), 0, st, n, (const int *)cls, (const Mig3 *)off,
-                         counts[1], counts[2], len, listed ? (const int *)plist[cur] : nullptr,
-                         (const double *)cdl[cur], plist[1 - cur], cdl[1 - cur]);
+                         counts[1], counts[2], len, listed ? (const int *)plist[cur] : nullptr, plist[1 - cur], cols);
       PM_HIP(hipGetLastError());
       cur = 1 - cur;
       listed = true;
@@ -448,17 +468,18 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
       n -= counts[3];
       if (!per || counts[1] + counts[2] == 0) break;
     }
-    if (wrapped && n > 0)
-      hipLaunchKernelGGL(k_list_store_coord, grid_for(n), dim3(256), 0, st, n, (const int *)plist[cur],
-                         (const double *)cdl[cur], d, arr[0]);
+    carried[d] = wrapped;
   }
   if (listed && n > 0) {
     Arr4 src{}, dst{};
     for (int a = 0; a < narr; ++a) {
       src.a[a] = arr[carry[a]];
-      dst.a[a] = w.migbuf.as<double>() + 2 * ni + (size_t)a * 3 * ni;
+      dst.a[a] = w.migbuf.as<double>() + 6 * ni + (size_t)a * 3 * ni;
     }
-    hipLaunchKernelGGL(k_list_gather, grid_for(3 * n), dim3(256), 0, st, n, (const int *)plist[cur], src, dst, narr);
+    ListCols col{};
+    for (int d = 0; d < 3; ++d) col.src[d] = carried[d] ? colbuf[d][cur] : nullptr;
+    hipLaunchKernelGGL(k_list_gather, grid_for(3 * n), dim3(256), 0, st, n, (const int *)plist[cur], src, dst, narr,
+                       col);
     for (int a = 0; a < narr; ++a)
       PM_HIP(hipMemcpyAsync(src.a[a], dst.a[a], sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
   }

@@ -571,30 +571,54 @@ __global__ void k_remesh_flags(PGeom g, const T *vort, const T *dvort, const uin
 }
 
 // create particles at cell centres in (k, j, i) order from the scanned flags;
-// vel / dvort are the fields copied (NULL: zero), per the vel / dvort flags
+// vel / dvort are the fields copied (NULL: zero), per the vel / dvort flags.
+// A workgroup takes 256 consecutive cells at a time: their particles are one
+// contiguous range of the outputs, staged in LDS and then written with
+// consecutive threads on consecutive words.  Launched with 256 threads.
+constexpr int kRemeshTile = 256;
 template <typename T>
-__global__ void k_remesh_create(PGeom g, const int *flag, const int *offs, const T *vort, const T *dvort,
-                                const T *vel, double *pos, double *pvort, double *pvel, double *pdvort,
-                                double *vel_rk, double *dvort_rk, int64_t cap) {
+__global__ void __launch_bounds__(kRemeshTile) k_remesh_create(PGeom g, const int *flag, const int *offs,
+                                                               const T *vort, const T *dvort, const T *vel,
+                                                               double *pos, double *pvort, double *pvel,
+                                                               double *pdvort, double *vel_rk, double *dvort_rk,
+                                                               int64_t cap) {
+  __shared__ double stage[4][3 * kRemeshTile];   // pos, vort, vel, dvort of the tile's particles
   const int64_t nn = (int64_t)g.E[0] * g.E[1] * g.E[2];
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nn;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    if (!flag[idx]) continue;
-    int64_t p = offs[idx];
-    if (p >= cap) continue;  // over capacity: the caller reports it
-    int ii[3];
-    ii[0] = (int)(idx % g.E[0]) + g.lo[0];
-    ii[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
-    ii[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
-    for (int d = 0; d < 3; ++d)
-      pos[3 * p + d] = __dadd_rn(g.xmin[d], __dmul_rn(__dadd_rn((double)(ii[d] - 1), 0.5), g.dx[d]));
-    for (int c = 0; c < 3; ++c) {
-      pvort[3 * p + c] = (double)vort[3 * idx + c];
-      if (pvel) pvel[3 * p + c] = vel ? (double)vel[3 * idx + c] : 0.0;
-      if (pdvort) pdvort[3 * p + c] = dvort ? (double)dvort[3 * idx + c] : 0.0;
-      if (vel_rk) vel_rk[3 * p + c] = 0.0;
-      if (dvort_rk) dvort_rk[3 * p + c] = 0.0;
+  const int64_t ntiles = (nn + kRemeshTile - 1) / kRemeshTile;
+  const int tid = threadIdx.x;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t c0 = tile * kRemeshTile, idx = c0 + tid;
+    const int64_t p0 = offs[c0], p1 = offs[min(c0 + kRemeshTile, nn)];
+    if (idx < nn && flag[idx]) {
+      const int s = (int)(offs[idx] - p0);
+      int ii[3];
+      ii[0] = (int)(idx % g.E[0]) + g.lo[0];
+      ii[1] = (int)((idx / g.E[0]) % g.E[1]) + g.lo[1];
+      ii[2] = (int)(idx / ((int64_t)g.E[0] * g.E[1])) + g.lo[2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        stage[0][3 * s + d] = __dadd_rn(g.xmin[d], __dmul_rn(__dadd_rn((double)(ii[d] - 1), 0.5), g.dx[d]));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        stage[1][3 * s + c] = (double)vort[3 * idx + c];
+        stage[2][3 * s + c] = vel ? (double)vel[3 * idx + c] : 0.0;
+        stage[3][3 * s + c] = dvort ? (double)dvort[3 * idx + c] : 0.0;
+      }
     }
+    __syncthreads();
+    // words [3 p0, 3 p1) of every output, past the capacity dropped (the
+    // caller reports an over-capacity count)
+    const int64_t w0 = 3 * p0, w1 = 3 * (p1 < cap ? p1 : (p0 < cap ? cap : p0));
+    for (int64_t wd = w0 + tid; wd < w1; wd += kRemeshTile) {
+      const int k = (int)(wd - w0);
+      pos[wd] = stage[0][k];
+      pvort[wd] = stage[1][k];
+      if (pvel) pvel[wd] = stage[2][k];
+      if (pdvort) pdvort[wd] = stage[3][k];
+      if (vel_rk) vel_rk[wd] = 0.0;
+      if (dvort_rk) dvort_rk[wd] = 0.0;
+    }
+    __syncthreads();   // the stage is free for the next tile
   }
 }
 
