@@ -382,16 +382,13 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
   PM_CHECK(stage >= 1 && stage <= rk_order, "pmgpu: RK stage out of range");
   PM_CHECK(np >= 0 && np < (int64_t)1 << 31, "pmgpu: particle count out of range");
   RkCoef k = rk_coef(rk_order, stage, euler_vort);
-  if (np > 0)
-    hipLaunchKernelGGL(k_rk_update, grid_for(3 * np), dim3(256), 0, st, k, dt, np, arr[0], arr[1], arr[2], arr[3],
-                       arr[4], arr[5]);
-  PM_HIP(hipGetLastError());
   PartWS &w = ws();
   int64_t n = np;
   if (n > 0) {
-    // one pass over the three axes first: when no particle is outside the
-    // patch on any axis, no axis below moves or drops one (its classification
-    // sees the same positions), so one read-back decides for all three
+    // the update flags a particle it leaves outside the patch on any axis:
+    // when there is none, no axis below moves or drops one (its
+    // classification sees the same positions), so one read-back decides for
+    // all three
     PBox box;
     for (int d = 0; d < 3; ++d) {
       box.lo[d] = geom->xmin[d];
@@ -399,7 +396,8 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
     }
     if (w.count.ensure(sizeof(int) * 8)) return -1;
     PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_any_outside, grid_for(n), dim3(256), 0, st, n, arr[0], box, w.count.as<int>());
+    hipLaunchKernelGGL(k_rk_update, grid_for(3 * n), dim3(256), 0, st, k, dt, n, arr[0], arr[1], arr[2], arr[3],
+                       arr[4], arr[5], box, w.count.as<int>());
     PM_HIP(hipGetLastError());
     int any = 0;
     PM_HIP(hipMemcpyAsync(&any, w.count.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -668,26 +666,84 @@ __global__ void k_fill_int(int *p, int64_t n, int v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
-// class indicators of the live particles (zero past the count)
-__global__ void k_mig_keys(NCount npc, int64_t cap, const int *cls, Mig3 *key) {
+// Three-way stable partition of the recorded step, gated: when the stage's
+// update left no particle outside the patch (or the axis moves none), every
+// kernel below returns at once.  The class offsets come from a two-level scan
+// over tiles of kPartTile particles (tile sums, one block scanning the tile
+// sums, then each tile's own block scans in index order), so that a step with
+// nothing to migrate launches no full-size scan.
+constexpr int kPartThreads = 256, kPartItems = 8, kPartTile = kPartThreads * kPartItems;
+__device__ __forceinline__ Mig3 m3_of(int c) { return Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0}; }
+__device__ __forceinline__ Mig3 m3_add(const Mig3 &a, const Mig3 &b) { return Mig3{a.w + b.w, a.e + b.e, a.d + b.d}; }
+// exclusive scan of one Mig3 per thread over a kPartThreads-thread block;
+// every thread of the block calls it
+__device__ Mig3 block_scan3(Mig3 v, Mig3 &total) {
+  __shared__ Mig3 wsum[kPartThreads / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  Mig3 x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const Mig3 y{__shfl_up(x.w, o), __shfl_up(x.e, o), __shfl_up(x.d, o)};
+    if (lane >= o) x = m3_add(x, y);
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  Mig3 base{0, 0, 0};
+  total = Mig3{0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < kPartThreads / 64; ++k) {
+    if (k < wv) base = m3_add(base, wsum[k]);
+    total = m3_add(total, wsum[k]);
+  }
+  __syncthreads();   // wsum is free for the next call
+  return Mig3{base.w + x.w - v.w, base.e + x.e - v.e, base.d + x.d - v.d};
+}
+__device__ __forceinline__ bool part_idle(const int *counts) { return counts[1] + counts[2] + counts[3] == 0; }
+// class counts of each tile
+__global__ void __launch_bounds__(kPartThreads) k_part_tiles(NCount npc, const int *counts, const int *cls,
+                                                             Mig3 *tsum) {
+  if (part_idle(counts)) return;
   const int64_t np = npc.get();
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < cap; q += (int64_t)gridDim.x * blockDim.x) {
-    const int c = q < np ? cls[q] : 0;
-    key[q] = Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0};
+  Mig3 s{0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < kPartItems; ++j) {
+    const int64_t q = (int64_t)blockIdx.x * kPartTile + j * kPartThreads + threadIdx.x;
+    if (q < np) s = m3_add(s, m3_of(cls[q]));
+  }
+  Mig3 tot;
+  (void)block_scan3(s, tot);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+// tile sums -> exclusive tile offsets (one block)
+__global__ void __launch_bounds__(kPartThreads) k_part_tile_scan(const int *counts, Mig3 *tsum, int ntiles) {
+  if (part_idle(counts)) return;
+  Mig3 carry{0, 0, 0};
+  for (int b = 0; b < ntiles; b += kPartThreads) {
+    const int t = b + threadIdx.x;
+    const Mig3 v = t < ntiles ? tsum[t] : Mig3{0, 0, 0};
+    Mig3 tot;
+    const Mig3 ex = block_scan3(v, tot);
+    if (t < ntiles) tsum[t] = m3_add(carry, ex);
+    carry = m3_add(carry, tot);
   }
 }
-
 // west movers, east movers, then stayers, each in their original order; a
 // stayer's rank is q minus the movers and drops before it
-__global__ void k_migrate_dev(NCount npc, const int *cls, const Mig3 *off, const int *counts, int d, double len,
-                              Arr6 src, Arr6 dst) {
-  if (counts[1] + counts[2] + counts[3] == 0) return;
+__global__ void __launch_bounds__(kPartThreads) k_part_migrate(NCount npc, const int *cls, const Mig3 *toff,
+                                                               const int *counts, int d, double len, Arr6 src,
+                                                               Arr6 dst) {
+  if (part_idle(counts)) return;
   const int64_t np = npc.get();
   const int n1 = counts[1], n2 = counts[2];
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    const int c = cls[q];
-    if (c == 3) continue;
-    const Mig3 o = off[q];
+  Mig3 run = toff[blockIdx.x];
+  for (int j = 0; j < kPartItems; ++j) {
+    const int64_t q = (int64_t)blockIdx.x * kPartTile + j * kPartThreads + threadIdx.x;
+    const int c = q < np ? cls[q] : 0;
+    Mig3 tot;
+    const Mig3 ex = block_scan3(q < np ? m3_of(c) : Mig3{0, 0, 0}, tot);
+    const Mig3 o = m3_add(run, ex);
+    run = m3_add(run, tot);
+    if (q >= np || c == 3) continue;
     const int64_t to = c == 1 ? o.w : (c == 2 ? n1 + o.e : n1 + n2 + (q - o.w - o.e - o.d));
     for (int a = 0; a < 6; ++a) {
       double v0 = src.a[a][3 * q], v1 = src.a[a][3 * q + 1], v2 = src.a[a][3 * q + 2];
@@ -711,7 +767,9 @@ __global__ void k_copy_back(NCount npc, const int *counts, Arr6 src, Arr6 dst) {
 __global__ void k_np_drop(int *np, const int *counts) { *np -= counts[3]; }
 // a particle still outside a periodic axis after one wrap moved more than a
 // period in one stage: flagged (bit 2 of the status word)
-__global__ void k_check_inside(NCount npc, const double *pos, int d, double lo, double hi, int *err) {
+__global__ void k_check_inside(NCount npc, const int *outside, const double *pos, int d, double lo, double hi,
+                               int *err) {
+  if (!*outside) return;
   const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     const double x = pos[3 * q + d];
@@ -736,9 +794,8 @@ struct VStep {
   Scratch parts;    // the particle set: 6 arrays of 3 * cap doubles
   Scratch tmp;      // migration target: 6 arrays
   Scratch mesh;     // vorticity and velocity meshes
-  Scratch mig;      // cls, counts, keys, offsets
-  Scratch scan3;    // rocPRIM temporary of the offset scan
-  Scratch state;    // np and the status word
+  Scratch mig;      // cls, counts, tile offsets
+  Scratch state;    // np, the status word and the stage's outside flag
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double g_dt = 0.0, g_min = 0.0;
@@ -751,6 +808,8 @@ struct VStep {
   }
   int *d_np() { return state.as<int>(); }
   int *d_err() { return state.as<int>() + 1; }
+  int *d_outside() { return state.as<int>() + 2; }
+  int ntiles() const { return (int)((cap + kPartTile - 1) / kPartTile); }
   void *mvort() { return mesh.p; }
   void *mvel() { return (char *)mesh.p + mesh_elems() * (prec == PMGPU_FP64 ? 8 : 4); }
 
@@ -773,23 +832,21 @@ struct VStep {
         w.rec.ensure(rec_bytes * ((size_t)cap + 1)) || w.flag.ensure(sizeof(int) * (nn + 1)) ||
         w.offs.ensure(sizeof(int) * (nn + 1)))
       return -1;
-    size_t b1 = 0, b2 = 0, b3 = 0;
+    size_t b1 = 0, b2 = 0;
     PM_HIP(rocprim::inclusive_scan(nullptr, b1, (int *)nullptr, (int *)nullptr, (size_t)nbins, rocprim::plus<int>(),
                                    st));
     PM_HIP(rocprim::inclusive_scan(nullptr, b2, (int *)nullptr, (int *)nullptr, (size_t)nn, rocprim::plus<int>(), st));
-    PM_HIP(rocprim::inclusive_scan(nullptr, b3, (Mig3 *)nullptr, (Mig3 *)nullptr, (size_t)cap, Mig3Plus(), st));
-    if (w.scan.ensure((b1 > b2 ? b1 : b2) + 16) || scan3.ensure(b3 + 16)) return -1;
+    if (w.scan.ensure((b1 > b2 ? b1 : b2) + 16)) return -1;
     const size_t pb = sizeof(double) * 3 * (size_t)cap * 6;
     if (parts.ensure(pb) || tmp.ensure(pb) || mesh.ensure(2 * mesh_elems() * (prec == PMGPU_FP64 ? 8 : 4)) ||
-        mig.ensure(sizeof(int) * ((size_t)cap + 8) + sizeof(Mig3) * (2 * (size_t)cap + 2)) ||
+        mig.ensure(sizeof(int) * ((size_t)cap + 8) + sizeof(Mig3) * ((size_t)ntiles() + 1)) ||
         state.ensure(sizeof(int) * 8))
       return -1;
     return 0;
   }
   int *cls() { return mig.as<int>(); }
   int *counts() { return mig.as<int>() + cap; }
-  Mig3 *key3() { return reinterpret_cast<Mig3 *>(mig.as<int>() + cap + 8); }
-  Mig3 *off3() { return key3() + cap; }
+  Mig3 *toff() { return reinterpret_cast<Mig3 *>(mig.as<int>() + cap + 8); }
 
   // deposition of the particle vorticity onto the vorticity mesh (clear)
   int p2m() {
@@ -831,19 +888,19 @@ struct VStep {
     const double lo = geom.xmin[d], hi = geom.xmax[d], len = geom.xmax[d] - geom.xmin[d];
     const int per = geom.bound_cond[d] != 0;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, counts(), (int64_t)4, 0);
-    hipLaunchKernelGGL(k_classify, grid_for(cap), dim3(256), 0, st, n, (const double *)A.a[0], d, lo, hi, per, cls(),
-                       counts());
-    hipLaunchKernelGGL(k_mig_keys, grid_for(cap), dim3(256), 0, st, n, cap, (const int *)cls(), key3());
-    hipLaunchKernelGGL(k_zero_first3, dim3(1), dim3(1), 0, st, off3());
-    size_t bytes = scan3.cap - 16;
-    PM_HIP(rocprim::inclusive_scan(scan3.p, bytes, key3(), off3() + 1, (size_t)cap, Mig3Plus(), st));
-    hipLaunchKernelGGL(k_migrate_dev, grid_for(cap), dim3(256), 0, st, n, (const int *)cls(), (const Mig3 *)off3(),
+    hipLaunchKernelGGL(k_classify, grid_for(cap), dim3(256), 0, st, n, (const int *)d_outside(),
+                       (const double *)A.a[0], d, lo, hi, per, cls(), counts());
+    const dim3 tiles((unsigned)ntiles());
+    hipLaunchKernelGGL(k_part_tiles, tiles, dim3(kPartThreads), 0, st, n, (const int *)counts(), (const int *)cls(),
+                       toff());
+    hipLaunchKernelGGL(k_part_tile_scan, dim3(1), dim3(kPartThreads), 0, st, (const int *)counts(), toff(), ntiles());
+    hipLaunchKernelGGL(k_part_migrate, tiles, dim3(kPartThreads), 0, st, n, (const int *)cls(), (const Mig3 *)toff(),
                        (const int *)counts(), d, len, A, T);
     hipLaunchKernelGGL(k_copy_back, grid_for(3 * cap), dim3(256), 0, st, n, (const int *)counts(), T, A);
     hipLaunchKernelGGL(k_np_drop, dim3(1), dim3(1), 0, st, d_np(), (const int *)counts());
     if (per)
-      hipLaunchKernelGGL(k_check_inside, grid_for(cap), dim3(256), 0, st, NCount(cap, d_np()), (const double *)A.a[0],
-                         d, lo, hi, d_err());
+      hipLaunchKernelGGL(k_check_inside, grid_for(cap), dim3(256), 0, st, NCount(cap, d_np()),
+                         (const int *)d_outside(), (const double *)A.a[0], d, lo, hi, d_err());
     PM_HIP(hipGetLastError());
     return 0;
   }
@@ -871,8 +928,14 @@ struct VStep {
                              (const double *)A.a[0], n, A.a[1]);
       }
       const RkCoef k = rk_coef(rk, s, 0);
+      PBox box;
+      for (int d = 0; d < 3; ++d) {
+        box.lo[d] = geom.xmin[d];
+        box.hi[d] = geom.xmax[d];
+      }
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d_outside(), (int64_t)1, 0);
       hipLaunchKernelGGL(k_rk_update, grid_for(3 * cap), dim3(256), 0, st, k, dt, n, A.a[0], (const double *)A.a[1],
-                         A.a[2], (const double *)A.a[3], A.a[4], A.a[5]);
+                         A.a[2], (const double *)A.a[3], A.a[4], A.a[5], box, d_outside());
       PM_HIP(hipGetLastError());
       for (int d = 0; d < 3; ++d)
         if (migrate(d)) return -1;

@@ -627,39 +627,40 @@ struct RkCoef {
   double a1, b1, c1, a2, b2, c2;
 };
 
+// the patch box [lo, hi) per axis
+struct PBox {
+  double lo[3], hi[3];
+};
+
+// the RK stage update of every particle word; with `outside` given, also
+// flags (1) a particle that the update leaves outside the box on any axis
 static __global__ void k_rk_update(RkCoef k, double dt, NCount npc, double *pos, const double *vel, double *vort,
-                            const double *dvort, double *vel_rk, double *dvort_rk) {
+                                   const double *dvort, double *vel_rk, double *dvort_rk, PBox box, int *outside) {
   const int64_t np = npc.get();
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 3 * np; i += (int64_t)gridDim.x * blockDim.x) {
+  // axis of word i, advanced with the grid stride (no 64-bit modulo)
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, s3 = (int)(((int64_t)gridDim.x * blockDim.x) % 3);
+  int d = i0 % 3;
+  for (int64_t i = i0; i < 3 * np; i += (int64_t)gridDim.x * blockDim.x, d = (d + s3) % 3) {
     double vr = __dadd_rn(__dmul_rn(k.a1, vel_rk[i]), __dmul_rn(__dmul_rn(k.b1, dt), vel[i]));
     double dr = __dadd_rn(__dmul_rn(k.a2, dvort_rk[i]), __dmul_rn(__dmul_rn(k.b2, dt), dvort[i]));
     vel_rk[i] = vr;
     dvort_rk[i] = dr;
-    pos[i] = __dadd_rn(pos[i], __dmul_rn(k.c1, vr));
+    const double x = __dadd_rn(pos[i], __dmul_rn(k.c1, vr));
+    pos[i] = x;
     vort[i] = __dadd_rn(vort[i], __dmul_rn(k.c2, dr));
-  }
-}
-
-// any particle outside [lo, hi) on any axis (flag set to 1)
-struct PBox {
-  double lo[3], hi[3];
-};
-static __global__ void k_any_outside(NCount npc, const double *pos, PBox b, int *flag) {
-  const int64_t np = npc.get();
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
-    bool out = false;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double x = pos[3 * q + d];
-      out = out || x < b.lo[d] || x >= b.hi[d];
+    if (outside) {
+      const bool o = x < box.lo[d] || x >= box.hi[d];
+      const uint64_t m = __ballot(o);
+      if (m && (threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) atomicOr(outside, 1);
     }
-    if (out) atomicOr(flag, 1);
   }
 }
 
-// migration class along axis d: 0 stay, 1 west (wrap), 2 east (wrap), 3 drop
-static __global__ void k_classify(NCount npc, const double *pos, int d, double lo, double hi, int per, int *cls,
-                           int *counts) {
+// migration class along axis d: 0 stay, 1 west (wrap), 2 east (wrap), 3 drop;
+// nothing to do when the stage's update left every particle inside
+static __global__ void k_classify(NCount npc, const int *outside, const double *pos, int d, double lo, double hi,
+                                  int per, int *cls, int *counts) {
+  if (!*outside) return;
   const int64_t np = npc.get();
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     double x = pos[3 * q + d];
