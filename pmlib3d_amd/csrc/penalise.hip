@@ -5,10 +5,10 @@
 // Restates one iteration of penalisation_solve
 // (vortex_codes/vortex_penalisation/source/penalisation/penalisation_solve.f:
 // 96-111 for f, :146-193 / :195-273 for the 2nd / 4th order curl).  f is never
-// stored: each thread forms it at its 12 stencil neighbours from chi, u and us
-// (the neighbours along i are the adjacent lanes, so the re-reads hit L1/L2).
-// Every floating-point operation is an explicitly rounded one, in the order
-// the reference writes it, so the FP64 result is bit-identical to it.
+// stored in HBM: a workgroup forms it plane by plane for its tile and halo in
+// LDS while it walks along z.  Every floating-point operation is an
+// explicitly rounded one, in the order the reference writes it, so the FP64
+// result is bit-identical to it.
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -40,18 +40,17 @@ __device__ __forceinline__ T f_at(const PenArgs<T> &a, int64_t q, int c) {
   return rmul(a.chi[q], rsub(s, u));
 }
 
-// central difference of component c along a stride st (in cells)
+// central difference from the values at offsets -2, -1, +1, +2 (FD = 4) or
+// -1, +1 (FD = 2; m2 and p2 unused)
 template <typename T, int FD>
-__device__ __forceinline__ T cdiff(const PenArgs<T> &a, int64_t q, int64_t st, int c, T cc) {
+__device__ __forceinline__ T cdiff_v(T m2, T m1, T p1, T p2, T cc) {
   if (FD == 4) {
     const T c1 = T(1.0 / 6.0), c2 = T(4.0 / 3.0);
-    T m2 = f_at(a, q - 2 * st, c), m1 = f_at(a, q - st, c), p1 = f_at(a, q + st, c), p2 = f_at(a, q + 2 * st, c);
     T t = rsub(rmul(c1, m2), rmul(c2, m1));
     t = radd(t, rmul(c2, p1));
     t = rsub(t, rmul(c1, p2));
     return rmul(t, cc);
   } else {
-    T m1 = f_at(a, q - st, c), p1 = f_at(a, q + st, c);
     return rmul(radd(-m1, p1), cc);
   }
 }
@@ -62,63 +61,145 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 constexpr int kTx = 64, kTy = 4;
+constexpr int kZc = 16;   // planes per workgroup
 
+// A workgroup of kTx x kTy threads owns an (i, j) tile of interior cells and
+// walks the planes of its z chunk (kZc planes).  Per plane it forms f once
+// for the tile and an H = fd/2 halo in x and y into LDS; the x and y
+// differences come from there, the z differences from a per-thread queue of
+// the cell's f0 and f1 over the 2H + 1 planes around it, so a plane's
+// vorticity is finished H planes after it was formed.  Per cell the operations
+// and their order are those of the reference's loop, so the FP64 result is
+// bit-identical to it; the sums are formed per (i tile, j tile, plane) and
+// reduced in a fixed order.
 template <typename T, int FD>
 __global__ void __launch_bounds__(kTx *kTy) k_penalise(PenArgs<T> a) {
-  const BGeom &g = a.g;
-  const int i = blockIdx.x * kTx + threadIdx.x + 1;  // interior 1..N
-  const int j = blockIdx.y * kTy + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  double s[5] = {0, 0, 0, 0, 0};
-  if (i <= g.N[0] && j <= g.N[1]) {
-    const int64_t sx = 1, sy = g.E[0], sz = (int64_t)g.E[0] * g.E[1];
-    const int64_t q = (int64_t)(i - 1 + g.ng[0]) + sy * (j - 1 + g.ng[1]) + sz * (k - 1 + g.ng[2]);
-    T dvdx = cdiff<T, FD>(a, q, sx, 1, a.cx), dwdx = cdiff<T, FD>(a, q, sx, 2, a.cx);
-    T dudy = cdiff<T, FD>(a, q, sy, 0, a.cy), dwdy = cdiff<T, FD>(a, q, sy, 2, a.cy);
-    T dudz = cdiff<T, FD>(a, q, sz, 0, a.cz), dvdz = cdiff<T, FD>(a, q, sz, 1, a.cz);
-    T w0 = rmul(a.relax, rsub(dwdy, dvdz));
-    T w1 = rmul(a.relax, rsub(dudz, dwdx));
-    T w2 = rmul(a.relax, rsub(dvdx, dudy));
-    T *v = a.vort + 3 * q;
-    T v0 = radd(v[0], w0), v1 = radd(v[1], w1), v2 = radd(v[2], w2);
-    v[0] = v0;
-    v[1] = v1;
-    v[2] = v2;
-    if (a.partial) {
-      const double px = __dadd_rn(g.xmin[0], __dmul_rn(__dadd_rn((double)(i - 1), 0.5), g.dx[0]));
-      const double py = __dadd_rn(g.xmin[1], __dmul_rn(__dadd_rn((double)(j - 1), 0.5), g.dx[1]));
-      const double pz = __dadd_rn(g.xmin[2], __dmul_rn(__dadd_rn((double)(k - 1), 0.5), g.dx[2]));
-      const double d0 = w0, d1 = w1, d2 = w2, e0 = v0, e1 = v1, e2 = v2;
-      s[0] = __dsub_rn(__dmul_rn(py, d2), __dmul_rn(pz, d1));
-      s[1] = __dsub_rn(__dmul_rn(pz, d0), __dmul_rn(px, d2));
-      s[2] = __dsub_rn(__dmul_rn(px, d1), __dmul_rn(py, d0));
-      s[3] = __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)), __dmul_rn(d2, d2));
-      s[4] = __dadd_rn(__dadd_rn(__dmul_rn(e0, e0), __dmul_rn(e1, e1)), __dmul_rn(e2, e2));
-    }
-  }
-  if (!a.partial) return;
+  constexpr int H = FD / 2, TW = kTx + 2 * H, TH = kTy + 2 * H, Q = 2 * H + 1;
+  __shared__ T fl[3][TH][TW];
   __shared__ double red[kTy][5];
-  for (int m = 0; m < 5; ++m) {
-    double t = wave_sum(s[m]);
-    if (threadIdx.x == 0) red[threadIdx.y][m] = t;
-  }
-  __syncthreads();
-  if (threadIdx.y == 0 && threadIdx.x < 5) {
-    double t = 0.0;
-    for (int w = 0; w < kTy; ++w) t += red[w][threadIdx.x];
-    const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-    a.partial[5 * b + threadIdx.x] = t;
+  const BGeom &g = a.g;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * kTx + tx;
+  const int i0 = blockIdx.x * kTx + 1, j0 = blockIdx.y * kTy + 1;   // interior indices are 1..N
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = blockIdx.z * kZc + 1, k1 = min(k0 + kZc - 1, g.N[2]);
+  const bool own = i <= g.N[0] && j <= g.N[1];
+  const int64_t sy = g.E[0], sz = (int64_t)g.E[0] * g.E[1];
+  auto cell = [&](int ii, int jj, int kk) {
+    return (int64_t)(ii - 1 + g.ng[0]) + sy * (jj - 1 + g.ng[1]) + sz * (kk - 1 + g.ng[2]);
+  };
+  T q0[Q], q1[Q];        // f0, f1 of the own cell, oldest plane first
+  T pend[H + 1][4];      // dvdx, dwdx, dudy, dwdy of the planes not yet finished
+#pragma unroll
+  for (int t = 0; t < Q; ++t) q0[t] = q1[t] = T(0);
+#pragma unroll
+  for (int t = 0; t <= H; ++t)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pend[t][c] = T(0);
+  for (int kk = k0 - H; kk <= k1 + H; ++kk) {
+    // f of the tile and its halo at plane kk (cells no owned cell reaches are
+    // skipped: they may lie past the ghost layers)
+    for (int idx = tid; idx < TW * TH; idx += kTx * kTy) {
+      const int r = idx / TW, c = idx - r * TW;
+      const int ii = i0 - H + c, jj = j0 - H + r;
+      if (ii <= g.N[0] + H && jj <= g.N[1] + H) {
+        const int64_t q = cell(ii, jj, kk);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) fl[m][r][c] = f_at(a, q, m);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t + 1 < Q; ++t) {
+      q0[t] = q0[t + 1];
+      q1[t] = q1[t + 1];
+    }
+    q0[Q - 1] = fl[0][ty + H][tx + H];
+    q1[Q - 1] = fl[1][ty + H][tx + H];
+    if (kk >= k0 && kk <= k1) {
+      const int r = ty + H, c = tx + H;
+      const T x2 = H == 2 ? fl[1][r][c - 2] : T(0), x_2 = H == 2 ? fl[1][r][c + 2] : T(0);
+      const T z2 = H == 2 ? fl[2][r][c - 2] : T(0), z_2 = H == 2 ? fl[2][r][c + 2] : T(0);
+      pend[H][0] = cdiff_v<T, FD>(x2, fl[1][r][c - 1], fl[1][r][c + 1], x_2, a.cx);   // dv/dx
+      pend[H][1] = cdiff_v<T, FD>(z2, fl[2][r][c - 1], fl[2][r][c + 1], z_2, a.cx);   // dw/dx
+      const T y2 = H == 2 ? fl[0][r - 2][c] : T(0), y_2 = H == 2 ? fl[0][r + 2][c] : T(0);
+      const T w2 = H == 2 ? fl[2][r - 2][c] : T(0), w_2 = H == 2 ? fl[2][r + 2][c] : T(0);
+      pend[H][2] = cdiff_v<T, FD>(y2, fl[0][r - 1][c], fl[0][r + 1][c], y_2, a.cy);   // du/dy
+      pend[H][3] = cdiff_v<T, FD>(w2, fl[2][r - 1][c], fl[2][r + 1][c], w_2, a.cy);   // dw/dy
+    }
+    __syncthreads();   // the tile is read; the next plane may overwrite it
+    const int kf = kk - H;   // plane finished now
+    if (kf >= k0) {
+      double s[5] = {0, 0, 0, 0, 0};
+      if (own) {
+        const T dvdx = pend[0][0], dwdx = pend[0][1], dudy = pend[0][2], dwdy = pend[0][3];
+        const T dudz = cdiff_v<T, FD>(q0[0], q0[H - 1], q0[H + 1], q0[Q - 1], a.cz);
+        const T dvdz = cdiff_v<T, FD>(q1[0], q1[H - 1], q1[H + 1], q1[Q - 1], a.cz);
+        T w0 = rmul(a.relax, rsub(dwdy, dvdz));
+        T w1 = rmul(a.relax, rsub(dudz, dwdx));
+        T w2 = rmul(a.relax, rsub(dvdx, dudy));
+        T *v = a.vort + 3 * cell(i, j, kf);
+        T v0 = radd(v[0], w0), v1 = radd(v[1], w1), v2 = radd(v[2], w2);
+        v[0] = v0;
+        v[1] = v1;
+        v[2] = v2;
+        if (a.partial) {
+          const double px = __dadd_rn(g.xmin[0], __dmul_rn(__dadd_rn((double)(i - 1), 0.5), g.dx[0]));
+          const double py = __dadd_rn(g.xmin[1], __dmul_rn(__dadd_rn((double)(j - 1), 0.5), g.dx[1]));
+          const double pz = __dadd_rn(g.xmin[2], __dmul_rn(__dadd_rn((double)(kf - 1), 0.5), g.dx[2]));
+          const double d0 = w0, d1 = w1, d2 = w2, e0 = v0, e1 = v1, e2 = v2;
+          s[0] = __dsub_rn(__dmul_rn(py, d2), __dmul_rn(pz, d1));
+          s[1] = __dsub_rn(__dmul_rn(pz, d0), __dmul_rn(px, d2));
+          s[2] = __dsub_rn(__dmul_rn(px, d1), __dmul_rn(py, d0));
+          s[3] = __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)), __dmul_rn(d2, d2));
+          s[4] = __dadd_rn(__dadd_rn(__dmul_rn(e0, e0), __dmul_rn(e1, e1)), __dmul_rn(e2, e2));
+        }
+      }
+      if (a.partial) {
+        for (int m = 0; m < 5; ++m) {
+          double t = wave_sum(s[m]);
+          if (tx == 0) red[ty][m] = t;
+        }
+        __syncthreads();
+        if (ty == 0 && tx < 5) {
+          double t = 0.0;
+          for (int w = 0; w < kTy; ++w) t += red[w][tx];
+          // the block index of the (i tile, j tile, plane) partial
+          const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * (kf - 1));
+          a.partial[5 * b + tx] = t;
+        }
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < H; ++t)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) pend[t][c] = pend[t + 1][c];
   }
 }
 
-// fixed-order reduction of the per-block sums (one block)
+// fixed-order reduction of the per-block sums (one block): thread t adds the
+// blocks t, t + 256, ... in that order, then a fixed tree; the loads of 8
+// blocks are issued ahead of their adds
 __global__ void __launch_bounds__(256) k_penalise_sums(const double *partial, int64_t nb, double *out) {
   __shared__ double sh[5][256];
-  for (int m = 0; m < 5; ++m) {
-    double t = 0.0;
-    for (int64_t b = threadIdx.x; b < nb; b += 256) t += partial[5 * b + m];
-    sh[m][threadIdx.x] = t;
+  double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int64_t b = threadIdx.x;
+  for (; b + 7 * 256 < nb; b += 8 * 256) {
+    double v[8][5];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) v[u][m] = partial[5 * (b + u * 256) + m];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) t[m] += v[u][m];
   }
+  for (; b < nb; b += 256)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) t[m] += partial[5 * b + m];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) sh[m][threadIdx.x] = t[m];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o)
@@ -130,8 +211,14 @@ __global__ void __launch_bounds__(256) k_penalise_sums(const double *partial, in
 
 constexpr int kMaxDevices = 64;
 
+// workgroups of k_penalise; the partial sums are one per (i tile, j tile,
+// plane), partial_blocks of them
 static dim3 penalise_grid(const pmgpu_geom *geom) {
-  return dim3((geom->ncell[0] + kTx - 1) / kTx, (geom->ncell[1] + kTy - 1) / kTy, geom->ncell[2]);
+  return dim3((geom->ncell[0] + kTx - 1) / kTx, (geom->ncell[1] + kTy - 1) / kTy, (geom->ncell[2] + kZc - 1) / kZc);
+}
+static int64_t partial_blocks(const pmgpu_geom *geom) {
+  const dim3 g = penalise_grid(geom);
+  return (int64_t)g.x * g.y * geom->ncell[2];
 }
 
 struct PenWS {
@@ -185,7 +272,7 @@ static int penalise_run(const pmgpu_geom *geom, const void *vel, const void *chi
   a.vort = (T *)vort;
   PM_CHECK(geom->ncell[2] < 65536 && geom->ncell[1] < (1 << 20), "pmgpu_penalise: mesh too large");
   const dim3 grid = penalise_grid(geom);
-  const int64_t nb = (int64_t)grid.x * grid.y * grid.z;
+  const int64_t nb = partial_blocks(geom);
   PM_CHECK(!partial || nb <= cap, "pmgpu_penalise: more blocks than the partial-sum buffer holds");
   a.partial = partial;
   if (fd_order == 4)
@@ -210,8 +297,7 @@ static int penalise_launch(const pmgpu_geom *geom, const void *vel, const void *
   double *partial = nullptr;
   int64_t cap = 0;
   if (diag) {
-    const dim3 grid = penalise_grid(geom);
-    const int64_t nb = (int64_t)grid.x * grid.y * grid.z;
+    const int64_t nb = partial_blocks(geom);
     if (!g_pen) g_pen = new PenWS;
     if (g_pen->cap < nb) {
       if (g_pen->partial) (void)hipFree(g_pen->partial);
@@ -489,8 +575,7 @@ int pmgpu_penalise_device_noalloc(const pmgpu_geom *g, int32_t precision, const 
   PM_CHECK(g, "pmgpu_penalise_device_noalloc: null argument");
   double *partial = nullptr;
   if (d_diag) {
-    const dim3 grid = penalise_grid(g);
-    const int64_t nb = (int64_t)grid.x * grid.y * grid.z;
+    const int64_t nb = partial_blocks(g);
     PM_CHECK(nb <= kNoAllocBlocks, "pmgpu_penalise_device_noalloc: the mesh needs " + std::to_string(nb) +
                                        " partial-sum blocks, more than the " + std::to_string(kNoAllocBlocks) +
                                        " its fixed scratch holds (use pmgpu_penalise_device)");
