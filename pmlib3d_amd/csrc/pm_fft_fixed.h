@@ -150,10 +150,16 @@ template <int L, typename V> __device__ __forceinline__ void twid_load(Twid<L, V
 }
 
 // v[r] *= w^r (conjugated for S > 0), r = 1 .. R - 1, for butterfly m of
-// stage I, the powers of the base twiddle by a multiplication chain
+// stage I, the powers of the base twiddle by a multiplication chain.  For
+// lines of 12 or more elements per thread the chain is formed anew at every
+// use: the empty asm hides the base value's identity, so that the compiler
+// does not keep the powers of every stage live across the transforms of a
+// kernel (at length 768 that held the three-component z pass at one
+// wavefront per SIMD; forming them again costs 7 % more VALU and gives two)
 template <int L, int I, int S, int R, typename V>
 __device__ __forceinline__ void twiddle(V *v, const Twid<L, V> &t, int m) {
   V w1 = t.w[I][m];
+  if constexpr (L / Geo<L>::P >= 12) asm volatile("" : "+v"(w1.x), "+v"(w1.y));
   if (S > 0) w1.y = -w1.y;
   V w = w1;
 #pragma unroll
