@@ -297,22 +297,31 @@ int remesh_device(const pmgpu_geom *geom, int prec, const void *vort_m, const vo
 // wrapped coordinate are those of permuting the arrays at every pass, which
 // is what the reference's map does.
 
-// coordinate d of the listed particles (the identity list when perm is null)
-__global__ void k_list_coord(int64_t n, const int *perm, const double *pos, int d, double *cd) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    cd[i] = pos[3 * (int64_t)(perm ? perm[i] : (int)i) + d];
-}
 // migration class of each listed particle (0 stay, 1 west, 2 east, 3 drop)
-// with its class indicators for the offset scan and the class counts
-__global__ void k_list_classify(int64_t n, const double *cd, double lo, double hi, int per, int *cls, Mig3 *key,
-                                int *counts) {
+// with its class indicators for the offset scan and the class counts; with
+// pos given, the coordinate column cd is first gathered from the positions
+// (coordinate d of the listed particles, the identity list when perm is null)
+__global__ void k_list_classify(int64_t n, const int *perm, const double *pos, int d, double *cd, double lo,
+                                double hi, int per, int *cls, Mig3 *key, int *counts) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double x = cd[i];
+    double x;
+    if (pos) {
+      x = pos[3 * (int64_t)(perm ? perm[i] : (int)i) + d];
+      cd[i] = x;
+    } else {
+      x = cd[i];
+    }
     int c = x < lo ? 1 : (x >= hi ? 2 : 0);
     if (c != 0 && !per) c = 3;
     cls[i] = c;
     key[i] = Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0};
-    if (c != 0) atomicAdd(&counts[c], 1);
+    // one atomic per wavefront and class
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 1; k <= 3; ++k) {
+      const uint64_t m = __ballot(c == k);
+      if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&counts[k], __popcll(m));
+    }
   }
 }
 // coordinate columns moved with the list: the current axis' column (index 0)
@@ -341,6 +350,16 @@ __global__ void k_list_partition(int64_t n, const int *cls, const Mig3 *off, int
 struct Arr4 {
   double *a[4];
 };
+// n words of each of narr arrays, src -> dst, two words per thread step
+// (every pointer 16-byte aligned)
+__global__ void k_copy_back4(int64_t n, const Arr4 src, Arr4 dst, int narr) {
+  const int64_t n2 = n / 2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    for (int a = 0; a < narr; ++a)
+      reinterpret_cast<double2 *>(dst.a[a])[i] = reinterpret_cast<const double2 *>(src.a[a])[i];
+  if (n % 2 && blockIdx.x == 0 && threadIdx.x == 0)
+    for (int a = 0; a < narr; ++a) dst.a[a][n - 1] = src.a[a][n - 1];
+}
 // row i of each carried array from row perm[i]; the position rows take the
 // wrapped coordinates from the columns of the axes that wrapped (col[d] not
 // null)
@@ -416,7 +435,7 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
   // list workspace: cls, two index lists (ints), keys and offsets (Mig3),
   // two buffers of each axis' coordinate column and the gathered rows
   // (doubles)
-  const size_t ni = (size_t)n0 + 1;
+  const size_t ni = ((size_t)n0 + 2) & ~(size_t)1;   // even: 16-byte aligned rows in migbuf
   if (w.cls.ensure(sizeof(int) * 3 * ni + sizeof(Mig3) * 2 * ni + 16) || w.count.ensure(sizeof(int) * 8) ||
       w.migbuf.ensure(sizeof(double) * (6 * ni + 3 * ni * narr)))
     return -1;
@@ -432,13 +451,13 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
   for (int d = 0; d < 3 && n > 0; ++d) {
     const double lo = geom->xmin[d], hi = geom->xmax[d], len = geom->xmax[d] - geom->xmin[d];
     const int per = geom->bound_cond[d] != 0;
-    hipLaunchKernelGGL(k_list_coord, grid_for(n), dim3(256), 0, st, n, listed ? (const int *)plist[cur] : nullptr,
-                       (const double *)arr[0], d, colbuf[d][cur]);
     bool wrapped = false;
     for (int pass = 0; pass < 64; ++pass) {
       PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * 4, st));
-      hipLaunchKernelGGL(k_list_classify, grid_for(n), dim3(256), 0, st, n, (const double *)colbuf[d][cur], lo, hi,
-                         per, cls, key, w.count.as<int>());
+      // the first pass of an axis gathers its coordinate column
+      hipLaunchKernelGGL(k_list_classify, grid_for(n), dim3(256), 0, st, n,
+                         listed ? (const int *)plist[cur] : nullptr, pass == 0 ? (const double *)arr[0] : nullptr, d,
+                         colbuf[d][cur], lo, hi, per, cls, key, w.count.as<int>());
       int counts[4];
       PM_HIP(hipMemcpyAsync(counts, w.count.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
       PM_HIP(hipStreamSynchronize(st));
@@ -478,8 +497,17 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
     for (int d = 0; d < 3; ++d) col.src[d] = carried[d] ? colbuf[d][cur] : nullptr;
     hipLaunchKernelGGL(k_list_gather, grid_for(3 * n), dim3(256), 0, st, n, (const int *)plist[cur], src, dst, narr,
                        col);
-    for (int a = 0; a < narr; ++a)
-      PM_HIP(hipMemcpyAsync(src.a[a], dst.a[a], sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+    // back into the caller's arrays: one launch of 16-byte words when every
+    // pointer allows it
+    bool al = true;
+    for (int a = 0; a < narr; ++a) al = al && ((uintptr_t)src.a[a] % 16 == 0) && ((uintptr_t)dst.a[a] % 16 == 0);
+    if (al) {
+      hipLaunchKernelGGL(k_copy_back4, grid_for((3 * n + 1) / 2), dim3(256), 0, st, 3 * n, dst, src,
+                         narr);
+    } else {
+      for (int a = 0; a < narr; ++a)
+        PM_HIP(hipMemcpyAsync(src.a[a], dst.a[a], sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+    }
   }
   PM_HIP(hipGetLastError());
   *np_out = n;
