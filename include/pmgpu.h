@@ -230,6 +230,17 @@ int pmgpu_particles_advance(const pmgpu_geom *g, int32_t rk_order, int32_t stage
 int pmgpu_particles_advance_device(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
                                    int32_t euler_vort, void *const d_arrays[6], int64_t np,
                                    int64_t *np_out, void *stream);
+/* The same with a second set of arrays, d_spare, of at least np particles
+ * each and distinct from d_arrays (vel and dvort may be NULL).  No reference
+ * counterpart: it is for time loops that own two particle sets and swap
+ * them.  When the map reorders particles, the carried arrays' new contents
+ * are written to d_spare instead of back into d_arrays and *in_spare = 1; the
+ * caller then continues with d_spare (its vel and dvort are not written).
+ * Otherwise the update is in place in d_arrays and *in_spare = 0. */
+int pmgpu_particles_advance_device_spare(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
+                                         int32_t euler_vort, void *const d_arrays[6],
+                                         void *const d_spare[6], int64_t np, int64_t *np_out,
+                                         int32_t *in_spare, void *stream);
 
 /* The particle -> cell map of every interpolation and mapping routine:
  * idx(d, q) = NINT((pos(d, q) - xmin(d)) * (1 / dx(d)))

@@ -73,6 +73,9 @@ SIGNATURES = {
                                           _P, C.c_int64, C.POINTER(C.c_int64)]),
     "pmgpu_particles_advance_device": (C.c_int, [C.POINTER(Geom), _I, _I, C.c_double, _I,
                                                  C.POINTER(_P), C.c_int64, C.POINTER(C.c_int64), _P]),
+    "pmgpu_particles_advance_device_spare": (C.c_int, [C.POINTER(Geom), _I, _I, C.c_double, _I, C.POINTER(_P),
+                                                       C.POINTER(_P), C.c_int64, C.POINTER(C.c_int64),
+                                                       C.POINTER(_I), _P]),
     "pmgpu_cell_index": (C.c_int, [C.POINTER(Geom), _P, C.c_int64, _P]),
     "pmgpu_penalise": (C.c_int, [C.POINTER(Geom), _P, _P, _P, _P, C.c_double, _I, _P, _P]),
     "pmgpu_penalise_device": (C.c_int, [C.POINTER(Geom), _I, _P, _P, _P, _P, C.c_double, _I, _P, _P, _P]),

@@ -395,8 +395,12 @@ static RkCoef rk_coef(int order, int stage, int euler_vort) {
 
 // arrays: pos, vel, vort, dvort, vel_rk, dvort_rk (3, np) on the device; the
 // survivors are compacted in place (through a scratch copy)
+// spare (may be null): a second set of the six arrays of the same capacity;
+// when the map reorders particles, the carried arrays' new contents go there
+// instead of back into arr, and *in_spare is set to 1 (else 0)
 int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, int euler_vort, double **arr,
-                   int64_t np, int64_t *np_out, hipStream_t st) {
+                   int64_t np, int64_t *np_out, hipStream_t st, double **spare, int *in_spare) {
+  if (in_spare) *in_spare = 0;
   PM_CHECK(rk_order >= 1 && rk_order <= 3, "pmgpu: time integration order must be 1, 2 or 3");
   PM_CHECK(stage >= 1 && stage <= rk_order, "pmgpu: RK stage out of range");
   PM_CHECK(np >= 0 && np < (int64_t)1 << 31, "pmgpu: particle count out of range");
@@ -491,12 +495,18 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
     Arr4 src{}, dst{};
     for (int a = 0; a < narr; ++a) {
       src.a[a] = arr[carry[a]];
-      dst.a[a] = w.migbuf.as<double>() + 6 * ni + (size_t)a * 3 * ni;
+      dst.a[a] = spare ? spare[carry[a]] : w.migbuf.as<double>() + 6 * ni + (size_t)a * 3 * ni;
     }
     ListCols col{};
     for (int d = 0; d < 3; ++d) col.src[d] = carried[d] ? colbuf[d][cur] : nullptr;
     hipLaunchKernelGGL(k_list_gather, grid_for(3 * n), dim3(256), 0, st, n, (const int *)plist[cur], src, dst, narr,
                        col);
+    PM_HIP(hipGetLastError());
+    if (spare) {
+      *in_spare = 1;
+      *np_out = n;
+      return 0;
+    }
     // back into the caller's arrays: one launch of 16-byte words when every
     // pointer allows it
     bool al = true;
@@ -648,7 +658,7 @@ int pmgpu_particles_advance(const pmgpu_geom *g, int32_t rk_order, int32_t stage
     dp[a] = (double *)d[a].p;
   }
   int64_t n = 0;
-  if (advance_device(g, rk_order, stage, dt, euler_vort, dp, np, &n, nullptr)) return -1;
+  if (advance_device(g, rk_order, stage, dt, euler_vort, dp, np, &n, nullptr, nullptr, nullptr)) return -1;
   for (int a = 0; a < 6; ++a)
     if (n) PM_HIP(hipMemcpy(h[a], dp[a], sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost));
   *np_out = n;
@@ -661,7 +671,26 @@ int pmgpu_particles_advance_device(const pmgpu_geom *g, int32_t rk_order, int32_
   PM_CHECK(g && d_arrays && np_out, "pmgpu_particles_advance_device: null argument");
   double *dp[6];
   for (int a = 0; a < 6; ++a) dp[a] = (double *)d_arrays[a];
-  return advance_device(g, rk_order, stage, dt, euler_vort, dp, np, np_out, (hipStream_t)stream);
+  return advance_device(g, rk_order, stage, dt, euler_vort, dp, np, np_out, (hipStream_t)stream, nullptr, nullptr);
+}
+
+int pmgpu_particles_advance_device_spare(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
+                                         int32_t euler_vort, void *const d_arrays[6], void *const d_spare[6],
+                                         int64_t np, int64_t *np_out, int32_t *in_spare, void *stream) {
+  PM_CHECK(g && d_arrays && d_spare && np_out && in_spare, "pmgpu_particles_advance_device_spare: null argument");
+  double *dp[6], *ds[6];
+  for (int a = 0; a < 6; ++a) {
+    dp[a] = (double *)d_arrays[a];
+    ds[a] = (double *)d_spare[a];
+  }
+  const int carried[4] = {0, 2, 4, 5};
+  for (int a = 0; a < (rk_order > 1 ? 4 : 2); ++a)
+    PM_CHECK(ds[carried[a]] && ds[carried[a]] != dp[carried[a]],
+             "pmgpu_particles_advance_device_spare: each carried spare array must be given and distinct");
+  int flag = 0;
+  const int rc = advance_device(g, rk_order, stage, dt, euler_vort, dp, np, np_out, (hipStream_t)stream, ds, &flag);
+  *in_spare = flag;
+  return rc;
 }
 
 int pmgpu_cell_index(const pmgpu_geom *g, const double *pos, int64_t np, int32_t *idx) {

@@ -195,15 +195,26 @@ class ParticleOps:
                    "pmgpu_remesh_device")
         return n.value
 
-    def advance(self, rk_order, stage, dt, arrays, npart, euler_vort=False, stream=None):
+    def advance(self, rk_order, stage, dt, arrays, npart, euler_vort=False, stream=None, spare=None):
         """arrays = (pos, vel, vort, dvort, vel_rk, dvort_rk) device addresses;
-        returns the new particle count."""
+        returns the new particle count.  With spare (a second such set, at
+        least npart particles each), returns (count, in_spare): in_spare True
+        when the reordered carried arrays were written to spare, which then
+        holds the particle set."""
         n = C.c_int64()
         ptrs = (C.c_void_p * 6)(*arrays)
-        _lib.check(self.lib.pmgpu_particles_advance_device(C.byref(self.g), int(rk_order), int(stage), float(dt),
-                                                           int(euler_vort), ptrs, int(npart), C.byref(n),
-                                                           stream), "pmgpu_particles_advance_device")
-        return n.value
+        if spare is None:
+            _lib.check(self.lib.pmgpu_particles_advance_device(C.byref(self.g), int(rk_order), int(stage),
+                                                               float(dt), int(euler_vort), ptrs, int(npart),
+                                                               C.byref(n), stream), "pmgpu_particles_advance_device")
+            return n.value
+        sp = (C.c_void_p * 6)(*spare)
+        flag = C.c_int32()
+        _lib.check(self.lib.pmgpu_particles_advance_device_spare(C.byref(self.g), int(rk_order), int(stage),
+                                                                 float(dt), int(euler_vort), ptrs, sp, int(npart),
+                                                                 C.byref(n), C.byref(flag), stream),
+                   "pmgpu_particles_advance_device_spare")
+        return n.value, bool(flag.value)
 
 
 __all__ = ["cell_index", "pmlib_interp_particle_mesh", "pmlib_interp_mesh_particle", "interp_mesh_field",

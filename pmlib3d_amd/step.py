@@ -115,7 +115,13 @@ class VortexStep:
                                  self.mesh_dvort.data_ptr(), self.visc, fd_order=self.fd_order,
                                  interp_order=self.interp_order)
             self.ops.m2p(self.mesh_dvort.data_ptr(), a["pos"].data_ptr(), n, a["dvort"].data_ptr())
-        self.np = self.ops.advance(self.rk_order, s, dt, [a[k].data_ptr() for k in _FIELDS], n)
+        # the map writes a reordered set into the other buffer set, which then
+        # becomes the current one (no copy back)
+        spare = self.buf[1 - self.cur]
+        self.np, swapped = self.ops.advance(self.rk_order, s, dt, [a[k].data_ptr() for k in _FIELDS], n,
+                                            spare=[spare[k].data_ptr() for k in _FIELDS])
+        if swapped:
+            self.cur = 1 - self.cur
 
     def remesh(self, min_vort, min_dvort=0.0):
         a = self.arrays
