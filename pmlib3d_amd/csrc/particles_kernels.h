@@ -307,7 +307,12 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
   constexpr bool F64 = sizeof(T) == sizeof(double);
   __shared__ T lds[NV][kRowCap];
   const int lane = threadIdx.x;
-  const int x0 = blockIdx.x * 64, iy = blockIdx.y, iz = blockIdx.z;
+  // XCD-aware line order: the workgroups of one XCD take consecutive lines
+  // (x fastest, then y, then z), so that the record rows neighbouring lines
+  // share are fetched into that XCD's L2 once rather than into every XCD's
+  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y * gridDim.z);
+  const int x0 = (int)(lt % gridDim.x) * 64, iy = (int)((lt / gridDim.x) % gridDim.y),
+            iz = (int)(lt / ((int64_t)gridDim.x * gridDim.y));
   const int ix = x0 + lane;
   const bool node_ok = ix < g.E[0];
   const int x1 = min(x0 + 64 + NS - 2, g.nb[0] - 1);
@@ -443,7 +448,10 @@ __global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const doubl
   const int64_t np = npc.get();
   constexpr bool F64 = sizeof(T) == sizeof(double);
   constexpr bool M6 = NS == 6;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+  // XCD-aware: the workgroups of one XCD take consecutive particle ranges
+  // (neighbours in the mesh after remeshing), sharing the nodes they gather
+  const int64_t b0 = xcd_tile(blockIdx.x, gridDim.x);
+  for (int64_t q = b0 * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     int idx[3][NS];
     T w[3][NS];
 #pragma unroll
