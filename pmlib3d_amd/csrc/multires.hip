@@ -316,9 +316,13 @@ struct PassArgs {
 // output cell (all three components).
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(256) k_m2m_pass(const Tin *__restrict__ in, Tout *__restrict__ out, PassArgs a) {
-  const int xy = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  // XCD-aware order: the workgroups of one XCD take consecutive (xy, z)
+  // tiles, so a z pass's taps on neighbouring planes are shared in its L2
+  const int64_t lt = xcd_tile(blockIdx.x + (int64_t)gridDim.x * blockIdx.y, (int64_t)gridDim.x * gridDim.y);
+  const int bx = (int)(lt % gridDim.x), bz = (int)(lt / gridDim.x);
+  const int xy = (int)(bx * blockDim.x + threadIdx.x);
   if (xy >= a.Eout[0] * a.Eout[1]) return;
-  int x[3] = {xy % a.Eout[0], xy / a.Eout[0], (int)blockIdx.y};
+  int x[3] = {xy % a.Eout[0], xy / a.Eout[0], bz};
   const int64_t c = (int64_t)x[0] + (int64_t)a.Eout[0] * (x[1] + (int64_t)a.Eout[1] * x[2]);
   const int64_t sin[3] = {3, 3 * (int64_t)a.Ein[0], 3 * (int64_t)a.Ein[0] * a.Ein[1]};
   {

@@ -79,9 +79,14 @@ __global__ void __launch_bounds__(kTx *kTy) k_penalise(PenArgs<T> a) {
   __shared__ double red[kTy][5];
   const BGeom &g = a.g;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * kTx + tx;
-  const int i0 = blockIdx.x * kTx + 1, j0 = blockIdx.y * kTy + 1;   // interior indices are 1..N
+  // XCD-aware tile order: the workgroups of one XCD take consecutive tiles
+  // (i fastest, then j, then the z chunk), whose halo rows they share in L2
+  const int64_t lt = xcd_tile(block_linear(), (int64_t)gridDim.x * gridDim.y * gridDim.z);
+  const int bx = (int)(lt % gridDim.x), by = (int)((lt / gridDim.x) % gridDim.y),
+            bz = (int)(lt / ((int64_t)gridDim.x * gridDim.y));
+  const int i0 = bx * kTx + 1, j0 = by * kTy + 1;   // interior indices are 1..N
   const int i = i0 + tx, j = j0 + ty;
-  const int k0 = blockIdx.z * kZc + 1, k1 = min(k0 + kZc - 1, g.N[2]);
+  const int k0 = bz * kZc + 1, k1 = min(k0 + kZc - 1, g.N[2]);
   const bool own = i <= g.N[0] && j <= g.N[1];
   const int64_t sy = g.E[0], sz = (int64_t)g.E[0] * g.E[1];
   auto cell = [&](int ii, int jj, int kk) {
@@ -164,7 +169,7 @@ __global__ void __launch_bounds__(kTx *kTy) k_penalise(PenArgs<T> a) {
           double t = 0.0;
           for (int w = 0; w < kTy; ++w) t += red[w][tx];
           // the block index of the (i tile, j tile, plane) partial
-          const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * (kf - 1));
+          const int64_t b = bx + (int64_t)gridDim.x * (by + (int64_t)gridDim.y * (kf - 1));
           a.partial[5 * b + tx] = t;
         }
         __syncthreads();
