@@ -448,10 +448,7 @@ __global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const doubl
   const int64_t np = npc.get();
   constexpr bool F64 = sizeof(T) == sizeof(double);
   constexpr bool M6 = NS == 6;
-  // XCD-aware: the workgroups of one XCD take consecutive particle ranges
-  // (neighbours in the mesh after remeshing), sharing the nodes they gather
-  const int64_t b0 = xcd_tile(blockIdx.x, gridDim.x);
-  for (int64_t q = b0 * blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
     int idx[3][NS];
     T w[3][NS];
 #pragma unroll
