@@ -25,6 +25,9 @@ CASES = [
     ((12, 12, 12), 0, (1, 1, 1), 4, "const"),
     ((10, 14, 9), 1, (0, 0, 0), 2, "field"),
     ((9, 10, 12), 2, (1, 0, 1), 2, "const"),
+    # several 16-plane z chunks of the z-marching kernel, the last one partial
+    ((20, 11, 37), 2, (0, 0, 0), 4, "field"),
+    ((13, 9, 33), 1, (1, 0, 1), 2, None),
 ]
 
 
