@@ -329,7 +329,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     PM_CHECK(refine == 0.0 || prm.poisson_order != 100,
              "pmgpu_poisson_setup: Gaussian regularisation must be used for multi-patch simulations "
              "(poisson/pmlib_poisson_setup.f:1514-1518)");
-    if (allper) return 0;  // G = zeta / (4 pi^2 k^2) is formed in the z pass
+    if (allper) return tune_vector();  // G = zeta / (4 pi^2 k^2) is formed in the z pass
     if (ker2) return setup_kernel2();
     // FP64 kernel grid and its transform (x R2C, y and z C2C), then the octant
     double *Gr = nullptr;
@@ -390,7 +390,7 @@ template <typename R> struct PoissonGPU : PoissonBase {
     (void)hipFree(Gr);
     (void)hipFree(Gc);
     (void)hipFree(dt);
-    return 0;
+    return tune_vector();
   }
 
   // Kernel 2: grad G on the kernel grid (three interleaved components), its
@@ -480,33 +480,9 @@ template <typename R> struct PoissonGPU : PoissonBase {
     return 0;
   }
 
-  // ---- pmlib_poisson_solve ---------------------------------------------
-  int solve_device(const void *vort, void *vel, void *vort_out, void *vecpot, int reg, int reproj,
-                   int psi) override {
-    const bool want_vort = reg || reproj;
-    const bool want_psi = psi && prm.poisson_kernel == 1;
-    PM_CHECK(allper || (ker2 ? Khat != nullptr : Ghat != nullptr), "pmgpu: pmgpu_poisson_setup was not called");
-    PM_CHECK(vort && vel, "pmgpu: null mesh pointer");
-    PM_CHECK(!want_vort || vort_out, "pmgpu: vort_out required with reg_vort/reproj");
-    PM_CHECK(!want_vort || prm.poisson_order != 100,
-             "pmgpu: order 100 has no regularised or reprojected vorticity output "
-             "(poisson/pmlib_poisson_solve.f:311-329 defines no filter for it)");
-    PM_CHECK(!want_psi || vecpot, "pmgpu: vecpot required with calc_vecpot");
-    if (want_psi && !Bpsi) PM_HIP(hipMalloc(&Bpsi, nB * sizeof(V)));
-    if (want_vort && !Bvort) PM_HIP(hipMalloc(&Bvort, nB * sizeof(V)));
-    if (record(0)) return -1;
-    const int64_t off = 3 * ((int64_t)lo[0] + (int64_t)E[0] * (lo[1] + (int64_t)E[1] * lo[2]));
-    const int64_t cs = 1, xs = 3, ys = 3 * (int64_t)E[0], zs = 3 * (int64_t)E[0] * E[1];
-    const R *vin = reinterpret_cast<const R *>(vort) + off;
-    int e = 0;
-    // component-interleaved spectra: A[((z*My + y)*Hx + kx)*3 + c],
-    // B[((z*Ly + ky)*Hx + kx)*3 + c]
+  // arguments of the three-component z pass
+  ZArgs z3_args(bool want_psi, bool want_vort, int reproj) const {
     const int64_t H3 = H3p;
-    const int ncols = 3 * Hx;
-    e |= launch_xfwd<R>(vin, A, cs, xs, ys, zs, 1, 3, H3, H3 * M[1], 3, M[0], M[1], M[2], Tx, pNh, twLx_off, d_tw);
-    e |= launch_strided<R, -1>(A, B, H3, H3, H3 * M[1], H3 * L[1], ncols, M[1], L[1], M[2], 1, 1.0, Ts, pLy,
-                               d_tw);
-    if (e) return -1;
     ZArgs za;
     za.B = B;
     za.Bpsi = Bpsi;
@@ -541,6 +517,80 @@ template <typename R> struct PoissonGPU : PoissonBase {
     za.h3 = g.dx[0] * g.dx[1] * g.dx[2];
     za.eps2pi = prm.regularisation_radius * dmax * 2.0 * 3.14159265358979323846;
     za.plan = pLz;
+    return za;
+  }
+
+  // Autotuner of the three-component velocity z pass: times the register
+  // form at 2, 4 and 8 columns per block (two sweeps of 5 launches, a
+  // candidate's time its faster sweep) on the handle's own spectra buffer and
+  // keeps the fastest, unless the built-in 4 is within 3 % of it; cached per
+  // transform shape like the scalar pass.  PMGPU_Z_TC fixes the width.
+  int tune_vector() {
+    if (tune_tc > 0 || ker2 || zkernel == 1) return 0;
+    char k[128];
+    snprintf(k, sizeof(k), "z3:L%d:Ly%d:Hx%d:%s", L[2], L[1], Hx, sizeof(R) == 4 ? "f32" : "f64");
+    if (int v = tune_cache().get(k)) {
+      tune_tc = v;
+      return 0;
+    }
+    PM_HIP(hipMemsetAsync(B, 0, nB * sizeof(V), stream));
+    hipEvent_t e0, e1;
+    PM_HIP(hipEventCreate(&e0));
+    PM_HIP(hipEventCreate(&e1));
+    const int cands[3] = {2, 4, 8};
+    float best[3] = {1e30f, 1e30f, 1e30f};
+    for (int sweep = 0; sweep < 2; ++sweep)
+      for (int c = 0; c < 3; ++c) {
+        ZArgs za = z3_args(false, false, 0);
+        za.tune_tc = cands[c];
+        if (fx_z<R>(L[2], za, L[1], d_tw, stream) != 1) continue;   // warm-up / availability
+        PM_HIP(hipEventRecord(e0, stream));
+        for (int r = 0; r < 5; ++r) (void)fx_z<R>(L[2], za, L[1], d_tw, stream);
+        PM_HIP(hipEventRecord(e1, stream));
+        PM_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        PM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best[c]) best[c] = ms;
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int pick = 1;
+    for (int c = 0; c < 3; ++c)
+      if (best[c] < best[pick] * 0.97f) pick = c;
+    if (best[pick] >= 1e30f) return 0;   // no register form at this length: the built-in path
+    tune_tc = cands[pick];
+    tune_cache().put(k, tune_tc);
+    return 0;
+  }
+
+  // ---- pmlib_poisson_solve ---------------------------------------------
+  int solve_device(const void *vort, void *vel, void *vort_out, void *vecpot, int reg, int reproj,
+                   int psi) override {
+    const bool want_vort = reg || reproj;
+    const bool want_psi = psi && prm.poisson_kernel == 1;
+    PM_CHECK(allper || (ker2 ? Khat != nullptr : Ghat != nullptr), "pmgpu: pmgpu_poisson_setup was not called");
+    PM_CHECK(vort && vel, "pmgpu: null mesh pointer");
+    PM_CHECK(!want_vort || vort_out, "pmgpu: vort_out required with reg_vort/reproj");
+    PM_CHECK(!want_vort || prm.poisson_order != 100,
+             "pmgpu: order 100 has no regularised or reprojected vorticity output "
+             "(poisson/pmlib_poisson_solve.f:311-329 defines no filter for it)");
+    PM_CHECK(!want_psi || vecpot, "pmgpu: vecpot required with calc_vecpot");
+    if (want_psi && !Bpsi) PM_HIP(hipMalloc(&Bpsi, nB * sizeof(V)));
+    if (want_vort && !Bvort) PM_HIP(hipMalloc(&Bvort, nB * sizeof(V)));
+    if (record(0)) return -1;
+    const int64_t off = 3 * ((int64_t)lo[0] + (int64_t)E[0] * (lo[1] + (int64_t)E[1] * lo[2]));
+    const int64_t cs = 1, xs = 3, ys = 3 * (int64_t)E[0], zs = 3 * (int64_t)E[0] * E[1];
+    const R *vin = reinterpret_cast<const R *>(vort) + off;
+    int e = 0;
+    // component-interleaved spectra: A[((z*My + y)*Hx + kx)*3 + c],
+    // B[((z*Ly + ky)*Hx + kx)*3 + c]
+    const int64_t H3 = H3p;
+    const int ncols = 3 * Hx;
+    e |= launch_xfwd<R>(vin, A, cs, xs, ys, zs, 1, 3, H3, H3 * M[1], 3, M[0], M[1], M[2], Tx, pNh, twLx_off, d_tw);
+    e |= launch_strided<R, -1>(A, B, H3, H3, H3 * M[1], H3 * L[1], ncols, M[1], L[1], M[2], 1, 1.0, Ts, pLy,
+                               d_tw);
+    if (e) return -1;
+    ZArgs za = z3_args(want_psi, want_vort, reproj);
     if (record(1)) return -1;
     int fz = fx_z<R>(L[2], za, L[1], d_tw, stream);
     PM_CHECK(fz >= 0, "pmgpu: z-pass launch failed");
