@@ -138,8 +138,9 @@ int pmgpu_poisson_batch_destroy(pmgpu_poisson_batch_t b);
  * meshes; the *_device variants take device pointers, a hipStream_t (NULL:
  * the default stream) and the mesh precision: PMGPU_FP64 reproduces the
  * reference's arithmetic, PMGPU_FP32 uses an FP32 mesh with FP32 kernel
- * weights, Kahan-compensated FP32 sums in deposition and FP64 sums of the
- * weighted FP32 node values in the gather.
+ * weights, compensated FP32 sums in deposition (each stencil row's terms of
+ * a node fused into one FP32 partial, the row partials Kahan-added) and FP64
+ * sums of the weighted FP32 node values in the gather.
  * ------------------------------------------------------------------------ */
 
 /* Replaces pmlib_interp_particle_mesh(topo, xpart, part, mesh, ierr, clear,

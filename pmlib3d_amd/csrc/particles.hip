@@ -111,7 +111,8 @@ void particles_release() {
 
 // ---------------------------------------------------------------- P2M
 // prec: PMGPU_FP64 (double mesh, reference arithmetic) or PMGPU_FP32 (float
-// mesh, FP32 weights with Kahan-compensated sums); particles are FP64.
+// mesh, FP32 weights, per-row partial sums Kahan-added across the stencil
+// rows); particles are FP64.
 // Order each bin's particles by id (one thread per bin), stage their records
 // (one thread per sorted particle), then gather them line by line of 64 nodes
 // along x (one wavefront per line).

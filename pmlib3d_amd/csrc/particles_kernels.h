@@ -268,8 +268,9 @@ __global__ void __launch_bounds__(256) k_p2m_stage(PGeom g, const double *pos, c
 // current row is summed.  Per node the terms arrive in the order kz
 // ascending (sz descending), ky ascending (sy descending), records
 // ascending (bins ascending, particles by id within a bin); FP64 terms are
-// ((wx wy) wz) v rounded per operation as the reference does, FP32 sums are
-// Kahan-compensated.
+// ((wx wy) wz) v rounded per operation as the reference does; FP32 terms
+// are wx (wy wz) v, fused into one partial sum per stencil row, and the ns^2
+// row partials are Kahan-added.
 constexpr int kRowCap = 128;   // records staged per row pass (two per lane)
 
 template <typename T, int NS> struct RowStage {
@@ -290,8 +291,14 @@ __device__ __forceinline__ void row_fetch(RowStage<T, NS> &st, const T *rec, int
     const T *r = rec + t;
 #pragma unroll
     for (int s = 0; s < NS; ++s) st.val[h][s] = r[s * rs];
-    st.val[h][NS] = r[(NS + sy) * rs];
-    st.val[h][NS + 1] = r[(2 * NS + sz) * rs];
+    if constexpr (sizeof(T) == sizeof(double)) {
+      st.val[h][NS] = r[(NS + sy) * rs];
+      st.val[h][NS + 1] = r[(2 * NS + sz) * rs];
+    } else {
+      // FP32: the record's y-z weight product, formed once for its ns lanes
+      st.val[h][NS] = r[(NS + sy) * rs] * r[(2 * NS + sz) * rs];
+      st.val[h][NS + 1] = (T)0;
+    }
     st.val[h][NS + 2] = r[KV * rs];
     st.val[h][NS + 3] = r[(KV + 1) * rs];
     st.val[h][NS + 4] = r[(KV + 2) * rs];
@@ -321,6 +328,7 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
   auto krow = [&](int r) { return g.nb[0] * ((iy + r % NS) + g.nb[1] * (iz + r / NS)); };
   double d0 = 0.0, d1 = 0.0, d2 = 0.0;
   Kahan f0, f1, f2;
+  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
   auto to_lds = [&](const RowStage<T, NS> &st, int m) {
 #pragma unroll
     for (int h = 0; h < kRowCap / 64; ++h) {
@@ -343,10 +351,20 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
       d1 = __dadd_rn(d1, __dmul_rn(w, lds[NS + 3][i]));
       d2 = __dadd_rn(d2, __dmul_rn(w, lds[NS + 4][i]));
     } else {
-      const float w = wx * lds[NS][i] * lds[NS + 1][i];
-      f0.add(w * lds[NS + 2][i]);
-      f1.add(w * lds[NS + 3][i]);
-      f2.add(w * lds[NS + 4][i]);
+      const float w = wx * lds[NS][i];
+      r0 = __fmaf_rn(w, lds[NS + 2][i], r0);
+      r1 = __fmaf_rn(w, lds[NS + 3][i], r1);
+      r2 = __fmaf_rn(w, lds[NS + 4][i], r2);
+    }
+  };
+  // FP32: a stencil row's terms (at most ns records per lane on a remeshed
+  // set) go into one partial sum, and the row partials are Kahan-added
+  auto row_done = [&]() {
+    if constexpr (!F64) {
+      f0.add(r0);
+      f1.add(r1);
+      f2.add(r2);
+      r0 = r1 = r2 = 0.0f;
     }
   };
   auto sum = [&](int lo, int hi, int base) {
@@ -397,6 +415,7 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
           sum(max(a0, T0 + p), min(a1, T0 + p + m), T0 + p);
           __syncthreads();
         }
+        row_done();
       }
       cur = nxt;
     }

@@ -23,8 +23,8 @@ class VortexStep:
     def __init__(self, patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3, capacity=None,
                  device="cuda:0", precision=_lib.FP64, visc=None, fd_order=4):
         """precision selects the mesh side (deposition, solve, gather,
-        remesh thresholds): FP64, or FP32 (Kahan-compensated deposition
-        sums, FP64 gather sums).  Particles are FP64 either way.  visc: each stage also forms
+        remesh thresholds): FP64, or FP32 (compensated deposition sums:
+        per-row partials, Kahan-added; FP64 gather sums).  Particles are FP64 either way.  visc: each stage also forms
         the vorticity right-hand side visc lap(w) + (w . grad) u on the mesh
         (finite differences of order fd_order; needs fd_order / 2 ghost
         layers) and interpolates it to the particles' dvort; None: dvort = 0
