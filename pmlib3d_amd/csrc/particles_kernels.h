@@ -514,17 +514,34 @@ __global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const doubl
     int xo[NS];
 #pragma unroll
     for (int sx = 0; sx < NS; ++sx) xo[sx] = 3 * (idx[0][sx] - g.lo[0]);
+    // stencil rows r = (sz, sy) in order, G rows at a time: a group's node
+    // triples are all loaded before any is summed (G ns gathers in flight per
+    // wavefront rather than one), and the scheduling barriers keep the
+    // compiler from moving loads back beside their uses or hoisting the next
+    // group's loads over this group's sums (register use stays bounded)
+    constexpr int G = F64 ? (M6 ? 1 : 2) : (M6 ? 2 : 4);
 #pragma unroll
-    for (int sz = 0; sz < NS; ++sz)
+    for (int r0 = 0; r0 < NS * NS; r0 += G) {
+      T mv[G][NS][3];
 #pragma unroll
-      for (int sy = 0; sy < NS; ++sy) {
+      for (int k = 0; k < G; ++k) {
+        const int sy = (r0 + k) % NS, sz = (r0 + k) / NS;
         const int64_t row = (int64_t)g.E[0] * ((idx[1][sy] - g.lo[1]) + (int64_t)g.E[1] * (idx[2][sz] - g.lo[2]));
         const T *mrow = mesh + 3 * row;
+#pragma unroll
+        for (int sx = 0; sx < NS; ++sx)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) mv[k][sx][c] = mrow[xo[sx] + c];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        const int sy = (r0 + k) % NS, sz = (r0 + k) / NS;
         // FP32 form: the FP32 weights' product in FP64, y and z once per row
         const double wyz = F64 ? 0.0 : (double)w[1][sy] * (double)w[2][sz];
 #pragma unroll
         for (int sx = 0; sx < NS; ++sx) {
-          const T *m = mrow + xo[sx];
+          const T *m = mv[k][sx];
           if constexpr (F64) {
             const double wt = __dmul_rn(__dmul_rn(w[0][sx], w[1][sy]), w[2][sz]);
             const double t0 = __dmul_rn(wt, m[0]), t1 = __dmul_rn(wt, m[1]), t2 = __dmul_rn(wt, m[2]);
@@ -546,6 +563,8 @@ __global__ void __launch_bounds__(256) k_m2p(PGeom g, const T *mesh, const doubl
           }
         }
       }
+      __builtin_amdgcn_sched_barrier(0);
+    }
     out[3 * q] = a0;
     out[3 * q + 1] = a1;
     out[3 * q + 2] = a2;
