@@ -274,8 +274,9 @@ __global__ void __launch_bounds__(256) k_p2m_stage(PGeom g, const double *pos, c
 constexpr int kRowCap = 128;   // records staged per row pass (two per lane)
 
 template <typename T, int NS> struct RowStage {
-  static constexpr int NV = NS + 6;   // wx[ns], wy, wz, v0, v1, v2, x slot base
+  static constexpr int NV = NS + 5;   // wx[ns], wy, wz, v0, v1, v2
   T val[kRowCap / 64][NV];
+  int xs[kRowCap / 64];               // x slot base
 };
 
 template <typename T, int NS>
@@ -303,7 +304,7 @@ __device__ __forceinline__ void row_fetch(RowStage<T, NS> &st, const T *rec, int
     st.val[h][NS + 3] = r[(KV + 1) * rs];
     st.val[h][NS + 4] = r[(KV + 2) * rs];
     // x slot of lane 0: sx = lane + (x0 + ns - 1 - kx)
-    st.val[h][NS + 5] = (T)(x0 + NS - 1 - ((int)r[KB * rs] - blo0));
+    st.xs[h] = x0 + NS - 1 - ((int)r[KB * rs] - blo0);
   }
 }
 
@@ -313,6 +314,7 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
   constexpr int NV = RowStage<T, NS>::NV, NR = NS * NS;
   constexpr bool F64 = sizeof(T) == sizeof(double);
   __shared__ T lds[NV][kRowCap];
+  __shared__ int ldsx[kRowCap];
   const int lane = threadIdx.x;
   // XCD-aware line order: the workgroups of one XCD take consecutive lines
   // (x fastest, then y, then z), so that the record rows neighbouring lines
@@ -336,6 +338,7 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
       if (i < m) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) lds[v][i] = st.val[h][v];
+        ldsx[i] = st.xs[h];
       }
     }
   };
@@ -343,7 +346,7 @@ __global__ void __launch_bounds__(64) k_p2m_rows(PGeom g, const int *start, cons
   // first 8 as straight-line predicated code (no loop for the wait counters
   // to drain pending loads in front of), the rest in a loop
   auto term = [&](int i) {
-    const int sx = lane + (int)lds[NS + 5][i];   // in [0, ns) for the lane's bins
+    const int sx = lane + ldsx[i];   // in [0, ns) for the lane's bins
     const T wx = lds[sx][i];
     if constexpr (F64) {
       const double w = __dmul_rn(__dmul_rn(wx, lds[NS][i]), lds[NS + 1][i]);
