@@ -259,7 +259,7 @@ def step_benchmark(steps, n=128):
     d_vort = torch.from_numpy(vort.ravel(order="F")).to(dev)
     res = {"workload": "two-ring vortex step, 128^3 free, all 128^3 = %d jittered particles every step "
                        "(re-loaded before each), order 6, eps 2h, M'4, RK2 dt=h/2, remesh min_vort=1e-6; "
-                       "FP32 = FP32 meshes, solve and interpolation (FP32 weights, Kahan-summed deposition), FP64 particles" % npart}
+                       "FP32 = FP32 meshes, solve and interpolation (FP32 weights; deposition in per-row FP32 partials, Kahan-added), FP64 particles" % npart}
     fetched = {}
     for name, prec in (("fp64", pm.FP64), ("fp32", pm.FP32)):
         st = pm.VortexStep(patch, order=6, eps_radius=2.0, rk_order=2, interp_order=3,

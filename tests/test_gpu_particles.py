@@ -375,7 +375,7 @@ def test_device_entry_points_match_host_entry_points():
 # ---------------------------------------------------------- FP32 meshes
 @pytest.mark.parametrize("case", [GEOMS[0], GEOMS[2], GEOMS[5]], ids=["m4_free", "m4_periodic", "m6_periodic"])
 def test_fp32_mesh_interpolation_against_fp64(case):
-    """FP32 meshes (FP32 weights; Kahan sums in deposition, FP64 sums in
+    """FP32 meshes (FP32 weights; Kahan-added row partials in deposition, FP64 sums in
     the gather) against the FP64 oracle:
     relative L2 <= 1e-6 for deposition and gather; the FP32 remesh of an
     FP32 mesh places the same particles as the FP64 remesh of its values."""

@@ -83,7 +83,7 @@ def test_step_with_vorticity_rhs_matches_oracle(fd_order):
 
 def test_step_fp32_against_fp64_64():
     """The FP32 step (FP32 meshes, solve and interpolation with FP32
-    weights, Kahan-summed deposition, FP64 particles): the same particle set and velocities within 1e-5
+    weights, Kahan-added row partials in deposition, FP64 particles): the same particle set and velocities within 1e-5
     relative L2 of the FP64 step after one step."""
     import pmlib3d_amd as pm
     n = 64
