@@ -47,16 +47,77 @@ static int scan_exclusive(const int *in, int *out, int64_t n, Scratch &tmp, hipS
   return 0;
 }
 
-// Per-particle counts of west movers, east movers and drops (the migration
-// classes 1, 2, 3) summed by one scan: exclusive offsets off[0] = 0,
-// off[q + 1] = inclusive scan.
+// Counts of west movers, east movers and drops (the migration classes 1, 2,
+// 3).  The migrations scan them in two levels over tiles of kPartTile list
+// entries: class totals per tile, one block scanning the tile totals into
+// tile offsets, then each tile's own block scans in index order.
 struct Mig3 {
   int w, e, d;
 };
-struct Mig3Plus {
-  __host__ __device__ Mig3 operator()(const Mig3 &x, const Mig3 &y) const { return Mig3{x.w + y.w, x.e + y.e, x.d + y.d}; }
-};
-__global__ void k_zero_first3(Mig3 *off) { off[0] = Mig3{0, 0, 0}; }
+constexpr int kPartThreads = 256, kPartItems = 8, kPartTile = kPartThreads * kPartItems;
+__device__ __forceinline__ Mig3 m3_of(int c) { return Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0}; }
+__device__ __forceinline__ Mig3 m3_add(const Mig3 &a, const Mig3 &b) { return Mig3{a.w + b.w, a.e + b.e, a.d + b.d}; }
+// exclusive scan of one Mig3 per thread over a kPartThreads-thread block;
+// every thread of the block calls it
+__device__ Mig3 block_scan3(Mig3 v, Mig3 &total) {
+  __shared__ Mig3 wsum[kPartThreads / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  Mig3 x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const Mig3 y{__shfl_up(x.w, o), __shfl_up(x.e, o), __shfl_up(x.d, o)};
+    if (lane >= o) x = m3_add(x, y);
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  Mig3 base{0, 0, 0};
+  total = Mig3{0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < kPartThreads / 64; ++k) {
+    if (k < wv) base = m3_add(base, wsum[k]);
+    total = m3_add(total, wsum[k]);
+  }
+  __syncthreads();   // wsum is free for the next call
+  return Mig3{base.w + x.w - v.w, base.e + x.e - v.e, base.d + x.d - v.d};
+}
+__device__ __forceinline__ bool part_idle(const int *counts) { return counts[1] + counts[2] + counts[3] == 0; }
+// tile sums -> exclusive tile offsets, in place (one block of
+// kTileScanThreads): each wavefront owns a contiguous share of the tiles,
+// sums it, and after the totals are combined walks it 64 tiles at a time
+// with a wavefront scan, the next 64 loaded while the current are scanned
+// (one block barrier in all)
+constexpr int kTileScanThreads = 1024;
+__global__ void __launch_bounds__(kTileScanThreads) k_part_tile_scan(const int *counts, Mig3 *tsum, int ntiles) {
+  if (part_idle(counts)) return;
+  constexpr int NW = kTileScanThreads / 64;
+  __shared__ Mig3 wtot[NW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int per = (ntiles + NW - 1) / NW;
+  const int t0 = min(wv * per, ntiles), t1 = min(t0 + per, ntiles);
+  const Mig3 zero{0, 0, 0};
+  Mig3 s = zero;
+  for (int t = t0 + lane; t < t1; t += 64) s = m3_add(s, tsum[t]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s = m3_add(s, Mig3{__shfl_xor(s.w, o), __shfl_xor(s.e, o), __shfl_xor(s.d, o)});
+  if (lane == 0) wtot[wv] = s;
+  __syncthreads();
+  Mig3 run = zero;
+  for (int k = 0; k < wv; ++k) run = m3_add(run, wtot[k]);
+  Mig3 vn = t0 + lane < t1 ? tsum[t0 + lane] : zero;
+  for (int b = t0; b < t1; b += 64) {
+    const int t = b + lane;
+    const Mig3 v = vn;
+    vn = t + 64 < t1 ? tsum[t + 64] : zero;
+    Mig3 x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const Mig3 y{__shfl_up(x.w, o), __shfl_up(x.e, o), __shfl_up(x.d, o)};
+      if (lane >= o) x = m3_add(x, y);
+    }
+    if (t < t1) tsum[t] = Mig3{run.w + x.w - v.w, run.e + x.e - v.e, run.d + x.d - v.d};
+    run = m3_add(run, Mig3{__shfl(x.w, 63), __shfl(x.e, 63), __shfl(x.d, 63)});
+  }
+}
 
 // ------------------------------------------------------------ geometry
 static int make_pgeom(const pmgpu_geom *g, PGeom &pg) {
@@ -298,13 +359,19 @@ int remesh_device(const pmgpu_geom *geom, int prec, const void *vort_m, const vo
 // wrapped coordinate are those of permuting the arrays at every pass, which
 // is what the reference's map does.
 
-// migration class of each listed particle (0 stay, 1 west, 2 east, 3 drop)
-// with its class indicators for the offset scan and the class counts; with
-// pos given, the coordinate column cd is first gathered from the positions
-// (coordinate d of the listed particles, the identity list when perm is null)
-__global__ void k_list_classify(int64_t n, const int *perm, const double *pos, int d, double *cd, double lo,
-                                double hi, int per, int *cls, Mig3 *key, int *counts) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+// migration class of each listed particle (0 stay, 1 west, 2 east, 3 drop),
+// the class totals of each tile of kPartTile entries (block b: entries
+// [b kPartTile, (b + 1) kPartTile)) and the class counts; with pos given,
+// the coordinate column cd is first gathered from the positions (coordinate
+// d of the listed particles, the identity list when perm is null)
+__global__ void __launch_bounds__(kPartThreads) k_list_classify(int64_t n, const int *perm, const double *pos, int d,
+                                                                double *cd, double lo, double hi, int per, int *cls,
+                                                                Mig3 *tsum, int *counts) {
+  Mig3 s{0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < kPartItems; ++j) {
+    const int64_t i = (int64_t)blockIdx.x * kPartTile + j * kPartThreads + threadIdx.x;
+    if (i >= n) break;
     double x;
     if (pos) {
       x = pos[3 * (int64_t)(perm ? perm[i] : (int)i) + d];
@@ -315,14 +382,15 @@ __global__ void k_list_classify(int64_t n, const int *perm, const double *pos, i
     int c = x < lo ? 1 : (x >= hi ? 2 : 0);
     if (c != 0 && !per) c = 3;
     cls[i] = c;
-    key[i] = Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0};
-    // one atomic per wavefront and class
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 1; k <= 3; ++k) {
-      const uint64_t m = __ballot(c == k);
-      if (m && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&counts[k], __popcll(m));
-    }
+    s = m3_add(s, m3_of(c));
+  }
+  Mig3 tot;
+  (void)block_scan3(s, tot);
+  if (threadIdx.x == 0) {
+    tsum[blockIdx.x] = tot;
+    if (tot.w) atomicAdd(&counts[1], tot.w);
+    if (tot.e) atomicAdd(&counts[2], tot.e);
+    if (tot.d) atomicAdd(&counts[3], tot.d);
   }
 }
 // coordinate columns moved with the list: the current axis' column (index 0)
@@ -332,13 +400,20 @@ struct ListCols {
   double *dst[3];
   int n;
 };
-// one partition pass of the list; movers' coordinates wrapped by one period
-__global__ void k_list_partition(int64_t n, const int *cls, const Mig3 *off, int n1, int n2, double len,
-                                 const int *perm, int *perm2, ListCols cols) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int c = cls[i];
-    if (c == 3) continue;
-    const Mig3 o = off[i];
+// one partition pass of the list, block b on tile b from its offset toff[b];
+// movers' coordinates wrapped by one period
+__global__ void __launch_bounds__(kPartThreads) k_list_partition(int64_t n, const int *cls, const Mig3 *toff, int n1,
+                                                                 int n2, double len, const int *perm, int *perm2,
+                                                                 ListCols cols) {
+  Mig3 run = toff[blockIdx.x];
+  for (int j = 0; j < kPartItems; ++j) {
+    const int64_t i = (int64_t)blockIdx.x * kPartTile + j * kPartThreads + threadIdx.x;
+    const int c = i < n ? cls[i] : 0;
+    Mig3 tot;
+    const Mig3 ex = block_scan3(m3_of(c), tot);
+    const Mig3 o = m3_add(run, ex);
+    run = m3_add(run, tot);
+    if (i >= n || c == 3) continue;
     const int64_t to = c == 1 ? o.w : (c == 2 ? n1 + o.e : n1 + n2 + (i - o.w - o.e - o.d));
     double x = cols.src[0][i];
     if (c == 1) x = x + len;
@@ -437,16 +512,17 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
   const int carry[4] = {0, 2, 4, 5};
   const int narr = rk_order > 1 ? 4 : 2;
   const int64_t n0 = n;
-  // list workspace: cls, two index lists (ints), keys and offsets (Mig3),
-  // two buffers of each axis' coordinate column and the gathered rows
-  // (doubles)
+  // list workspace: cls, two index lists (ints), the tile totals and
+  // offsets (Mig3), two buffers of each axis' coordinate column and the
+  // gathered rows (doubles)
   const size_t ni = ((size_t)n0 + 2) & ~(size_t)1;   // even: 16-byte aligned rows in migbuf
-  if (w.cls.ensure(sizeof(int) * 3 * ni + sizeof(Mig3) * 2 * ni + 16) || w.count.ensure(sizeof(int) * 8) ||
+  const size_t nt0 = (size_t)((n0 + kPartTile - 1) / kPartTile) + 1;
+  if (w.cls.ensure(sizeof(int) * 3 * ni + sizeof(Mig3) * nt0 + 16) || w.count.ensure(sizeof(int) * 8) ||
       w.migbuf.ensure(sizeof(double) * (6 * ni + 3 * ni * narr)))
     return -1;
   int *cls = w.cls.as<int>();
   int *plist[2] = {cls + ni, cls + 2 * ni};
-  Mig3 *key = reinterpret_cast<Mig3 *>(cls + 3 * ni), *off = key + ni;
+  Mig3 *tsum = reinterpret_cast<Mig3 *>(cls + 3 * ni);
   double *colbuf[3][2];
   for (int d = 0; d < 3; ++d)
     for (int b = 0; b < 2; ++b) colbuf[d][b] = w.migbuf.as<double>() + (size_t)(2 * d + b) * ni;
@@ -460,18 +536,16 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
     for (int pass = 0; pass < 64; ++pass) {
       PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * 4, st));
       // the first pass of an axis gathers its coordinate column
-      hipLaunchKernelGGL(k_list_classify, grid_for(n), dim3(256), 0, st, n,
+      const int ntl = (int)((n + kPartTile - 1) / kPartTile);
+      hipLaunchKernelGGL(k_list_classify, dim3(ntl), dim3(kPartThreads), 0, st, n,
                          listed ? (const int *)plist[cur] : nullptr, pass == 0 ? (const double *)arr[0] : nullptr, d,
-                         colbuf[d][cur], lo, hi, per, cls, key, w.count.as<int>());
+                         colbuf[d][cur], lo, hi, per, cls, tsum, w.count.as<int>());
       int counts[4];
       PM_HIP(hipMemcpyAsync(counts, w.count.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
       PM_HIP(hipStreamSynchronize(st));
       if (counts[1] == 0 && counts[2] == 0 && counts[3] == 0) break;  // nobody moves
-      hipLaunchKernelGGL(k_zero_first3, dim3(1), dim3(1), 0, st, off);
-      size_t bytes = 0;
-      PM_HIP(rocprim::inclusive_scan(nullptr, bytes, key, off + 1, (size_t)n, Mig3Plus(), st));
-      if (w.scan.ensure(bytes + 16)) return -1;
-      PM_HIP(rocprim::inclusive_scan(w.scan.p, bytes, key, off + 1, (size_t)n, Mig3Plus(), st));
+      hipLaunchKernelGGL(k_part_tile_scan, dim3(1), dim3(kTileScanThreads), 0, st, (const int *)w.count.as<int>(), tsum,
+                         ntl);
       ListCols cols{};
       cols.src[0] = colbuf[d][cur];
       cols.dst[0] = colbuf[d][1 - cur];
@@ -481,7 +555,7 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
           cols.src[cols.n] = colbuf[e][cur];
           cols.dst[cols.n++] = colbuf[e][1 - cur];
         }
-      hipLaunchKernelGGL(k_list_partition, grid_for(n), dim3(256), 0, st, n, (const int *)cls, (const Mig3 *)off,
+      hipLaunchKernelGGL(k_list_partition, dim3(ntl), dim3(kPartThreads), 0, st, n, (const int *)cls, (const Mig3 *)tsum,
                          counts[1], counts[2], len, listed ? (const int *)plist[cur] : nullptr, plist[1 - cur], cols);
       PM_HIP(hipGetLastError());
       cur = 1 - cur;
@@ -730,33 +804,6 @@ __global__ void k_fill_int(int *p, int64_t n, int v) {
 // over tiles of kPartTile particles (tile sums, one block scanning the tile
 // sums, then each tile's own block scans in index order), so that a step with
 // nothing to migrate launches no full-size scan.
-constexpr int kPartThreads = 256, kPartItems = 8, kPartTile = kPartThreads * kPartItems;
-__device__ __forceinline__ Mig3 m3_of(int c) { return Mig3{c == 1 ? 1 : 0, c == 2 ? 1 : 0, c == 3 ? 1 : 0}; }
-__device__ __forceinline__ Mig3 m3_add(const Mig3 &a, const Mig3 &b) { return Mig3{a.w + b.w, a.e + b.e, a.d + b.d}; }
-// exclusive scan of one Mig3 per thread over a kPartThreads-thread block;
-// every thread of the block calls it
-__device__ Mig3 block_scan3(Mig3 v, Mig3 &total) {
-  __shared__ Mig3 wsum[kPartThreads / 64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  Mig3 x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const Mig3 y{__shfl_up(x.w, o), __shfl_up(x.e, o), __shfl_up(x.d, o)};
-    if (lane >= o) x = m3_add(x, y);
-  }
-  if (lane == 63) wsum[wv] = x;
-  __syncthreads();
-  Mig3 base{0, 0, 0};
-  total = Mig3{0, 0, 0};
-#pragma unroll
-  for (int k = 0; k < kPartThreads / 64; ++k) {
-    if (k < wv) base = m3_add(base, wsum[k]);
-    total = m3_add(total, wsum[k]);
-  }
-  __syncthreads();   // wsum is free for the next call
-  return Mig3{base.w + x.w - v.w, base.e + x.e - v.e, base.d + x.d - v.d};
-}
-__device__ __forceinline__ bool part_idle(const int *counts) { return counts[1] + counts[2] + counts[3] == 0; }
 // class counts of each tile
 __global__ void __launch_bounds__(kPartThreads) k_part_tiles(NCount npc, const int *counts, const int *cls,
                                                              Mig3 *tsum) {
@@ -771,19 +818,6 @@ __global__ void __launch_bounds__(kPartThreads) k_part_tiles(NCount npc, const i
   Mig3 tot;
   (void)block_scan3(s, tot);
   if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
-}
-// tile sums -> exclusive tile offsets (one block)
-__global__ void __launch_bounds__(kPartThreads) k_part_tile_scan(const int *counts, Mig3 *tsum, int ntiles) {
-  if (part_idle(counts)) return;
-  Mig3 carry{0, 0, 0};
-  for (int b = 0; b < ntiles; b += kPartThreads) {
-    const int t = b + threadIdx.x;
-    const Mig3 v = t < ntiles ? tsum[t] : Mig3{0, 0, 0};
-    Mig3 tot;
-    const Mig3 ex = block_scan3(v, tot);
-    if (t < ntiles) tsum[t] = m3_add(carry, ex);
-    carry = m3_add(carry, tot);
-  }
 }
 // west movers, east movers, then stayers, each in their original order; a
 // stayer's rank is q minus the movers and drops before it
@@ -951,7 +985,7 @@ struct VStep {
     const dim3 tiles((unsigned)ntiles());
     hipLaunchKernelGGL(k_part_tiles, tiles, dim3(kPartThreads), 0, st, n, (const int *)counts(), (const int *)cls(),
                        toff());
-    hipLaunchKernelGGL(k_part_tile_scan, dim3(1), dim3(kPartThreads), 0, st, (const int *)counts(), toff(), ntiles());
+    hipLaunchKernelGGL(k_part_tile_scan, dim3(1), dim3(kTileScanThreads), 0, st, (const int *)counts(), toff(), ntiles());
     hipLaunchKernelGGL(k_part_migrate, tiles, dim3(kPartThreads), 0, st, n, (const int *)cls(), (const Mig3 *)toff(),
                        (const int *)counts(), d, len, A, T);
     hipLaunchKernelGGL(k_copy_back, grid_for(3 * cap), dim3(256), 0, st, n, (const int *)counts(), T, A);
