@@ -108,9 +108,10 @@ class VortexStep:
         self.ops.p2m(a["pos"].data_ptr(), a["vort"].data_ptr(), n, self.mesh_vort.data_ptr(), clear=True)
         self.solver.solve_device(self.mesh_vort.data_ptr(), self.mesh_vel.data_ptr())
         self.ops.m2p(self.mesh_vel.data_ptr(), a["pos"].data_ptr(), n, a["vel"].data_ptr())
-        if self.visc is None:
-            a["dvort"][:3 * n].zero_()
-        else:
+        # Without a right-hand side dvort stays zero in both buffer sets:
+        # they start zeroed, load zeroes every field, remesh writes zero
+        # dvort, the RK update only reads it and the map does not carry it.
+        if self.visc is not None:
             vorticity_rhs_device(self.patch, self.precision, self.mesh_vort.data_ptr(), self.mesh_vel.data_ptr(),
                                  self.mesh_dvort.data_ptr(), self.visc, fd_order=self.fd_order,
                                  interp_order=self.interp_order)
