@@ -401,10 +401,12 @@ struct ListCols {
   int n;
 };
 // one partition pass of the list, block b on tile b from its offset toff[b];
-// movers' coordinates wrapped by one period
+// movers' coordinates wrapped by one period, and those still outside
+// [lo, hi) after the wrap counted in *still (the next pass's movers)
 __global__ void __launch_bounds__(kPartThreads) k_list_partition(int64_t n, const int *cls, const Mig3 *toff, int n1,
-                                                                 int n2, double len, const int *perm, int *perm2,
-                                                                 ListCols cols) {
+                                                                 int n2, double len, double lo, double hi,
+                                                                 const int *perm, int *perm2, ListCols cols,
+                                                                 int *still) {
   Mig3 run = toff[blockIdx.x];
   for (int j = 0; j < kPartItems; ++j) {
     const int64_t i = (int64_t)blockIdx.x * kPartTile + j * kPartThreads + threadIdx.x;
@@ -418,6 +420,7 @@ __global__ void __launch_bounds__(kPartThreads) k_list_partition(int64_t n, cons
     double x = cols.src[0][i];
     if (c == 1) x = x + len;
     if (c == 2) x = x - len;
+    if (c != 0 && (x < lo || x >= hi)) atomicAdd(still, 1);
     perm2[to] = perm ? perm[i] : (int)i;
     cols.dst[0][to] = x;
     for (int k = 1; k < cols.n; ++k) cols.dst[k][to] = cols.src[k][i];
@@ -534,7 +537,7 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
     const int per = geom->bound_cond[d] != 0;
     bool wrapped = false;
     for (int pass = 0; pass < 64; ++pass) {
-      PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * 4, st));
+      PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int) * 5, st));
       // the first pass of an axis gathers its coordinate column
       const int ntl = (int)((n + kPartTile - 1) / kPartTile);
       hipLaunchKernelGGL(k_list_classify, dim3(ntl), dim3(kPartThreads), 0, st, n,
@@ -556,13 +559,20 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
           cols.dst[cols.n++] = colbuf[e][1 - cur];
         }
       hipLaunchKernelGGL(k_list_partition, dim3(ntl), dim3(kPartThreads), 0, st, n, (const int *)cls, (const Mig3 *)tsum,
-                         counts[1], counts[2], len, listed ? (const int *)plist[cur] : nullptr, plist[1 - cur], cols);
+                         counts[1], counts[2], len, lo, hi, listed ? (const int *)plist[cur] : nullptr, plist[1 - cur],
+                         cols, w.count.as<int>() + 4);
       PM_HIP(hipGetLastError());
       cur = 1 - cur;
       listed = true;
       wrapped = wrapped || counts[1] + counts[2] > 0;
       n -= counts[3];
       if (!per || counts[1] + counts[2] == 0) break;
+      // a further pass only when a wrapped particle is still outside (it
+      // would find nothing to move otherwise)
+      int still = 0;
+      PM_HIP(hipMemcpyAsync(&still, w.count.as<int>() + 4, sizeof(int), hipMemcpyDeviceToHost, st));
+      PM_HIP(hipStreamSynchronize(st));
+      if (still == 0) break;
     }
     carried[d] = wrapped;
   }
