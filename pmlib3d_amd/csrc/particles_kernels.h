@@ -287,24 +287,36 @@ __device__ __forceinline__ void row_fetch(RowStage<T, NS> &st, const T *rec, int
   for (int h = 0; h < kRowCap / 64; ++h) {
     const int i = lane + 64 * h;
     // unconditional loads (no branch for the wait counters to merge over):
-    // an empty row reads record t0 <= np, inside the allocation of np + 1
-    const int t = t0 + (i < n ? i : (n > 0 ? n - 1 : 0));
-    const T *r = rec + t;
+    // an empty row reads record t0 <= np, inside the allocation of np + 1.
+    // FP32: field f of record t0 + li is read as (rec + f rs + t0)[li], a
+    // wave-uniform base (t0 is) and a 32-bit byte offset, so that no load
+    // needs per-lane 64-bit address arithmetic (gather 193 -> 188 us at 2.1M
+    // particles); FP64 keeps per-lane addresses (measured faster there)
+    const int li = i < n ? i : (n > 0 ? n - 1 : 0);
+    const T *rb = rec + t0, *r = rb + li;
+    const uint32_t lb = (uint32_t)li * (uint32_t)sizeof(T);
+    auto fld = [&](int f) -> T {
+      if constexpr (sizeof(T) == sizeof(float))
+        return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(rb + (int64_t)f * rs) + lb);
+      else
+        return r[f * rs];
+    };
 #pragma unroll
-    for (int s = 0; s < NS; ++s) st.val[h][s] = r[s * rs];
+    for (int s = 0; s < NS; ++s) st.val[h][s] = fld(s);
     if constexpr (sizeof(T) == sizeof(double)) {
-      st.val[h][NS] = r[(NS + sy) * rs];
-      st.val[h][NS + 1] = r[(2 * NS + sz) * rs];
+      st.val[h][NS] = fld(NS + sy);
+      st.val[h][NS + 1] = fld(2 * NS + sz);
     } else {
       // FP32: the record's y-z weight product, formed once for its ns lanes
-      st.val[h][NS] = r[(NS + sy) * rs] * r[(2 * NS + sz) * rs];
+      st.val[h][NS] = fld(NS + sy) * fld(2 * NS + sz);
       st.val[h][NS + 1] = (T)0;
     }
-    st.val[h][NS + 2] = r[KV * rs];
-    st.val[h][NS + 3] = r[(KV + 1) * rs];
-    st.val[h][NS + 4] = r[(KV + 2) * rs];
+    st.val[h][NS + 2] = fld(KV);
+    st.val[h][NS + 3] = fld(KV + 1);
+    st.val[h][NS + 4] = fld(KV + 2);
+    const T bx = fld(KB);
     // x slot of lane 0: sx = lane + (x0 + ns - 1 - kx)
-    st.xs[h] = x0 + NS - 1 - ((int)r[KB * rs] - blo0);
+    st.xs[h] = x0 + NS - 1 - ((int)bx - blo0);
   }
 }
 
