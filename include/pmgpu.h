@@ -227,12 +227,17 @@ int pmgpu_remesh_device(const pmgpu_geom *g, int32_t precision, const void *d_vo
 int pmgpu_particles_advance(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
                             int32_t euler_vort, double *pos, double *vel, double *vort, double *dvort,
                             double *vel_rk, double *dvort_rk, int64_t np, int64_t *np_out);
-/* Device variant; d_arrays = {pos, vel, vort, dvort, vel_rk, dvort_rk}. */
+/* Device variant; d_arrays = {pos, vel, vort, dvort, vel_rk, dvort_rk}.
+ * A NULL dvort means a zero right-hand side (dvort = dvort_rk = +0; no
+ * reference counterpart, for callers without one): the update then neither
+ * reads dvort nor reads, writes or carries dvort_rk (which may be NULL as
+ * well), with the same results as zero arrays. */
 int pmgpu_particles_advance_device(const pmgpu_geom *g, int32_t rk_order, int32_t stage, double dt,
                                    int32_t euler_vort, void *const d_arrays[6], int64_t np,
                                    int64_t *np_out, void *stream);
 /* The same with a second set of arrays, d_spare, of at least np particles
- * each and distinct from d_arrays (vel and dvort may be NULL).  No reference
+ * each and distinct from d_arrays (vel and dvort may be NULL, and dvort_rk
+ * with a zero right-hand side).  No reference
  * counterpart: it is for time loops that own two particle sets and swap
  * them.  When the map reorders particles, the carried arrays' new contents
  * are written to d_spare instead of back into d_arrays and *in_spare = 1; the

@@ -483,6 +483,10 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
   PM_CHECK(rk_order >= 1 && rk_order <= 3, "pmgpu: time integration order must be 1, 2 or 3");
   PM_CHECK(stage >= 1 && stage <= rk_order, "pmgpu: RK stage out of range");
   PM_CHECK(np >= 0 && np < (int64_t)1 << 31, "pmgpu: particle count out of range");
+  // dvort null: a zero right-hand side, dvort_rk not used (may be null too)
+  const bool rhs = arr[3] != nullptr;
+  PM_CHECK(np == 0 || (arr[0] && arr[1] && arr[2] && arr[4] && (!rhs || arr[5])),
+           "pmgpu_particles_advance: null array (only dvort, and with it dvort_rk, may be null)");
   RkCoef k = rk_coef(rk_order, stage, euler_vort);
   PartWS &w = ws();
   int64_t n = np;
@@ -499,7 +503,7 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
     if (w.count.ensure(sizeof(int) * 8)) return -1;
     PM_HIP(hipMemsetAsync(w.count.p, 0, sizeof(int), st));
     hipLaunchKernelGGL(k_rk_update, grid_for(3 * n), dim3(256), 0, st, k, dt, n, arr[0], arr[1], arr[2], arr[3],
-                       arr[4], arr[5], box, w.count.as<int>());
+                       arr[4], rhs ? arr[5] : nullptr, box, w.count.as<int>());
     PM_HIP(hipGetLastError());
     int any = 0;
     PM_HIP(hipMemcpyAsync(&any, w.count.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -510,10 +514,11 @@ int advance_device(const pmgpu_geom *geom, int rk_order, int stage, double dt, i
     }
   }
   // the arrays the reference's map carries (pmlib_particles_advance.f:121-166):
-  // pos and vort, and above first order vel_rk and dvort_rk; vel and dvort
-  // keep what the update left in them
+  // pos and vort, and above first order vel_rk and dvort_rk (not with a zero
+  // right-hand side: it stays zero); vel and dvort keep what the update left
+  // in them
   const int carry[4] = {0, 2, 4, 5};
-  const int narr = rk_order > 1 ? 4 : 2;
+  const int narr = rk_order > 1 ? (rhs ? 4 : 3) : 2;
   const int64_t n0 = n;
   // list workspace: cls, two index lists (ints), the tile totals and
   // offsets (Mig3), two buffers of each axis' coordinate column and the
@@ -769,7 +774,7 @@ int pmgpu_particles_advance_device_spare(const pmgpu_geom *g, int32_t rk_order, 
     ds[a] = (double *)d_spare[a];
   }
   const int carried[4] = {0, 2, 4, 5};
-  for (int a = 0; a < (rk_order > 1 ? 4 : 2); ++a)
+  for (int a = 0; a < (rk_order > 1 ? (dp[3] ? 4 : 3) : 2); ++a)
     PM_CHECK(ds[carried[a]] && ds[carried[a]] != dp[carried[a]],
              "pmgpu_particles_advance_device_spare: each carried spare array must be given and distinct");
   int flag = 0;
@@ -1036,8 +1041,9 @@ struct VStep {
         box.hi[d] = geom.xmax[d];
       }
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d_outside(), (int64_t)1, 0);
+      // no right-hand side in this step: dvort and dvort_rk stay zero
       hipLaunchKernelGGL(k_rk_update, grid_for(3 * cap), dim3(256), 0, st, k, dt, n, A.a[0], (const double *)A.a[1],
-                         A.a[2], (const double *)A.a[3], A.a[4], A.a[5], box, d_outside());
+                         A.a[2], (const double *)nullptr, A.a[4], (double *)nullptr, box, d_outside());
       PM_HIP(hipGetLastError());
       for (int d = 0; d < 3; ++d)
         if (migrate(d)) return -1;

@@ -692,20 +692,29 @@ struct PBox {
 
 // the RK stage update of every particle word; with `outside` given, also
 // flags (1) a particle that the update leaves outside the box on any axis
+// dvort null: a zero right-hand side (dvort = dvort_rk = +0), dvort_rk is
+// neither read nor written and vort takes the increment c2 dr that the full
+// form rounds for those zeros, dr = a2 (+0) + (b2 dt) (+0)
 static __global__ void k_rk_update(RkCoef k, double dt, NCount npc, double *pos, const double *vel, double *vort,
                                    const double *dvort, double *vel_rk, double *dvort_rk, PBox box, int *outside) {
   const int64_t np = npc.get();
+  const bool rhs = dvort != nullptr;
+  const double dinc = __dmul_rn(k.c2, __dadd_rn(__dmul_rn(k.a2, 0.0), __dmul_rn(__dmul_rn(k.b2, dt), 0.0)));
   // axis of word i, advanced with the grid stride (no 64-bit modulo)
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x, s3 = (int)(((int64_t)gridDim.x * blockDim.x) % 3);
   int d = i0 % 3;
   for (int64_t i = i0; i < 3 * np; i += (int64_t)gridDim.x * blockDim.x, d = (d + s3) % 3) {
     double vr = __dadd_rn(__dmul_rn(k.a1, vel_rk[i]), __dmul_rn(__dmul_rn(k.b1, dt), vel[i]));
-    double dr = __dadd_rn(__dmul_rn(k.a2, dvort_rk[i]), __dmul_rn(__dmul_rn(k.b2, dt), dvort[i]));
     vel_rk[i] = vr;
-    dvort_rk[i] = dr;
     const double x = __dadd_rn(pos[i], __dmul_rn(k.c1, vr));
     pos[i] = x;
-    vort[i] = __dadd_rn(vort[i], __dmul_rn(k.c2, dr));
+    if (rhs) {
+      double dr = __dadd_rn(__dmul_rn(k.a2, dvort_rk[i]), __dmul_rn(__dmul_rn(k.b2, dt), dvort[i]));
+      dvort_rk[i] = dr;
+      vort[i] = __dadd_rn(vort[i], __dmul_rn(k.c2, dr));
+    } else {
+      vort[i] = __dadd_rn(vort[i], dinc);
+    }
     if (outside) {
       const bool o = x < box.lo[d] || x >= box.hi[d];
       const uint64_t m = __ballot(o);

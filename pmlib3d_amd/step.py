@@ -119,8 +119,13 @@ class VortexStep:
         # the map writes a reordered set into the other buffer set, which then
         # becomes the current one (no copy back)
         spare = self.buf[1 - self.cur]
-        self.np, swapped = self.ops.advance(self.rk_order, s, dt, [a[k].data_ptr() for k in _FIELDS], n,
-                                            spare=[spare[k].data_ptr() for k in _FIELDS])
+        ptrs = [a[k].data_ptr() for k in _FIELDS]
+        sp = [spare[k].data_ptr() for k in _FIELDS]
+        if self.visc is None:
+            # a zero right-hand side: the update neither reads dvort nor
+            # reads, writes or carries dvort_rk (both stay zero)
+            ptrs[3] = ptrs[5] = sp[3] = sp[5] = None
+        self.np, swapped = self.ops.advance(self.rk_order, s, dt, ptrs, n, spare=sp)
         if swapped:
             self.cur = 1 - self.cur
 
