@@ -209,7 +209,8 @@ int pmgpu_remesh(const pmgpu_geom *g, const double *vort_m, const double *dvort_
                  int32_t vel_flag, int32_t incl_ghost, double *pos, double *vort, double *vel,
                  double *dvort, int64_t cap, int64_t *np_out);
 /* Device variant (meshes in the given precision, particles FP64); also
- * zeroes d_vel_rk and d_dvort_rk (either may be NULL). */
+ * zeroes d_vel_rk and d_dvort_rk; d_vel, d_dvort, d_vel_rk and d_dvort_rk
+ * may each be NULL (not written). */
 int pmgpu_remesh_device(const pmgpu_geom *g, int32_t precision, const void *d_vort_m, const void *d_dvort_m,
                         const void *d_vel_m, const void *d_mask, double min_vort, double min_dvort,
                         int32_t dvort_flag, int32_t vel_flag, int32_t incl_ghost, void *d_pos,

@@ -133,10 +133,11 @@ class VortexStep:
         a = self.arrays
         self.ops.p2m(a["pos"].data_ptr(), a["vort"].data_ptr(), self.np, self.mesh_vort.data_ptr(), clear=True)
         nxt = self.buf[1 - self.cur]
+        out = [nxt[k].data_ptr() for k in ("pos", "vort", "vel", "dvort", "vel_rk", "dvort_rk")]
+        if self.visc is None:
+            out[3] = out[5] = None   # zero already and never written without a right-hand side
         n = self.ops.remesh(self.mesh_vort.data_ptr(), None, self.mesh_vel.data_ptr(), min_vort, min_dvort,
-                            out=[nxt[k].data_ptr() for k in ("pos", "vort", "vel", "dvort", "vel_rk",
-                                                             "dvort_rk")],
-                            cap=self.cap, dvort=False, vel=True)
+                            out=out, cap=self.cap, dvort=False, vel=True)
         if n > self.cap:
             raise _lib.PmgpuError("remesh made %d particles, capacity %d" % (n, self.cap))
         self.cur = 1 - self.cur
