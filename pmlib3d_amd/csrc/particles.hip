@@ -6,7 +6,6 @@
 #include "../../include/pmgpu.h"
 #include "particles_kernels.h"
 
-#include <rocprim/device/device_scan.hpp>
 
 namespace pmgpu {
 
@@ -31,20 +30,128 @@ struct Scratch {
   template <typename T> T *as() { return reinterpret_cast<T *>(p); }
 };
 
-// exclusive scan of n ints (in -> out, not aliased); out[n] receives the
-// total: out[0] = 0 and an inclusive scan of in into out + 1
-__global__ void k_zero_first(int *out) { out[0] = 0; }
-static int scan_exclusive(const int *in, int *out, int64_t n, Scratch &tmp, hipStream_t st) {
-  PM_CHECK(in != out, "pmgpu: internal scan aliasing");
-  hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, st, out);
-  if (n > 0) {
-    size_t bytes = 0;
-    PM_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out + 1, (size_t)n, rocprim::plus<int>(), st));
-    if (tmp.ensure(bytes + 16)) return -1;
-    PM_HIP(rocprim::inclusive_scan(tmp.p, bytes, in, out + 1, (size_t)n, rocprim::plus<int>(), st));
+// Exclusive scan of n ints (in -> out, not aliased): out[0] = 0 and out[i +
+// 1] = in[0] + ... + in[i], so out[n] is the total.  Reduce, then scan, over
+// tiles of kIScanTile ints: the tile sums, one block turning them into tile
+// offsets, then each tile scanned through LDS (each thread a run of
+// kIScanItems) from its offset.  No memset, so a step may record it.
+constexpr int kIScanThreads = 256, kIScanItems = 16, kIScanTile = kIScanThreads * kIScanItems;
+constexpr int kIScanOffThreads = 1024;
+__device__ __forceinline__ int block_excl_int(int v, int &total) {
+  __shared__ int ws[kIScanThreads / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
   }
+  if (lane == 63) ws[wv] = x;
+  __syncthreads();
+  int base = 0;
+  total = 0;
+#pragma unroll
+  for (int k = 0; k < kIScanThreads / 64; ++k) {
+    if (k < wv) base += ws[k];
+    total += ws[k];
+  }
+  __syncthreads();   // ws is free for the next call
+  return base + x - v;
+}
+__global__ void __launch_bounds__(kIScanThreads) k_iscan_tiles(const int *in, int64_t n, int *tsum) {
+  const int64_t b = (int64_t)blockIdx.x * kIScanTile;
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < kIScanItems; ++j) {
+    const int64_t i = b + j * kIScanThreads + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  int tot;
+  (void)block_excl_int(s, tot);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+// tile sums -> exclusive tile offsets in place: each wavefront a contiguous
+// share, summed, then walked 64 tiles at a time (the next 64 prefetched)
+__global__ void __launch_bounds__(kIScanOffThreads) k_iscan_offsets(int *tsum, int ntiles) {
+  constexpr int NW = kIScanOffThreads / 64;
+  __shared__ int wtot[NW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int per = (ntiles + NW - 1) / NW;
+  const int t0 = min(wv * per, ntiles), t1 = min(t0 + per, ntiles);
+  int s = 0;
+  for (int t = t0 + lane; t < t1; t += 64) s += tsum[t];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) wtot[wv] = s;
+  __syncthreads();
+  int run = 0;
+  for (int k = 0; k < wv; ++k) run += wtot[k];
+  int vn = t0 + lane < t1 ? tsum[t0 + lane] : 0;
+  for (int b = t0; b < t1; b += 64) {
+    const int t = b + lane;
+    const int v = vn;
+    vn = t + 64 < t1 ? tsum[t + 64] : 0;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (t < t1) tsum[t] = run + x - v;
+    run += __shfl(x, 63);
+  }
+}
+__global__ void __launch_bounds__(kIScanThreads) k_iscan_apply(const int *in, int64_t n, const int *toff, int *out) {
+  __shared__ int sh[kIScanTile + kIScanTile / 32];
+  const int64_t b = (int64_t)blockIdx.x * kIScanTile;
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+#pragma unroll
+  for (int j = 0; j < kIScanItems; ++j) {
+    const int i = j * kIScanThreads + threadIdx.x;
+    sh[i + i / 32] = b + i < n ? in[b + i] : 0;
+  }
+  __syncthreads();
+  int v[kIScanItems], sum = 0;
+#pragma unroll
+  for (int k = 0; k < kIScanItems; ++k) {
+    const int i = threadIdx.x * kIScanItems + k;
+    v[k] = sh[i + i / 32];
+    sum += v[k];
+  }
+  int tot;
+  int run = block_excl_int(sum, tot) + toff[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kIScanItems; ++k) {
+    const int i = threadIdx.x * kIScanItems + k;
+    run += v[k];
+    sh[i + i / 32] = run;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kIScanItems; ++j) {
+    const int i = j * kIScanThreads + threadIdx.x;
+    if (b + i < n) out[1 + b + i] = sh[i + i / 32];
+  }
+}
+static int64_t iscan_tiles(int64_t n) { return (n + kIScanTile - 1) / kIScanTile; }
+// tsum: iscan_tiles(n) ints of scratch
+static int iscan_launch(const int *in, int *out, int64_t n, int *tsum, hipStream_t st) {
+  const int64_t nt = iscan_tiles(n);
+  PM_CHECK(nt < (int64_t)1 << 31, "pmgpu: scan too large");
+  if (nt == 0) {
+    PM_HIP(hipMemsetAsync(out, 0, sizeof(int), st));
+    return 0;
+  }
+  hipLaunchKernelGGL(k_iscan_tiles, dim3((unsigned)nt), dim3(kIScanThreads), 0, st, in, n, tsum);
+  hipLaunchKernelGGL(k_iscan_offsets, dim3(1), dim3(kIScanOffThreads), 0, st, tsum, (int)nt);
+  hipLaunchKernelGGL(k_iscan_apply, dim3((unsigned)nt), dim3(kIScanThreads), 0, st, in, n, (const int *)tsum, out);
   PM_HIP(hipGetLastError());
   return 0;
+}
+static int scan_exclusive(const int *in, int *out, int64_t n, Scratch &tmp, hipStream_t st) {
+  PM_CHECK(in != out, "pmgpu: internal scan aliasing");
+  if (tmp.ensure(sizeof(int) * ((size_t)iscan_tiles(n) + 1))) return -1;
+  return iscan_launch(in, out, n, tmp.as<int>(), st);
 }
 
 // Counts of west movers, east movers and drops (the migration classes 1, 2,
@@ -939,11 +1046,8 @@ struct VStep {
         w.rec.ensure(rec_bytes * ((size_t)cap + 1)) || w.flag.ensure(sizeof(int) * (nn + 1)) ||
         w.offs.ensure(sizeof(int) * (nn + 1)))
       return -1;
-    size_t b1 = 0, b2 = 0;
-    PM_HIP(rocprim::inclusive_scan(nullptr, b1, (int *)nullptr, (int *)nullptr, (size_t)nbins, rocprim::plus<int>(),
-                                   st));
-    PM_HIP(rocprim::inclusive_scan(nullptr, b2, (int *)nullptr, (int *)nullptr, (size_t)nn, rocprim::plus<int>(), st));
-    if (w.scan.ensure((b1 > b2 ? b1 : b2) + 16)) return -1;
+    const int64_t ts = iscan_tiles(nbins > nn ? nbins : nn);
+    if (w.scan.ensure(sizeof(int) * ((size_t)ts + 1))) return -1;
     const size_t pb = sizeof(double) * 3 * (size_t)cap * 6;
     if (parts.ensure(pb) || tmp.ensure(pb) || mesh.ensure(2 * mesh_elems() * (prec == PMGPU_FP64 ? 8 : 4)) ||
         mig.ensure(sizeof(int) * ((size_t)cap + 8) + sizeof(Mig3) * ((size_t)ntiles() + 1)) ||
@@ -964,10 +1068,7 @@ struct VStep {
     hipLaunchKernelGGL(k_fill_int, grid_for(nbins + 1), dim3(256), 0, st, w.count.as<int>(), (int64_t)nbins + 1, 0);
     hipLaunchKernelGGL(k_p2m_keys, grid_for(cap), dim3(256), 0, st, g, (const double *)A.a[0], n, nbins, key, rank,
                        w.count.as<int>());
-    hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, st, w.start.as<int>());
-    size_t bytes = w.scan.cap - 16;
-    PM_HIP(rocprim::inclusive_scan(w.scan.p, bytes, w.count.as<int>(), w.start.as<int>() + 1, (size_t)nbins,
-                                   rocprim::plus<int>(), st));
+    if (iscan_launch(w.count.as<int>(), w.start.as<int>(), nbins, w.scan.as<int>(), st)) return -1;
     hipLaunchKernelGGL(k_p2m_scatter, grid_for(cap), dim3(256), 0, st, n, nbins, key, rank, w.start.as<int>(),
                        id_out);
     if (prec == PMGPU_FP64) {
@@ -1055,10 +1156,7 @@ struct VStep {
       remesh_flags_launch<double>(g, nn, mvort(), nullptr, nullptr, min_vort, 0.0, 0, w.flag.as<int>(), st);
     else
       remesh_flags_launch<float>(g, nn, mvort(), nullptr, nullptr, min_vort, 0.0, 0, w.flag.as<int>(), st);
-    hipLaunchKernelGGL(k_zero_first, dim3(1), dim3(1), 0, st, w.offs.as<int>());
-    size_t bytes = w.scan.cap - 16;
-    PM_HIP(rocprim::inclusive_scan(w.scan.p, bytes, w.flag.as<int>(), w.offs.as<int>() + 1, (size_t)nn,
-                                   rocprim::plus<int>(), st));
+    if (iscan_launch(w.flag.as<int>(), w.offs.as<int>(), nn, w.scan.as<int>(), st)) return -1;
     if (prec == PMGPU_FP64)
       hipLaunchKernelGGL(k_remesh_create<double>, grid_for(nn), dim3(256), 0, st, g, (const int *)w.flag.as<int>(),
                          (const int *)w.offs.as<int>(), (const double *)mvort(), (const double *)nullptr,
